@@ -1,0 +1,109 @@
+/*
+ * speigs_hip.h -- MI355X extension of the speigs C API (libspeigs_hip.so).
+ *
+ * speigs.h stays ABI-identical to the reference; everything specific to the
+ * GPU build lives here, in separate structs, so stack-allocated speigs_opts /
+ * speigs_info of existing callers keep their layout.
+ *
+ * Two levels are exported, both plain C (no C++ or HIP types cross):
+ *   1. speigs_ex(): speigs() plus precision / start / telemetry knobs.
+ *   2. the device operator (speigs_hip_op_*): the reference's pluggable
+ *      operator seam -- speigs_op, op_apply, op_multA at
+ *      src/speigs_core.c:57-116 -- as an opaque handle owning the device CSR
+ *      matrix and the device supernodal factor.
+ * Error convention as in speigs.h: 0 or a negative SPEIGS_ERR_* code; nothing
+ * is printed and no process state is changed.  Every call is re-entrant: all
+ * state is per call or per handle, and each handle owns its HIP stream.
+ */
+#ifndef SPEIGS_HIP_H
+#define SPEIGS_HIP_H
+
+#include <stdint.h>
+#include "speigs.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* arithmetic of the shift-invert factor */
+#define SPEIGS_HIP_FP64     0   /* FP64 factor and solves: the drop-in path            */
+#define SPEIGS_HIP_FP32_IR  1   /* FP32 factor on MFMA, each apply refined to FP64 by
+                                 * iterative refinement on an FP64 SpMV residual      */
+
+/* start vector of the Lanczos recurrence */
+#define SPEIGS_HIP_START_UNIFORM  0  /* xorshift64* uniform [-0.5,0.5): the reference's
+                                      * start (src/speigs_core.c:185-191)               */
+#define SPEIGS_HIP_START_VECTOR   1  /* caller-supplied start_vec[n]                    */
+#define SPEIGS_HIP_START_LEARNED  2  /* learned start block (falls back to uniform)     */
+
+typedef struct {
+    int           precision;    /* SPEIGS_HIP_FP64 (default) or SPEIGS_HIP_FP32_IR          */
+    double        ir_tol;       /* refined apply: ||b - (A+dI)x|| <= ir_tol ||b||; 0 -> 1e-13 */
+    int           ir_maxit;     /* refinement sweeps per apply; 0 -> 12                      */
+    int           start;        /* SPEIGS_HIP_START_*                                        */
+    int           start_block;  /* learned start: number of proposed vectors; 0 -> k        */
+    int           use_graph;    /* 1 -> capture the triangular solves in a hipGraph          */
+    int           device;       /* HIP device ordinal; default 0                             */
+    int           verbose;      /* 0 silent                                                  */
+    const double *start_vec;    /* n values for SPEIGS_HIP_START_VECTOR                      */
+    int64_t       reserved[8];
+} speigs_hip_opts;
+
+typedef struct {
+    int64_t nnz_L;          /* stored entries of the supernodal factor (dense fronts)  */
+    double  factor_flops;   /* flops of the numeric factorization                      */
+    int64_t nsuper;         /* supernodes (fronts)                                     */
+    int64_t nlevels;        /* levels of the supernodal elimination tree               */
+    int64_t ir_steps;       /* refinement corrections over all applies                 */
+    int64_t solves;         /* triangular solve pairs executed                         */
+    int     precision_used; /* factor precision actually used                          */
+    int     fallback;       /* 1 if FP32 factor/refinement failed and FP64 took over    */
+    double  t_order, t_symbolic, t_upload, t_numeric, t_solve, t_spmv;
+    double  bytes_solve;    /* algorithmic bytes of one solve pair (factor + vectors)  */
+    double  bytes_spmv;     /* algorithmic bytes of one SpMV                           */
+    double  reserved[8];
+} speigs_hip_info;
+
+void speigs_hip_default_opts(speigs_hip_opts *o);
+
+/* speigs() with the extension knobs; hopts/hinfo may be NULL. */
+int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const double *Ax,
+              speigs_int k, speigs_which which, const speigs_opts *opts,
+              const speigs_hip_opts *hopts, double *lambda, double *V, double *resid,
+              speigs_info *info, speigs_hip_info *hinfo);
+
+/* number of visible HIP devices (0 when none); never fails */
+int speigs_hip_device_count(void);
+
+/* Ordering + supernodal symbolic analysis only: host code, no device needed.
+ * Replaces cholmod_l_analyze (src/speigs_core.c:933-936).  perm[n] (may be
+ * NULL) receives the fill-reducing order, perm[new] = old; snode[n] (may be
+ * NULL) the supernode of each new column; hinfo receives nnz_L, factor_flops,
+ * nsuper, nlevels, t_order and t_symbolic. */
+int speigs_hip_analyze(speigs_int n, const speigs_int *Ap, const speigs_int *Ai,
+                       speigs_int *perm, speigs_int *snode, speigs_hip_info *hinfo);
+
+/* ---- device operator: the pluggable seam (src/speigs_core.c:57-116) ------ */
+typedef struct speigs_hip_op speigs_hip_op;
+
+#define SPEIGS_HIP_OP_MULT  0   /* y = A x                        (op_multA, mode 0) */
+#define SPEIGS_HIP_OP_SOLVE 1   /* y = (A + delta I)^-1 x         (op_apply, mode 1) */
+
+/* Build the operator for A (CSC, upper triangle read).  For OP_SOLVE this runs
+ * ordering + symbolic analysis on the host and the numeric factorization on the
+ * device; returns SPEIGS_ERR_FACTOR if A + delta I is not positive definite. */
+int  speigs_hip_op_create(speigs_int n, const speigs_int *Ap, const speigs_int *Ai,
+                          const double *Ax, int mode, int precision, double delta,
+                          speigs_hip_op **op, speigs_hip_info *hinfo);
+/* refactor with a new shift (same pattern) */
+int  speigs_hip_op_refactor(speigs_hip_op *op, double delta);
+/* y[:,j] = M x[:,j] for nvec host vectors (column-major, leading dim n) */
+int  speigs_hip_op_apply(speigs_hip_op *op, const double *x, double *y, speigs_int nvec);
+/* y = A x (always A, never the shifted matrix) */
+int  speigs_hip_op_multA(speigs_hip_op *op, const double *x, double *y, speigs_int nvec);
+void speigs_hip_op_destroy(speigs_hip_op *op);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPEIGS_HIP_H */

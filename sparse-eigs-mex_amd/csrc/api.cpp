@@ -1,0 +1,782 @@
+// api.cpp -- the public C ABI of libspeigs_hip.so: speigs(), speigs_z(),
+// speigs_default_opts(), speigs_errmsg() (reference include/speigs.h:70-99,
+// driver src/speigs_core.c:783-1024) and the speigs_hip_* extension.
+//
+// The driver keeps the reference's host-side structure analysis -- symmetry
+// check, diagonal / dense short-circuits, connected components, banded gate,
+// shift escalation -- and hands the Krylov loop a device operator:
+//   'lm'        -> SELL-64 SpMV on the GPU            (path DIRECT)
+//   'sm'        -> supernodal Cholesky on the GPU     (path SHIFTINV)
+//   banded / LU -> host band Cholesky / dense LU solve behind the same seam
+// There is no CPU fallback for the GPU paths: without a HIP device they fail
+// with SPEIGS_ERR_INTERNAL.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "dev.h"
+#include "factor.h"
+
+using namespace sehip;
+
+namespace {
+
+struct KeyIdx {
+    double key;
+    int64_t idx;
+};
+int key_asc(const void *a, const void *b)
+{
+    double x = ((const KeyIdx *)a)->key, y = ((const KeyIdx *)b)->key;
+    return (x > y) - (x < y);
+}
+int key_desc(const void *a, const void *b) { return -key_asc(a, b); }
+
+// 1 full symmetric, 2 upper only, 3 lower only, 0 not symmetric (src/speigs_core.c:363-401)
+int symmetry_kind(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax)
+{
+    bool up = false, lo = false;
+    for (int64_t c = 0; c < n && !(up && lo); c++)
+        for (int64_t q = Ap[c]; q < Ap[c + 1]; q++) {
+            if (Ai[q] < c) up = true;
+            else if (Ai[q] > c) lo = true;
+        }
+    if (up && !lo) return 2;
+    if (lo && !up) return 3;
+    if (!up && !lo) return 1;
+    const int64_t nnz = Ap[n];
+    std::vector<int64_t> Tp(n + 1, 0), Ti(nnz), nx(n, 0);
+    std::vector<double> Tx(nnz);
+    for (int64_t q = 0; q < nnz; q++) nx[Ai[q]]++;
+    for (int64_t i = 0; i < n; i++) { Tp[i + 1] = Tp[i] + nx[i]; nx[i] = Tp[i]; }
+    for (int64_t c = 0; c < n; c++)
+        for (int64_t q = Ap[c]; q < Ap[c + 1]; q++) {
+            int64_t d = nx[Ai[q]]++;
+            Ti[d] = c;
+            Tx[d] = Ax[q];
+        }
+    for (int64_t i = 0; i <= n; i++)
+        if (Tp[i] != Ap[i]) return 0;
+    for (int64_t q = 0; q < nnz; q++) {
+        if (Ti[q] != Ai[q]) return 0;
+        double a = Ax[q], b = Tx[q], mx = std::max(std::fabs(a), std::fabs(b));
+        if (std::fabs(a - b) > 1e-12 * (mx > 0 ? mx : 1.0)) return 0;
+    }
+    return 1;
+}
+
+int diag_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax, int64_t k, speigs_which which,
+              double *lambda, double *V, double *resid, speigs_info *info)
+{
+    std::vector<double> d(n, 0.0);
+    std::vector<KeyIdx> ord(n);
+    for (int64_t c = 0; c < n; c++)
+        for (int64_t p = Ap[c]; p < Ap[c + 1]; p++)
+            if (Ai[p] == c) d[c] = Ax[p];
+    for (int64_t i = 0; i < n; i++) ord[i] = {std::fabs(d[i]), i};
+    qsort(ord.data(), (size_t)n, sizeof(KeyIdx), which == SPEIGS_SM ? key_asc : key_desc);
+    for (int64_t t = 0; t < k; t++) {
+        int64_t i = ord[t].idx;
+        lambda[t] = d[i];
+        if (V) {
+            std::memset(V + t * n, 0, (size_t)n * sizeof(double));
+            V[t * n + i] = 1.0;
+        }
+        if (resid) resid[t] = 0.0;
+    }
+    info->nconv = k;
+    info->flag = 0;
+    info->path = SPEIGS_PATH_DIAG;
+    return SPEIGS_OK;
+}
+
+int dense_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax, int64_t k, speigs_which which,
+               double *lambda, double *V, double *resid, speigs_info *info)
+{
+    std::vector<double> D((size_t)n * n, 0.0), ev(n);
+    for (int64_t c = 0; c < n; c++)
+        for (int64_t p = Ap[c]; p < Ap[c + 1]; p++) {
+            D[Ai[p] + c * n] = Ax[p];
+            D[c + Ai[p] * n] = Ax[p];
+        }
+    if (dense_syev(n, D.data(), ev.data())) return SPEIGS_ERR_LAPACK;
+    std::vector<KeyIdx> ord(n);
+    for (int64_t i = 0; i < n; i++) ord[i] = {std::fabs(ev[i]), i};
+    qsort(ord.data(), (size_t)n, sizeof(KeyIdx), which == SPEIGS_SM ? key_asc : key_desc);
+    for (int64_t t = 0; t < k; t++) {
+        int64_t i = ord[t].idx;
+        lambda[t] = ev[i];
+        if (V) std::memcpy(V + t * n, D.data() + i * n, (size_t)n * sizeof(double));
+        if (resid) resid[t] = 0.0;
+    }
+    info->nconv = k;
+    info->flag = 0;
+    info->path = SPEIGS_PATH_DENSE;
+    return SPEIGS_OK;
+}
+
+int64_t uf_root(std::vector<int64_t> &p, int64_t x)
+{
+    while (p[x] != x) {
+        p[x] = p[p[x]];
+        x = p[x];
+    }
+    return x;
+}
+
+// connected components over numerically nonzero entries (src/speigs_core.c:436-466)
+int64_t components(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax, std::vector<int64_t> &comp)
+{
+    std::vector<int64_t> p(n), lab(n, -1);
+    for (int64_t i = 0; i < n; i++) p[i] = i;
+    for (int64_t c = 0; c < n; c++)
+        for (int64_t q = Ap[c]; q < Ap[c + 1]; q++)
+            if (Ax[q] != 0.0) {
+                int64_t a = uf_root(p, Ai[q]), b = uf_root(p, c);
+                if (a != b) p[a] = b;
+            }
+    int64_t nc = 0;
+    comp.resize(n);
+    for (int64_t i = 0; i < n; i++) {
+        int64_t r = uf_root(p, i);
+        if (lab[r] < 0) lab[r] = nc++;
+        comp[i] = lab[r];
+    }
+    return nc;
+}
+
+// RCM over the stored column lists and the band width it yields (src/speigs_core.c:471-523)
+void rcm(int64_t n, const int64_t *Ap, const int64_t *Ai, std::vector<int64_t> &perm)
+{
+    std::vector<int64_t> deg(n), q(n);
+    std::vector<char> seen(n, 0);
+    for (int64_t c = 0; c < n; c++) deg[c] = Ap[c + 1] - Ap[c];
+    int64_t head = 0, tail = 0;
+    for (int64_t st = 0; st < n; st++) {
+        if (seen[st]) continue;
+        int64_t s = st, best = deg[st];
+        for (int64_t i = st; i < n; i++)
+            if (!seen[i] && deg[i] < best) { best = deg[i]; s = i; }
+        seen[s] = 1;
+        q[tail++] = s;
+        while (head < tail) {
+            int64_t v = q[head++], b0 = tail;
+            for (int64_t p = Ap[v]; p < Ap[v + 1]; p++) {
+                int64_t u = Ai[p];
+                if (!seen[u]) { seen[u] = 1; q[tail++] = u; }
+            }
+            for (int64_t a = b0 + 1; a < tail; a++) {
+                int64_t key = q[a], dk = deg[key], b = a - 1;
+                while (b >= b0 && deg[q[b]] > dk) { q[b + 1] = q[b]; b--; }
+                q[b + 1] = key;
+            }
+        }
+    }
+    perm.resize(n);
+    for (int64_t i = 0; i < n; i++) perm[i] = q[n - 1 - i];
+}
+
+double norm1_upper(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax)
+{
+    std::vector<double> cs(n, 0.0);
+    for (int64_t j = 0; j < n; j++)
+        for (int64_t q = Ap[j]; q < Ap[j + 1]; q++) {
+            int64_t i = Ai[q];
+            if (i > j) continue;
+            cs[j] += std::fabs(Ax[q]);
+            if (i != j) cs[i] += std::fabs(Ax[q]);
+        }
+    double m = 0.0;
+    for (double v : cs) m = std::max(m, v);
+    return m;
+}
+
+// ------------------------------------------------------------- operators
+struct MultOp : DevOperator {
+    SellA A;
+    int apply(const double *x, double *y) override
+    {
+        spmv(A, x, y, stream);
+        return 0;
+    }
+    int multA(const double *X, int64_t ldx, double *Y, int64_t ldy, int nv) override
+    {
+        spmv_multi(A, X, ldx, Y, ldy, nv, stream);
+        return 0;
+    }
+};
+
+// host-side solve behind the device seam (banded Cholesky or dense LU short-circuits)
+struct HostSolveOp : MultOp {
+    int kind = 0;                        // 0 banded, 1 LU
+    std::vector<double> fac, bw;
+    std::vector<int64_t> perm, piv;
+    int64_t kd = 0;
+    int apply(const double *x, double *y) override
+    {
+        std::vector<double> hx(n);
+        if (hipMemcpyAsync(hx.data(), x, n * 8, hipMemcpyDeviceToHost, stream) != hipSuccess) return SPEIGS_ERR_INTERNAL;
+        if (hipStreamSynchronize(stream) != hipSuccess) return SPEIGS_ERR_INTERNAL;
+        if (kind == 0) {
+            for (int64_t i = 0; i < n; i++) bw[i] = hx[perm[i]];
+            band_solve_u(n, kd, fac.data(), kd + 1, bw.data());
+            for (int64_t i = 0; i < n; i++) hx[perm[i]] = bw[i];
+        } else {
+            lu_solve(n, fac.data(), piv.data(), hx.data());
+        }
+        if (hipMemcpyAsync(y, hx.data(), n * 8, hipMemcpyHostToDevice, stream) != hipSuccess) return SPEIGS_ERR_INTERNAL;
+        if (hipStreamSynchronize(stream) != hipSuccess) return SPEIGS_ERR_INTERNAL;
+        return 0;
+    }
+};
+
+// supernodal Cholesky on the device
+struct FactorOp : MultOp {
+    Factor *F = nullptr;
+    ~FactorOp() override { factor_destroy(F); }
+    int apply(const double *x, double *y) override { return factor_apply(F, x, y); }
+};
+
+struct StreamGuard {
+    hipStream_t s = nullptr;
+    ~StreamGuard()
+    {
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+int try_banded(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax, const speigs_opts *o, double anorm,
+               HostSolveOp *op, speigs_info *info)
+{
+    int bmax = o->band_max ? o->band_max : 512;
+    if (bmax <= 0) return 0;
+    double tb = wall();
+    std::vector<int64_t> pm, ip(n);
+    rcm(n, Ap, Ai, pm);
+    for (int64_t i = 0; i < n; i++) ip[pm[i]] = i;
+    int64_t kd = 0;
+    for (int64_t c = 0; c < n; c++)
+        for (int64_t q = Ap[c]; q < Ap[c + 1]; q++) kd = std::max<int64_t>(kd, std::llabs(ip[Ai[q]] - ip[c]));
+    double cells = (double)(kd + 1) * (double)n;
+    double occ = kd > 0 ? (double)Ap[n] / ((double)n * (double)kd) : 1.0;
+    if (!(kd <= bmax && cells <= 2.0e7 && occ >= 0.05)) return 0;
+    const int64_t ld = kd + 1;
+    std::vector<double> AB((size_t)cells, 0.0);
+    for (int64_t c = 0; c < n; c++)
+        for (int64_t q = Ap[c]; q < Ap[c + 1]; q++) {
+            int64_t r = ip[Ai[q]], cc = ip[c];
+            int64_t lo = std::min(r, cc), hi = std::max(r, cc);
+            AB[(kd + lo - hi) + (size_t)hi * ld] = Ax[q];
+        }
+    std::vector<double> Wk(AB);
+    int bad = band_chol_u(n, kd, Wk.data(), ld);
+    double dl = 0.0;
+    if (bad) {
+        // the reference's loop steps the shift once more after success
+        // (src/speigs_core.c:908-912), so the reported shift is 10x the one used
+        double d0 = anorm * (o->shift0 > 0 ? o->shift0 : 1e-6), dcap = anorm * 1e-3;
+        if (d0 > dcap) dcap = d0;
+        for (dl = d0; dl <= dcap * 1.0000001 && bad; dl *= 10.0) {
+            Wk = AB;
+            for (int64_t j = 0; j < n; j++) Wk[kd + (size_t)j * ld] += dl;
+            bad = band_chol_u(n, kd, Wk.data(), ld);
+        }
+        if (bad) dl = 0.0;
+    }
+    if (bad) return 0;
+    op->kind = 0;
+    op->fac.swap(Wk);
+    op->perm = pm;
+    op->kd = kd;
+    op->bw.assign(n, 0.0);
+    info->shift = dl;
+    info->path = SPEIGS_PATH_BANDED;
+    info->t_factor = wall() - tb;
+    return 1;
+}
+
+int blocks_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax, int64_t k, speigs_which which,
+                const speigs_opts *opts, const speigs_hip_opts *ho, const std::vector<int64_t> &comp, int64_t ncomp,
+                double *lambda, double *V, double *resid, speigs_info *info);
+
+}  // namespace
+
+extern "C" {
+
+void speigs_default_opts(speigs_opts *o)
+{
+    std::memset(o, 0, sizeof *o);
+    o->dense_max = -1;
+    o->detect = 1;
+    o->check_sym = 1;
+}
+
+void speigs_hip_default_opts(speigs_hip_opts *o)
+{
+    std::memset(o, 0, sizeof *o);
+    o->precision = SPEIGS_HIP_FP64;
+    o->start = SPEIGS_HIP_START_UNIFORM;
+}
+
+const char *speigs_errmsg(int c)
+{
+    switch (c) {
+    case SPEIGS_OK: return "ok";
+    case SPEIGS_ERR_ARG: return "invalid argument";
+    case SPEIGS_ERR_MEM: return "out of memory";
+    case SPEIGS_ERR_FACTOR: return "factorization failed";
+    case SPEIGS_ERR_LAPACK: return "LAPACK failed";
+    case SPEIGS_ERR_NOTSYM: return "matrix is not symmetric (this solver is for symmetric/Hermitian matrices only)";
+    default: return "internal error";
+    }
+}
+
+int speigs_hip_device_count(void)
+{
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return c;
+}
+
+int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const double *Ax, speigs_int k,
+              speigs_which which, const speigs_opts *opts, const speigs_hip_opts *hopts, double *lambda, double *V,
+              double *resid, speigs_info *info, speigs_hip_info *hinfo)
+{
+    speigs_opts dflt;
+    if (!opts) {
+        speigs_default_opts(&dflt);
+        opts = &dflt;
+    }
+    speigs_hip_opts hdflt;
+    if (!hopts) {
+        speigs_hip_default_opts(&hdflt);
+        hopts = &hdflt;
+    }
+    speigs_info li;
+    if (!info) info = &li;
+    std::memset(info, 0, sizeof *info);
+    speigs_hip_info lhi;
+    if (!hinfo) hinfo = &lhi;
+    std::memset(hinfo, 0, sizeof *hinfo);
+
+    if (n <= 0 || !Ap || !Ai || !Ax || !lambda) return SPEIGS_ERR_ARG;
+    if (k <= 0 || k > n) return SPEIGS_ERR_ARG;
+    if (opts->check_sym != 0 && symmetry_kind(n, Ap, Ai, Ax) == 0) return SPEIGS_ERR_NOTSYM;
+
+    const double t0 = wall();
+    const int dmax = opts->dense_max < 0 ? 800 : opts->dense_max;
+    if (opts->detect) {
+        bool diag = true;
+        for (int64_t c = 0; c < n && diag; c++)
+            for (int64_t p = Ap[c]; p < Ap[c + 1]; p++)
+                if (Ai[p] != c) { diag = false; break; }
+        if (diag) return diag_path(n, Ap, Ai, Ax, k, which, lambda, V, resid, info);
+    }
+    if (dmax > 0 && (n <= dmax || 4 * k > n)) {
+        int rc = dense_path(n, Ap, Ai, Ax, k, which, lambda, V, resid, info);
+        info->t_iter = wall() - t0;
+        return rc;
+    }
+    if (opts->detect) {
+        std::vector<int64_t> comp;
+        int64_t nc = components(n, Ap, Ai, Ax, comp);
+        if (nc > 1) {
+            int rc = blocks_path(n, Ap, Ai, Ax, k, which, opts, hopts, comp, nc, lambda, V, resid, info);
+            info->t_iter = wall() - t0;
+            return rc;
+        }
+    }
+    double anorm = norm1_upper(n, Ap, Ai, Ax);
+    if (anorm <= 0) anorm = 1.0;
+    info->anorm = anorm;
+
+    // ---- the GPU part: needs a device, no CPU fallback
+    if (speigs_hip_device_count() <= 0) return SPEIGS_ERR_INTERNAL;
+    if (hipSetDevice(hopts->device) != hipSuccess) return SPEIGS_ERR_ARG;
+    StreamGuard sg;
+    if (hipStreamCreate(&sg.s) != hipSuccess) return SPEIGS_ERR_INTERNAL;
+    DevMem mem;
+    std::unique_ptr<MultOp> op;
+    int rc = SPEIGS_OK;
+
+    if (which == SPEIGS_LM) {
+        op.reset(new MultOp());
+        info->path = SPEIGS_PATH_DIRECT;
+    } else {
+        auto *hs = new HostSolveOp();
+        if (try_banded(n, Ap, Ai, Ax, opts, anorm, hs, info)) {
+            op.reset(hs);
+        } else {
+            delete hs;
+            auto *fo = new FactorOp();
+            op.reset(fo);
+            FactorOptions fopt;
+            fopt.precision = hopts->precision;
+            fopt.ir_tol = hopts->ir_tol > 0 ? hopts->ir_tol : 1e-13;
+            fopt.ir_maxit = hopts->ir_maxit > 0 ? hopts->ir_maxit : 12;
+            fopt.ordering = opts->ordering;
+            fopt.use_graph = hopts->use_graph;
+            double ta = wall();
+            rc = factor_create(n, Ap, Ai, Ax, fopt, sg.s, &fo->F, hinfo);
+            if (rc) return rc;
+            info->t_analyze = wall() - ta;
+            double tf = wall();
+            double delta = 0.0;
+            int st = factor_numeric(fo->F, 0.0);
+            if (st < 0) return st;
+            if (st == 1) {
+                double d0 = anorm * (opts->shift0 > 0 ? opts->shift0 : 1e-6), dcap = anorm * 1e-3;
+                if (d0 > dcap) dcap = d0;
+                for (delta = d0; delta <= dcap * 1.0000001; delta *= 10.0) {
+                    st = factor_numeric(fo->F, delta);
+                    if (st < 0) return st;
+                    if (st != 1) break;
+                }
+                if (st == 1) delta = 0.0;
+            }
+            if (st == 0) {
+                info->shift = delta;
+                info->path = SPEIGS_PATH_SHIFTINV;
+            } else {
+                // indefinite: LU at sigma = 0, dense on the host for small n
+                factor_destroy(fo->F);
+                fo->F = nullptr;
+                if (n > 6000) return SPEIGS_ERR_FACTOR;
+                auto *lu = new HostSolveOp();
+                op.reset(lu);
+                lu->kind = 1;
+                lu->fac.assign((size_t)n * n, 0.0);
+                lu->piv.assign(n, 0);
+                for (int64_t c = 0; c < n; c++)
+                    for (int64_t p = Ap[c]; p < Ap[c + 1]; p++) {
+                        int64_t r = Ai[p];
+                        if (r > c) continue;
+                        lu->fac[r + c * n] = Ax[p];
+                        lu->fac[c + r * n] = Ax[p];
+                    }
+                if (lu_factor(n, lu->fac.data(), lu->piv.data())) return SPEIGS_ERR_FACTOR;
+                info->shift = 0.0;
+                info->path = SPEIGS_PATH_LU;
+            }
+            info->t_factor = wall() - tf;
+            hinfo->t_numeric = info->t_factor;
+        }
+    }
+    op->n = n;
+    op->stream = sg.s;
+    double tu = wall();
+    if ((rc = sell_build(n, Ap, Ai, Ax, &op->A, mem, sg.s))) return rc;
+    hinfo->t_upload += wall() - tu;
+    hinfo->bytes_spmv = spmv_bytes(op->A);
+    if (auto *fo = dynamic_cast<FactorOp *>(op.get())) {
+        if (fo->F) factor_attach_matrix(fo->F, &op->A);
+    }
+
+    double ti = wall();
+    rc = lanczos_dev(op.get(), n, k, opts, hopts, anorm, lambda, V, resid, info);
+    info->t_iter = wall() - ti;
+    if (auto *fo = dynamic_cast<FactorOp *>(op.get())) {
+        if (fo->F) factor_stats(fo->F, hinfo);
+    }
+    if (rc == SPEIGS_OK) {
+        std::vector<KeyIdx> ord(k);
+        std::vector<double> lc(lambda, lambda + k), rc2(k), Vc;
+        if (resid) rc2.assign(resid, resid + k);
+        if (V) Vc.assign(V, V + (size_t)n * k);
+        for (int64_t i = 0; i < k; i++) ord[i] = {std::fabs(lambda[i]), i};
+        qsort(ord.data(), (size_t)k, sizeof(KeyIdx), which == SPEIGS_SM ? key_asc : key_desc);
+        for (int64_t t = 0; t < k; t++) {
+            int64_t i = ord[t].idx;
+            lambda[t] = lc[i];
+            if (resid) resid[t] = rc2[i];
+            if (V) std::memcpy(V + t * n, Vc.data() + i * n, (size_t)n * sizeof(double));
+        }
+    }
+    return rc;
+}
+
+int speigs(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const double *Ax, speigs_int k,
+           speigs_which which, const speigs_opts *opts, double *lambda, double *V, double *resid, speigs_info *info)
+{
+    return speigs_ex(n, Ap, Ai, Ax, k, which, opts, nullptr, lambda, V, resid, info, nullptr);
+}
+
+// complex Hermitian through the real embedding [[R,-S],[S,R]] (src/speigs_core.c:660-768)
+int speigs_z(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const double *Axr, const double *Axi,
+             speigs_int k, speigs_which which, const speigs_opts *opts, double *lambda, double *Vr, double *Vi,
+             double *resid, speigs_info *info)
+{
+    if (!Axi) return speigs(n, Ap, Ai, Axr, k, which, opts, lambda, Vr, resid, info);
+    if (n <= 0 || k <= 0 || k > n || !Ap || !Ai || !Axr || !lambda) return SPEIGS_ERR_ARG;
+    const int64_t n2 = 2 * n, nnz = Ap[n];
+    std::vector<int64_t> Bp(n2 + 1), Bi;
+    std::vector<double> Bx;
+    Bi.reserve(4 * nnz);
+    Bx.reserve(4 * nnz);
+    for (int64_t j = 0; j < n; j++) {
+        Bp[j] = (int64_t)Bi.size();
+        for (int64_t q = Ap[j]; q < Ap[j + 1]; q++)
+            if (Axr[q] != 0.0) { Bi.push_back(Ai[q]); Bx.push_back(Axr[q]); }
+        for (int64_t q = Ap[j]; q < Ap[j + 1]; q++)
+            if (Axi[q] != 0.0) { Bi.push_back(n + Ai[q]); Bx.push_back(Axi[q]); }
+    }
+    for (int64_t j = 0; j < n; j++) {
+        Bp[n + j] = (int64_t)Bi.size();
+        for (int64_t q = Ap[j]; q < Ap[j + 1]; q++)
+            if (Axi[q] != 0.0) { Bi.push_back(Ai[q]); Bx.push_back(-Axi[q]); }
+        for (int64_t q = Ap[j]; q < Ap[j + 1]; q++)
+            if (Axr[q] != 0.0) { Bi.push_back(n + Ai[q]); Bx.push_back(Axr[q]); }
+    }
+    Bp[n2] = (int64_t)Bi.size();
+    if (Bi.empty()) { Bi.push_back(0); Bx.push_back(0.0); }
+    const int64_t kk = std::min<int64_t>(2 * k + 2, n2);
+    std::vector<double> l2(kk), r2(kk), V2((size_t)n2 * kk), ar((size_t)n * k, 0.0), ai((size_t)n * k, 0.0), wr(n), wi(n);
+    int rc = speigs(n2, Bp.data(), Bi.data(), Bx.data(), kk, which, opts, l2.data(), V2.data(), r2.data(), info);
+    if (rc) return rc;
+    int64_t nacc = 0;
+    for (int64_t c = 0; c < kk && nacc < k; c++) {
+        const double *a = V2.data() + (size_t)c * n2, *b = a + n;
+        double nn = 0.0;
+        for (int64_t i = 0; i < n; i++) nn += a[i] * a[i] + b[i] * b[i];
+        if (nn <= 0.0) continue;
+        nn = 1.0 / std::sqrt(nn);
+        for (int64_t i = 0; i < n; i++) { wr[i] = a[i] * nn; wi[i] = b[i] * nn; }
+        for (int pass = 0; pass < 2; pass++)
+            for (int64_t u = 0; u < nacc; u++) {
+                const double *ur = ar.data() + u * n, *ui = ai.data() + u * n;
+                double re = 0.0, im = 0.0;
+                for (int64_t i = 0; i < n; i++) {
+                    re += ur[i] * wr[i] + ui[i] * wi[i];
+                    im += ur[i] * wi[i] - ui[i] * wr[i];
+                }
+                for (int64_t i = 0; i < n; i++) {
+                    wr[i] -= re * ur[i] - im * ui[i];
+                    wi[i] -= re * ui[i] + im * ur[i];
+                }
+            }
+        double wn = 0.0;
+        for (int64_t i = 0; i < n; i++) wn += wr[i] * wr[i] + wi[i] * wi[i];
+        wn = std::sqrt(wn);
+        if (wn <= 0.1) continue;     // an i-multiple of an accepted direction
+        for (int64_t i = 0; i < n; i++) {
+            ar[nacc * n + i] = wr[i] / wn;
+            ai[nacc * n + i] = wi[i] / wn;
+        }
+        lambda[nacc] = l2[c];
+        if (resid) resid[nacc] = r2[c];
+        nacc++;
+    }
+    if (Vr) std::memcpy(Vr, ar.data(), (size_t)n * k * sizeof(double));
+    if (Vi) std::memcpy(Vi, ai.data(), (size_t)n * k * sizeof(double));
+    if (info) {
+        info->nconv = nacc;
+        if (nacc < k) info->flag = 1;
+    }
+    return SPEIGS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int blocks_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax, int64_t k, speigs_which which,
+                const speigs_opts *opts, const speigs_hip_opts *ho, const std::vector<int64_t> &comp, int64_t ncomp,
+                double *lambda, double *V, double *resid, speigs_info *info)
+{
+    std::vector<int64_t> sz(ncomp, 0), off(ncomp + 1, 0), gidx(n), loc(n), fill(ncomp, 0);
+    for (int64_t i = 0; i < n; i++) sz[comp[i]]++;
+    for (int64_t c = 0; c < ncomp; c++) off[c + 1] = off[c] + sz[c];
+    for (int64_t i = 0; i < n; i++) {
+        int64_t c = comp[i], slot = off[c] + fill[c]++;
+        gidx[slot] = i;
+        loc[i] = slot - off[c];
+    }
+    std::vector<std::vector<double>> cv(ncomp);
+    std::vector<double> clam, cres;
+    std::vector<int64_t> ccmp, cslt;
+    std::vector<int64_t> sAp(n + 1), sAi(std::max<int64_t>(Ap[n], 1));
+    std::vector<double> sAx(std::max<int64_t>(Ap[n], 1));
+    for (int64_t c = 0; c < ncomp; c++) {
+        int64_t nc = sz[c], cnt = 0;
+        for (int64_t jj = 0; jj < nc; jj++) {
+            int64_t g = gidx[off[c] + jj];
+            sAp[jj] = cnt;
+            for (int64_t q = Ap[g]; q < Ap[g + 1]; q++)
+                if (comp[Ai[q]] == c) {
+                    sAi[cnt] = loc[Ai[q]];
+                    sAx[cnt] = Ax[q];
+                    cnt++;
+                }
+        }
+        sAp[nc] = cnt;
+        int64_t kc = std::min(k, nc);
+        std::vector<double> lv(kc), rv(kc);
+        if (V) cv[c].assign((size_t)nc * kc, 0.0);
+        speigs_info si;
+        int r2 = speigs_ex(nc, sAp.data(), sAi.data(), sAx.data(), kc, which, opts, ho, lv.data(),
+                           V ? cv[c].data() : nullptr, rv.data(), &si, nullptr);
+        if (r2) return r2;
+        info->nops += si.nops;
+        if (si.flag) info->flag = 1;
+        for (int64_t t = 0; t < kc; t++) {
+            clam.push_back(lv[t]);
+            cres.push_back(rv[t]);
+            ccmp.push_back(c);
+            cslt.push_back(t);
+        }
+    }
+    const int64_t ncand = (int64_t)clam.size();
+    std::vector<KeyIdx> ord(ncand);
+    for (int64_t i = 0; i < ncand; i++) ord[i] = {std::fabs(clam[i]), i};
+    qsort(ord.data(), (size_t)ncand, sizeof(KeyIdx), which == SPEIGS_SM ? key_asc : key_desc);
+    for (int64_t t = 0; t < k && t < ncand; t++) {
+        int64_t i = ord[t].idx, c = ccmp[i], sl = cslt[i], nc = sz[c];
+        lambda[t] = clam[i];
+        if (resid) resid[t] = cres[i];
+        if (V) {
+            std::memset(V + t * n, 0, (size_t)n * sizeof(double));
+            const double *src = cv[c].data() + sl * nc;
+            for (int64_t jj = 0; jj < nc; jj++) V[t * n + gidx[off[c] + jj]] = src[jj];
+        }
+    }
+    info->nconv = std::min(k, ncand);
+    info->ncomp = ncomp;
+    info->path = SPEIGS_PATH_BLOCKS;
+    return SPEIGS_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- operator ABI
+struct speigs_hip_op {
+    int mode = 0;
+    int64_t n = 0;
+    hipStream_t stream = nullptr;
+    DevMem mem;
+    SellA A;
+    Factor *F = nullptr;
+    double *x = nullptr, *y = nullptr;
+    int64_t cap = 0;
+    ~speigs_hip_op()
+    {
+        factor_destroy(F);
+        mem.release();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+extern "C" {
+
+int speigs_hip_op_create(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const double *Ax, int mode,
+                         int precision, double delta, speigs_hip_op **out, speigs_hip_info *hinfo)
+{
+    if (!out || n <= 0 || !Ap || !Ai || !Ax) return SPEIGS_ERR_ARG;
+    *out = nullptr;
+    if (mode != SPEIGS_HIP_OP_MULT && mode != SPEIGS_HIP_OP_SOLVE) return SPEIGS_ERR_ARG;
+    if (speigs_hip_device_count() <= 0) return SPEIGS_ERR_INTERNAL;
+    speigs_hip_info lhi;
+    if (!hinfo) hinfo = &lhi;
+    std::memset(hinfo, 0, sizeof *hinfo);
+    std::unique_ptr<speigs_hip_op> op(new speigs_hip_op());
+    op->mode = mode;
+    op->n = n;
+    if (hipStreamCreate(&op->stream) != hipSuccess) return SPEIGS_ERR_INTERNAL;
+    int rc = sell_build(n, Ap, Ai, Ax, &op->A, op->mem, op->stream);
+    if (rc) return rc;
+    hinfo->bytes_spmv = spmv_bytes(op->A);
+    if (mode == SPEIGS_HIP_OP_SOLVE) {
+        FactorOptions fo;
+        fo.precision = precision;
+        rc = factor_create(n, Ap, Ai, Ax, fo, op->stream, &op->F, hinfo);
+        if (rc) return rc;
+        factor_attach_matrix(op->F, &op->A);
+        int st = factor_numeric(op->F, delta);
+        if (st < 0) return st;
+        if (st == 1) return SPEIGS_ERR_FACTOR;
+        factor_stats(op->F, hinfo);
+    }
+    *out = op.release();
+    return SPEIGS_OK;
+}
+
+int speigs_hip_op_refactor(speigs_hip_op *op, double delta)
+{
+    if (!op || !op->F) return SPEIGS_ERR_ARG;
+    int st = factor_numeric(op->F, delta);
+    if (st < 0) return st;
+    return st == 1 ? SPEIGS_ERR_FACTOR : SPEIGS_OK;
+}
+
+static int op_buffers(speigs_hip_op *op, int64_t nvec)
+{
+    if (op->cap >= nvec) return 0;
+    int64_t ld = op->n;
+    int rc;
+    if ((rc = op->mem.alloc(&op->x, (size_t)ld * nvec))) return rc;
+    if ((rc = op->mem.alloc(&op->y, (size_t)ld * nvec))) return rc;
+    op->cap = nvec;
+    return 0;
+}
+
+int speigs_hip_op_multA(speigs_hip_op *op, const double *x, double *y, speigs_int nvec)
+{
+    if (!op || !x || !y || nvec <= 0) return SPEIGS_ERR_ARG;
+    int rc = op_buffers(op, nvec);
+    if (rc) return rc;
+    const size_t bytes = (size_t)op->n * nvec * sizeof(double);
+    SEHIP_CHECK(hipMemcpyAsync(op->x, x, bytes, hipMemcpyHostToDevice, op->stream));
+    spmv_multi(op->A, op->x, op->n, op->y, op->n, (int)nvec, op->stream);
+    SEHIP_CHECK(hipMemcpyAsync(y, op->y, bytes, hipMemcpyDeviceToHost, op->stream));
+    SEHIP_CHECK(hipStreamSynchronize(op->stream));
+    return SPEIGS_OK;
+}
+
+int speigs_hip_op_apply(speigs_hip_op *op, const double *x, double *y, speigs_int nvec)
+{
+    if (!op) return SPEIGS_ERR_ARG;
+    if (op->mode == SPEIGS_HIP_OP_MULT) return speigs_hip_op_multA(op, x, y, nvec);
+    if (!x || !y || nvec <= 0) return SPEIGS_ERR_ARG;
+    int rc = op_buffers(op, nvec);
+    if (rc) return rc;
+    const size_t bytes = (size_t)op->n * nvec * sizeof(double);
+    SEHIP_CHECK(hipMemcpyAsync(op->x, x, bytes, hipMemcpyHostToDevice, op->stream));
+    for (int64_t v = 0; v < nvec; v++)
+        if ((rc = factor_apply(op->F, op->x + v * op->n, op->y + v * op->n))) return rc;
+    SEHIP_CHECK(hipMemcpyAsync(y, op->y, bytes, hipMemcpyDeviceToHost, op->stream));
+    SEHIP_CHECK(hipStreamSynchronize(op->stream));
+    return SPEIGS_OK;
+}
+
+void speigs_hip_op_destroy(speigs_hip_op *op) { delete op; }
+
+}  // extern "C"
+
+extern "C" int speigs_hip_analyze(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, speigs_int *perm,
+                                  speigs_int *snode, speigs_hip_info *hinfo)
+{
+    if (n <= 0 || !Ap || !Ai) return SPEIGS_ERR_ARG;
+    Symbolic S;
+    SymbolicOptions so;
+    int rc = symbolic_analyze(n, Ap, Ai, so, &S);
+    if (rc) return rc;
+    if (perm) std::memcpy(perm, S.perm.data(), (size_t)n * sizeof(int64_t));
+    if (snode)
+        for (int64_t s = 0; s < S.ns; s++)
+            for (int64_t j = S.c0[s]; j < S.c1[s]; j++) snode[j] = s;
+    if (hinfo) {
+        std::memset(hinfo, 0, sizeof *hinfo);
+        hinfo->nnz_L = S.nnzL;
+        hinfo->factor_flops = S.flops;
+        hinfo->nsuper = S.ns;
+        hinfo->nlevels = S.nlevels;
+        hinfo->t_order = S.t_order;
+        hinfo->t_symbolic = S.t_symbolic;
+    }
+    return SPEIGS_OK;
+}

@@ -1,0 +1,132 @@
+// dev.h -- internal C++ interface of libspeigs_hip (not installed; the public
+// ABI is include/speigs.h + include/speigs_hip.h).
+//
+// Layout in HBM (one speigs() call, n rows):
+//   * A as SELL-64: rows in slices of 64, each slice stored column-major
+//     (entry k of all 64 rows contiguous), FP64 values + int32 columns; the
+//     full symmetric matrix is expanded from the stored upper triangle.
+//   * Krylov basis V and restart buffer Vn: n x (ncv+1) FP64, column-major,
+//     leading dimension padded to 32 doubles (256 B) so every column starts on
+//     a 256-byte boundary.
+//   * supernodal factor: dense fronts (column-major F x F) in one pool, FP64
+//     or FP32; the first c columns of a front are the factor panel read by the
+//     triangular solves.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/speigs.h"
+#include "../../include/speigs_hip.h"
+
+namespace sehip {
+
+#define SEHIP_CHECK(expr)                                                     \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess) return sehip::hip_error_code(e_);               \
+    } while (0)
+
+int hip_error_code(hipError_t e);
+double wall();
+
+constexpr int kBlock = 256;          // 4 wave64s
+constexpr int kRowsPerThread = 4;    // reduction kernels: rows per thread
+constexpr int64_t kRowsPerBlock = (int64_t)kBlock * kRowsPerThread;
+
+inline int64_t pad_ld(int64_t n) { return (n + 31) / 32 * 32; }
+inline int64_t nblocks_rows(int64_t n) { return (n + kRowsPerBlock - 1) / kRowsPerBlock; }
+
+// ------------------------------------------------------------ device memory
+struct DevMem {
+    std::vector<void *> ptrs;
+    ~DevMem() { release(); }
+    template <class T> int alloc(T **p, size_t count) {
+        void *q = nullptr;
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc(&q, count * sizeof(T));
+        if (e != hipSuccess) { (void)hipGetLastError(); *p = nullptr; return SPEIGS_ERR_MEM; }
+        ptrs.push_back(q);
+        *p = static_cast<T *>(q);
+        return 0;
+    }
+    void release() {
+        for (void *q : ptrs) (void)hipFree(q);
+        ptrs.clear();
+    }
+};
+
+// ------------------------------------------------------------ SELL matrix
+struct SellA {
+    int64_t n = 0, nslices = 0, stored = 0, nnz_full = 0;
+    int64_t *sptr = nullptr;   // slice start (entries)
+    int32_t *swid = nullptr;   // slice width
+    int32_t *col = nullptr;
+    double *val = nullptr;
+};
+// expand the upper triangle of CSC A to full symmetric SELL-64 and upload
+int sell_build(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax,
+               SellA *S, DevMem &mem, hipStream_t st);
+void spmv(const SellA &S, const double *x, double *y, hipStream_t st);
+void spmv_multi(const SellA &S, const double *X, int64_t ldx, double *Y, int64_t ldy, int nv,
+                hipStream_t st);
+double spmv_bytes(const SellA &S);
+
+// ------------------------------------------------------------ BLAS-1/2 on tall V
+// partial[b*m + c] = sum over row block b of V[:,c] . w
+void coldots_partial(const double *V, int64_t ldv, int m, const double *w, int64_t n,
+                     double *partial, hipStream_t st);
+// out[c] = sum_b partial[b*m + c] (fixed order); nb = number of row blocks
+void reduce_partial(const double *partial, int64_t nb, int m, double *out, hipStream_t st);
+// w -= V h; then partial dots V^T w (mode 1) or partial |w|^2 (mode 2) or nothing (0)
+void cgs_update(const double *V, int64_t ldv, int m, const double *h, double *w, int64_t n,
+                double *partial, int mode, hipStream_t st);
+// X[:, t] = V[:, 0:m] S[:, t], t < p (S column-major m x p, device)
+void tsgemm(const double *V, int64_t ldv, int m, const double *S, int p, double *X, int64_t ldx,
+            int64_t n, hipStream_t st);
+// partial[b*p + t] = sum X[:,t] . Y[:,t]
+void pairdots_partial(const double *X, int64_t ldx, const double *Y, int64_t ldy, int p, int64_t n,
+                      double *partial, hipStream_t st);
+// partial[b*p + t] = sum (Y[:,t] - lam[t] X[:,t])^2
+void residual_partial(const double *X, int64_t ldx, const double *Y, int64_t ldy, const double *lam,
+                      int p, int64_t n, double *partial, hipStream_t st);
+// X[:,t] *= s[t]
+void scale_cols(double *X, int64_t ldx, const double *s, int p, int64_t n, hipStream_t st);
+// y = a x
+void scale_copy(const double *x, double a, double *y, int64_t n, hipStream_t st);
+// y = b - A x - delta x  (FP64 residual of the shifted system)
+void axpby_residual(const double *b, const double *Ax, const double *x, double delta, double *r,
+                    int64_t n, hipStream_t st);
+// x += d
+void vadd(double *x, const double *d, int64_t n, hipStream_t st);
+
+// ------------------------------------------------------------ operator seam
+struct DevOperator {
+    virtual ~DevOperator() {}
+    // y = M x (device pointers), M = A ('lm') or (A + delta I)^-1 ('sm')
+    virtual int apply(const double *x, double *y) = 0;
+    // y = A x, nv vectors
+    virtual int multA(const double *X, int64_t ldx, double *Y, int64_t ldy, int nv) = 0;
+    int64_t nops = 0;
+    int64_t n = 0;
+    hipStream_t stream = nullptr;
+};
+
+// Thick-restart Lanczos over a device operator (reference src/speigs_core.c:146-315).
+int lanczos_dev(DevOperator *op, int64_t n, int64_t k, const speigs_opts *o,
+                const speigs_hip_opts *ho, double anorm, double *lambda, double *Vout, double *resid,
+                speigs_info *info);
+
+// host dense kernels (host_dense.cpp)
+int jacobi_eig(int64_t m, const double *Hupper, int64_t ldh, double *Z, double *w);
+int dense_syev(int64_t n, double *A, double *w);   // Householder + implicit QL, ascending
+int band_chol_u(int64_t n, int64_t kd, double *AB, int64_t ld);
+void band_solve_u(int64_t n, int64_t kd, const double *AB, int64_t ld, double *b);
+int lu_factor(int64_t n, double *A, int64_t *piv);
+void lu_solve(int64_t n, const double *LU, const int64_t *piv, double *b);
+
+uint64_t xorshift_next(uint64_t *s);
+void uniform_vec(uint64_t *s, int64_t n, double *v);
+
+}  // namespace sehip

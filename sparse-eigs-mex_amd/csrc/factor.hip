@@ -1,0 +1,1095 @@
+// factor.hip -- numeric multifrontal Cholesky and triangular solves on gfx950.
+//
+// Replaces cholmod_l_factorize_p and cholmod_l_solve2 (reference
+// src/speigs_core.c:938-970 and :86-100).  Data layout (HBM):
+//   pool   : every front s is a dense F_s x F_s column-major block (FP64 or
+//            FP32) at foff[s]; after factorization its first c_s columns hold
+//            the factor panel [L11; L21] and its trailing r_s x r_s block the
+//            update (Schur complement) passed to the parent by extend-add.
+//   ws     : per front a vector of F_s doubles used by the solves (the
+//            forward solve leaves the child's update vector in its tail).
+// Factorization runs level by level over the supernodal tree (leaves first).
+// Per level: extend-add of the children's update blocks (one launch per child
+// ordinal, so no two blocks of a launch write the same front), then per
+// 64-column panel k: POTRF of the diagonal tile, TRSM of the tiles below it,
+// and the Schur update of the trailing tiles -- each a batched launch over a
+// task list that spans every front of the level.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+
+#include "factor.h"
+
+namespace sehip {
+
+constexpr int TB = 64;   // tile edge
+
+// ------------------------------------------------------------------ kernels
+template <class T>
+__global__ void k_zero(T *p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = T(0);
+}
+
+// pool[dst[e]] = A value (+ delta on diagonals); src < 0 marks a diagonal entry ~src
+template <class T>
+__global__ void k_assemble(T *pool, const int64_t *__restrict__ dst, const int64_t *__restrict__ src,
+                           const double *__restrict__ Ax, int64_t ne, double delta)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= ne) return;
+    const int64_t s = src[e];
+    const double v = s >= 0 ? Ax[s] : (s == INT64_MIN ? 0.0 : Ax[~s]) + delta;
+    pool[dst[e]] = (T)v;
+}
+
+struct EaTask {   // extend-add: child front columns [cb*64, cb*64+64) into the parent
+    int32_t child, parent, cb, pad;
+};
+
+template <class T>
+__global__ void __launch_bounds__(256) k_extend_add(T *pool, const EaTask *__restrict__ tasks,
+                                                    const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
+                                                    const int32_t *__restrict__ ncol,
+                                                    const int64_t *__restrict__ relptr,
+                                                    const int32_t *__restrict__ rel)
+{
+    const EaTask t = tasks[blockIdx.x];
+    const int32_t Fc = Fs[t.child], cc = ncol[t.child], rc = Fc - cc, Fp = Fs[t.parent];
+    const T *U = pool + foff[t.child] + (int64_t)cc * Fc + cc;     // r x r, ld Fc
+    T *P = pool + foff[t.parent];
+    const int32_t *rl = rel + relptr[t.child];
+    const int j0 = t.cb * 64, j1 = min(rc, j0 + 64);
+    for (int j = j0; j < j1; j++) {
+        const int64_t pc = (int64_t)rl[j] * Fp;
+        for (int i = j + threadIdx.x; i < rc; i += 256) P[pc + rl[i]] += U[(int64_t)j * Fc + i];
+    }
+}
+
+struct TileTask {   // panel k of front s; tile row i, tile column j
+    int32_t s, k, i, j;
+};
+
+// diagonal tile of panel k: in-LDS unblocked Cholesky with the pivot test
+template <class T>
+__global__ void __launch_bounds__(256) k_potrf(T *pool, const TileTask *__restrict__ tasks,
+                                               const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
+                                               const int32_t *__restrict__ ncol, const int64_t *__restrict__ col0,
+                                               const double *__restrict__ diagA, double delta, double piv_rel,
+                                               int *flag)
+{
+    __shared__ T a[TB][TB + 1];
+    const TileTask t = tasks[blockIdx.x];
+    const int32_t F = Fs[t.s], c = ncol[t.s];
+    const int k0 = t.k * TB, w = min(TB, c - k0);
+    T *base = pool + foff[t.s] + (int64_t)k0 * F + k0;
+    for (int e = threadIdx.x; e < w * w; e += 256) {
+        const int r = e % w, q = e / w;
+        a[r][q] = (r >= q) ? base[(int64_t)q * F + r] : T(0);
+    }
+    __syncthreads();
+    for (int p = 0; p < w; p++) {
+        if (threadIdx.x == 0) {
+            T d = a[p][p];
+            const double ref = fabs(diagA[col0[t.s] + k0 + p] + delta);
+            if (!((double)d > piv_rel * ref) || !(d == d)) {
+                atomicOr(flag, 1);
+                d = T(1);
+            }
+            a[p][p] = sqrt(d);
+        }
+        __syncthreads();
+        const T dp = a[p][p];
+        for (int r = p + 1 + threadIdx.x; r < w; r += 256) a[r][p] /= dp;
+        __syncthreads();
+        const int m = w - p - 1;
+        for (int e = threadIdx.x; e < m * m; e += 256) {
+            const int r = p + 1 + e % m, q = p + 1 + e / m;
+            if (r >= q) a[r][q] -= a[r][p] * a[q][p];
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < w * w; e += 256) {
+        const int r = e % w, q = e / w;
+        if (r >= q) base[(int64_t)q * F + r] = a[r][q];
+    }
+}
+
+// rows of tile row i below the diagonal block: B <- B L_kk^-T
+template <class T>
+__global__ void __launch_bounds__(256) k_trsm(T *pool, const TileTask *__restrict__ tasks,
+                                              const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
+                                              const int32_t *__restrict__ ncol)
+{
+    __shared__ T L[TB][TB + 1];
+    __shared__ T b[TB][TB + 1];
+    const TileTask t = tasks[blockIdx.x];
+    const int32_t F = Fs[t.s], c = ncol[t.s];
+    const int k0 = t.k * TB, w = min(TB, c - k0);
+    const int r0 = max(t.i * TB, k0 + w), r1 = min(F, (t.i + 1) * TB), nr = r1 - r0;
+    if (nr <= 0) return;
+    T *fr = pool + foff[t.s];
+    for (int e = threadIdx.x; e < w * w; e += 256) {
+        const int r = e % w, q = e / w;
+        L[r][q] = fr[(int64_t)(k0 + q) * F + k0 + r];
+    }
+    for (int e = threadIdx.x; e < nr * w; e += 256) {
+        const int r = e % nr, q = e / nr;
+        b[r][q] = fr[(int64_t)(k0 + q) * F + r0 + r];
+    }
+    __syncthreads();
+    const int row = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    for (int p = 0; p < w; p++) {
+        if (grp == (p & 3) && row < nr) b[row][p] /= L[p][p];
+        __syncthreads();
+        if (row < nr) {
+            const T xp = b[row][p];
+            for (int q = p + 1 + ((grp - (p + 1)) & 3); q < w; q += 4) b[row][q] -= xp * L[q][p];
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < nr * w; e += 256) {
+        const int r = e % nr, q = e / nr;
+        fr[(int64_t)(k0 + q) * F + r0 + r] = b[r][q];
+    }
+}
+
+// Schur update of tile (i, j): C -= L_i L_j^T over the panel (vector FMA, 4x4 per thread)
+template <class T>
+__global__ void __launch_bounds__(256) k_update(T *pool, const TileTask *__restrict__ tasks,
+                                                const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
+                                                const int32_t *__restrict__ ncol)
+{
+    __shared__ T Ai[TB][TB + 4];   // [kk][row]
+    __shared__ T Aj[TB][TB + 4];
+    const TileTask t = tasks[blockIdx.x];
+    const int32_t F = Fs[t.s], c = ncol[t.s];
+    const int k0 = t.k * TB, w = min(TB, c - k0), kend = k0 + w;
+    const int r0 = max(t.i * TB, kend), r1 = min(F, (t.i + 1) * TB);
+    const int q0 = max(t.j * TB, kend), q1 = min(F, (t.j + 1) * TB);
+    const int nr = r1 - r0, nq = q1 - q0;
+    if (nr <= 0 || nq <= 0) return;
+    T *fr = pool + foff[t.s];
+    for (int e = threadIdx.x; e < TB * w; e += 256) {
+        const int r = e % TB, kk = e / TB;
+        Ai[kk][r] = r < nr ? fr[(int64_t)(k0 + kk) * F + r0 + r] : T(0);
+        Aj[kk][r] = r < nq ? fr[(int64_t)(k0 + kk) * F + q0 + r] : T(0);
+    }
+    __syncthreads();
+    const int ri = 4 * (threadIdx.x & 15), qj = 4 * (threadIdx.x >> 4);
+    T acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = T(0);
+    for (int kk = 0; kk < w; kk++) {
+        T x[4], y[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) { x[a] = Ai[kk][ri + a]; y[a] = Aj[kk][qj + a]; }
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) acc[a][b] = fma(x[a], y[b], acc[a][b]);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const int q = qj + b;
+        if (q >= nq) continue;
+        T *col = fr + (int64_t)(q0 + q) * F + r0;
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+            if (ri + a < nr) col[ri + a] -= acc[a][b];
+    }
+}
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+// Operand layout (both types, wave64, K = 4): lane l supplies A[l%16][l/16] and
+// B[l/16][l%16].  Result layout differs (checked on gfx950 with a lane probe):
+//   f64 16x16x4: lane l, value v holds D[l%16][l/16 + 4v]
+//   f32 16x16x4: lane l, value v holds D[4(l/16) + v][l%16]
+// blk(xi, xj) returns the 16x16 block of L_i L_j^T such that value v of lane l
+// is C[row = l%16][col = colof(l/16, v)] in both cases: f64 takes (L_i, L_j),
+// f32 the swapped pair (L_j, L_i), whose transpose puts rows on l%16.
+template <class T> struct MfmaAcc;
+template <> struct MfmaAcc<double> {
+    typedef dbl4 V;
+    static __device__ __forceinline__ V blk(double xi, double xj, V c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(xi, xj, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int colof(int lk, int v) { return lk + 4 * v; }
+};
+template <> struct MfmaAcc<float> {
+    typedef flt4 V;
+    static __device__ __forceinline__ V blk(float xi, float xj, V c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(xj, xi, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int colof(int lk, int v) { return 4 * lk + v; }
+};
+
+// Schur update of tile (i, j) on the matrix cores: each wave owns a 32 x 32
+// quarter of the 64 x 64 tile as 2 x 2 blocks of 16 x 16, K = 4 per MFMA.
+// Lanes 0-15 of each result value hold 16 consecutive rows of one column, so
+// the read-modify-write of the column-major front is coalesced.
+template <class T>
+__global__ void __launch_bounds__(256) k_update_mfma(T *pool, const TileTask *__restrict__ tasks,
+                                                     const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
+                                                     const int32_t *__restrict__ ncol)
+{
+    __shared__ T Ai[TB][TB + 4];   // [kk][row]
+    __shared__ T Aj[TB][TB + 4];
+    typedef typename MfmaAcc<T>::V V;
+    const TileTask t = tasks[blockIdx.x];
+    const int32_t F = Fs[t.s], c = ncol[t.s];
+    const int k0 = t.k * TB, w = min(TB, c - k0), kend = k0 + w;
+    const int r0 = max(t.i * TB, kend), r1 = min(F, (t.i + 1) * TB);
+    const int q0 = max(t.j * TB, kend), q1 = min(F, (t.j + 1) * TB);
+    const int nr = r1 - r0, nq = q1 - q0;
+    if (nr <= 0 || nq <= 0) return;
+    T *fr = pool + foff[t.s];
+    const int w4 = (w + 3) & ~3;
+    for (int e = threadIdx.x; e < TB * w4; e += 256) {
+        const int r = e % TB, kk = e / TB;
+        const bool in = kk < w;
+        Ai[kk][r] = (in && r < nr) ? fr[(int64_t)(k0 + kk) * F + r0 + r] : T(0);
+        Aj[kk][r] = (in && r < nq) ? fr[(int64_t)(k0 + kk) * F + q0 + r] : T(0);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int rb = (wv & 1) * 32, cb = (wv >> 1) * 32;
+    const int lr = lane & 15, lk = lane >> 4;
+    V acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = V{T(0), T(0), T(0), T(0)};
+    for (int kk = 0; kk < w4; kk += 4) {
+        T xi[2], xj[2];
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+            xi[a] = Ai[kk + lk][rb + 16 * a + lr];
+            xj[a] = Aj[kk + lk][cb + 16 * a + lr];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) acc[a][b] = MfmaAcc<T>::blk(xi[a], xj[b], acc[a][b]);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const int row = rb + 16 * a + lr;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int col = cb + 16 * b + MfmaAcc<T>::colof(lk, v);
+                if (row < nr && col < nq) fr[(int64_t)(q0 + col) * F + r0 + row] -= acc[a][b][v];
+            }
+        }
+}
+
+// ------------------------------------------------------------------ solves
+__global__ void k_gather_perm(const double *__restrict__ x, const int64_t *__restrict__ perm, double *__restrict__ y,
+                              int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] = x[perm[i]];
+}
+
+__global__ void k_scatter_perm(const double *__restrict__ x, const int64_t *__restrict__ perm, double *__restrict__ y,
+                               int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[perm[i]] = x[i];
+}
+
+struct SolveArgs {
+    const int64_t *foff, *wsoff, *col0, *rptr, *relptr, *chptr;
+    const int32_t *Fs, *ncol, *rows, *rel, *chl;
+    double *ws;
+};
+
+// forward: L y = b for every front of a level (one block per front)
+template <class T>
+__global__ void __launch_bounds__(256) k_fwd(const T *__restrict__ pool, const int32_t *__restrict__ fronts,
+                                             SolveArgs a, const double *__restrict__ b, double *__restrict__ y)
+{
+    __shared__ double yp[TB];
+    const int s = fronts[blockIdx.x];
+    const int32_t F = a.Fs[s], c = a.ncol[s];
+    double *w = a.ws + a.wsoff[s];
+    const int64_t c0 = a.col0[s];
+    for (int i = threadIdx.x; i < F; i += 256) w[i] = i < c ? b[c0 + i] : 0.0;
+    __syncthreads();
+    for (int64_t p = a.chptr[s]; p < a.chptr[s + 1]; p++) {
+        const int ch = a.chl[p];
+        const int32_t cc = a.ncol[ch], rc = a.Fs[ch] - cc;
+        const double *u = a.ws + a.wsoff[ch] + cc;
+        const int32_t *rl = a.rel + a.relptr[ch];
+        for (int t = threadIdx.x; t < rc; t += 256) w[rl[t]] += u[t];
+        __syncthreads();
+    }
+    const T *L = pool + a.foff[s];
+    for (int k0 = 0; k0 < c; k0 += TB) {
+        const int wd = min(TB, c - k0);
+        if (threadIdx.x < 64) {
+            const int q = threadIdx.x;
+            double v = q < wd ? w[k0 + q] : 0.0;
+            for (int p = 0; p < wd; p++) {
+                if (q == p) v = v / (double)L[(int64_t)(k0 + p) * F + k0 + p];
+                const double yv = __shfl(v, p, 64);
+                if (q > p && q < wd) v -= (double)L[(int64_t)(k0 + p) * F + k0 + q] * yv;
+            }
+            if (q < wd) { w[k0 + q] = v; yp[q] = v; }
+        }
+        __syncthreads();
+        for (int r = k0 + wd + threadIdx.x; r < F; r += 256) {
+            double s2 = 0.0;
+            for (int p = 0; p < wd; p++) s2 = fma((double)L[(int64_t)(k0 + p) * F + r], yp[p], s2);
+            w[r] -= s2;
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < c; i += 256) y[c0 + i] = w[i];
+}
+
+// backward: L^T x = y for every front of a level
+template <class T>
+__global__ void __launch_bounds__(256) k_bwd(const T *__restrict__ pool, const int32_t *__restrict__ fronts,
+                                             SolveArgs a, const double *__restrict__ y, double *__restrict__ x)
+{
+    __shared__ double z[TB];
+    const int s = fronts[blockIdx.x];
+    const int32_t F = a.Fs[s], c = a.ncol[s];
+    double *w = a.ws + a.wsoff[s];
+    const int64_t c0 = a.col0[s];
+    const int32_t *R = a.rows + a.rptr[s];
+    for (int i = threadIdx.x; i < F; i += 256) w[i] = i < c ? y[c0 + i] : x[R[i - c]];
+    __syncthreads();
+    const T *L = pool + a.foff[s];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int npan = (c + TB - 1) / TB;
+    for (int kp = npan - 1; kp >= 0; kp--) {
+        const int k0 = kp * TB, wd = min(TB, c - k0);
+        for (int p = wv; p < wd; p += 4) {
+            const T *colp = L + (int64_t)(k0 + p) * F;
+            double s2 = 0.0;
+            for (int r = k0 + wd + lane; r < F; r += 64) s2 = fma((double)colp[r], w[r], s2);
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) s2 += __shfl_xor(s2, off, 64);
+            if (lane == 0) z[p] = w[k0 + p] - s2;
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const int q = threadIdx.x;
+            double v = q < wd ? z[q] : 0.0;
+            for (int p = wd - 1; p >= 0; p--) {
+                if (q == p) v = v / (double)L[(int64_t)(k0 + p) * F + k0 + p];
+                const double xv = __shfl(v, p, 64);
+                if (q < p) v -= (double)L[(int64_t)(k0 + q) * F + k0 + p] * xv;
+            }
+            if (q < wd) w[k0 + q] = v;
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < c; i += 256) x[c0 + i] = w[i];
+}
+
+// ---- blocked multi-workgroup solves ------------------------------------------
+// A front of order F is cut into 64-row blocks (forward) or 64-column panels
+// (backward).  Work items take tickets from a per-launch counter in dependency
+// order, so an item only ever waits on items holding smaller tickets -- items
+// that are already running -- and the launch cannot deadlock whatever the
+// residency.  Completion of a panel is published as flag[panel] = epoch (the
+// solve's sequence number, so flags never need clearing); the panel values
+// themselves cross workgroups through agent-scope atomic loads and stores.
+// Every wait is bounded: after ~2^22 polls the item records an error and
+// continues, so a logic error shows up as SPEIGS_ERR_INTERNAL, never a hang.
+struct BlkTask {
+    int32_t s, b;
+};
+
+__device__ __forceinline__ void wait_epoch(const int *f, int epoch, int *err)
+{
+    for (int it = 0; it < (1 << 22); it++) {
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;   // already failed
+        if (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == epoch) return;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    atomicOr(err, 2);
+}
+
+__device__ __forceinline__ double ld_coherent(const double *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_coherent(double *p, double v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double bcast_lane(double v, int src)
+{
+    const int64_t bits = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffff), src);
+    const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), src);
+    return __longlong_as_double(((int64_t)hi << 32) | (uint32_t)lo);
+}
+
+__global__ void k_zero_counters(unsigned *c, int n)
+{
+    if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
+}
+
+// ws[s] = [b(c0 .. c0+c) ; 0] for the fronts of a level
+__global__ void __launch_bounds__(256) k_fwd_init(const int32_t *__restrict__ fronts, SolveArgs a,
+                                                  const double *__restrict__ b)
+{
+    const int s = fronts[blockIdx.x];
+    const int32_t F = a.Fs[s], c = a.ncol[s];
+    double *w = a.ws + a.wsoff[s];
+    const int64_t c0 = a.col0[s];
+    for (int i = threadIdx.x; i < F; i += 256) w[i] = i < c ? b[c0 + i] : 0.0;
+}
+
+// ws[parent][rel[t]] += u_child[t] for one child ordinal (EaTask: child, parent)
+__global__ void __launch_bounds__(256) k_fwd_ea(const EaTask *__restrict__ tasks, SolveArgs a)
+{
+    const EaTask t = tasks[blockIdx.x];
+    const int32_t cc = a.ncol[t.child], rc = a.Fs[t.child] - cc;
+    const double *u = a.ws + a.wsoff[t.child] + cc;
+    double *w = a.ws + a.wsoff[t.parent];
+    const int32_t *rl = a.rel + a.relptr[t.child];
+    for (int i = threadIdx.x; i < rc; i += 256) w[rl[i]] += u[i];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_fwd_blk(const T *__restrict__ pool, const BlkTask *__restrict__ tasks,
+                                                 SolveArgs a, const int64_t *__restrict__ panbase, int *flags,
+                                                 int epoch, unsigned *ticket, int *err, double *y)
+{
+    __shared__ double wl[64];
+    __shared__ double yp[64];
+    __shared__ double part[4][64];
+    __shared__ unsigned tk;
+    if (threadIdx.x == 0) tk = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const BlkTask t = tasks[tk];
+    const int s = t.s, b = t.b;
+    const int32_t F = a.Fs[s], c = a.ncol[s];
+    const int npan = (c + 63) / 64;
+    const int r0 = b * 64, nr = min(64, F - r0);
+    double *ws = a.ws + a.wsoff[s];
+    const T *L = pool + a.foff[s];
+    const int64_t c0 = a.col0[s];
+    int *fl = flags + panbase[s];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (threadIdx.x < 64) wl[lane] = lane < nr ? ws[r0 + lane] : 0.0;
+    const bool has_diag = b < npan;
+    const int wd = has_diag ? min(64, c - r0) : 0;
+    double Lr[64];
+    double dinv = 0.0;
+    if (has_diag && wv == 0) {
+#pragma unroll
+        for (int p = 0; p < 64; p++)
+            Lr[p] = (p < wd && lane < nr) ? (double)L[(int64_t)(r0 + p) * F + r0 + lane] : 0.0;
+        if (lane < wd) dinv = 1.0 / (double)L[(int64_t)(r0 + lane) * F + r0 + lane];
+    }
+    __syncthreads();
+    const int pend = min(b, npan);
+    for (int p = 0; p < pend; p++) {
+        if (threadIdx.x == 0) wait_epoch(&fl[p], epoch, err);
+        __syncthreads();
+        const int pw = min(64, c - p * 64);
+        if (threadIdx.x < 64) yp[lane] = lane < pw ? ld_coherent(&y[c0 + p * 64 + lane]) : 0.0;
+        __syncthreads();
+        double acc = 0.0;
+        if (lane < nr)
+            for (int j = wv; j < pw; j += 4) acc = fma((double)L[(int64_t)(p * 64 + j) * F + r0 + lane], yp[j], acc);
+        part[wv][lane] = acc;
+        __syncthreads();
+        if (threadIdx.x < 64) wl[lane] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+        __syncthreads();
+    }
+    if (has_diag) {
+        if (wv == 0) {
+            double v = wl[lane];
+#pragma unroll
+            for (int p = 0; p < 64; p++) {
+                if (p < wd) {
+                    const double tq = v * dinv;
+                    const double yv = bcast_lane(tq, p);
+                    if (lane == p) v = tq;
+                    else if (lane > p) v = fma(-Lr[p], yv, v);
+                }
+            }
+            if (lane < wd) st_coherent(&y[c0 + r0 + lane], v);
+            else if (lane < nr) ws[r0 + lane] = v;
+            __threadfence();
+            if (lane == 0) __hip_atomic_store(&fl[b], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    } else if (threadIdx.x < 64 && lane < nr) {
+        ws[r0 + lane] = wl[lane];
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_bwd_blk(const T *__restrict__ pool, const BlkTask *__restrict__ tasks,
+                                                 SolveArgs a, const int64_t *__restrict__ panbase, int *flags,
+                                                 int epoch, unsigned *ticket, int *err, const double *__restrict__ y,
+                                                 double *x)
+{
+    __shared__ double xs[256];
+    __shared__ double z[64];
+    __shared__ double Ld[64][65];
+    __shared__ unsigned tk;
+    if (threadIdx.x == 0) tk = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const BlkTask t = tasks[tk];
+    const int s = t.s, p = t.b;
+    const int32_t F = a.Fs[s], c = a.ncol[s];
+    const int npan = (c + 63) / 64;
+    const int k0 = p * 64, wd = min(64, c - k0);
+    const T *L = pool + a.foff[s];
+    const int64_t c0 = a.col0[s];
+    const int32_t *R = a.rows + a.rptr[s];
+    int *fl = flags + panbase[s];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    // diagonal block (column-major) into LDS; z from the forward solution
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int r = e & 63, q = e >> 6;
+        Ld[q][r] = (r < wd && q < wd) ? (double)L[(int64_t)(k0 + q) * F + k0 + r] : 0.0;
+    }
+    if (threadIdx.x < 64) z[lane] = lane < wd ? y[c0 + k0 + lane] : 0.0;
+    // each wave accumulates its 16 columns q = wv + 4i over the rows below the panel
+    double acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) acc[i] = 0.0;
+    // update rows (>= c): x values from ancestors, in chunks of 256 rows through LDS
+    for (int rb = c; rb < F; rb += 256) {
+        const int nrr = min(256, F - rb);
+        __syncthreads();
+        if ((int)threadIdx.x < nrr) xs[threadIdx.x] = x[R[rb - c + threadIdx.x]];
+        __syncthreads();
+        for (int r = lane; r < nrr; r += 64) {
+            const double xv = xs[r];
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int q = wv + 4 * i;
+                if (q < wd) acc[i] = fma((double)L[(int64_t)(k0 + q) * F + rb + r], xv, acc[i]);
+            }
+        }
+    }
+    // panel rows of later panels, as their solutions are published
+    for (int pp = npan - 1; pp > p; pp--) {
+        __syncthreads();
+        if (threadIdx.x == 0) wait_epoch(&fl[pp], epoch, err);
+        __syncthreads();
+        const int pw = min(64, c - pp * 64);
+        if (threadIdx.x < 64) xs[lane] = lane < pw ? ld_coherent(&x[c0 + pp * 64 + lane]) : 0.0;
+        __syncthreads();
+        if (lane < pw) {
+            const double xv = xs[lane];
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int q = wv + 4 * i;
+                if (q < wd) acc[i] = fma((double)L[(int64_t)(k0 + q) * F + pp * 64 + lane], xv, acc[i]);
+            }
+        }
+    }
+    // rows inside the diagonal block below row q are handled by the substitution
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        double v = acc[i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        acc[i] = v;
+    }
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int q = wv + 4 * i;
+            if (q < wd) z[q] -= acc[i];
+        }
+    }
+    __syncthreads();
+    if (wv == 0) {
+        double Lc[64];   // lane q: column q of the diagonal block, rows 0..63
+#pragma unroll
+        for (int r = 0; r < 64; r++) Lc[r] = Ld[lane][r];
+        const double dinv = lane < wd ? 1.0 / Ld[lane][lane] : 0.0;
+        double v = z[lane];
+#pragma unroll
+        for (int r = 63; r >= 0; r--) {
+            if (r < wd) {
+                const double tq = v * dinv;
+                const double xv = bcast_lane(tq, r);
+                if (lane == r) v = tq;
+                else if (lane < r) v = fma(-Lc[r], xv, v);
+            }
+        }
+        if (lane < wd) st_coherent(&x[c0 + k0 + lane], v);
+        __threadfence();
+        if (lane == 0) __hip_atomic_store(&fl[p], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__global__ void k_norm2_partial(const double *x, int64_t n, double *partial)
+{
+    __shared__ double lds[256];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) s = fma(x[i], x[i], s);
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) lds[threadIdx.x] += lds[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = lds[0];
+}
+
+// ------------------------------------------------------------------ host side
+struct LevelPlan {
+    std::vector<int64_t> ea_off;      // per child ordinal: offset in the extend-add task array
+    std::vector<int64_t> ea_cnt;
+    std::vector<int64_t> k_potrf_off, k_potrf_cnt, k_trsm_off, k_trsm_cnt, k_upd_off, k_upd_cnt;
+    int64_t fr_off = 0, fr_cnt = 0;   // fronts of the level in the fronts array
+    std::vector<int64_t> sea_off, sea_cnt;   // solve extend-add (one task per child) per ordinal
+    int64_t fb_off = 0, fb_cnt = 0, bb_off = 0, bb_cnt = 0;   // blocked solve work items
+};
+
+struct Factor {
+    Symbolic S;
+    FactorOptions opt;
+    hipStream_t st = nullptr;
+    DevMem mem;
+    int64_t n = 0, nnzA = 0;
+    bool fp32 = false;
+    void *pool = nullptr;
+    double *Ax = nullptr, *diagA = nullptr;
+    int64_t *asm_dst = nullptr, *asm_src = nullptr, nasm = 0;
+    int64_t *foff = nullptr, *wsoff = nullptr, *col0 = nullptr, *rptr = nullptr, *relptr = nullptr, *chptr = nullptr;
+    int32_t *Fs = nullptr, *ncol = nullptr, *rows = nullptr, *rel = nullptr, *chl = nullptr, *fronts = nullptr;
+    int64_t *perm = nullptr;
+    EaTask *ea = nullptr;
+    TileTask *tiles = nullptr;
+    std::vector<LevelPlan> plan;
+    int *flag = nullptr;
+    double *ws = nullptr, *xp = nullptr, *yp = nullptr, *r = nullptr, *d = nullptr, *ax = nullptr, *partial = nullptr;
+    double *hpart = nullptr;
+    int64_t wssize = 0;
+    double delta = 0.0;
+    const SellA *A = nullptr;
+    int64_t solves = 0, ir_steps = 0;
+    bool factored = false;
+    bool use_mfma = false;
+    int solve_mode = 1;               // 0: one workgroup per front, 1: blocked multi-workgroup
+    EaTask *sea = nullptr;
+    BlkTask *fblk = nullptr, *bblk = nullptr;
+    int64_t *panbase = nullptr;
+    int *fflag = nullptr, *bflag = nullptr, *err = nullptr, *herr = nullptr;
+    unsigned *counters = nullptr;
+    int epoch = 0;
+    ~Factor()
+    {
+        if (hpart) (void)hipHostFree(hpart);
+        if (herr) (void)hipHostFree(herr);
+    }
+};
+
+namespace {
+
+template <class T>
+int run_numeric(Factor *F, double delta)
+{
+    const Symbolic &S = F->S;
+    hipStream_t st = F->st;
+    T *pool = (T *)F->pool;
+    const double piv_rel = F->fp32 ? 1e-5 : 1e-11;
+    SEHIP_CHECK(hipMemsetAsync(F->flag, 0, sizeof(int), st));
+    k_zero<T><<<dim3((unsigned)((S.pool + 255) / 256)), dim3(256), 0, st>>>(pool, S.pool);
+    k_assemble<T><<<dim3((unsigned)((F->nasm + 255) / 256)), dim3(256), 0, st>>>(pool, F->asm_dst, F->asm_src, F->Ax,
+                                                                                F->nasm, delta);
+    for (int32_t L = 0; L < S.nlevels; L++) {
+        const LevelPlan &P = F->plan[L];
+        for (size_t o = 0; o < P.ea_cnt.size(); o++)
+            if (P.ea_cnt[o] > 0)
+                k_extend_add<T><<<dim3((unsigned)P.ea_cnt[o]), dim3(256), 0, st>>>(pool, F->ea + P.ea_off[o], F->foff,
+                                                                                   F->Fs, F->ncol, F->relptr, F->rel);
+        for (size_t k = 0; k < P.k_potrf_cnt.size(); k++) {
+            if (P.k_potrf_cnt[k] > 0)
+                k_potrf<T><<<dim3((unsigned)P.k_potrf_cnt[k]), dim3(256), 0, st>>>(
+                    pool, F->tiles + P.k_potrf_off[k], F->foff, F->Fs, F->ncol, F->col0, F->diagA, delta, piv_rel, F->flag);
+            if (P.k_trsm_cnt[k] > 0)
+                k_trsm<T><<<dim3((unsigned)P.k_trsm_cnt[k]), dim3(256), 0, st>>>(pool, F->tiles + P.k_trsm_off[k],
+                                                                                 F->foff, F->Fs, F->ncol);
+            if (P.k_upd_cnt[k] > 0) {
+                if (F->use_mfma)
+                    k_update_mfma<T><<<dim3((unsigned)P.k_upd_cnt[k]), dim3(256), 0, st>>>(
+                        pool, F->tiles + P.k_upd_off[k], F->foff, F->Fs, F->ncol);
+                else
+                    k_update<T><<<dim3((unsigned)P.k_upd_cnt[k]), dim3(256), 0, st>>>(pool, F->tiles + P.k_upd_off[k],
+                                                                                      F->foff, F->Fs, F->ncol);
+            }
+        }
+    }
+    int hflag = 0;
+    SEHIP_CHECK(hipMemcpyAsync(&hflag, F->flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    SEHIP_CHECK(hipStreamSynchronize(st));
+    return hflag ? 1 : 0;
+}
+
+template <class T>
+void run_solve(Factor *F, const double *b, double *x)
+{
+    const Symbolic &S = F->S;
+    hipStream_t st = F->st;
+    const T *pool = (const T *)F->pool;
+    SolveArgs a{F->foff, F->wsoff, F->col0, F->rptr, F->relptr, F->chptr, F->Fs, F->ncol, F->rows, F->rel, F->chl, F->ws};
+    const unsigned nb = (unsigned)((F->n + 255) / 256);
+    k_gather_perm<<<dim3(nb), dim3(256), 0, st>>>(b, F->perm, F->xp, F->n);
+    if (F->solve_mode == 1) {
+        const int nc = 2 * S.nlevels;
+        k_zero_counters<<<dim3(1), dim3(256), 0, st>>>(F->counters, nc);
+        const int ep = ++F->epoch;
+        for (int32_t L = 0; L < S.nlevels; L++) {
+            const LevelPlan &P = F->plan[L];
+            if (!P.fr_cnt) continue;
+            k_fwd_init<<<dim3((unsigned)P.fr_cnt), dim3(256), 0, st>>>(F->fronts + P.fr_off, a, F->xp);
+            for (size_t o = 0; o < P.sea_cnt.size(); o++)
+                if (P.sea_cnt[o])
+                    k_fwd_ea<<<dim3((unsigned)P.sea_cnt[o]), dim3(256), 0, st>>>(F->sea + P.sea_off[o], a);
+            k_fwd_blk<T><<<dim3((unsigned)P.fb_cnt), dim3(256), 0, st>>>(pool, F->fblk + P.fb_off, a, F->panbase,
+                                                                        F->fflag, ep, F->counters + L, F->err, F->yp);
+        }
+        for (int32_t L = S.nlevels - 1; L >= 0; L--) {
+            const LevelPlan &P = F->plan[L];
+            if (!P.fr_cnt) continue;
+            k_bwd_blk<T><<<dim3((unsigned)P.bb_cnt), dim3(256), 0, st>>>(pool, F->bblk + P.bb_off, a, F->panbase,
+                                                                        F->bflag, ep, F->counters + S.nlevels + L,
+                                                                        F->err, F->yp, F->xp);
+        }
+        k_scatter_perm<<<dim3(nb), dim3(256), 0, st>>>(F->xp, F->perm, x, F->n);
+        (void)hipMemcpyAsync(F->herr, F->err, sizeof(int), hipMemcpyDeviceToHost, st);
+        F->solves++;
+        return;
+    }
+    for (int32_t L = 0; L < S.nlevels; L++) {
+        const LevelPlan &P = F->plan[L];
+        if (P.fr_cnt) k_fwd<T><<<dim3((unsigned)P.fr_cnt), dim3(256), 0, st>>>(pool, F->fronts + P.fr_off, a, F->xp, F->yp);
+    }
+    for (int32_t L = S.nlevels - 1; L >= 0; L--) {
+        const LevelPlan &P = F->plan[L];
+        if (P.fr_cnt) k_bwd<T><<<dim3((unsigned)P.fr_cnt), dim3(256), 0, st>>>(pool, F->fronts + P.fr_off, a, F->yp, F->xp);
+    }
+    k_scatter_perm<<<dim3(nb), dim3(256), 0, st>>>(F->xp, F->perm, x, F->n);
+    F->solves++;
+}
+
+double device_norm(Factor *F, const double *v)
+{
+    const int nb = 256;
+    k_norm2_partial<<<dim3(nb), dim3(256), 0, F->st>>>(v, F->n, F->partial);
+    if (hipMemcpyAsync(F->hpart, F->partial, nb * sizeof(double), hipMemcpyDeviceToHost, F->st) != hipSuccess) return -1;
+    if (hipStreamSynchronize(F->st) != hipSuccess) return -1;
+    double s = 0.0;
+    for (int i = 0; i < nb; i++) s += F->hpart[i];
+    return std::sqrt(s);
+}
+
+template <class D>
+int upload(Factor *F, D **dst, const std::vector<D> &src)
+{
+    int rc = F->mem.alloc(dst, src.size());
+    if (rc) return rc;
+    if (!src.empty())
+        SEHIP_CHECK(hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(D), hipMemcpyHostToDevice, F->st));
+    return 0;
+}
+
+}  // namespace
+
+int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax, const FactorOptions &o,
+                  hipStream_t st, Factor **out, speigs_hip_info *hinfo)
+{
+    std::unique_ptr<Factor> F(new Factor());
+    F->opt = o;
+    F->st = st;
+    F->n = n;
+    F->fp32 = o.precision == SPEIGS_HIP_FP32_IR;
+    // debug knobs for A/B checks of kernel variants
+    if (const char *e = getenv("SPEIGS_HIP_SOLVE_MODE")) F->solve_mode = atoi(e);
+    if (const char *e = getenv("SPEIGS_HIP_MFMA")) F->use_mfma = atoi(e) != 0;
+    SymbolicOptions so;
+    int rc = symbolic_analyze(n, Ap, Ai, so, &F->S);
+    if (rc) return rc;
+    const Symbolic &S = F->S;
+    double tu = wall();
+    // per-supernode arrays
+    const int64_t ns = S.ns;
+    std::vector<int32_t> Fs(ns), nc(ns), rows32(std::max<size_t>(S.rows.size(), 1)), chl32(std::max<size_t>(S.chl.size(), 1));
+    std::vector<int64_t> wsoff(ns + 1, 0);
+    for (int64_t s = 0; s < ns; s++) {
+        if (S.F[s] >= ((int64_t)1 << 31) / 2) return SPEIGS_ERR_MEM;
+        Fs[s] = (int32_t)S.F[s];
+        nc[s] = (int32_t)(S.c1[s] - S.c0[s]);
+        wsoff[s + 1] = wsoff[s] + S.F[s];
+    }
+    for (size_t i = 0; i < S.rows.size(); i++) rows32[i] = (int32_t)S.rows[i];
+    for (size_t i = 0; i < S.chl.size(); i++) chl32[i] = (int32_t)S.chl[i];
+    // assembly of A's entries: upper triangle as stored (i <= j)
+    std::vector<int64_t> snode(n);
+    for (int64_t s = 0; s < ns; s++)
+        for (int64_t j = S.c0[s]; j < S.c1[s]; j++) snode[j] = s;
+    std::vector<int64_t> dst, src;
+    std::vector<char> hasdiag(n, 0);
+    std::vector<double> diagA(n, 0.0);
+    auto local_row = [&](int64_t s, int64_t row) -> int64_t {
+        if (row < S.c1[s]) return row - S.c0[s];
+        const int64_t *R = S.rows.data() + S.rptr[s], nr = S.rptr[s + 1] - S.rptr[s];
+        const int64_t *pos = std::lower_bound(R, R + nr, row);
+        if (pos == R + nr || *pos != row) return -1;
+        return (S.c1[s] - S.c0[s]) + (pos - R);
+    };
+    for (int64_t j = 0; j < n; j++)
+        for (int64_t q = Ap[j]; q < Ap[j + 1]; q++) {
+            const int64_t i = Ai[q];
+            if (i > j) continue;
+            const int64_t ni = S.iperm[i], nj = S.iperm[j];
+            const int64_t col = std::min(ni, nj), row = std::max(ni, nj);
+            const int64_t s = snode[col];
+            const int64_t lr = local_row(s, row);
+            if (lr < 0) return SPEIGS_ERR_INTERNAL;
+            dst.push_back(S.foff[s] + lr + (col - S.c0[s]) * S.F[s]);
+            if (i == j) {
+                src.push_back(~q);
+                hasdiag[col] = 1;
+                diagA[col] = Ax[q];
+            } else {
+                src.push_back(q);
+            }
+        }
+    for (int64_t j = 0; j < n; j++)
+        if (!hasdiag[j]) {
+            const int64_t s = snode[j], l = j - S.c0[s];
+            dst.push_back(S.foff[s] + l + l * S.F[s]);
+            src.push_back(INT64_MIN);
+        }
+    F->nasm = (int64_t)dst.size();
+    F->nnzA = Ap[n];
+    // plans: extend-add tasks, tile tasks, fronts per level
+    std::vector<EaTask> ea;
+    std::vector<TileTask> tiles;
+    std::vector<int32_t> fronts;
+    std::vector<std::vector<int64_t>> bylevel(S.nlevels);
+    for (int64_t s = 0; s < ns; s++) bylevel[S.level[s]].push_back(s);
+    F->plan.resize(S.nlevels);
+    for (int32_t L = 0; L < S.nlevels; L++) {
+        LevelPlan &P = F->plan[L];
+        P.fr_off = (int64_t)fronts.size();
+        for (int64_t s : bylevel[L]) fronts.push_back((int32_t)s);
+        P.fr_cnt = (int64_t)bylevel[L].size();
+        int64_t maxch = 0, maxpan = 0;
+        for (int64_t s : bylevel[L]) {
+            maxch = std::max(maxch, S.chptr[s + 1] - S.chptr[s]);
+            maxpan = std::max<int64_t>(maxpan, (S.c1[s] - S.c0[s] + TB - 1) / TB);
+        }
+        for (int64_t o2 = 0; o2 < maxch; o2++) {
+            P.ea_off.push_back((int64_t)ea.size());
+            for (int64_t s : bylevel[L]) {
+                if (S.chptr[s] + o2 >= S.chptr[s + 1]) continue;
+                const int64_t ch = S.chl[S.chptr[s] + o2];
+                const int64_t rc2 = S.F[ch] - (S.c1[ch] - S.c0[ch]);
+                for (int64_t cb = 0; cb * 64 < rc2; cb++) ea.push_back({(int32_t)ch, (int32_t)s, (int32_t)cb, 0});
+            }
+            P.ea_cnt.push_back((int64_t)ea.size() - P.ea_off.back());
+        }
+        for (int64_t k = 0; k < maxpan; k++) {
+            P.k_potrf_off.push_back((int64_t)tiles.size());
+            for (int64_t s : bylevel[L])
+                if ((S.c1[s] - S.c0[s]) > k * TB) tiles.push_back({(int32_t)s, (int32_t)k, (int32_t)k, (int32_t)k});
+            P.k_potrf_cnt.push_back((int64_t)tiles.size() - P.k_potrf_off.back());
+            P.k_trsm_off.push_back((int64_t)tiles.size());
+            for (int64_t s : bylevel[L]) {
+                const int64_t c = S.c1[s] - S.c0[s], Fv = S.F[s];
+                if (c <= k * TB) continue;
+                const int64_t kend = k * TB + std::min<int64_t>(TB, c - k * TB);
+                const int64_t nt = (Fv + TB - 1) / TB;
+                for (int64_t i = k; i < nt; i++)
+                    if (std::max(i * TB, kend) < std::min(Fv, (i + 1) * TB))
+                        tiles.push_back({(int32_t)s, (int32_t)k, (int32_t)i, 0});
+            }
+            P.k_trsm_cnt.push_back((int64_t)tiles.size() - P.k_trsm_off.back());
+            P.k_upd_off.push_back((int64_t)tiles.size());
+            for (int64_t s : bylevel[L]) {
+                const int64_t c = S.c1[s] - S.c0[s], Fv = S.F[s];
+                if (c <= k * TB) continue;
+                const int64_t kend = k * TB + std::min<int64_t>(TB, c - k * TB);
+                const int64_t nt = (Fv + TB - 1) / TB;
+                for (int64_t j = k; j < nt; j++) {
+                    if (std::max(j * TB, kend) >= std::min(Fv, (j + 1) * TB)) continue;
+                    for (int64_t i = j; i < nt; i++)
+                        tiles.push_back({(int32_t)s, (int32_t)k, (int32_t)i, (int32_t)j});
+                }
+            }
+            P.k_upd_cnt.push_back((int64_t)tiles.size() - P.k_upd_off.back());
+        }
+    }
+    for (auto &P : F->plan) {
+        (void)P;
+    }
+    // blocked solve plans: forward row blocks in increasing order per front,
+    // backward panels in decreasing order per front (ticket = dependency order)
+    std::vector<EaTask> sea;
+    std::vector<BlkTask> fblk, bblk;
+    std::vector<int64_t> panbase(ns + 1, 0);
+    for (int64_t s2 = 0; s2 < ns; s2++) panbase[s2 + 1] = panbase[s2] + (S.c1[s2] - S.c0[s2] + 63) / 64;
+    for (int32_t L = 0; L < S.nlevels; L++) {
+        LevelPlan &P = F->plan[L];
+        int64_t maxch = 0;
+        for (int64_t s2 : bylevel[L]) maxch = std::max(maxch, S.chptr[s2 + 1] - S.chptr[s2]);
+        for (int64_t o2 = 0; o2 < maxch; o2++) {
+            P.sea_off.push_back((int64_t)sea.size());
+            for (int64_t s2 : bylevel[L])
+                if (S.chptr[s2] + o2 < S.chptr[s2 + 1])
+                    sea.push_back({(int32_t)S.chl[S.chptr[s2] + o2], (int32_t)s2, 0, 0});
+            P.sea_cnt.push_back((int64_t)sea.size() - P.sea_off.back());
+        }
+        P.fb_off = (int64_t)fblk.size();
+        for (int64_t s2 : bylevel[L])
+            for (int64_t b = 0; b * 64 < S.F[s2]; b++) fblk.push_back({(int32_t)s2, (int32_t)b});
+        P.fb_cnt = (int64_t)fblk.size() - P.fb_off;
+        P.bb_off = (int64_t)bblk.size();
+        for (int64_t s2 : bylevel[L])
+            for (int64_t pp = (S.c1[s2] - S.c0[s2] + 63) / 64 - 1; pp >= 0; pp--) bblk.push_back({(int32_t)s2, (int32_t)pp});
+        P.bb_cnt = (int64_t)bblk.size() - P.bb_off;
+    }
+    if (sea.empty()) sea.push_back({0, 0, 0, 0});
+    if (ea.empty()) ea.push_back({0, 0, 0, 0});
+    if (tiles.empty()) tiles.push_back({0, 0, 0, 0});
+    if ((rc = upload(F.get(), &F->sea, sea))) return rc;
+    if ((rc = upload(F.get(), &F->fblk, fblk))) return rc;
+    if ((rc = upload(F.get(), &F->bblk, bblk))) return rc;
+    if ((rc = upload(F.get(), &F->panbase, panbase))) return rc;
+    if ((rc = F->mem.alloc(&F->fflag, (size_t)panbase[ns] + 1))) return rc;
+    if ((rc = F->mem.alloc(&F->bflag, (size_t)panbase[ns] + 1))) return rc;
+    SEHIP_CHECK(hipMemsetAsync(F->fflag, 0, ((size_t)panbase[ns] + 1) * sizeof(int), st));
+    SEHIP_CHECK(hipMemsetAsync(F->bflag, 0, ((size_t)panbase[ns] + 1) * sizeof(int), st));
+    if ((rc = F->mem.alloc(&F->counters, (size_t)2 * S.nlevels + 2))) return rc;
+    if ((rc = F->mem.alloc(&F->err, 1))) return rc;
+    SEHIP_CHECK(hipMemsetAsync(F->err, 0, sizeof(int), st));
+    if (hipHostMalloc((void **)&F->herr, sizeof(int)) != hipSuccess) return SPEIGS_ERR_MEM;
+    *F->herr = 0;
+    // uploads
+    const size_t esz = F->fp32 ? sizeof(float) : sizeof(double);
+    if (F->fp32) {
+        if ((rc = F->mem.alloc((float **)&F->pool, (size_t)S.pool))) return rc;
+    } else {
+        if ((rc = F->mem.alloc((double **)&F->pool, (size_t)S.pool))) return rc;
+    }
+    (void)esz;
+    std::vector<double> axv(Ax, Ax + std::max<int64_t>(Ap[n], 1));
+    if (Ap[n] == 0) axv.assign(1, 0.0);
+    if ((rc = upload(F.get(), &F->Ax, axv))) return rc;
+    if ((rc = upload(F.get(), &F->diagA, diagA))) return rc;
+    if ((rc = upload(F.get(), &F->asm_dst, dst))) return rc;
+    if ((rc = upload(F.get(), &F->asm_src, src))) return rc;
+    if ((rc = upload(F.get(), &F->foff, S.foff))) return rc;
+    if ((rc = upload(F.get(), &F->wsoff, wsoff))) return rc;
+    if ((rc = upload(F.get(), &F->col0, S.c0))) return rc;
+    if ((rc = upload(F.get(), &F->rptr, S.rptr))) return rc;
+    if ((rc = upload(F.get(), &F->relptr, S.relptr))) return rc;
+    if ((rc = upload(F.get(), &F->chptr, S.chptr))) return rc;
+    if ((rc = upload(F.get(), &F->Fs, Fs))) return rc;
+    if ((rc = upload(F.get(), &F->ncol, nc))) return rc;
+    if ((rc = upload(F.get(), &F->rows, rows32))) return rc;
+    if ((rc = upload(F.get(), &F->rel, S.rel))) return rc;
+    if ((rc = upload(F.get(), &F->chl, chl32))) return rc;
+    if ((rc = upload(F.get(), &F->fronts, fronts))) return rc;
+    if ((rc = upload(F.get(), &F->perm, S.perm))) return rc;
+    if ((rc = upload(F.get(), &F->ea, ea))) return rc;
+    if ((rc = upload(F.get(), &F->tiles, tiles))) return rc;
+    if ((rc = F->mem.alloc(&F->flag, 1))) return rc;
+    F->wssize = wsoff[ns];
+    if ((rc = F->mem.alloc(&F->ws, (size_t)std::max<int64_t>(F->wssize, 1)))) return rc;
+    if ((rc = F->mem.alloc(&F->xp, (size_t)n))) return rc;
+    if ((rc = F->mem.alloc(&F->yp, (size_t)n))) return rc;
+    if ((rc = F->mem.alloc(&F->r, (size_t)n))) return rc;
+    if ((rc = F->mem.alloc(&F->d, (size_t)n))) return rc;
+    if ((rc = F->mem.alloc(&F->ax, (size_t)n))) return rc;
+    if ((rc = F->mem.alloc(&F->partial, 256))) return rc;
+    if (hipHostMalloc((void **)&F->hpart, 256 * sizeof(double)) != hipSuccess) return SPEIGS_ERR_MEM;
+    SEHIP_CHECK(hipStreamSynchronize(st));
+    if (hinfo) {
+        hinfo->t_order = S.t_order;
+        hinfo->t_symbolic = S.t_symbolic;
+        hinfo->t_upload = wall() - tu;
+    }
+    *out = F.release();
+    return SPEIGS_OK;
+}
+
+int factor_numeric(Factor *F, double delta)
+{
+    F->delta = delta;
+    int st = F->fp32 ? run_numeric<float>(F, delta) : run_numeric<double>(F, delta);
+    F->factored = (st == 0);
+    return st;
+}
+
+void factor_attach_matrix(Factor *F, const SellA *A) { F->A = A; }
+
+int factor_apply(Factor *F, const double *x, double *y)
+{
+    if (!F->factored) return SPEIGS_ERR_FACTOR;
+    if (F->herr && *F->herr) return SPEIGS_ERR_INTERNAL;   // a bounded wait gave up in an earlier solve
+    if (!F->fp32) {
+        run_solve<double>(F, x, y);
+        return 0;
+    }
+    // FP32 factor: iterative refinement on the FP64 residual of (A + delta I)
+    run_solve<float>(F, x, y);
+    if (!F->A) return 0;
+    const double bn = device_norm(F, x);
+    if (bn < 0) return SPEIGS_ERR_INTERNAL;
+    if (bn == 0.0) return 0;
+    for (int it = 0; it < F->opt.ir_maxit; it++) {
+        spmv(*F->A, y, F->ax, F->st);
+        axpby_residual(x, F->ax, y, F->delta, F->r, F->n, F->st);
+        const double rn = device_norm(F, F->r);
+        if (rn < 0) return SPEIGS_ERR_INTERNAL;
+        if (rn <= F->opt.ir_tol * bn) break;
+        run_solve<float>(F, F->r, F->d);
+        vadd(y, F->d, F->n, F->st);
+        F->ir_steps++;
+    }
+    return 0;
+}
+
+void factor_stats(const Factor *F, speigs_hip_info *h)
+{
+    if (!h) return;
+    h->nnz_L = F->S.nnzL;
+    h->factor_flops = F->S.flops;
+    h->nsuper = F->S.ns;
+    h->nlevels = F->S.nlevels;
+    h->ir_steps = F->ir_steps;
+    h->solves = F->solves;
+    h->precision_used = F->fp32 ? SPEIGS_HIP_FP32_IR : SPEIGS_HIP_FP64;
+    const double es = F->fp32 ? 4.0 : 8.0;
+    h->bytes_solve = 2.0 * (double)F->S.nnzL * es + 2.0 * 2.0 * (double)F->n * 8.0;
+}
+
+void factor_destroy(Factor *F) { delete F; }
+
+const Symbolic *factor_symbolic(const Factor *F) { return &F->S; }
+
+}  // namespace sehip

@@ -1,0 +1,250 @@
+// host_dense.cpp -- small dense kernels that stay on the host CPU:
+//   * jacobi_eig: the m x m projected Lanczos problem (the reference calls
+//     dsyev_("V","U"), src/speigs_core.c:232-237).  Cyclic two-sided Jacobi
+//     with the relative stopping test |a_pq| <= eps sqrt(|a_pp a_qq|) keeps the
+//     small Ritz values of a graded positive definite projection to relative
+//     accuracy (Demmel and Veselic 1992); m <= ncv is at most a few hundred.
+//   * dense_syev: the dense short-circuit (src/speigs_core.c:318-353):
+//     Householder tridiagonalisation, implicit QL with shifts, ascending order.
+//   * band_chol_u / band_solve_u: the banded short-circuit (dpbtrf/dpbtrs "U",
+//     src/speigs_core.c:869-931, :101-110), LAPACK's upper band storage.
+//   * lu_factor / lu_solve: dense partial-pivoting LU for the indefinite
+//     fallback at small n (the reference uses UMFPACK there, :974-984).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "dev.h"
+
+namespace sehip {
+
+static const double kEps = 2.220446049250313e-16;
+
+static void sort_ascending(int64_t m, double *w, double *Z, int64_t ldz)
+{
+    for (int64_t i = 0; i + 1 < m; i++) {
+        int64_t b = i;
+        for (int64_t j = i + 1; j < m; j++)
+            if (w[j] < w[b]) b = j;
+        if (b != i) {
+            std::swap(w[i], w[b]);
+            for (int64_t r = 0; r < m; r++) std::swap(Z[r + i * ldz], Z[r + b * ldz]);
+        }
+    }
+}
+
+int jacobi_eig(int64_t m, const double *Hu, int64_t ldh, double *Z, double *w)
+{
+    std::vector<double> a((size_t)m * m);
+    for (int64_t c = 0; c < m; c++)
+        for (int64_t r = 0; r < m; r++) a[r + c * m] = (r <= c) ? Hu[r + c * ldh] : Hu[c + r * ldh];
+    for (int64_t c = 0; c < m; c++)
+        for (int64_t r = 0; r < m; r++) Z[r + c * m] = (r == c) ? 1.0 : 0.0;
+    int sweep = 0;
+    for (; sweep < 100; sweep++) {
+        bool any = false;
+        for (int64_t p = 0; p + 1 < m; p++)
+            for (int64_t q = p + 1; q < m; q++) {
+                const double apq = a[p + q * m], app = a[p + p * m], aqq = a[q + q * m];
+                if (apq == 0.0 || std::fabs(apq) <= kEps * std::sqrt(std::fabs(app * aqq))) continue;
+                any = true;
+                const double th = (aqq - app) / (2.0 * apq);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int64_t r = 0; r < m; r++) {
+                    const double x = a[r + p * m], y = a[r + q * m];
+                    a[r + p * m] = c * x - s * y;
+                    a[r + q * m] = s * x + c * y;
+                }
+                for (int64_t r = 0; r < m; r++) {
+                    const double x = a[p + r * m], y = a[q + r * m];
+                    a[p + r * m] = c * x - s * y;
+                    a[q + r * m] = s * x + c * y;
+                }
+                a[p + p * m] = app - t * apq;
+                a[q + q * m] = aqq + t * apq;
+                a[p + q * m] = a[q + p * m] = 0.0;
+                for (int64_t r = 0; r < m; r++) {
+                    const double x = Z[r + p * m], y = Z[r + q * m];
+                    Z[r + p * m] = c * x - s * y;
+                    Z[r + q * m] = s * x + c * y;
+                }
+            }
+        if (!any) break;
+    }
+    if (sweep >= 100) return 1;
+    for (int64_t i = 0; i < m; i++) w[i] = a[i + i * m];
+    sort_ascending(m, w, Z, m);
+    return 0;
+}
+
+int dense_syev(int64_t n, double *A, double *d)
+{
+    if (n <= 0) return 0;
+    if (n == 1) { d[0] = A[0]; A[0] = 1.0; return 0; }
+    for (int64_t c = 0; c < n; c++)
+        for (int64_t r = c + 1; r < n; r++) A[r + c * n] = A[c + r * n];
+    std::vector<double> Q((size_t)n * n, 0.0), v(n), p(n), e(n, 0.0);
+    for (int64_t i = 0; i < n; i++) Q[i + i * n] = 1.0;
+    for (int64_t k = 0; k + 2 < n; k++) {
+        double xn = 0.0;
+        for (int64_t i = k + 1; i < n; i++) xn += A[i + k * n] * A[i + k * n];
+        xn = std::sqrt(xn);
+        const double x0 = A[(k + 1) + k * n];
+        if (xn == 0.0) { e[k] = 0.0; continue; }
+        const double alpha = x0 > 0 ? -xn : xn;
+        for (int64_t i = k + 1; i < n; i++) v[i] = A[i + k * n];
+        v[k + 1] -= alpha;
+        double vn = 0.0;
+        for (int64_t i = k + 1; i < n; i++) vn += v[i] * v[i];
+        vn = std::sqrt(vn);
+        if (vn == 0.0) { e[k] = x0; continue; }
+        for (int64_t i = k + 1; i < n; i++) v[i] /= vn;
+        double K = 0.0;
+        for (int64_t i = k + 1; i < n; i++) p[i] = 0.0;
+        for (int64_t j = k + 1; j < n; j++) {      // p = A22 v, column sweep
+            const double vj = v[j];
+            const double *aj = A + j * n;
+            for (int64_t i = k + 1; i < n; i++) p[i] += aj[i] * vj;
+        }
+        for (int64_t i = k + 1; i < n; i++) K += v[i] * p[i];
+        for (int64_t i = k + 1; i < n; i++) p[i] = 2.0 * p[i] - 2.0 * K * v[i];
+        for (int64_t j = k + 1; j < n; j++) {
+            double *aj = A + j * n;
+            const double vj = v[j], pj = p[j];
+            for (int64_t i = k + 1; i < n; i++) aj[i] -= v[i] * pj + p[i] * vj;
+        }
+        e[k] = alpha;
+        for (int64_t r = 0; r < n; r++) {
+            double s = 0.0;
+            for (int64_t i = k + 1; i < n; i++) s += Q[r + i * n] * v[i];
+            s *= 2.0;
+            for (int64_t i = k + 1; i < n; i++) Q[r + i * n] -= s * v[i];
+        }
+    }
+    for (int64_t i = 0; i < n; i++) d[i] = A[i + i * n];
+    if (n >= 2) {
+        // e[k] for k <= n-3 were set above; the last sub-diagonal entry is read off
+        e[n - 2] = A[(n - 1) + (n - 2) * n];
+    }
+    e[n - 1] = 0.0;
+    // implicit QL
+    for (int64_t l = 0; l < n; l++) {
+        int iter = 0;
+        for (;;) {
+            int64_t mm = l;
+            for (; mm < n - 1; mm++)
+                if (std::fabs(e[mm]) <= kEps * (std::fabs(d[mm]) + std::fabs(d[mm + 1]))) break;
+            if (mm == l) break;
+            if (++iter > 60) return (int)(l + 1);
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = std::hypot(g, 1.0);
+            g = d[mm] - d[l] + e[l] / (g + (g >= 0 ? r : -r));
+            double s = 1.0, c = 1.0, pp = 0.0;
+            bool under = false;
+            for (int64_t i = mm - 1; i >= l; i--) {
+                const double f = s * e[i], b = c * e[i];
+                r = std::hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) { d[i + 1] -= pp; e[mm] = 0.0; under = true; break; }
+                s = f / r;
+                c = g / r;
+                g = d[i + 1] - pp;
+                r = (d[i] - g) * s + 2.0 * c * b;
+                pp = s * r;
+                d[i + 1] = g + pp;
+                g = c * r - b;
+                double *zi = Q.data() + i * n, *zi1 = Q.data() + (i + 1) * n;
+                for (int64_t q = 0; q < n; q++) {
+                    const double zf = zi1[q];
+                    zi1[q] = s * zi[q] + c * zf;
+                    zi[q] = c * zi[q] - s * zf;
+                }
+            }
+            if (under) continue;
+            d[l] -= pp;
+            e[l] = g;
+            e[mm] = 0.0;
+        }
+    }
+    std::memcpy(A, Q.data(), (size_t)n * n * sizeof(double));
+    sort_ascending(n, d, A, n);
+    return 0;
+}
+
+#define UB(i, j) AB[kd + (i) - (j) + (size_t)(j) * ld]
+int band_chol_u(int64_t n, int64_t kd, double *AB, int64_t ld)
+{
+    for (int64_t j = 0; j < n; j++) {
+        const int64_t i0 = std::max<int64_t>(0, j - kd);
+        for (int64_t i = i0; i < j; i++) {
+            double s = UB(i, j);
+            for (int64_t p = i0; p < i; p++) s -= UB(p, i) * UB(p, j);
+            UB(i, j) = s / UB(i, i);
+        }
+        double dd = UB(j, j);
+        for (int64_t p = i0; p < j; p++) dd -= UB(p, j) * UB(p, j);
+        if (!(dd > 0.0)) return (int)(j + 1);
+        UB(j, j) = std::sqrt(dd);
+    }
+    return 0;
+}
+
+void band_solve_u(int64_t n, int64_t kd, const double *AB, int64_t ld, double *b)
+{
+    for (int64_t j = 0; j < n; j++) {
+        const int64_t i0 = std::max<int64_t>(0, j - kd);
+        double s = b[j];
+        for (int64_t i = i0; i < j; i++) s -= UB(i, j) * b[i];
+        b[j] = s / UB(j, j);
+    }
+    for (int64_t j = n - 1; j >= 0; j--) {
+        b[j] /= UB(j, j);
+        const int64_t i0 = std::max<int64_t>(0, j - kd);
+        const double bj = b[j];
+        for (int64_t i = i0; i < j; i++) b[i] -= UB(i, j) * bj;
+    }
+}
+#undef UB
+
+int lu_factor(int64_t n, double *A, int64_t *piv)
+{
+    for (int64_t k = 0; k < n; k++) {
+        int64_t p = k;
+        double best = std::fabs(A[k + k * n]);
+        for (int64_t i = k + 1; i < n; i++)
+            if (std::fabs(A[i + k * n]) > best) { best = std::fabs(A[i + k * n]); p = i; }
+        piv[k] = p;
+        if (best == 0.0) return (int)(k + 1);
+        if (p != k)
+            for (int64_t j = 0; j < n; j++) std::swap(A[k + j * n], A[p + j * n]);
+        const double inv = 1.0 / A[k + k * n];
+        for (int64_t i = k + 1; i < n; i++) A[i + k * n] *= inv;
+        for (int64_t j = k + 1; j < n; j++) {
+            const double a = A[k + j * n];
+            if (a == 0.0) continue;
+            for (int64_t i = k + 1; i < n; i++) A[i + j * n] -= A[i + k * n] * a;
+        }
+    }
+    return 0;
+}
+
+void lu_solve(int64_t n, const double *LU, const int64_t *piv, double *b)
+{
+    for (int64_t k = 0; k < n; k++)
+        if (piv[k] != k) std::swap(b[k], b[piv[k]]);
+    for (int64_t j = 0; j < n; j++) {
+        const double bj = b[j];
+        if (bj != 0.0)
+            for (int64_t i = j + 1; i < n; i++) b[i] -= LU[i + j * n] * bj;
+    }
+    for (int64_t j = n - 1; j >= 0; j--) {
+        b[j] /= LU[j + j * n];
+        const double bj = b[j];
+        if (bj != 0.0)
+            for (int64_t i = 0; i < j; i++) b[i] -= LU[i + j * n] * bj;
+    }
+}
+
+}  // namespace sehip

@@ -1,0 +1,78 @@
+"""The C-ABI library loads and exports every symbol include/*.h declares, and
+its host-only short-circuits behave like the reference (no GPU needed)."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import ref_cases as rc
+from conftest import ROOT, load_pkg
+
+pkg = load_pkg()
+
+
+def _declared():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        txt = h.read_text()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"\b([a-z_][a-z0-9_]*)\s*\(", txt):
+            nm = m.group(1)
+            if nm.startswith("speigs") and not nm.endswith("_t"):
+                names.add(nm)
+    return names
+
+
+def test_exports(hiplib):
+    declared = _declared()
+    assert {"speigs", "speigs_z", "speigs_default_opts", "speigs_errmsg", "speigs_ex",
+            "speigs_hip_op_create", "speigs_hip_op_apply"} <= declared
+    for nm in sorted(declared):
+        assert hasattr(hiplib, nm), nm
+
+
+def test_errmsg_and_defaults(hiplib):
+    assert pkg.errmsg(0) == "ok"
+    assert "symmetric" in pkg.errmsg(-6)
+    o = pkg.default_opts()
+    assert (o.dense_max, o.detect, o.check_sym, o.tol, o.ncv, o.seed) == (-1, 1, 1, 0.0, 0, 0)
+
+
+def _solve(A, k, which, opts=None, vectors=True, resid=True):
+    return pkg.speigs(A, k, which, opts=opts, vectors=vectors, resid=resid)
+
+
+def test_arg_errors(hiplib):
+    A = pkg.gen.lap3d(4, True)
+    assert _solve(A, 0, "sm").rc == -1
+    assert _solve(A, 65, "sm").rc == -1
+
+
+def test_host_diag_path(hiplib):
+    rc.check_diagonal(_solve)
+
+
+def test_host_dense_path(hiplib):
+    A = pkg.gen.lap3d(8, True)                  # n = 512 <= 800: dense short-circuit
+    ex = pkg.gen.exact_spectrum3(8, True)
+    r = _solve(A, 6, "sm")
+    assert r.info["path"] == 1
+    assert np.allclose(r.lam, ex[:6], rtol=1e-12)
+    D = pkg.gen.csc_to_dense(*A)
+    assert np.linalg.norm(D @ r.V - r.V * r.lam, axis=0).max() < 1e-11
+
+
+def test_host_nonsymmetric_rejected(hiplib):
+    n = 5000
+    Ap, Ai, Ax = rc.tridiag_1d(n, 5.0, -1.0)
+    col = np.repeat(np.arange(n), np.diff(Ap))
+    Ax = np.where(Ai > col, -4.0, Ax)
+    assert _solve((Ap, Ai, Ax), 3, "sm", vectors=False).rc == -6
+
+
+def test_gpu_paths_fail_loudly_without_device(hiplib):
+    if hiplib.speigs_hip_device_count() > 0:
+        pytest.skip("a HIP device is present")
+    A = pkg.gen.lap3d(12, True)
+    assert _solve(A, 4, "lm").rc == -5          # SPEIGS_ERR_INTERNAL, no CPU fallback

@@ -1,0 +1,154 @@
+"""'sm' SHIFTINV path on the GPU: device multifrontal factor + triangular solves
+inside the thick-restart Lanczos, against closed forms, numpy and the oracle."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import ref_cases as rc
+from conftest import load_pkg
+
+pkg = load_pkg()
+gen = pkg.gen
+pytestmark = pytest.mark.gpu
+
+
+def _solve(A, k, which, opts=None, vectors=True, resid=True, hip_opts=None):
+    print(f"[case] n={len(A[0]) - 1} k={k} which={which} opts={opts} hip={hip_opts}", flush=True)
+    return pkg.speigs(A, k, which, opts=opts, vectors=vectors, resid=resid, hip_opts=hip_opts)
+
+
+def _op(hiplib, A, mode, precision=0, delta=0.0):
+    hiplib.speigs_hip_op_create.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                            ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(pkg.HipInfo)]
+    hiplib.speigs_hip_op_apply.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+    hiplib.speigs_hip_op_destroy.argtypes = [ctypes.c_void_p]
+    Ap, Ai, Ax = (np.ascontiguousarray(a) for a in A)
+    op = ctypes.c_void_p()
+    hi = pkg.HipInfo()
+    rc_ = hiplib.speigs_hip_op_create(len(Ap) - 1, Ap.ctypes.data, Ai.ctypes.data, Ax.ctypes.data, mode, precision,
+                                      delta, ctypes.byref(op), ctypes.byref(hi))
+    return rc_, op, hi
+
+
+@pytest.mark.parametrize("upper", [False, True])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_solve_vs_numpy(hiplib, upper, precision):
+    A = gen.weighted_lap3d(11, seed=5, upper=upper)
+    n = len(A[0]) - 1
+    delta = 0.05
+    rc_, op, hi = _op(hiplib, A, 1, precision, delta)
+    assert rc_ == 0, rc_
+    D = gen.csc_to_dense(*A, symmetrize_upper=True) + delta * np.eye(n)
+    B = np.asfortranarray(np.random.default_rng(2).standard_normal((n, 3)))
+    X = np.zeros((n, 3), order="F")
+    assert hiplib.speigs_hip_op_apply(op, B.ctypes.data, X.ctypes.data, 3) == 0
+    Xref = np.linalg.solve(D, B)
+    err = np.abs(X - Xref).max() / np.abs(Xref).max()
+    print("solve rel err", err, "nnzL", hi.nnz_L, "nsuper", hi.nsuper)
+    assert err <= 1e-11
+    hiplib.speigs_hip_op_destroy(op)
+
+
+def test_solve_known_answer(hiplib):
+    """(A)^-1 v = v / lambda for a separable Dirichlet eigenvector (20^3)."""
+    N = 20
+    rc_, op, hi = _op(hiplib, gen.lap3d(N, True), 1)
+    assert rc_ == 0
+    v = np.ascontiguousarray(gen.dirichlet_mode3(N, 3, 1, 2))
+    lam = sum(2 - 2 * np.cos(p * np.pi / (N + 1)) for p in (3, 1, 2))
+    x = np.zeros_like(v)
+    assert hiplib.speigs_hip_op_apply(op, v.ctypes.data, x.ctypes.data, 1) == 0
+    assert np.abs(x - v / lam).max() <= 1e-12 / lam
+    hiplib.speigs_hip_op_destroy(op)
+
+
+def test_not_pd_detected(hiplib):
+    rc_, op, hi = _op(hiplib, gen.lap3d(10, False), 1)   # singular Neumann Laplacian
+    assert rc_ == -3
+
+
+def test_ref1_dirichlet(hiplib):
+    r = rc.check_dirichlet(_solve)
+    assert r.info["path"] == 2
+
+
+def test_ref2_neumann_singular(hiplib):
+    rc.check_neumann_singular(_solve)
+
+
+def test_ref3_multiplicity(hiplib):
+    r = rc.check_multiplicity(_solve)
+    ex = gen.exact_spectrum3(16, True)
+    assert rc.subspace_angle_max(r.V[:, 1:4], rc.dirichlet_eigenspace(16, ex[1])) <= 1e-8
+
+
+def test_ref8_blocks(hiplib):
+    rc.check_blocks(_solve)
+
+
+def test_ref13_upper_only(hiplib):
+    rc.check_upper_only(_solve)
+
+
+def test_config1_closed_form(hiplib, oracle):
+    A = gen.upper_only(*gen.lap3d(20, True))
+    want = np.array([0.067015042649, 0.133531083527, 0.133531083527, 0.133531083527,
+                     0.200047124405, 0.200047124405])
+    r = _solve(A, 6, "sm")
+    assert r.rc == 0 and r.info["flag"] == 0 and r.info["path"] == 2
+    assert np.abs(r.lam - want).max() <= 1e-11
+    o = pkg.speigs(A, 6, "sm", lib=oracle, prefix="oracle_")
+    assert np.abs(r.lam - o.lam).max() <= 1e-10 * np.abs(o.lam).max()
+    ex = gen.exact_spectrum3(20, True)
+    assert rc.subspace_angle_max(r.V[:, 1:4], rc.dirichlet_eigenspace(20, ex[1])) <= 1e-8
+    U3 = rc.dirichlet_eigenspace(20, ex[4])
+    assert np.linalg.norm(r.V[:, 4:6] - U3 @ (U3.T @ r.V[:, 4:6])) <= 1e-8
+
+
+def test_config2_sm_closed_form(hiplib):
+    A = gen.lap3d(46, True)
+    want = np.array([0.013398728261, 0.026777509198, 0.026777509198, 0.026777509198,
+                     0.040156290135, 0.040156290135])
+    r = _solve(A, 6, "sm", hip_opts={})
+    assert r.rc == 0 and r.info["flag"] == 0
+    assert np.abs(r.lam - want).max() <= 1e-10 * want.max() + 1e-12
+    ex = gen.exact_spectrum3(46, True)
+    assert np.abs(r.lam - ex[:6]).max() <= 1e-10 * ex[5]
+    R = gen.spmv_upper(*A, r.V) - r.V * r.lam
+    assert np.linalg.norm(R, axis=0).max() <= 1e-10 * 12
+    print("info", r.info, "hip", r.hip_info)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_weighted_vs_oracle(hiplib, oracle, precision):
+    """A = L_w + 1e-3 I at 16^3: lambda_0 = 1e-3 exactly, the rest equal the oracle."""
+    A = gen.weighted_lap3d(16, seed=21)
+    r = _solve(A, 10, "sm", opts={"tol": 1e-12}, hip_opts={"precision": precision})
+    o = pkg.speigs(A, 10, "sm", opts={"tol": 1e-12}, lib=oracle, prefix="oracle_")
+    assert r.rc == 0 and r.info["flag"] == 0
+    assert abs(r.lam[0] - 1e-3) <= 1e-13
+    assert np.abs(r.lam - o.lam).max() <= 1e-10 * np.abs(o.lam).max()
+    assert r.resid.max() <= 1e-12 * r.info["anorm"]
+    print("precision", precision, "nops", r.info["nops"], o.info["nops"], "hip", r.hip_info)
+
+
+@pytest.mark.parametrize("mode,mfma", [("0", "0"), ("1", "0"), ("1", "1")])
+def test_kernel_variants_agree(hiplib, monkeypatch, mode, mfma):
+    """one-workgroup-per-front vs blocked multi-workgroup solves, FMA vs MFMA
+    Schur updates: the same solution to rounding (46^3 has fronts of ~2000)."""
+    monkeypatch.setenv("SPEIGS_HIP_SOLVE_MODE", mode)
+    monkeypatch.setenv("SPEIGS_HIP_MFMA", mfma)
+    A = gen.weighted_lap3d(24, seed=9, upper=True)
+    n = len(A[0]) - 1
+    rc_, op, hi = _op(hiplib, A, 1, 0, 0.0)
+    assert rc_ == 0
+    B = np.asfortranarray(np.random.default_rng(4).standard_normal((n, 2)))
+    X = np.zeros((n, 2), order="F")
+    assert hiplib.speigs_hip_op_apply(op, B.ctypes.data, X.ctypes.data, 2) == 0
+    R = gen.spmv_upper(*A, X) - B
+    rel = np.linalg.norm(R) / np.linalg.norm(B)
+    print("mode", mode, "mfma", mfma, "residual", rel)
+    assert rel <= 1e-12
+    hiplib.speigs_hip_op_destroy(op)
