@@ -55,20 +55,20 @@ def run_once(A, k, tol, precision):
 
 def cpu_baseline(seconds_budget: float, k: int, tol: float, seed: int):
     """The CPU oracle (plain-C restatement of the reference algorithm with its own
-    sparse Cholesky, one thread) on a bounded sample: the same family at 30^3."""
+    sparse Cholesky, one thread) on a bounded sample: the same family at 40^3."""
     pkg = _pkg()
     lib_path = ROOT / "oracle" / "build" / "liboracle.so"
     if not lib_path.exists():
         subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)
     lib = ctypes.CDLL(str(lib_path))
     pkg.bind_api(lib, "oracle_")
-    N = 30
+    N = 40
     A = pkg.gen.weighted_lap3d(N, seed=seed, wlo=0.1, whi=10.0, shift=1e-3, upper=True)
     t0 = time.perf_counter()
     r = pkg.speigs(A, k, "sm", opts={"tol": tol}, lib=lib, prefix="oracle_")
     dt = time.perf_counter() - t0
     return {"value": round(dt, 4), "unit": "s", "cores": 1, "kind": "port",
-            "sample": f"same workload family at 30^3 (n=27,000), k={k}, tol={tol}: oracle end to end, "
+            "sample": f"same workload family at 40^3 (n=64,000), k={k}, tol={tol}: oracle end to end, "
                       f"nops={r.info['nops']}, flag={r.info['flag']}"}
 
 

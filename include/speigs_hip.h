@@ -83,6 +83,10 @@ int speigs_hip_device_count(void);
 int speigs_hip_analyze(speigs_int n, const speigs_int *Ap, const speigs_int *Ai,
                        speigs_int *perm, speigs_int *snode, speigs_hip_info *hinfo);
 
+/* The reverse Cuthill-McKee order the banded gate uses (src/speigs_core.c:471-503),
+ * perm[new] = old.  Host code, no device needed. */
+int speigs_hip_rcm(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, speigs_int *perm);
+
 /* ---- device operator: the pluggable seam (src/speigs_core.c:57-116) ------ */
 typedef struct speigs_hip_op speigs_hip_op;
 

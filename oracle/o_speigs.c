@@ -447,9 +447,15 @@ static void rcm(int64_t n, const int64_t *Ap, const int64_t *Ai, int64_t *perm)
             }
         }
     }
+    /* a start the BFS never reached is skipped by the loop above (as in the
+     * reference, which then reverses a partly filled queue: undefined); the
+     * unreached nodes are appended in index order to keep a permutation */
+    for (int64_t c = 0; c < n && tail < n; c++) if (!seen[c]) q[tail++] = c;
     for (int64_t i = 0; i < n; i++) perm[i] = q[n - 1 - i];
     free(deg); free(seen); free(q);
 }
+
+void oracle_rcm(int64_t n, const int64_t *Ap, const int64_t *Ai, int64_t *perm) { rcm(n, Ap, Ai, perm); }
 
 static int64_t bandwidth(int64_t n, const int64_t *Ap, const int64_t *Ai,
                          const int64_t *perm, int64_t *iperm)

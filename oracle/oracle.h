@@ -72,6 +72,10 @@ void    oracle_chol_free(oracle_chol *F);
 /* nested-dissection ordering (perm[new] = old) of the graph of A (upper or full) */
 int oracle_nd_order(int64_t n, const int64_t *Ap, const int64_t *Ai, int64_t *perm);
 
+/* reverse Cuthill-McKee of the banded gate, the reference's seed scan kept (O(n^2)
+ * on upper-triangle storage) */
+void oracle_rcm(int64_t n, const int64_t *Ap, const int64_t *Ai, int64_t *perm);
+
 /* dense helpers used by the short-circuit paths */
 int  oracle_lu_factor(int64_t n, double *A, int64_t *piv);         /* 0 ok, >0 singular */
 void oracle_lu_solve(int64_t n, const double *LU, const int64_t *piv, double *b);

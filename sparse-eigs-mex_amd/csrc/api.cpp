@@ -148,18 +148,44 @@ int64_t components(int64_t n, const int64_t *Ap, const int64_t *Ai, const double
     return nc;
 }
 
-// RCM over the stored column lists and the band width it yields (src/speigs_core.c:471-523)
+// RCM over the stored column lists (src/speigs_core.c:471-503).  Same order as
+// the reference: an outer index `start` runs over 0..n-1; when it is unvisited
+// a BFS is seeded at the lowest-index node of minimum degree among the
+// unvisited nodes >= start, neighbours are appended in column order and
+// insertion-sorted (stably) by degree, and the queue is reversed.  The
+// reference finds each seed by a scan to n, O(n^2) on upper-triangle storage
+// (column lists reach only lower-numbered neighbours, so nearly every node
+// seeds a BFS); degree buckets with cursors that only move forward (past
+// visited nodes and past indices < start) give the identical seeds in
+// O(n + nnz).  A `start` the BFS never reaches is skipped, as in the reference;
+// the reference then reverses a partly filled queue (undefined), here such
+// nodes are appended in index order so the result stays a permutation.
 void rcm(int64_t n, const int64_t *Ap, const int64_t *Ai, std::vector<int64_t> &perm)
 {
     std::vector<int64_t> deg(n), q(n);
     std::vector<char> seen(n, 0);
-    for (int64_t c = 0; c < n; c++) deg[c] = Ap[c + 1] - Ap[c];
-    int64_t head = 0, tail = 0;
-    for (int64_t st = 0; st < n; st++) {
-        if (seen[st]) continue;
-        int64_t s = st, best = deg[st];
-        for (int64_t i = st; i < n; i++)
-            if (!seen[i] && deg[i] < best) { best = deg[i]; s = i; }
+    int64_t maxdeg = 0;
+    for (int64_t c = 0; c < n; c++) {
+        deg[c] = Ap[c + 1] - Ap[c];
+        maxdeg = std::max(maxdeg, deg[c]);
+    }
+    std::vector<int64_t> bptr(maxdeg + 2, 0), bnode(n), cur(maxdeg + 1);
+    for (int64_t c = 0; c < n; c++) bptr[deg[c] + 1]++;
+    for (int64_t d = 0; d <= maxdeg; d++) bptr[d + 1] += bptr[d];
+    for (int64_t d = 0; d <= maxdeg; d++) cur[d] = bptr[d];
+    {
+        std::vector<int64_t> fillp(bptr.begin(), bptr.end() - 1);
+        for (int64_t c = 0; c < n; c++) bnode[fillp[deg[c]]++] = c;
+    }
+    int64_t dmin = 0, head = 0, tail = 0;
+    for (int64_t start = 0; start < n; start++) {
+        if (seen[start]) continue;
+        int64_t s = -1;
+        for (int64_t d = dmin; d <= maxdeg; d++) {
+            while (cur[d] < bptr[d + 1] && (seen[bnode[cur[d]]] || bnode[cur[d]] < start)) cur[d]++;
+            if (cur[d] < bptr[d + 1]) { s = bnode[cur[d]]; break; }
+            if (d == dmin) dmin++;           // bucket exhausted for good
+        }
         seen[s] = 1;
         q[tail++] = s;
         while (head < tail) {
@@ -175,6 +201,8 @@ void rcm(int64_t n, const int64_t *Ap, const int64_t *Ai, std::vector<int64_t> &
             }
         }
     }
+    for (int64_t c = 0; c < n && tail < n; c++)
+        if (!seen[c]) q[tail++] = c;
     perm.resize(n);
     for (int64_t i = 0; i < n; i++) perm[i] = q[n - 1 - i];
 }
@@ -778,5 +806,14 @@ extern "C" int speigs_hip_analyze(speigs_int n, const speigs_int *Ap, const spei
         hinfo->t_order = S.t_order;
         hinfo->t_symbolic = S.t_symbolic;
     }
+    return SPEIGS_OK;
+}
+
+extern "C" int speigs_hip_rcm(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, speigs_int *perm)
+{
+    if (n <= 0 || !Ap || !Ai || !perm) return SPEIGS_ERR_ARG;
+    std::vector<int64_t> pm;
+    rcm(n, Ap, Ai, pm);
+    std::memcpy(perm, pm.data(), (size_t)n * sizeof(int64_t));
     return SPEIGS_OK;
 }

@@ -69,6 +69,50 @@ __global__ void __launch_bounds__(256) k_extend_add(T *pool, const EaTask *__res
     }
 }
 
+struct PcbTask {   // a 64-column block of a parent front that receives extend-adds
+    int32_t parent, pcb;
+};
+
+__device__ __forceinline__ int lower_bound_i32(const int32_t *a, int n, int v)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// extend-add into parent columns [pcb*64, pcb*64+64): children in order, so the
+// sum is deterministic; rel maps are increasing, so each child contributes a
+// contiguous range of its columns
+template <class T>
+__global__ void __launch_bounds__(256) k_extend_add_pcb(T *pool, const PcbTask *__restrict__ tasks,
+                                                        const int64_t *__restrict__ foff,
+                                                        const int32_t *__restrict__ Fs, const int32_t *__restrict__ ncol,
+                                                        const int64_t *__restrict__ relptr,
+                                                        const int32_t *__restrict__ rel,
+                                                        const int64_t *__restrict__ chptr,
+                                                        const int32_t *__restrict__ chl)
+{
+    const PcbTask t = tasks[blockIdx.x];
+    const int32_t Fp = Fs[t.parent];
+    T *P = pool + foff[t.parent];
+    const int c0 = t.pcb * 64, c1 = min(Fp, c0 + 64);
+    for (int64_t q = chptr[t.parent]; q < chptr[t.parent + 1]; q++) {
+        const int ch = chl[q];
+        const int32_t Fc = Fs[ch], cc = ncol[ch], rc = Fc - cc;
+        const int32_t *rl = rel + relptr[ch];
+        const int jlo = lower_bound_i32(rl, rc, c0), jhi = lower_bound_i32(rl, rc, c1);
+        const T *U = pool + foff[ch] + (int64_t)cc * Fc + cc;
+        for (int j = jlo; j < jhi; j++) {
+            const int64_t pc = (int64_t)rl[j] * Fp;
+            for (int i = j + threadIdx.x; i < rc; i += 256) P[pc + rl[i]] += U[(int64_t)j * Fc + i];
+        }
+        __syncthreads();
+    }
+}
+
 struct TileTask {   // panel k of front s; tile row i, tile column j
     int32_t s, k, i, j;
 };
@@ -209,18 +253,17 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
 // Operand layout (both types, wave64, K = 4): lane l supplies A[l%16][l/16] and
-// B[l/16][l%16].  Result layout differs (checked on gfx950 with a lane probe):
-//   f64 16x16x4: lane l, value v holds D[l%16][l/16 + 4v]
+// B[l/16][l%16].  Result layout (measured on gfx950 with an asymmetric lane probe):
+//   f64 16x16x4: lane l, value v holds D[l/16 + 4v][l%16]
 //   f32 16x16x4: lane l, value v holds D[4(l/16) + v][l%16]
-// blk(xi, xj) returns the 16x16 block of L_i L_j^T such that value v of lane l
-// is C[row = l%16][col = colof(l/16, v)] in both cases: f64 takes (L_i, L_j),
-// f32 the swapped pair (L_j, L_i), whose transpose puts rows on l%16.
+// blk(xi, xj) computes D = L_j L_i^T = C^T, so value v of lane l is
+// C[row = l%16][col = colof(l/16, v)]: 16 lanes cover 16 consecutive rows.
 template <class T> struct MfmaAcc;
 template <> struct MfmaAcc<double> {
     typedef dbl4 V;
     static __device__ __forceinline__ V blk(double xi, double xj, V c)
     {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(xi, xj, c, 0, 0, 0);
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(xj, xi, c, 0, 0, 0);
     }
     static __device__ __forceinline__ int colof(int lk, int v) { return lk + 4 * v; }
 };
@@ -417,10 +460,12 @@ struct BlkTask {
 
 __device__ __forceinline__ void wait_epoch(const int *f, int epoch, int *err)
 {
+    // relaxed polling: an agent-scope acquire here would invalidate the L2 of
+    // the XCD on every poll; the published data is read with coherent loads
     for (int it = 0; it < (1 << 22); it++) {
-        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;   // already failed
-        if (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == epoch) return;
-        __builtin_amdgcn_s_sleep(2);
+        if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch) return;
+        if ((it & 63) == 63 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+        __builtin_amdgcn_s_sleep(1);
     }
     atomicOr(err, 2);
 }
@@ -433,6 +478,15 @@ __device__ __forceinline__ double ld_coherent(const double *p)
 __device__ __forceinline__ void st_coherent(double *p, double v)
 {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// after coherent (write-through) stores of a panel: wait until every memory
+// access the wave issued is acknowledged (s_waitcnt 0: vmcnt, expcnt and
+// lgkmcnt all zero), then raise the flag
+__device__ __forceinline__ void publish(int *f, int epoch)
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    __hip_atomic_store(f, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ double bcast_lane(double v, int src)
@@ -448,15 +502,28 @@ __global__ void k_zero_counters(unsigned *c, int n)
     if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
 }
 
-// ws[s] = [b(c0 .. c0+c) ; 0] for the fronts of a level
-__global__ void __launch_bounds__(256) k_fwd_init(const int32_t *__restrict__ fronts, SolveArgs a,
+// ws[s] rows [ck*256, ck*256+256) = [b(c0 .. c0+c) ; 0] plus the children's
+// update vectors (children in order: deterministic)
+__global__ void __launch_bounds__(256) k_fwd_init(const PcbTask *__restrict__ tasks, SolveArgs a,
                                                   const double *__restrict__ b)
 {
-    const int s = fronts[blockIdx.x];
+    const PcbTask t = tasks[blockIdx.x];
+    const int s = t.parent;
     const int32_t F = a.Fs[s], c = a.ncol[s];
     double *w = a.ws + a.wsoff[s];
     const int64_t c0 = a.col0[s];
-    for (int i = threadIdx.x; i < F; i += 256) w[i] = i < c ? b[c0 + i] : 0.0;
+    const int r0 = t.pcb * 256, r1 = min(F, r0 + 256);
+    const int i = r0 + threadIdx.x;
+    if (i < r1) w[i] = i < c ? b[c0 + i] : 0.0;
+    for (int64_t q = a.chptr[s]; q < a.chptr[s + 1]; q++) {
+        __syncthreads();
+        const int ch = a.chl[q];
+        const int32_t cc = a.ncol[ch], rc = a.Fs[ch] - cc;
+        const int32_t *rl = a.rel + a.relptr[ch];
+        const int lo = lower_bound_i32(rl, rc, r0), hi = lower_bound_i32(rl, rc, r1);
+        const double *u = a.ws + a.wsoff[ch] + cc;
+        for (int e = lo + threadIdx.x; e < hi; e += 256) w[rl[e]] += u[e];
+    }
 }
 
 // ws[parent][rel[t]] += u_child[t] for one child ordinal (EaTask: child, parent)
@@ -478,41 +545,55 @@ __global__ void __launch_bounds__(256) k_fwd_blk(const T *__restrict__ pool, con
     __shared__ double wl[64];
     __shared__ double yp[64];
     __shared__ double part[4][64];
+    __shared__ double Ld[64][65];        // diagonal block, Ld[p][q] = L[r0+q][r0+p]
     __shared__ unsigned tk;
-    if (threadIdx.x == 0) tk = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const BlkTask t = tasks[tk];
-    const int s = t.s, b = t.b;
+    const int s = tasks[blockIdx.x].s;
     const int32_t F = a.Fs[s], c = a.ncol[s];
     const int npan = (c + 63) / 64;
+    const unsigned nblk = (unsigned)((F + 63) / 64);
+    if (threadIdx.x == 0) tk = nblk == 1 ? 0u : atomicAdd(&ticket[s], 1u) % nblk;
+    __syncthreads();
+    const int b = (int)tk;
     const int r0 = b * 64, nr = min(64, F - r0);
     double *ws = a.ws + a.wsoff[s];
     const T *L = pool + a.foff[s];
     const int64_t c0 = a.col0[s];
     int *fl = flags + panbase[s];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (threadIdx.x < 64) wl[lane] = lane < nr ? ws[r0 + lane] : 0.0;
     const bool has_diag = b < npan;
     const int wd = has_diag ? min(64, c - r0) : 0;
-    double Lr[64];
-    double dinv = 0.0;
-    if (has_diag && wv == 0) {
-#pragma unroll
-        for (int p = 0; p < 64; p++)
-            Lr[p] = (p < wd && lane < nr) ? (double)L[(int64_t)(r0 + p) * F + r0 + lane] : 0.0;
-        if (lane < wd) dinv = 1.0 / (double)L[(int64_t)(r0 + lane) * F + r0 + lane];
-    }
-    __syncthreads();
+    if (threadIdx.x < 64) wl[lane] = lane < nr ? ws[r0 + lane] : 0.0;
+    if (has_diag)
+        for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+            const int q = e & 63, p = e >> 6;
+            Ld[p][q] = (p < wd && q < nr) ? (double)L[(int64_t)(r0 + p) * F + r0 + q] : 0.0;
+        }
     const int pend = min(b, npan);
+    // rows r0+lane, columns p*64 + wv + 4j: prefetched one panel ahead of its flag
+    T pre[16];
+    auto fetch = [&](int p) {
+        const int pw = min(64, c - p * 64);
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+            const int col = wv + 4 * j;
+            pre[j] = (lane < nr && col < pw) ? L[(int64_t)(p * 64 + col) * F + r0 + lane] : T(0);
+        }
+    };
+    if (pend > 0) fetch(0);
+    __syncthreads();
     for (int p = 0; p < pend; p++) {
+        T cur[16];
+#pragma unroll
+        for (int j = 0; j < 16; j++) cur[j] = pre[j];
+        if (p + 1 < pend) fetch(p + 1);
         if (threadIdx.x == 0) wait_epoch(&fl[p], epoch, err);
         __syncthreads();
         const int pw = min(64, c - p * 64);
         if (threadIdx.x < 64) yp[lane] = lane < pw ? ld_coherent(&y[c0 + p * 64 + lane]) : 0.0;
         __syncthreads();
         double acc = 0.0;
-        if (lane < nr)
-            for (int j = wv; j < pw; j += 4) acc = fma((double)L[(int64_t)(p * 64 + j) * F + r0 + lane], yp[j], acc);
+#pragma unroll
+        for (int j = 0; j < 16; j++) acc = fma((double)cur[j], yp[wv + 4 * j], acc);
         part[wv][lane] = acc;
         __syncthreads();
         if (threadIdx.x < 64) wl[lane] -= (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
@@ -521,22 +602,70 @@ __global__ void __launch_bounds__(256) k_fwd_blk(const T *__restrict__ pool, con
     if (has_diag) {
         if (wv == 0) {
             double v = wl[lane];
-#pragma unroll
-            for (int p = 0; p < 64; p++) {
-                if (p < wd) {
-                    const double tq = v * dinv;
-                    const double yv = bcast_lane(tq, p);
-                    if (lane == p) v = tq;
-                    else if (lane > p) v = fma(-Lr[p], yv, v);
-                }
+            const double dinv = lane < wd ? 1.0 / Ld[lane][lane] : 0.0;
+#pragma unroll 8
+            for (int p = 0; p < wd; p++) {
+                const double lq = Ld[p][lane];
+                const double tq = v * dinv;
+                const double yv = bcast_lane(tq, p);
+                if (lane == p) v = tq;
+                else if (lane > p) v = fma(-lq, yv, v);
             }
             if (lane < wd) st_coherent(&y[c0 + r0 + lane], v);
             else if (lane < nr) ws[r0 + lane] = v;
-            __threadfence();
-            if (lane == 0) __hip_atomic_store(&fl[b], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) publish(&fl[b], epoch);
         }
     } else if (threadIdx.x < 64 && lane < nr) {
         ws[r0 + lane] = wl[lane];
+    }
+}
+
+// backward, update rows: part[s][p][j][q] = sum over rows r of chunk j (512 rows
+// of the front's update part) of L[r][p*64+q] * x[R[r-c]] -- fully parallel
+struct UpdTask {
+    int32_t s, p, j, pad;
+};
+
+template <class T>
+__global__ void __launch_bounds__(256) k_bwd_upd(const T *__restrict__ pool, const UpdTask *__restrict__ tasks,
+                                                 SolveArgs a, const double *__restrict__ x, double *__restrict__ part,
+                                                 const int64_t *__restrict__ partoff)
+{
+    __shared__ double xs[512];
+    __shared__ double red[4][64];
+    const UpdTask t = tasks[blockIdx.x];
+    const int s = t.s, p = t.p;
+    const int32_t F = a.Fs[s], c = a.ncol[s];
+    const int k0 = p * 64, wd = min(64, c - k0);
+    const int rb = c + t.j * 512, nrr = min(512, F - rb);
+    const T *L = pool + a.foff[s];
+    const int32_t *R = a.rows + a.rptr[s];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < nrr; i += 256) xs[i] = x[R[rb - c + i]];
+    __syncthreads();
+    double acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) acc[i] = 0.0;
+    for (int r = lane; r < nrr; r += 64) {
+        const double xv = xs[r];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int q = wv + 4 * i;
+            if (q < wd) acc[i] = fma((double)L[(int64_t)(k0 + q) * F + rb + r], xv, acc[i]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        double v = acc[i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (lane == i) red[wv][i] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int nch = (F - c + 511) / 512;
+        const int q = lane;
+        part[partoff[s] + ((int64_t)p * nch + t.j) * 64 + q] = (q < wd) ? red[q & 3][q >> 2] : 0.0;
     }
 }
 
@@ -544,101 +673,98 @@ template <class T>
 __global__ void __launch_bounds__(256) k_bwd_blk(const T *__restrict__ pool, const BlkTask *__restrict__ tasks,
                                                  SolveArgs a, const int64_t *__restrict__ panbase, int *flags,
                                                  int epoch, unsigned *ticket, int *err, const double *__restrict__ y,
-                                                 double *x)
+                                                 double *x, const double *__restrict__ part,
+                                                 const int64_t *__restrict__ partoff)
 {
     __shared__ double xs[256];
     __shared__ double z[64];
-    __shared__ double Ld[64][65];
+    __shared__ double Ld[64][65];        // Ld[q][r] = L[k0+r][k0+q] (column q of the diagonal block)
+    __shared__ double red[4][64];
     __shared__ unsigned tk;
-    if (threadIdx.x == 0) tk = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const BlkTask t = tasks[tk];
-    const int s = t.s, p = t.b;
+    const int s = tasks[blockIdx.x].s;
     const int32_t F = a.Fs[s], c = a.ncol[s];
     const int npan = (c + 63) / 64;
+    if (threadIdx.x == 0) tk = npan == 1 ? 0u : atomicAdd(&ticket[s], 1u) % (unsigned)npan;
+    __syncthreads();
+    const int p = npan - 1 - (int)tk;
     const int k0 = p * 64, wd = min(64, c - k0);
     const T *L = pool + a.foff[s];
     const int64_t c0 = a.col0[s];
     const int32_t *R = a.rows + a.rptr[s];
     int *fl = flags + panbase[s];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    // diagonal block (column-major) into LDS; z from the forward solution
     for (int e = threadIdx.x; e < 64 * 64; e += 256) {
         const int r = e & 63, q = e >> 6;
         Ld[q][r] = (r < wd && q < wd) ? (double)L[(int64_t)(k0 + q) * F + k0 + r] : 0.0;
     }
     if (threadIdx.x < 64) z[lane] = lane < wd ? y[c0 + k0 + lane] : 0.0;
-    // each wave accumulates its 16 columns q = wv + 4i over the rows below the panel
+    // thread (lane, wv) accumulates columns q = wv + 4i over rows lane, lane+64, ...
     double acc[16];
 #pragma unroll
     for (int i = 0; i < 16; i++) acc[i] = 0.0;
-    // update rows (>= c): x values from ancestors, in chunks of 256 rows through LDS
-    for (int rb = c; rb < F; rb += 256) {
-        const int nrr = min(256, F - rb);
+    // update rows (>= c): partial sums from k_bwd_upd, added in fixed chunk order
+    {
+        const int nch = (F - c + 511) / 512;
+        const double *pt = part + partoff[s] + (int64_t)p * nch * 64;
         __syncthreads();
-        if ((int)threadIdx.x < nrr) xs[threadIdx.x] = x[R[rb - c + threadIdx.x]];
-        __syncthreads();
-        for (int r = lane; r < nrr; r += 64) {
-            const double xv = xs[r];
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const int q = wv + 4 * i;
-                if (q < wd) acc[i] = fma((double)L[(int64_t)(k0 + q) * F + rb + r], xv, acc[i]);
-            }
+        if (threadIdx.x < 64) {
+            double sum = 0.0;
+            for (int j = 0; j < nch; j++) sum += pt[(int64_t)j * 64 + lane];
+            if (lane < wd) z[lane] -= sum;
         }
     }
-    // panel rows of later panels, as their solutions are published
+    // later panels of this front, prefetching each L block before its flag
+    T pre[16];
+    auto fetch = [&](int pp) {
+        const int pw = min(64, c - pp * 64);
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int q = wv + 4 * i;
+            pre[i] = (lane < pw && q < wd) ? L[(int64_t)(k0 + q) * F + pp * 64 + lane] : T(0);
+        }
+    };
+    if (npan - 1 > p) fetch(npan - 1);
     for (int pp = npan - 1; pp > p; pp--) {
+        T cur[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) cur[i] = pre[i];
+        if (pp - 1 > p) fetch(pp - 1);
         __syncthreads();
         if (threadIdx.x == 0) wait_epoch(&fl[pp], epoch, err);
         __syncthreads();
-        const int pw = min(64, c - pp * 64);
-        if (threadIdx.x < 64) xs[lane] = lane < pw ? ld_coherent(&x[c0 + pp * 64 + lane]) : 0.0;
+        if (threadIdx.x < 64) xs[lane] = (pp * 64 + lane < c) ? ld_coherent(&x[c0 + pp * 64 + lane]) : 0.0;
         __syncthreads();
-        if (lane < pw) {
-            const double xv = xs[lane];
+        const double xv = xs[lane];
 #pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const int q = wv + 4 * i;
-                if (q < wd) acc[i] = fma((double)L[(int64_t)(k0 + q) * F + pp * 64 + lane], xv, acc[i]);
-            }
-        }
+        for (int i = 0; i < 16; i++) acc[i] = fma((double)cur[i], xv, acc[i]);
     }
-    // rows inside the diagonal block below row q are handled by the substitution
+    // reduce the 64 lanes of each column: transpose through LDS, 16 columns per wave
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         double v = acc[i];
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-        acc[i] = v;
+        if (lane == i) red[wv][i] = v;
     }
     __syncthreads();
-    if (lane == 0) {
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const int q = wv + 4 * i;
-            if (q < wd) z[q] -= acc[i];
-        }
+    if (threadIdx.x < 64) {
+        const int q = lane, w2 = q & 3, i = q >> 2;
+        if (q < wd) z[q] -= red[w2][i];
     }
     __syncthreads();
     if (wv == 0) {
-        double Lc[64];   // lane q: column q of the diagonal block, rows 0..63
-#pragma unroll
-        for (int r = 0; r < 64; r++) Lc[r] = Ld[lane][r];
         const double dinv = lane < wd ? 1.0 / Ld[lane][lane] : 0.0;
         double v = z[lane];
-#pragma unroll
-        for (int r = 63; r >= 0; r--) {
-            if (r < wd) {
-                const double tq = v * dinv;
-                const double xv = bcast_lane(tq, r);
-                if (lane == r) v = tq;
-                else if (lane < r) v = fma(-Lc[r], xv, v);
-            }
+#pragma unroll 8
+        for (int r = wd - 1; r >= 0; r--) {
+            const double lq = Ld[lane][r];
+            const double tq = v * dinv;
+            const double xv = bcast_lane(tq, r);
+            if (lane == r) v = tq;
+            else if (lane < r) v = fma(-lq, xv, v);
         }
         if (lane < wd) st_coherent(&x[c0 + k0 + lane], v);
-        __threadfence();
-        if (lane == 0) __hip_atomic_store(&fl[p], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (lane == 0) publish(&fl[p], epoch);
     }
 }
 
@@ -662,8 +788,11 @@ struct LevelPlan {
     std::vector<int64_t> ea_cnt;
     std::vector<int64_t> k_potrf_off, k_potrf_cnt, k_trsm_off, k_trsm_cnt, k_upd_off, k_upd_cnt;
     int64_t fr_off = 0, fr_cnt = 0;   // fronts of the level in the fronts array
-    std::vector<int64_t> sea_off, sea_cnt;   // solve extend-add (one task per child) per ordinal
+    std::vector<int64_t> sea_off, sea_cnt;   // (unused) per-ordinal solve extend-add
+    int64_t pcb_off = 0, pcb_cnt = 0;        // factor extend-add: parent column blocks
+    int64_t fin_off = 0, fin_cnt = 0;        // solve init: (front, 256-row chunk)
     int64_t fb_off = 0, fb_cnt = 0, bb_off = 0, bb_cnt = 0;   // blocked solve work items
+    int64_t ub_off = 0, ub_cnt = 0;                            // backward update-row chunks
 };
 
 struct Factor {
@@ -690,22 +819,69 @@ struct Factor {
     const SellA *A = nullptr;
     int64_t solves = 0, ir_steps = 0;
     bool factored = false;
-    bool use_mfma = false;
+    bool use_mfma = true;
     int solve_mode = 1;               // 0: one workgroup per front, 1: blocked multi-workgroup
     EaTask *sea = nullptr;
+    PcbTask *pcb = nullptr, *fin = nullptr;
+    UpdTask *upd = nullptr;
+    double *bpart = nullptr;
+    int64_t *partoff = nullptr;
+    unsigned *tick_f = nullptr, *tick_b = nullptr;
     BlkTask *fblk = nullptr, *bblk = nullptr;
     int64_t *panbase = nullptr;
     int *fflag = nullptr, *bflag = nullptr, *err = nullptr, *herr = nullptr;
     unsigned *counters = nullptr;
     int epoch = 0;
+    // HIP events around each solve pair, harvested at the next apply (the
+    // Krylov loop synchronizes between applies) or at factor_stats
+    std::vector<hipEvent_t> ev;
+    std::vector<int> ev_pending;      // indices of recorded pairs not yet read
+    int ev_next = 0;
+    double t_solve = 0.0;
     ~Factor()
     {
         if (hpart) (void)hipHostFree(hpart);
         if (herr) (void)hipHostFree(herr);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     }
 };
 
 namespace {
+
+constexpr int kEvPairs = 32;
+
+void ev_harvest(Factor *F, bool wait)
+{
+    std::vector<int> keep;
+    for (int i : F->ev_pending) {
+        if (!wait && hipEventQuery(F->ev[2 * i + 1]) != hipSuccess) { keep.push_back(i); continue; }
+        if (wait) (void)hipEventSynchronize(F->ev[2 * i + 1]);
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, F->ev[2 * i], F->ev[2 * i + 1]) == hipSuccess) F->t_solve += 1e-3 * ms;
+    }
+    F->ev_pending.swap(keep);
+}
+
+int ev_begin(Factor *F)
+{
+    if (F->ev.empty()) return -1;
+    for (int tries = 0; tries < kEvPairs; tries++) {
+        const int i = F->ev_next;
+        F->ev_next = (F->ev_next + 1) % kEvPairs;
+        if (std::find(F->ev_pending.begin(), F->ev_pending.end(), i) == F->ev_pending.end()) {
+            (void)hipEventRecord(F->ev[2 * i], F->st);
+            return i;
+        }
+    }
+    return -1;
+}
+
+void ev_end(Factor *F, int i)
+{
+    if (i < 0) return;
+    (void)hipEventRecord(F->ev[2 * i + 1], F->st);
+    F->ev_pending.push_back(i);
+}
 
 template <class T>
 int run_numeric(Factor *F, double delta)
@@ -720,10 +896,9 @@ int run_numeric(Factor *F, double delta)
                                                                                 F->nasm, delta);
     for (int32_t L = 0; L < S.nlevels; L++) {
         const LevelPlan &P = F->plan[L];
-        for (size_t o = 0; o < P.ea_cnt.size(); o++)
-            if (P.ea_cnt[o] > 0)
-                k_extend_add<T><<<dim3((unsigned)P.ea_cnt[o]), dim3(256), 0, st>>>(pool, F->ea + P.ea_off[o], F->foff,
-                                                                                   F->Fs, F->ncol, F->relptr, F->rel);
+        if (P.pcb_cnt > 0)
+            k_extend_add_pcb<T><<<dim3((unsigned)P.pcb_cnt), dim3(256), 0, st>>>(
+                pool, F->pcb + P.pcb_off, F->foff, F->Fs, F->ncol, F->relptr, F->rel, F->chptr, F->chl);
         for (size_t k = 0; k < P.k_potrf_cnt.size(); k++) {
             if (P.k_potrf_cnt[k] > 0)
                 k_potrf<T><<<dim3((unsigned)P.k_potrf_cnt[k]), dim3(256), 0, st>>>(
@@ -757,25 +932,23 @@ void run_solve(Factor *F, const double *b, double *x)
     const unsigned nb = (unsigned)((F->n + 255) / 256);
     k_gather_perm<<<dim3(nb), dim3(256), 0, st>>>(b, F->perm, F->xp, F->n);
     if (F->solve_mode == 1) {
-        const int nc = 2 * S.nlevels;
-        k_zero_counters<<<dim3(1), dim3(256), 0, st>>>(F->counters, nc);
         const int ep = ++F->epoch;
         for (int32_t L = 0; L < S.nlevels; L++) {
             const LevelPlan &P = F->plan[L];
             if (!P.fr_cnt) continue;
-            k_fwd_init<<<dim3((unsigned)P.fr_cnt), dim3(256), 0, st>>>(F->fronts + P.fr_off, a, F->xp);
-            for (size_t o = 0; o < P.sea_cnt.size(); o++)
-                if (P.sea_cnt[o])
-                    k_fwd_ea<<<dim3((unsigned)P.sea_cnt[o]), dim3(256), 0, st>>>(F->sea + P.sea_off[o], a);
+            k_fwd_init<<<dim3((unsigned)P.fin_cnt), dim3(256), 0, st>>>(F->fin + P.fin_off, a, F->xp);
             k_fwd_blk<T><<<dim3((unsigned)P.fb_cnt), dim3(256), 0, st>>>(pool, F->fblk + P.fb_off, a, F->panbase,
-                                                                        F->fflag, ep, F->counters + L, F->err, F->yp);
+                                                                        F->fflag, ep, F->tick_f, F->err, F->yp);
         }
         for (int32_t L = S.nlevels - 1; L >= 0; L--) {
             const LevelPlan &P = F->plan[L];
             if (!P.fr_cnt) continue;
+            if (P.ub_cnt)
+                k_bwd_upd<T><<<dim3((unsigned)P.ub_cnt), dim3(256), 0, st>>>(pool, F->upd + P.ub_off, a, F->xp,
+                                                                            F->bpart, F->partoff);
             k_bwd_blk<T><<<dim3((unsigned)P.bb_cnt), dim3(256), 0, st>>>(pool, F->bblk + P.bb_off, a, F->panbase,
-                                                                        F->bflag, ep, F->counters + S.nlevels + L,
-                                                                        F->err, F->yp, F->xp);
+                                                                        F->bflag, ep, F->tick_b, F->err, F->yp, F->xp,
+                                                                        F->bpart, F->partoff);
         }
         k_scatter_perm<<<dim3(nb), dim3(256), 0, st>>>(F->xp, F->perm, x, F->n);
         (void)hipMemcpyAsync(F->herr, F->err, sizeof(int), hipMemcpyDeviceToHost, st);
@@ -972,6 +1145,50 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
             for (int64_t pp = (S.c1[s2] - S.c0[s2] + 63) / 64 - 1; pp >= 0; pp--) bblk.push_back({(int32_t)s2, (int32_t)pp});
         P.bb_cnt = (int64_t)bblk.size() - P.bb_off;
     }
+    std::vector<PcbTask> pcb, fin;
+    for (int32_t L = 0; L < S.nlevels; L++) {
+        LevelPlan &P = F->plan[L];
+        P.pcb_off = (int64_t)pcb.size();
+        for (int64_t s2 : bylevel[L])
+            if (S.chptr[s2 + 1] > S.chptr[s2])
+                for (int64_t cb = 0; cb * 64 < S.F[s2]; cb++) pcb.push_back({(int32_t)s2, (int32_t)cb});
+        P.pcb_cnt = (int64_t)pcb.size() - P.pcb_off;
+        P.fin_off = (int64_t)fin.size();
+        for (int64_t s2 : bylevel[L])
+            for (int64_t cb = 0; cb * 256 < S.F[s2]; cb++) fin.push_back({(int32_t)s2, (int32_t)cb});
+        P.fin_cnt = (int64_t)fin.size() - P.fin_off;
+    }
+    // backward update-row chunks and their partial-sum slots (reused across levels)
+    std::vector<UpdTask> updv;
+    std::vector<int64_t> partoff(ns + 1, 0);
+    int64_t partmax = 1;
+    for (int32_t L = 0; L < S.nlevels; L++) {
+        LevelPlan &P = F->plan[L];
+        P.ub_off = (int64_t)updv.size();
+        int64_t off = 0;
+        for (int64_t s2 : bylevel[L]) {
+            const int64_t c = S.c1[s2] - S.c0[s2], r = S.F[s2] - c;
+            const int64_t npan = (c + 63) / 64, nch = (r + 511) / 512;
+            partoff[s2] = off;
+            off += npan * nch * 64;
+            for (int64_t pp = 0; pp < npan; pp++)
+                for (int64_t j = 0; j < nch; j++) updv.push_back({(int32_t)s2, (int32_t)pp, (int32_t)j, 0});
+        }
+        partmax = std::max(partmax, off);
+        P.ub_cnt = (int64_t)updv.size() - P.ub_off;
+    }
+    if (updv.empty()) updv.push_back({0, 0, 0, 0});
+    if ((rc = upload(F.get(), &F->upd, updv))) return rc;
+    if ((rc = upload(F.get(), &F->partoff, partoff))) return rc;
+    if ((rc = F->mem.alloc(&F->bpart, (size_t)partmax))) return rc;
+    if ((rc = F->mem.alloc(&F->tick_f, (size_t)ns + 1))) return rc;
+    if ((rc = F->mem.alloc(&F->tick_b, (size_t)ns + 1))) return rc;
+    SEHIP_CHECK(hipMemsetAsync(F->tick_f, 0, ((size_t)ns + 1) * sizeof(unsigned), st));
+    SEHIP_CHECK(hipMemsetAsync(F->tick_b, 0, ((size_t)ns + 1) * sizeof(unsigned), st));
+    if (pcb.empty()) pcb.push_back({0, 0});
+    if (fin.empty()) fin.push_back({0, 0});
+    if ((rc = upload(F.get(), &F->pcb, pcb))) return rc;
+    if ((rc = upload(F.get(), &F->fin, fin))) return rc;
     if (sea.empty()) sea.push_back({0, 0, 0, 0});
     if (ea.empty()) ea.push_back({0, 0, 0, 0});
     if (tiles.empty()) tiles.push_back({0, 0, 0, 0});
@@ -988,6 +1205,9 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     SEHIP_CHECK(hipMemsetAsync(F->err, 0, sizeof(int), st));
     if (hipHostMalloc((void **)&F->herr, sizeof(int)) != hipSuccess) return SPEIGS_ERR_MEM;
     *F->herr = 0;
+    F->ev.resize(2 * kEvPairs);
+    for (auto &e : F->ev)
+        if (hipEventCreate(&e) != hipSuccess) return SPEIGS_ERR_INTERNAL;
     // uploads
     const size_t esz = F->fp32 ? sizeof(float) : sizeof(double);
     if (F->fp32) {
@@ -1051,12 +1271,19 @@ int factor_apply(Factor *F, const double *x, double *y)
 {
     if (!F->factored) return SPEIGS_ERR_FACTOR;
     if (F->herr && *F->herr) return SPEIGS_ERR_INTERNAL;   // a bounded wait gave up in an earlier solve
+    ev_harvest(F, false);
     if (!F->fp32) {
+        const int e = ev_begin(F);
         run_solve<double>(F, x, y);
+        ev_end(F, e);
         return 0;
     }
     // FP32 factor: iterative refinement on the FP64 residual of (A + delta I)
-    run_solve<float>(F, x, y);
+    {
+        const int e = ev_begin(F);
+        run_solve<float>(F, x, y);
+        ev_end(F, e);
+    }
     if (!F->A) return 0;
     const double bn = device_norm(F, x);
     if (bn < 0) return SPEIGS_ERR_INTERNAL;
@@ -1067,16 +1294,21 @@ int factor_apply(Factor *F, const double *x, double *y)
         const double rn = device_norm(F, F->r);
         if (rn < 0) return SPEIGS_ERR_INTERNAL;
         if (rn <= F->opt.ir_tol * bn) break;
+        const int e = ev_begin(F);
         run_solve<float>(F, F->r, F->d);
+        ev_end(F, e);
         vadd(y, F->d, F->n, F->st);
         F->ir_steps++;
     }
     return 0;
 }
 
-void factor_stats(const Factor *F, speigs_hip_info *h)
+void factor_stats(const Factor *Fc, speigs_hip_info *h)
 {
     if (!h) return;
+    Factor *F = const_cast<Factor *>(Fc);
+    ev_harvest(F, true);
+    h->t_solve = F->t_solve;
     h->nnz_L = F->S.nnzL;
     h->factor_flops = F->S.flops;
     h->nsuper = F->S.ns;

@@ -76,3 +76,26 @@ def test_gpu_paths_fail_loudly_without_device(hiplib):
         pytest.skip("a HIP device is present")
     A = pkg.gen.lap3d(12, True)
     assert _solve(A, 4, "lm").rc == -5          # SPEIGS_ERR_INTERNAL, no CPU fallback
+
+
+@pytest.mark.parametrize("case", ["full", "upper", "weighted_upper", "blocks", "chain"])
+def test_fast_rcm_equals_reference_scan(hiplib, oracle, case):
+    """The O(n+nnz) RCM of the banded gate reproduces the reference's O(n^2)
+    seed-scan order exactly (oracle restatement), on full and upper-only storage."""
+    import ctypes
+    gen = pkg.gen
+    A = {"full": lambda: gen.lap3d(9, True),
+         "upper": lambda: gen.upper_only(*gen.lap3d(9, True)),
+         "weighted_upper": lambda: gen.weighted_lap3d(8, seed=2, upper=True),
+         "blocks": lambda: gen.block_diag([gen.lap3d(5, True), gen.lap2d(7, True), gen.lap3d(4, False)]),
+         "chain": lambda: rc.tridiag_1d(300)}[case]()
+    Ap, Ai, _ = (np.ascontiguousarray(a) for a in A)
+    n = len(Ap) - 1
+    p1 = np.zeros(n, np.int64)
+    p2 = np.zeros(n, np.int64)
+    hiplib.speigs_hip_rcm.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    oracle.oracle_rcm.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    assert hiplib.speigs_hip_rcm(n, Ap.ctypes.data, Ai.ctypes.data, p1.ctypes.data) == 0
+    oracle.oracle_rcm(n, Ap.ctypes.data, Ai.ctypes.data, p2.ctypes.data)
+    assert np.array_equal(np.sort(p1), np.arange(n))
+    assert np.array_equal(p1, p2)

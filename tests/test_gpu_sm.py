@@ -134,15 +134,15 @@ def test_weighted_vs_oracle(hiplib, oracle, precision):
     print("precision", precision, "nops", r.info["nops"], o.info["nops"], "hip", r.hip_info)
 
 
-@pytest.mark.parametrize("mode,mfma", [("0", "0"), ("1", "0"), ("1", "1")])
-def test_kernel_variants_agree(hiplib, monkeypatch, mode, mfma):
+@pytest.mark.parametrize("mode,mfma,prec", [("0", "0", 0), ("1", "0", 0), ("1", "1", 0), ("1", "0", 1), ("1", "1", 1)])
+def test_kernel_variants_agree(hiplib, monkeypatch, mode, mfma, prec):
     """one-workgroup-per-front vs blocked multi-workgroup solves, FMA vs MFMA
     Schur updates: the same solution to rounding (46^3 has fronts of ~2000)."""
     monkeypatch.setenv("SPEIGS_HIP_SOLVE_MODE", mode)
     monkeypatch.setenv("SPEIGS_HIP_MFMA", mfma)
     A = gen.weighted_lap3d(24, seed=9, upper=True)
     n = len(A[0]) - 1
-    rc_, op, hi = _op(hiplib, A, 1, 0, 0.0)
+    rc_, op, hi = _op(hiplib, A, 1, prec, 0.0)
     assert rc_ == 0
     B = np.asfortranarray(np.random.default_rng(4).standard_normal((n, 2)))
     X = np.zeros((n, 2), order="F")
