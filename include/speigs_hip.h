@@ -29,6 +29,10 @@ extern "C" {
 #define SPEIGS_HIP_FP64     0   /* FP64 factor and solves: the drop-in path            */
 #define SPEIGS_HIP_FP32_IR  1   /* FP32 factor on MFMA, each apply refined to FP64 by
                                  * iterative refinement on an FP64 SpMV residual      */
+#define SPEIGS_HIP_FP32_BLOCK 2 /* FP32 factor (L11 blocks inverted) as the preconditioner
+                                 * of a block eigensolver (LOBPCG, block size <= 32): one
+                                 * multi-right-hand-side FP32 solve per iteration, Ritz
+                                 * pairs and residuals against A in FP64 ('sm', A+dI SPD) */
 
 /* start vector of the Lanczos recurrence */
 #define SPEIGS_HIP_START_UNIFORM  0  /* xorshift64* uniform [-0.5,0.5): the reference's
@@ -37,7 +41,7 @@ extern "C" {
 #define SPEIGS_HIP_START_LEARNED  2  /* learned start block (falls back to uniform)     */
 
 typedef struct {
-    int           precision;    /* SPEIGS_HIP_FP64 (default) or SPEIGS_HIP_FP32_IR          */
+    int           precision;    /* SPEIGS_HIP_FP64 (default), _FP32_IR or _FP32_BLOCK        */
     double        ir_tol;       /* refined apply: ||b - (A+dI)x|| <= ir_tol ||b||; 0 -> 1e-13 */
     int           ir_maxit;     /* refinement sweeps per apply; 0 -> 12                      */
     int           start;        /* SPEIGS_HIP_START_*                                        */
@@ -101,7 +105,10 @@ int  speigs_hip_op_create(speigs_int n, const speigs_int *Ap, const speigs_int *
                           speigs_hip_op **op, speigs_hip_info *hinfo);
 /* refactor with a new shift (same pattern) */
 int  speigs_hip_op_refactor(speigs_hip_op *op, double delta);
-/* y[:,j] = M x[:,j] for nvec host vectors (column-major, leading dim n) */
+/* y[:,j] = M x[:,j] for nvec host vectors (column-major, leading dim n).
+ * precision SPEIGS_HIP_FP32_BLOCK: one unrefined FP32 multi-right-hand-side
+ * solve per 32 columns (accuracy ~kappa * 6e-8, the block eigensolver's
+ * preconditioner); FP64 and FP32_IR solve each column to FP64 accuracy. */
 int  speigs_hip_op_apply(speigs_hip_op *op, const double *x, double *y, speigs_int nvec);
 /* y = A x (always A, never the shifted matrix) */
 int  speigs_hip_op_multA(speigs_hip_op *op, const double *x, double *y, speigs_int nvec);

@@ -18,7 +18,7 @@
 #include <vector>
 
 #include "dev.h"
-#include "factor.h"
+#include "factor_impl.h"
 
 using namespace sehip;
 
@@ -507,10 +507,18 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
     }
 
     double ti = wall();
-    rc = lanczos_dev(op.get(), n, k, opts, hopts, anorm, lambda, V, resid, info);
+    auto *fop = dynamic_cast<FactorOp *>(op.get());
+    if (fop && fop->F && hopts->precision == SPEIGS_HIP_FP32_BLOCK && info->path == SPEIGS_PATH_SHIFTINV)
+        rc = lobpcg_dev(op->A, fop->F, sg.s, n, k, opts, hopts, anorm, lambda, V, resid, info, hinfo);
+    else
+        rc = lanczos_dev(op.get(), n, k, opts, hopts, anorm, lambda, V, resid, info);
     info->t_iter = wall() - ti;
     if (auto *fo = dynamic_cast<FactorOp *>(op.get())) {
-        if (fo->F) factor_stats(fo->F, hinfo);
+        if (fo->F) {
+            const int64_t bsolves = hinfo->solves;
+            factor_stats(fo->F, hinfo);
+            if (hopts->precision == SPEIGS_HIP_FP32_BLOCK) hinfo->solves = bsolves;   // block solves
+        }
     }
     if (rc == SPEIGS_OK) {
         std::vector<KeyIdx> ord(k);
@@ -690,7 +698,9 @@ struct speigs_hip_op {
     SellA A;
     Factor *F = nullptr;
     double *x = nullptr, *y = nullptr;
+    float *xf = nullptr, *yf = nullptr;
     int64_t cap = 0;
+    int precision = 0;
     ~speigs_hip_op()
     {
         factor_destroy(F);
@@ -714,6 +724,7 @@ int speigs_hip_op_create(speigs_int n, const speigs_int *Ap, const speigs_int *A
     std::unique_ptr<speigs_hip_op> op(new speigs_hip_op());
     op->mode = mode;
     op->n = n;
+    op->precision = precision;
     if (hipStreamCreate(&op->stream) != hipSuccess) return SPEIGS_ERR_INTERNAL;
     int rc = sell_build(n, Ap, Ai, Ax, &op->A, op->mem, op->stream);
     if (rc) return rc;
@@ -727,6 +738,7 @@ int speigs_hip_op_create(speigs_int n, const speigs_int *Ap, const speigs_int *A
         int st = factor_numeric(op->F, delta);
         if (st < 0) return st;
         if (st == 1) return SPEIGS_ERR_FACTOR;
+        if (precision == SPEIGS_HIP_FP32_BLOCK && (rc = factor_invert_l11(op->F))) return rc;
         factor_stats(op->F, hinfo);
     }
     *out = op.release();
@@ -738,7 +750,9 @@ int speigs_hip_op_refactor(speigs_hip_op *op, double delta)
     if (!op || !op->F) return SPEIGS_ERR_ARG;
     int st = factor_numeric(op->F, delta);
     if (st < 0) return st;
-    return st == 1 ? SPEIGS_ERR_FACTOR : SPEIGS_OK;
+    if (st == 1) return SPEIGS_ERR_FACTOR;
+    if (op->precision == SPEIGS_HIP_FP32_BLOCK) return factor_invert_l11(op->F);
+    return SPEIGS_OK;
 }
 
 static int op_buffers(speigs_hip_op *op, int64_t nvec)
@@ -748,6 +762,10 @@ static int op_buffers(speigs_hip_op *op, int64_t nvec)
     int rc;
     if ((rc = op->mem.alloc(&op->x, (size_t)ld * nvec))) return rc;
     if ((rc = op->mem.alloc(&op->y, (size_t)ld * nvec))) return rc;
+    if (op->precision == SPEIGS_HIP_FP32_BLOCK) {
+        if ((rc = op->mem.alloc(&op->xf, (size_t)ld * nvec))) return rc;
+        if ((rc = op->mem.alloc(&op->yf, (size_t)ld * nvec))) return rc;
+    }
     op->cap = nvec;
     return 0;
 }
@@ -774,8 +792,27 @@ int speigs_hip_op_apply(speigs_hip_op *op, const double *x, double *y, speigs_in
     if (rc) return rc;
     const size_t bytes = (size_t)op->n * nvec * sizeof(double);
     SEHIP_CHECK(hipMemcpyAsync(op->x, x, bytes, hipMemcpyHostToDevice, op->stream));
-    for (int64_t v = 0; v < nvec; v++)
-        if ((rc = factor_apply(op->F, op->x + v * op->n, op->y + v * op->n))) return rc;
+    if (op->precision == SPEIGS_HIP_FP32_BLOCK) {
+        // one unrefined FP32 solve per block of <= 32 columns (the block
+        // eigensolver's preconditioner); SPEIGS_HIP_BLOCK_GENERIC=1 selects the
+        // generic FMA tile kernels with FP64 vectors instead of the MFMA ones
+        const char *g = getenv("SPEIGS_HIP_BLOCK_GENERIC");
+        const bool generic = g && atoi(g) == 1;
+        for (int64_t v0 = 0; v0 < nvec; v0 += 32) {
+            const int nb = (int)std::min<int64_t>(32, nvec - v0);
+            double *xb = op->x + v0 * op->n, *yb = op->y + v0 * op->n;
+            if (generic) {
+                if ((rc = factor_block_solve_d(op->F, xb, yb, nb))) return rc;
+            } else {
+                block_to_f32(xb, op->n, op->xf, nb, op->n, op->stream);
+                if ((rc = factor_block_solve(op->F, op->xf, op->yf, nb))) return rc;
+                block_to_f64(op->yf, yb, op->n, nb, op->n, op->stream);
+            }
+        }
+    } else {
+        for (int64_t v = 0; v < nvec; v++)
+            if ((rc = factor_apply(op->F, op->x + v * op->n, op->y + v * op->n))) return rc;
+    }
     SEHIP_CHECK(hipMemcpyAsync(y, op->y, bytes, hipMemcpyDeviceToHost, op->stream));
     SEHIP_CHECK(hipStreamSynchronize(op->stream));
     return SPEIGS_OK;

@@ -1,0 +1,751 @@
+// blocksolve.hip -- chain-free multi-right-hand-side solves with the supernodal
+// factor: each front's diagonal block L11 is inverted in place after the
+// factorization, so a front's forward solve is two GEMM-shaped products
+//     Y = L11^-1 W1,   U = W2 - L21 Y
+// and its backward solve two more
+//     Z = Y - L21^T X_R,   X1 = L11^-T Z,
+// with no sequential dependence inside a front; only the tree levels remain.
+//
+// In-place inverse (lower triangular, 64-column tiles, from the last tile up):
+//     X_JJ = L_JJ^-1 ;  X_IJ = -(sum_{J<K<=I} X_IK L_KJ) X_JJ   for I > J,
+// where X_IK (K > J) are the already-inverted trailing tiles.  Per step J the
+// diagonal tiles of all fronts of a level are inverted in LDS, the products
+// T_I = sum_K X_IK L_KJ go to scratch, then X_IJ = -T_I X_JJ overwrite L_IJ.
+//
+// The block solve serves as the preconditioner of the block eigensolver: the
+// FP32 factor and FP32 vectors give ~kappa u_32 accuracy, which the outer
+// iteration (residuals against A in FP64) absorbs.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "factor_impl.h"
+
+namespace sehip {
+
+// ------------------------------------------------------------------ inverse
+struct InvTask {
+    int32_t s, J, I, pad;
+};
+
+// tile (I, K) of front s, rows/cols clipped to the L11 order c
+__device__ __forceinline__ int tile_w(int c, int K) { return min(TB, c - K * TB); }
+
+template <class T>
+__global__ void __launch_bounds__(256) k_inv_diag(T *pool, const InvTask *__restrict__ tasks,
+                                                  const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
+                                                  const int32_t *__restrict__ ncol)
+{
+    __shared__ T a[TB][TB + 1];
+    __shared__ T x[TB][TB + 1];
+    const InvTask t = tasks[blockIdx.x];
+    const int32_t F = Fs[t.s], c = ncol[t.s];
+    const int k0 = t.J * TB, w = tile_w(c, t.J);
+    T *base = pool + foff[t.s] + (int64_t)k0 * F + k0;
+    for (int e = threadIdx.x; e < w * w; e += 256) {
+        const int r = e % w, q = e / w;
+        a[r][q] = r >= q ? base[(int64_t)q * F + r] : T(0);
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < w) {           // column j of the inverse by forward substitution
+        const int j = threadIdx.x;
+        for (int i = 0; i < j; i++) x[i][j] = T(0);
+        for (int i = j; i < w; i++) {
+            T s = (i == j) ? T(1) : T(0);
+            for (int k = j; k < i; k++) s -= a[i][k] * x[k][j];
+            x[i][j] = s / a[i][i];
+        }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < w * w; e += 256) {
+        const int r = e % w, q = e / w;
+        if (r >= q) base[(int64_t)q * F + r] = x[r][q];
+    }
+}
+
+// scratch_s[I] (64 x w) = sum_{J<K<=I} X_IK L_KJ   (X_II lower triangular)
+template <class T>
+__global__ void __launch_bounds__(256) k_inv_trmm(const T *__restrict__ pool, const InvTask *__restrict__ tasks,
+                                                  const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
+                                                  const int32_t *__restrict__ ncol, T *scratch,
+                                                  const int64_t *__restrict__ soff)
+{
+    __shared__ T Xt[TB][TB + 4];   // [kk][row]
+    __shared__ T Lt[TB][TB + 4];   // [kk][col]
+    const InvTask t = tasks[blockIdx.x];
+    const int32_t F = Fs[t.s], c = ncol[t.s];
+    const int J = t.J, I = t.I;
+    const int wJ = tile_w(c, J), wI = tile_w(c, I);
+    const T *fr = pool + foff[t.s];
+    const int ri = 4 * (threadIdx.x & 15), qj = 4 * (threadIdx.x >> 4);
+    T acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = T(0);
+    for (int K = J + 1; K <= I; K++) {
+        const int wK = tile_w(c, K);
+        __syncthreads();
+        for (int e = threadIdx.x; e < TB * TB; e += 256) {
+            const int r = e % TB, kk = e / TB;
+            // X_IK[r][kk] = entry (I*64 + r, K*64 + kk); lower triangular on the diagonal tile
+            T xv = T(0);
+            if (r < wI && kk < wK && (K < I || r >= kk)) xv = fr[(int64_t)(K * TB + kk) * F + I * TB + r];
+            Xt[kk][r] = xv;
+            // L_KJ[kk][q] = entry (K*64 + kk, J*64 + q), stored as Lt[kk][q]
+            const int q = e % TB, k2 = e / TB;
+            Lt[k2][q] = (k2 < wK && q < wJ) ? fr[(int64_t)(J * TB + q) * F + K * TB + k2] : T(0);
+        }
+        __syncthreads();
+        for (int kk = 0; kk < wK; kk++) {
+            T xa[4], lb[4];
+#pragma unroll
+            for (int a = 0; a < 4; a++) { xa[a] = Xt[kk][ri + a]; lb[a] = Lt[kk][qj + a]; }
+#pragma unroll
+            for (int a = 0; a < 4; a++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) acc[a][b] = fma(xa[a], lb[b], acc[a][b]);
+        }
+    }
+    T *out = scratch + soff[blockIdx.x];
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+        for (int a = 0; a < 4; a++) out[(qj + b) * TB + ri + a] = acc[a][b];
+}
+
+// L_IJ = -T_I X_JJ
+template <class T>
+__global__ void __launch_bounds__(256) k_inv_scale(T *pool, const InvTask *__restrict__ tasks,
+                                                   const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
+                                                   const int32_t *__restrict__ ncol, const T *__restrict__ scratch,
+                                                   const int64_t *__restrict__ soff)
+{
+    __shared__ T Tt[TB][TB + 4];   // [kk][row]
+    __shared__ T Xd[TB][TB + 4];   // [kk][col] = X_JJ[kk][col]
+    const InvTask t = tasks[blockIdx.x];
+    const int32_t F = Fs[t.s], c = ncol[t.s];
+    const int J = t.J, I = t.I;
+    const int wJ = tile_w(c, J), wI = tile_w(c, I);
+    T *fr = pool + foff[t.s];
+    const T *in = scratch + soff[blockIdx.x];
+    for (int e = threadIdx.x; e < TB * TB; e += 256) {
+        const int r = e % TB, kk = e / TB;
+        Tt[kk][r] = in[kk * TB + r];
+        const int q = e % TB, k2 = e / TB;
+        Xd[k2][q] = (k2 < wJ && q < wJ && k2 >= q) ? fr[(int64_t)(J * TB + q) * F + J * TB + k2] : T(0);
+    }
+    __syncthreads();
+    const int ri = 4 * (threadIdx.x & 15), qj = 4 * (threadIdx.x >> 4);
+    T acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = T(0);
+    for (int kk = 0; kk < wJ; kk++) {
+        T ta[4], xb[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) { ta[a] = Tt[kk][ri + a]; xb[a] = Xd[kk][qj + a]; }
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) acc[a][b] = fma(ta[a], xb[b], acc[a][b]);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const int q = qj + b;
+        if (q >= wJ) continue;
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+            if (ri + a < wI) fr[(int64_t)(J * TB + q) * F + I * TB + ri + a] = -acc[a][b];
+    }
+}
+
+template <class T>
+static int invert_impl(Factor *F)
+{
+    const Symbolic &S = F->S;
+    hipStream_t st = F->st;
+    T *pool = (T *)F->pool;
+    std::vector<std::vector<int64_t>> bylevel(S.nlevels);
+    for (int64_t s = 0; s < S.ns; s++) bylevel[S.level[s]].push_back(s);
+    // all task lists of all steps, uploaded once
+    std::vector<InvTask> diag, prod;
+    std::vector<int64_t> soff;
+    struct Step { int64_t d_off, d_cnt, p_off, p_cnt; };
+    std::vector<std::vector<Step>> steps(S.nlevels);
+    int64_t smax = 1;
+    for (int32_t L = 0; L < S.nlevels; L++) {
+        int64_t nbmax = 0;
+        for (int64_t s : bylevel[L]) nbmax = std::max<int64_t>(nbmax, (S.c1[s] - S.c0[s] + TB - 1) / TB);
+        for (int64_t st2 = 0; st2 < nbmax; st2++) {
+            Step sp;
+            sp.d_off = (int64_t)diag.size();
+            sp.p_off = (int64_t)prod.size();
+            int64_t so = 0;
+            for (int64_t s : bylevel[L]) {
+                const int64_t nb = (S.c1[s] - S.c0[s] + TB - 1) / TB;
+                if (st2 >= nb) continue;
+                const int32_t J = (int32_t)(nb - 1 - st2);
+                diag.push_back({(int32_t)s, J, J, 0});
+                for (int64_t I = J + 1; I < nb; I++) {
+                    prod.push_back({(int32_t)s, J, (int32_t)I, 0});
+                    soff.push_back(so);
+                    so += TB * TB;
+                }
+            }
+            smax = std::max(smax, so);
+            sp.d_cnt = (int64_t)diag.size() - sp.d_off;
+            sp.p_cnt = (int64_t)prod.size() - sp.p_off;
+            steps[L].push_back(sp);
+        }
+    }
+    if (diag.empty()) return 0;
+    if (prod.empty()) { prod.push_back({0, 0, 0, 0}); soff.push_back(0); }
+    DevMem tmp;
+    InvTask *d_diag = nullptr, *d_prod = nullptr;
+    int64_t *d_soff = nullptr;
+    T *scratch = nullptr;
+    int rc;
+    if ((rc = tmp.alloc(&d_diag, diag.size()))) return rc;
+    if ((rc = tmp.alloc(&d_prod, prod.size()))) return rc;
+    if ((rc = tmp.alloc(&d_soff, soff.size()))) return rc;
+    if ((rc = tmp.alloc(&scratch, (size_t)smax))) return rc;
+    SEHIP_CHECK(hipMemcpyAsync(d_diag, diag.data(), diag.size() * sizeof(InvTask), hipMemcpyHostToDevice, st));
+    SEHIP_CHECK(hipMemcpyAsync(d_prod, prod.data(), prod.size() * sizeof(InvTask), hipMemcpyHostToDevice, st));
+    SEHIP_CHECK(hipMemcpyAsync(d_soff, soff.data(), soff.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    for (int32_t L = 0; L < S.nlevels; L++)
+        for (const Step &sp : steps[L]) {
+            k_inv_diag<T><<<dim3((unsigned)sp.d_cnt), dim3(256), 0, st>>>(pool, d_diag + sp.d_off, F->foff, F->Fs,
+                                                                         F->ncol);
+            if (sp.p_cnt) {
+                k_inv_trmm<T><<<dim3((unsigned)sp.p_cnt), dim3(256), 0, st>>>(pool, d_prod + sp.p_off, F->foff, F->Fs,
+                                                                             F->ncol, scratch, d_soff + sp.p_off);
+                k_inv_scale<T><<<dim3((unsigned)sp.p_cnt), dim3(256), 0, st>>>(pool, d_prod + sp.p_off, F->foff, F->Fs,
+                                                                              F->ncol, scratch, d_soff + sp.p_off);
+            }
+        }
+    SEHIP_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
+int factor_invert_l11(Factor *F)
+{
+    if (!F->factored) return SPEIGS_ERR_FACTOR;
+    if (F->inverted) return 0;
+    int rc = F->fp32 ? invert_impl<float>(F) : invert_impl<double>(F);
+    if (rc == 0) F->inverted = true;
+    return rc;
+}
+
+// ------------------------------------------------------------------ block solve
+// Vectors: FP32 (or FP64) n x nb column-major with leading dimension n (new
+// order), nb <= 32; per front a workspace of F x nb (ld F).
+constexpr int NBMAX = 32;
+
+struct BsArgs {
+    const int64_t *foff, *wsoff, *col0, *rptr, *relptr, *chptr;
+    const int32_t *Fs, *ncol, *rows, *rel, *chl;
+};
+
+struct RowTask {   // front s, 64-row tile I (of L11 when I < ceil(c/64), else of the update part)
+    int32_t s, I;
+};
+
+__device__ __forceinline__ int lb_i32(const int32_t *a, int n, int v)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// W_s rows [ck*256, +256) = [B(c0..c0+c) ; 0] + children's U (children in order)
+template <class V>
+__global__ void __launch_bounds__(256) k_bs_init(const RowTask *__restrict__ tasks, BsArgs a, const V *__restrict__ B,
+                                                 int64_t n, int nb, V *ws)
+{
+    const RowTask t = tasks[blockIdx.x];
+    const int s = t.s;
+    const int32_t F = a.Fs[s], c = a.ncol[s];
+    V *w = ws + a.wsoff[s] * NBMAX;
+    const int64_t c0 = a.col0[s];
+    const int r0 = t.I * 256, r1 = min(F, r0 + 256);
+    const int i = r0 + threadIdx.x;
+    if (i < r1)
+        for (int v = 0; v < nb; v++) w[(int64_t)v * F + i] = i < c ? B[(int64_t)v * n + c0 + i] : V(0);
+    for (int64_t q = a.chptr[s]; q < a.chptr[s + 1]; q++) {
+        __syncthreads();
+        const int ch = a.chl[q];
+        const int32_t Fc = a.Fs[ch], cc = a.ncol[ch], rc = Fc - cc;
+        const int32_t *rl = a.rel + a.relptr[ch];
+        const int lo = lb_i32(rl, rc, r0), hi = lb_i32(rl, rc, r1);
+        const V *u = ws + a.wsoff[ch] * NBMAX;
+        for (int e = lo + threadIdx.x; e < hi; e += 256)
+            for (int v = 0; v < nb; v++) w[(int64_t)v * F + rl[e]] += u[(int64_t)v * Fc + cc + e];
+    }
+}
+
+// generic tile-row product on one front (vectors n x nb, column-major, ld n):
+//   acc[r][v] (64 x nb) = sum over K-tiles of op(A)[r][k] * Xk[k][v]
+// MODE 0 forward L11:    A = X11 tile row I, tiles K <= I (lower on K == I), Xk = W1 (ws) -> Y (global)
+// MODE 1 forward L21:    A = L21 tile row I (rows c + 64I ..), all K-tiles of L11, Xk = Y -> W2 -= acc (ws)
+// MODE 2 backward L21^T: A = L21 column tile I, K over update tiles, Xk = X at rows R -> Z = Y - acc (ws rows < c)
+// MODE 3 backward L11^T: A = X11 column tile I, tiles K >= I (lower on K == I), Xk = Z (ws) -> X (global)
+template <class T, class V, int MODE>
+__global__ void __launch_bounds__(256) k_bs_tile(const T *__restrict__ pool, const RowTask *__restrict__ tasks,
+                                                 BsArgs a, int64_t n, int nb, V *ws, V *Yg, V *Xg)
+{
+    __shared__ V At[TB][TB + 1];     // At[r][k] = op(A)[r][k]
+    __shared__ V Xs[TB][NBMAX + 1];  // Xs[k][v]
+    const RowTask t = tasks[blockIdx.x];
+    const int s = t.s, I = t.I;
+    const int32_t F = a.Fs[s], c = a.ncol[s];
+    const int nt11 = (c + TB - 1) / TB;
+    const int nupd = (F - c + TB - 1) / TB;
+    const T *L = pool + a.foff[s];
+    V *w = ws + a.wsoff[s] * NBMAX;
+    const int64_t c0 = a.col0[s];
+    const int32_t *R = a.rows + a.rptr[s];
+    int orow0, on;
+    if (MODE == 1) { orow0 = c + I * TB; on = min(TB, F - orow0); }
+    else { orow0 = I * TB; on = min(TB, c - orow0); }
+    int Kbeg, Kend;
+    if (MODE == 0) { Kbeg = 0; Kend = I + 1; }
+    else if (MODE == 1) { Kbeg = 0; Kend = nt11; }
+    else if (MODE == 2) { Kbeg = 0; Kend = nupd; }
+    else { Kbeg = I; Kend = nt11; }
+    const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
+    V acc[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) acc[j] = V(0);
+    for (int K = Kbeg; K < Kend; K++) {
+        int krow0, kn;
+        if (MODE == 2) { krow0 = c + K * TB; kn = min(TB, F - krow0); }
+        else { krow0 = K * TB; kn = min(TB, c - krow0); }
+        __syncthreads();
+        for (int e = threadIdx.x; e < TB * TB; e += 256) {
+            const int p = e % TB, q = e / TB;   // p fastest: consecutive rows of one column
+            if (MODE == 0 || MODE == 1) {       // A[r][k] = L(orow0 + r, krow0 + k): column krow0+q, row orow0+p
+                V av = V(0);
+                if (p < on && q < kn && (MODE == 1 || K < I || p >= q)) av = (V)L[(int64_t)(krow0 + q) * F + orow0 + p];
+                At[p][q] = av;
+            } else {                            // op(A)[r][k] = L(krow0 + k, orow0 + r): column orow0+q, row krow0+p
+                V av = V(0);
+                if (p < kn && q < on && (MODE == 2 || K > I || p >= q)) av = (V)L[(int64_t)(orow0 + q) * F + krow0 + p];
+                At[q][p] = av;
+            }
+        }
+        for (int e = threadIdx.x; e < TB * nb; e += 256) {
+            const int kk = e % TB, v = e / TB;
+            V xv = V(0);
+            if (kk < kn) {
+                if (MODE == 0 || MODE == 3) xv = w[(int64_t)v * F + krow0 + kk];
+                else if (MODE == 1) xv = Yg[(int64_t)v * n + c0 + krow0 + kk];
+                else xv = Xg[(int64_t)v * n + R[krow0 - c + kk]];
+            }
+            Xs[kk][v] = xv;
+        }
+        __syncthreads();
+        for (int k = 0; k < kn; k++) {
+            const V av = At[r][k];
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+                const int v = g + 4 * j;
+                if (v < nb) acc[j] = fma(av, Xs[k][v], acc[j]);
+            }
+        }
+    }
+    if (r < on) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int v = g + 4 * j;
+            if (v >= nb) continue;
+            if (MODE == 0) Yg[(int64_t)v * n + c0 + orow0 + r] = acc[j];
+            else if (MODE == 1) w[(int64_t)v * F + orow0 + r] -= acc[j];
+            else if (MODE == 2) w[(int64_t)v * F + orow0 + r] = Yg[(int64_t)v * n + c0 + orow0 + r] - acc[j];
+            else Xg[(int64_t)v * n + c0 + orow0 + r] = acc[j];
+        }
+    }
+}
+
+// ---- fast path: FP32 vectors, split-K MFMA tile products + fixed-order epilogue
+// Part task (front s, output tile I, K tiles [k0, k1)): partial[task] (64 x 32,
+// row-major) = sum_K op(A)_IK X_K, on v_mfma_f32_16x16x4f32: wave w owns rows
+// 16w..16w+15, two 16-column blocks of right-hand sides; lane l supplies
+// A[16w + l%16][k + l/16] and X[k + l/16][16h + l%16] and receives
+// out[16w + 4(l/16) + v][16h + l%16].
+struct PartTask {
+    int32_t s, I, k0, k1;
+};
+struct FinTask {
+    int32_t s, I;
+    int64_t p0;      // first part task of this output tile
+    int32_t np, pad;
+};
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <class T, int MODE>
+__global__ void __launch_bounds__(256) k_bsf_part(const T *__restrict__ pool, const PartTask *__restrict__ tasks,
+                                                  BsArgs a, int64_t n, int nb, const float *__restrict__ ws,
+                                                  const float *__restrict__ Yg, float *__restrict__ part)
+{
+    __shared__ float At[TB][TB + 4];
+    __shared__ float Xs[TB][48];
+    const PartTask t = tasks[blockIdx.x];
+    const int s = t.s, I = t.I;
+    const int32_t F = a.Fs[s], c = a.ncol[s];
+    const T *L = pool + a.foff[s];
+    const float *w = ws + a.wsoff[s] * NBMAX;
+    const int64_t c0 = a.col0[s];
+    int orow0, on;
+    if (MODE == 1) { orow0 = c + I * TB; on = min(TB, F - orow0); }
+    else { orow0 = I * TB; on = min(TB, c - orow0); }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    for (int K = t.k0; K < t.k1; K++) {
+        int krow0, kn;
+        if (MODE == 2) { krow0 = c + K * TB; kn = min(TB, F - krow0); }
+        else { krow0 = K * TB; kn = min(TB, c - krow0); }
+        __syncthreads();
+        for (int e = threadIdx.x; e < TB * TB; e += 256) {
+            const int p = e & 63, q = e >> 6;
+            if (MODE == 0 || MODE == 1) {   // At[p][q] = L(orow0 + p, krow0 + q)
+                float av = 0.f;
+                if (p < on && q < kn && (MODE == 1 || K < I || p >= q)) av = (float)L[(int64_t)(krow0 + q) * F + orow0 + p];
+                At[p][q] = av;
+            } else {                        // At[q][p] = L(krow0 + p, orow0 + q)
+                float av = 0.f;
+                if (p < kn && q < on && (MODE == 2 || K > I || p >= q)) av = (float)L[(int64_t)(orow0 + q) * F + krow0 + p];
+                At[q][p] = av;
+            }
+        }
+        for (int e = threadIdx.x; e < TB * 32; e += 256) {
+            const int kk = e & 63, v = e >> 6;
+            float xv = 0.f;
+            if (kk < kn && v < nb) {
+                if (MODE == 1) xv = Yg[(int64_t)v * n + c0 + krow0 + kk];
+                else xv = w[(int64_t)v * F + krow0 + kk];
+            }
+            Xs[kk][v] = xv;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int k = 0; k < TB; k += 4) {
+            const float av = At[16 * wv + lr][k + lk];
+            const float b0 = Xs[k + lk][lr], b1 = Xs[k + lk][16 + lr];
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b1, acc1, 0, 0, 0);
+        }
+    }
+    float *out = part + (int64_t)blockIdx.x * (TB * 32);
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+        const int row = 16 * wv + 4 * lk + v;
+        out[row * 32 + lr] = acc0[v];
+        out[row * 32 + 16 + lr] = acc1[v];
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_bsf_fin(const FinTask *__restrict__ tasks, BsArgs a, int64_t n, int nb,
+                                                 float *__restrict__ ws, float *__restrict__ Yg,
+                                                 float *__restrict__ Xg, const float *__restrict__ part)
+{
+    const FinTask t = tasks[blockIdx.x];
+    const int s = t.s, I = t.I;
+    const int32_t F = a.Fs[s], c = a.ncol[s];
+    float *w = ws + a.wsoff[s] * NBMAX;
+    const int64_t c0 = a.col0[s];
+    int orow0, on;
+    if (MODE == 1) { orow0 = c + I * TB; on = min(TB, F - orow0); }
+    else { orow0 = I * TB; on = min(TB, c - orow0); }
+    for (int e = threadIdx.x; e < TB * 32; e += 256) {
+        const int r = e & 63, v = e >> 6;
+        if (r >= on || v >= nb) continue;
+        float sum = 0.f;
+        for (int q = 0; q < t.np; q++) sum += part[(t.p0 + q) * (TB * 32) + r * 32 + v];
+        if (MODE == 0) Yg[(int64_t)v * n + c0 + orow0 + r] = sum;
+        else if (MODE == 1) w[(int64_t)v * F + orow0 + r] -= sum;
+        else if (MODE == 2) w[(int64_t)v * F + orow0 + r] = Yg[(int64_t)v * n + c0 + orow0 + r] - sum;
+        else Xg[(int64_t)v * n + c0 + orow0 + r] = sum;
+    }
+}
+
+// backward: ws rows >= c of each front of the level = X at the front's update rows
+__global__ void __launch_bounds__(256) k_bsf_gather(const RowTask *__restrict__ tasks, BsArgs a, int64_t n, int nb,
+                                                    float *__restrict__ ws, const float *__restrict__ Xg)
+{
+    const RowTask t = tasks[blockIdx.x];
+    const int s = t.s;
+    const int32_t F = a.Fs[s], c = a.ncol[s];
+    float *w = ws + a.wsoff[s] * NBMAX;
+    const int32_t *R = a.rows + a.rptr[s];
+    const int i = c + t.I * 256 + threadIdx.x;
+    if (i >= F) return;
+    const int64_t g = R[i - c];
+    for (int v = 0; v < nb; v++) w[(int64_t)v * F + i] = Xg[(int64_t)v * n + g];
+}
+
+template <class V>
+__global__ void k_bs_perm(const V *__restrict__ src, const int64_t *__restrict__ perm, V *__restrict__ dst, int64_t n,
+                          int nb, int gather)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int64_t p = perm[i];
+    for (int v = 0; v < nb; v++) {
+        if (gather) dst[(int64_t)v * n + i] = src[(int64_t)v * n + p];
+        else dst[(int64_t)v * n + p] = src[(int64_t)v * n + i];
+    }
+}
+
+namespace {
+
+struct BsPlan {
+    std::vector<int64_t> init_off, init_cnt, m0_off, m0_cnt, m1_off, m1_cnt;
+};
+
+template <class V>
+struct BsState {
+    bool ready = false;
+    DevMem mem;
+    RowTask *tasks = nullptr;
+    BsPlan plan;
+    V *ws = nullptr, *Pg = nullptr, *Yg = nullptr, *Xg = nullptr;
+    // fast path (V = float)
+    PartTask *pt = nullptr;
+    FinTask *ft = nullptr;
+    float *part = nullptr;
+    RowTask *gt = nullptr;
+    std::vector<int64_t> pt_off[4], pt_cnt[4], ft_off[4], ft_cnt[4], gt_off, gt_cnt;
+};
+
+constexpr int KCHUNK = 6;   // K tiles per part task
+
+// part/fin task lists of one mode for the fronts of a level
+void plan_mode(const Symbolic &S, const std::vector<int64_t> &fronts, int mode, std::vector<PartTask> &pt,
+               std::vector<FinTask> &ft)
+{
+    for (int64_t s : fronts) {
+        const int64_t c = S.c1[s] - S.c0[s], F = S.F[s];
+        const int64_t nt11 = (c + TB - 1) / TB, nupd = (F - c + TB - 1) / TB;
+        const int64_t nout = mode == 1 ? nupd : nt11;
+        for (int64_t I = 0; I < nout; I++) {
+            int64_t kb, ke;
+            if (mode == 0) { kb = 0; ke = I + 1; }
+            else if (mode == 1) { kb = 0; ke = nt11; }
+            else if (mode == 2) { kb = 0; ke = nupd; }
+            else { kb = I; ke = nt11; }
+            // a front without update rows still needs Z = Y (a finish with no parts)
+            if (ke <= kb && mode != 2) continue;
+            FinTask f;
+            f.s = (int32_t)s;
+            f.I = (int32_t)I;
+            f.p0 = (int64_t)pt.size();
+            f.pad = 0;
+            for (int64_t k0 = kb; k0 < ke; k0 += KCHUNK)
+                pt.push_back({(int32_t)s, (int32_t)I, (int32_t)k0, (int32_t)std::min<int64_t>(ke, k0 + KCHUNK)});
+            f.np = (int32_t)((int64_t)pt.size() - f.p0);
+            ft.push_back(f);
+        }
+    }
+}
+
+int bsf_prepare(Factor *F, BsState<float> &B)
+{
+    const Symbolic &S = F->S;
+    std::vector<std::vector<int64_t>> bylevel(S.nlevels);
+    for (int64_t s = 0; s < S.ns; s++) bylevel[S.level[s]].push_back(s);
+    std::vector<PartTask> pt;
+    std::vector<FinTask> ft;
+    std::vector<RowTask> gt;
+    int64_t maxpart = 1;
+    for (int32_t L = 0; L < S.nlevels; L++) {
+        for (int mode = 0; mode < 4; mode++) {
+            const int64_t p0 = (int64_t)pt.size(), f0 = (int64_t)ft.size();
+            plan_mode(S, bylevel[L], mode, pt, ft);
+            B.pt_off[mode].push_back(p0);
+            B.pt_cnt[mode].push_back((int64_t)pt.size() - p0);
+            B.ft_off[mode].push_back(f0);
+            B.ft_cnt[mode].push_back((int64_t)ft.size() - f0);
+            // part indices of fin tasks are relative to the mode's launch
+            for (int64_t q = f0; q < (int64_t)ft.size(); q++) ft[q].p0 -= p0;
+            maxpart = std::max<int64_t>(maxpart, (int64_t)pt.size() - p0);
+        }
+        B.gt_off.push_back((int64_t)gt.size());
+        for (int64_t s : bylevel[L]) {
+            const int64_t c = S.c1[s] - S.c0[s], r = S.F[s] - c;
+            for (int64_t ck = 0; ck * 256 < r; ck++) gt.push_back({(int32_t)s, (int32_t)ck});
+        }
+        B.gt_cnt.push_back((int64_t)gt.size() - B.gt_off.back());
+    }
+    if (pt.empty()) pt.push_back({0, 0, 0, 0});
+    if (ft.empty()) ft.push_back({0, 0, 0, 0, 0});
+    if (gt.empty()) gt.push_back({0, 0});
+    int rc;
+    if ((rc = B.mem.alloc(&B.pt, pt.size()))) return rc;
+    if ((rc = B.mem.alloc(&B.ft, ft.size()))) return rc;
+    if ((rc = B.mem.alloc(&B.gt, gt.size()))) return rc;
+    if ((rc = B.mem.alloc(&B.part, (size_t)maxpart * TB * 32))) return rc;
+    SEHIP_CHECK(hipMemcpyAsync(B.pt, pt.data(), pt.size() * sizeof(PartTask), hipMemcpyHostToDevice, F->st));
+    SEHIP_CHECK(hipMemcpyAsync(B.ft, ft.data(), ft.size() * sizeof(FinTask), hipMemcpyHostToDevice, F->st));
+    SEHIP_CHECK(hipMemcpyAsync(B.gt, gt.data(), gt.size() * sizeof(RowTask), hipMemcpyHostToDevice, F->st));
+    SEHIP_CHECK(hipStreamSynchronize(F->st));
+    return 0;
+}
+
+template <class T>
+int bsf_run(Factor *F, BsState<float> &B, const float *Bin, float *Xout, int nb)
+{
+    const Symbolic &S = F->S;
+    hipStream_t st = F->st;
+    const T *pool = (const T *)F->pool;
+    BsArgs a{F->foff, F->wsoff, F->col0, F->rptr, F->relptr, F->chptr, F->Fs, F->ncol, F->rows, F->rel, F->chl};
+    const unsigned nbk = (unsigned)((F->n + 255) / 256);
+    k_bs_perm<float><<<dim3(nbk), dim3(256), 0, st>>>(Bin, F->perm, B.Pg, F->n, nb, 1);
+    auto mode_launch = [&](int mode, int32_t L) {
+        const int64_t pc = B.pt_cnt[mode][L], fc = B.ft_cnt[mode][L];
+        if (!fc) return;
+        const PartTask *pt = B.pt + B.pt_off[mode][L];
+        const FinTask *ft = B.ft + B.ft_off[mode][L];
+        const dim3 gp((unsigned)pc), gf((unsigned)fc), blk(256);
+        switch (mode) {
+        case 0: if (pc) k_bsf_part<T, 0><<<gp, blk, 0, st>>>(pool, pt, a, F->n, nb, B.ws, B.Yg, B.part);
+                k_bsf_fin<0><<<gf, blk, 0, st>>>(ft, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part); break;
+        case 1: if (pc) k_bsf_part<T, 1><<<gp, blk, 0, st>>>(pool, pt, a, F->n, nb, B.ws, B.Yg, B.part);
+                k_bsf_fin<1><<<gf, blk, 0, st>>>(ft, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part); break;
+        case 2: if (pc) k_bsf_part<T, 2><<<gp, blk, 0, st>>>(pool, pt, a, F->n, nb, B.ws, B.Yg, B.part);
+                k_bsf_fin<2><<<gf, blk, 0, st>>>(ft, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part); break;
+        default: if (pc) k_bsf_part<T, 3><<<gp, blk, 0, st>>>(pool, pt, a, F->n, nb, B.ws, B.Yg, B.part);
+                 k_bsf_fin<3><<<gf, blk, 0, st>>>(ft, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part);
+        }
+    };
+    for (int32_t L = 0; L < S.nlevels; L++) {
+        const BsPlan &P = B.plan;
+        if (P.init_cnt[L])
+            k_bs_init<float><<<dim3((unsigned)P.init_cnt[L]), dim3(256), 0, st>>>(B.tasks + P.init_off[L], a, B.Pg,
+                                                                                  F->n, nb, B.ws);
+        mode_launch(0, L);
+        mode_launch(1, L);
+    }
+    for (int32_t L = S.nlevels - 1; L >= 0; L--) {
+        if (B.gt_cnt[L])
+            k_bsf_gather<<<dim3((unsigned)B.gt_cnt[L]), dim3(256), 0, st>>>(B.gt + B.gt_off[L], a, F->n, nb, B.ws, B.Xg);
+        mode_launch(2, L);
+        mode_launch(3, L);
+    }
+    k_bs_perm<float><<<dim3(nbk), dim3(256), 0, st>>>(B.Xg, F->perm, Xout, F->n, nb, 0);
+    return 0;
+}
+
+template <class V>
+int bs_prepare(Factor *F, BsState<V> &B)
+{
+    if (B.ready) return 0;
+    const Symbolic &S = F->S;
+    std::vector<std::vector<int64_t>> bylevel(S.nlevels);
+    for (int64_t s = 0; s < S.ns; s++) bylevel[S.level[s]].push_back(s);
+    std::vector<RowTask> tk;
+    for (int32_t L = 0; L < S.nlevels; L++) {
+        B.plan.init_off.push_back((int64_t)tk.size());
+        for (int64_t s : bylevel[L])
+            for (int64_t ck = 0; ck * 256 < S.F[s]; ck++) tk.push_back({(int32_t)s, (int32_t)ck});
+        B.plan.init_cnt.push_back((int64_t)tk.size() - B.plan.init_off.back());
+        B.plan.m0_off.push_back((int64_t)tk.size());   // tiles of L11 (modes 0, 2, 3)
+        for (int64_t s : bylevel[L])
+            for (int64_t I = 0; I * TB < S.c1[s] - S.c0[s]; I++) tk.push_back({(int32_t)s, (int32_t)I});
+        B.plan.m0_cnt.push_back((int64_t)tk.size() - B.plan.m0_off.back());
+        B.plan.m1_off.push_back((int64_t)tk.size());   // tiles of the update rows (mode 1)
+        for (int64_t s : bylevel[L]) {
+            const int64_t r = S.F[s] - (S.c1[s] - S.c0[s]);
+            for (int64_t I = 0; I * TB < r; I++) tk.push_back({(int32_t)s, (int32_t)I});
+        }
+        B.plan.m1_cnt.push_back((int64_t)tk.size() - B.plan.m1_off.back());
+    }
+    if (tk.empty()) tk.push_back({0, 0});
+    int rc;
+    if ((rc = B.mem.alloc(&B.tasks, tk.size()))) return rc;
+    SEHIP_CHECK(hipMemcpyAsync(B.tasks, tk.data(), tk.size() * sizeof(RowTask), hipMemcpyHostToDevice, F->st));
+    if ((rc = B.mem.alloc(&B.ws, (size_t)std::max<int64_t>(F->wssize, 1) * NBMAX))) return rc;
+    if ((rc = B.mem.alloc(&B.Pg, (size_t)F->n * NBMAX))) return rc;
+    if ((rc = B.mem.alloc(&B.Yg, (size_t)F->n * NBMAX))) return rc;
+    if ((rc = B.mem.alloc(&B.Xg, (size_t)F->n * NBMAX))) return rc;
+    SEHIP_CHECK(hipStreamSynchronize(F->st));
+    B.ready = true;
+    return 0;
+}
+
+template <class T, class V>
+int bs_run(Factor *F, BsState<V> &B, const V *Bin, V *Xout, int nb)
+{
+    const Symbolic &S = F->S;
+    hipStream_t st = F->st;
+    const T *pool = (const T *)F->pool;
+    BsArgs a{F->foff, F->wsoff, F->col0, F->rptr, F->relptr, F->chptr, F->Fs, F->ncol, F->rows, F->rel, F->chl};
+    const unsigned nbk = (unsigned)((F->n + 255) / 256);
+    k_bs_perm<V><<<dim3(nbk), dim3(256), 0, st>>>(Bin, F->perm, B.Pg, F->n, nb, 1);
+    for (int32_t L = 0; L < S.nlevels; L++) {
+        const BsPlan &P = B.plan;
+        if (P.init_cnt[L])
+            k_bs_init<V><<<dim3((unsigned)P.init_cnt[L]), dim3(256), 0, st>>>(B.tasks + P.init_off[L], a, B.Pg, F->n,
+                                                                              nb, B.ws);
+        if (P.m0_cnt[L])
+            k_bs_tile<T, V, 0><<<dim3((unsigned)P.m0_cnt[L]), dim3(256), 0, st>>>(pool, B.tasks + P.m0_off[L], a, F->n,
+                                                                                  nb, B.ws, B.Yg, B.Xg);
+        if (P.m1_cnt[L])
+            k_bs_tile<T, V, 1><<<dim3((unsigned)P.m1_cnt[L]), dim3(256), 0, st>>>(pool, B.tasks + P.m1_off[L], a, F->n,
+                                                                                  nb, B.ws, B.Yg, B.Xg);
+    }
+    for (int32_t L = S.nlevels - 1; L >= 0; L--) {
+        const BsPlan &P = B.plan;
+        if (P.m0_cnt[L]) {
+            k_bs_tile<T, V, 2><<<dim3((unsigned)P.m0_cnt[L]), dim3(256), 0, st>>>(pool, B.tasks + P.m0_off[L], a, F->n,
+                                                                                  nb, B.ws, B.Yg, B.Xg);
+            k_bs_tile<T, V, 3><<<dim3((unsigned)P.m0_cnt[L]), dim3(256), 0, st>>>(pool, B.tasks + P.m0_off[L], a, F->n,
+                                                                                  nb, B.ws, B.Yg, B.Xg);
+        }
+    }
+    k_bs_perm<V><<<dim3(nbk), dim3(256), 0, st>>>(B.Xg, F->perm, Xout, F->n, nb, 0);
+    return 0;
+}
+
+}  // namespace
+
+int factor_block_solve(Factor *F, const float *Bin, float *Xout, int nb)
+{
+    if (nb < 1 || nb > NBMAX) return SPEIGS_ERR_ARG;
+    if (!F->inverted) return SPEIGS_ERR_FACTOR;
+    if (!F->bs_f) F->bs_f = new BsState<float>();
+    BsState<float> &B = *reinterpret_cast<BsState<float> *>(F->bs_f);
+    if (!B.ready) {
+        int rc = bs_prepare(F, B);
+        if (rc) return rc;
+        if ((rc = bsf_prepare(F, B))) return rc;
+    }
+    return F->fp32 ? bsf_run<float>(F, B, Bin, Xout, nb) : bsf_run<double>(F, B, Bin, Xout, nb);
+}
+
+int factor_block_solve_d(Factor *F, const double *Bin, double *Xout, int nb)
+{
+    if (nb < 1 || nb > NBMAX) return SPEIGS_ERR_ARG;
+    if (!F->inverted) return SPEIGS_ERR_FACTOR;
+    if (!F->bs_d) F->bs_d = new BsState<double>();
+    BsState<double> &B = *reinterpret_cast<BsState<double> *>(F->bs_d);
+    int rc = bs_prepare(F, B);
+    if (rc) return rc;
+    return F->fp32 ? bs_run<float, double>(F, B, Bin, Xout, nb) : bs_run<double, double>(F, B, Bin, Xout, nb);
+}
+
+void factor_block_free(Factor *F)
+{
+    delete reinterpret_cast<BsState<float> *>(F->bs_f);
+    delete reinterpret_cast<BsState<double> *>(F->bs_d);
+    F->bs_f = F->bs_d = nullptr;
+}
+
+}  // namespace sehip

@@ -85,6 +85,9 @@ void cgs_update(const double *V, int64_t ldv, int m, const double *h, double *w,
 // X[:, t] = V[:, 0:m] S[:, t], t < p (S column-major m x p, device)
 void tsgemm(const double *V, int64_t ldv, int m, const double *S, int p, double *X, int64_t ldx,
             int64_t n, hipStream_t st);
+// X = V S + beta X (X must not alias V)
+void tsgemm_beta(const double *V, int64_t ldv, int m, const double *S, int p, double *X, int64_t ldx,
+                 int64_t n, double beta, hipStream_t st);
 // partial[b*p + t] = sum X[:,t] . Y[:,t]
 void pairdots_partial(const double *X, int64_t ldx, const double *Y, int64_t ldy, int p, int64_t n,
                       double *partial, hipStream_t st);
@@ -100,6 +103,17 @@ void axpby_residual(const double *b, const double *Ax, const double *x, double d
                     int64_t n, hipStream_t st);
 // x += d
 void vadd(double *x, const double *d, int64_t n, hipStream_t st);
+
+// ------------------------------------------------------------ block kernels
+// G (p x q, column-major, ld p) = A^T B for p, q <= 96; partial must hold
+// gram_blocks(n) * p * q doubles
+int64_t gram_blocks(int64_t n);
+void gram(const double *A, int64_t lda, int p, const double *B, int64_t ldb, int q, int64_t n, double *partial,
+          double *G, hipStream_t st);
+void block_resid(const double *AX, const double *X, int64_t ld, const double *lam, int m, int64_t n, double *R,
+                 hipStream_t st);
+void block_to_f32(const double *X, int64_t ld, float *Y, int m, int64_t n, hipStream_t st);
+void block_to_f64(const float *Y, double *X, int64_t ld, int m, int64_t n, hipStream_t st);
 
 // ------------------------------------------------------------ operator seam
 struct DevOperator {

@@ -20,11 +20,10 @@
 #include <cstring>
 #include <memory>
 
-#include "factor.h"
+#include "factor_impl.h"
 
 namespace sehip {
 
-constexpr int TB = 64;   // tile edge
 
 // ------------------------------------------------------------------ kernels
 template <class T>
@@ -46,9 +45,7 @@ __global__ void k_assemble(T *pool, const int64_t *__restrict__ dst, const int64
     pool[dst[e]] = (T)v;
 }
 
-struct EaTask {   // extend-add: child front columns [cb*64, cb*64+64) into the parent
-    int32_t child, parent, cb, pad;
-};
+
 
 template <class T>
 __global__ void __launch_bounds__(256) k_extend_add(T *pool, const EaTask *__restrict__ tasks,
@@ -69,9 +66,7 @@ __global__ void __launch_bounds__(256) k_extend_add(T *pool, const EaTask *__res
     }
 }
 
-struct PcbTask {   // a 64-column block of a parent front that receives extend-adds
-    int32_t parent, pcb;
-};
+
 
 __device__ __forceinline__ int lower_bound_i32(const int32_t *a, int n, int v)
 {
@@ -113,9 +108,7 @@ __global__ void __launch_bounds__(256) k_extend_add_pcb(T *pool, const PcbTask *
     }
 }
 
-struct TileTask {   // panel k of front s; tile row i, tile column j
-    int32_t s, k, i, j;
-};
+
 
 // diagonal tile of panel k: in-LDS unblocked Cholesky with the pivot test
 template <class T>
@@ -352,11 +345,7 @@ __global__ void k_scatter_perm(const double *__restrict__ x, const int64_t *__re
     if (i < n) y[perm[i]] = x[i];
 }
 
-struct SolveArgs {
-    const int64_t *foff, *wsoff, *col0, *rptr, *relptr, *chptr;
-    const int32_t *Fs, *ncol, *rows, *rel, *chl;
-    double *ws;
-};
+
 
 // forward: L y = b for every front of a level (one block per front)
 template <class T>
@@ -454,9 +443,7 @@ __global__ void __launch_bounds__(256) k_bwd(const T *__restrict__ pool, const i
 // themselves cross workgroups through agent-scope atomic loads and stores.
 // Every wait is bounded: after ~2^22 polls the item records an error and
 // continues, so a logic error shows up as SPEIGS_ERR_INTERNAL, never a hang.
-struct BlkTask {
-    int32_t s, b;
-};
+
 
 __device__ __forceinline__ void wait_epoch(const int *f, int epoch, int *err)
 {
@@ -622,9 +609,7 @@ __global__ void __launch_bounds__(256) k_fwd_blk(const T *__restrict__ pool, con
 
 // backward, update rows: part[s][p][j][q] = sum over rows r of chunk j (512 rows
 // of the front's update part) of L[r][p*64+q] * x[R[r-c]] -- fully parallel
-struct UpdTask {
-    int32_t s, p, j, pad;
-};
+
 
 template <class T>
 __global__ void __launch_bounds__(256) k_bwd_upd(const T *__restrict__ pool, const UpdTask *__restrict__ tasks,
@@ -783,68 +768,9 @@ __global__ void k_norm2_partial(const double *x, int64_t n, double *partial)
 }
 
 // ------------------------------------------------------------------ host side
-struct LevelPlan {
-    std::vector<int64_t> ea_off;      // per child ordinal: offset in the extend-add task array
-    std::vector<int64_t> ea_cnt;
-    std::vector<int64_t> k_potrf_off, k_potrf_cnt, k_trsm_off, k_trsm_cnt, k_upd_off, k_upd_cnt;
-    int64_t fr_off = 0, fr_cnt = 0;   // fronts of the level in the fronts array
-    std::vector<int64_t> sea_off, sea_cnt;   // (unused) per-ordinal solve extend-add
-    int64_t pcb_off = 0, pcb_cnt = 0;        // factor extend-add: parent column blocks
-    int64_t fin_off = 0, fin_cnt = 0;        // solve init: (front, 256-row chunk)
-    int64_t fb_off = 0, fb_cnt = 0, bb_off = 0, bb_cnt = 0;   // blocked solve work items
-    int64_t ub_off = 0, ub_cnt = 0;                            // backward update-row chunks
-};
 
-struct Factor {
-    Symbolic S;
-    FactorOptions opt;
-    hipStream_t st = nullptr;
-    DevMem mem;
-    int64_t n = 0, nnzA = 0;
-    bool fp32 = false;
-    void *pool = nullptr;
-    double *Ax = nullptr, *diagA = nullptr;
-    int64_t *asm_dst = nullptr, *asm_src = nullptr, nasm = 0;
-    int64_t *foff = nullptr, *wsoff = nullptr, *col0 = nullptr, *rptr = nullptr, *relptr = nullptr, *chptr = nullptr;
-    int32_t *Fs = nullptr, *ncol = nullptr, *rows = nullptr, *rel = nullptr, *chl = nullptr, *fronts = nullptr;
-    int64_t *perm = nullptr;
-    EaTask *ea = nullptr;
-    TileTask *tiles = nullptr;
-    std::vector<LevelPlan> plan;
-    int *flag = nullptr;
-    double *ws = nullptr, *xp = nullptr, *yp = nullptr, *r = nullptr, *d = nullptr, *ax = nullptr, *partial = nullptr;
-    double *hpart = nullptr;
-    int64_t wssize = 0;
-    double delta = 0.0;
-    const SellA *A = nullptr;
-    int64_t solves = 0, ir_steps = 0;
-    bool factored = false;
-    bool use_mfma = true;
-    int solve_mode = 1;               // 0: one workgroup per front, 1: blocked multi-workgroup
-    EaTask *sea = nullptr;
-    PcbTask *pcb = nullptr, *fin = nullptr;
-    UpdTask *upd = nullptr;
-    double *bpart = nullptr;
-    int64_t *partoff = nullptr;
-    unsigned *tick_f = nullptr, *tick_b = nullptr;
-    BlkTask *fblk = nullptr, *bblk = nullptr;
-    int64_t *panbase = nullptr;
-    int *fflag = nullptr, *bflag = nullptr, *err = nullptr, *herr = nullptr;
-    unsigned *counters = nullptr;
-    int epoch = 0;
-    // HIP events around each solve pair, harvested at the next apply (the
-    // Krylov loop synchronizes between applies) or at factor_stats
-    std::vector<hipEvent_t> ev;
-    std::vector<int> ev_pending;      // indices of recorded pairs not yet read
-    int ev_next = 0;
-    double t_solve = 0.0;
-    ~Factor()
-    {
-        if (hpart) (void)hipHostFree(hpart);
-        if (herr) (void)hipHostFree(herr);
-        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-    }
-};
+
+
 
 namespace {
 
@@ -930,6 +856,10 @@ void run_solve(Factor *F, const double *b, double *x)
     const T *pool = (const T *)F->pool;
     SolveArgs a{F->foff, F->wsoff, F->col0, F->rptr, F->relptr, F->chptr, F->Fs, F->ncol, F->rows, F->rel, F->chl, F->ws};
     const unsigned nb = (unsigned)((F->n + 255) / 256);
+    if (F->solve_mode == 2) {           // inverse-based, chain-free (see blocksolve.hip)
+        if (factor_block_solve_d(F, b, x, 1) == 0) F->solves++;
+        return;
+    }
     k_gather_perm<<<dim3(nb), dim3(256), 0, st>>>(b, F->perm, F->xp, F->n);
     if (F->solve_mode == 1) {
         const int ep = ++F->epoch;
@@ -997,7 +927,7 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     F->opt = o;
     F->st = st;
     F->n = n;
-    F->fp32 = o.precision == SPEIGS_HIP_FP32_IR;
+    F->fp32 = o.precision == SPEIGS_HIP_FP32_IR || o.precision == SPEIGS_HIP_FP32_BLOCK;
     // debug knobs for A/B checks of kernel variants
     if (const char *e = getenv("SPEIGS_HIP_SOLVE_MODE")) F->solve_mode = atoi(e);
     if (const char *e = getenv("SPEIGS_HIP_MFMA")) F->use_mfma = atoi(e) != 0;
@@ -1260,6 +1190,7 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
 int factor_numeric(Factor *F, double delta)
 {
     F->delta = delta;
+    F->inverted = false;
     int st = F->fp32 ? run_numeric<float>(F, delta) : run_numeric<double>(F, delta);
     F->factored = (st == 0);
     return st;
@@ -1271,6 +1202,10 @@ int factor_apply(Factor *F, const double *x, double *y)
 {
     if (!F->factored) return SPEIGS_ERR_FACTOR;
     if (F->herr && *F->herr) return SPEIGS_ERR_INTERNAL;   // a bounded wait gave up in an earlier solve
+    if (F->solve_mode == 2 && !F->inverted) {
+        const int rc = factor_invert_l11(F);
+        if (rc) return rc;
+    }
     ev_harvest(F, false);
     if (!F->fp32) {
         const int e = ev_begin(F);
@@ -1315,12 +1250,16 @@ void factor_stats(const Factor *Fc, speigs_hip_info *h)
     h->nlevels = F->S.nlevels;
     h->ir_steps = F->ir_steps;
     h->solves = F->solves;
-    h->precision_used = F->fp32 ? SPEIGS_HIP_FP32_IR : SPEIGS_HIP_FP64;
+    h->precision_used = F->opt.precision == SPEIGS_HIP_FP32_BLOCK ? SPEIGS_HIP_FP32_BLOCK : F->fp32 ? SPEIGS_HIP_FP32_IR : SPEIGS_HIP_FP64;
     const double es = F->fp32 ? 4.0 : 8.0;
     h->bytes_solve = 2.0 * (double)F->S.nnzL * es + 2.0 * 2.0 * (double)F->n * 8.0;
 }
 
-void factor_destroy(Factor *F) { delete F; }
+void factor_destroy(Factor *F)
+{
+    if (F) factor_block_free(F);
+    delete F;
+}
 
 const Symbolic *factor_symbolic(const Factor *F) { return &F->S; }
 

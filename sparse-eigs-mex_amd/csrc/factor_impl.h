@@ -1,0 +1,117 @@
+// factor_impl.h -- internal state of the device supernodal factor, shared by
+// factor.hip (factorization, blocked solves), blocksolve.hip (in-place inverse
+// of the diagonal blocks, multi-right-hand-side solves) and lobpcg.cpp.
+#pragma once
+#include <vector>
+
+#include "factor.h"
+
+namespace sehip {
+
+constexpr int TB = 64;   // tile edge
+
+struct EaTask {   // extend-add: child front columns [cb*64, cb*64+64) into the parent
+    int32_t child, parent, cb, pad;
+};
+
+struct PcbTask {   // a 64-column block of a parent front that receives extend-adds
+    int32_t parent, pcb;
+};
+
+struct TileTask {   // panel k of front s; tile row i, tile column j
+    int32_t s, k, i, j;
+};
+
+struct SolveArgs {
+    const int64_t *foff, *wsoff, *col0, *rptr, *relptr, *chptr;
+    const int32_t *Fs, *ncol, *rows, *rel, *chl;
+    double *ws;
+};
+
+struct BlkTask {
+    int32_t s, b;
+};
+
+struct UpdTask {
+    int32_t s, p, j, pad;
+};
+
+struct LevelPlan {
+    std::vector<int64_t> ea_off;      // per child ordinal: offset in the extend-add task array
+    std::vector<int64_t> ea_cnt;
+    std::vector<int64_t> k_potrf_off, k_potrf_cnt, k_trsm_off, k_trsm_cnt, k_upd_off, k_upd_cnt;
+    int64_t fr_off = 0, fr_cnt = 0;   // fronts of the level in the fronts array
+    std::vector<int64_t> sea_off, sea_cnt;   // (unused) per-ordinal solve extend-add
+    int64_t pcb_off = 0, pcb_cnt = 0;        // factor extend-add: parent column blocks
+    int64_t fin_off = 0, fin_cnt = 0;        // solve init: (front, 256-row chunk)
+    int64_t fb_off = 0, fb_cnt = 0, bb_off = 0, bb_cnt = 0;   // blocked solve work items
+    int64_t ub_off = 0, ub_cnt = 0;                            // backward update-row chunks
+};
+
+struct Factor {
+    Symbolic S;
+    FactorOptions opt;
+    hipStream_t st = nullptr;
+    DevMem mem;
+    int64_t n = 0, nnzA = 0;
+    bool fp32 = false;
+    void *pool = nullptr;
+    double *Ax = nullptr, *diagA = nullptr;
+    int64_t *asm_dst = nullptr, *asm_src = nullptr, nasm = 0;
+    int64_t *foff = nullptr, *wsoff = nullptr, *col0 = nullptr, *rptr = nullptr, *relptr = nullptr, *chptr = nullptr;
+    int32_t *Fs = nullptr, *ncol = nullptr, *rows = nullptr, *rel = nullptr, *chl = nullptr, *fronts = nullptr;
+    int64_t *perm = nullptr;
+    EaTask *ea = nullptr;
+    TileTask *tiles = nullptr;
+    std::vector<LevelPlan> plan;
+    int *flag = nullptr;
+    double *ws = nullptr, *xp = nullptr, *yp = nullptr, *r = nullptr, *d = nullptr, *ax = nullptr, *partial = nullptr;
+    double *hpart = nullptr;
+    int64_t wssize = 0;
+    double delta = 0.0;
+    const SellA *A = nullptr;
+    int64_t solves = 0, ir_steps = 0;
+    bool factored = false;
+    bool use_mfma = true;
+    int solve_mode = 1;               // 0: one workgroup per front, 1: blocked multi-workgroup
+    EaTask *sea = nullptr;
+    PcbTask *pcb = nullptr, *fin = nullptr;
+    UpdTask *upd = nullptr;
+    double *bpart = nullptr;
+    int64_t *partoff = nullptr;
+    unsigned *tick_f = nullptr, *tick_b = nullptr;
+    BlkTask *fblk = nullptr, *bblk = nullptr;
+    int64_t *panbase = nullptr;
+    int *fflag = nullptr, *bflag = nullptr, *err = nullptr, *herr = nullptr;
+    unsigned *counters = nullptr;
+    int epoch = 0;
+    // HIP events around each solve pair, harvested at the next apply (the
+    // Krylov loop synchronizes between applies) or at factor_stats
+    std::vector<hipEvent_t> ev;
+    std::vector<int> ev_pending;      // indices of recorded pairs not yet read
+    int ev_next = 0;
+    double t_solve = 0.0;
+    bool inverted = false;            // L11 of every front holds its inverse
+    void *bs_f = nullptr, *bs_d = nullptr;   // block-solve state (FP32 / FP64 vectors)
+    ~Factor()
+    {
+        if (hpart) (void)hipHostFree(hpart);
+        if (herr) (void)hipHostFree(herr);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+// in-place inverse of every front's L11 (chain-free solves); 0 ok
+int factor_invert_l11(Factor *F);
+// multi-right-hand-side solve with the inverted factor: X = (L L^T)^-1 B for
+// nb <= 32 FP32 columns (ld n); requires factor_invert_l11
+int factor_block_solve(Factor *F, const float *B, float *X, int nb);
+int factor_block_solve_d(Factor *F, const double *B, double *X, int nb);
+void factor_block_free(Factor *F);
+
+// block eigensolver preconditioned by the FP32 factor (lobpcg.cpp)
+int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, const speigs_opts *o,
+               const speigs_hip_opts *ho, double anorm, double *lambda, double *Vout, double *resid,
+               speigs_info *info, speigs_hip_info *hinfo);
+
+}  // namespace sehip

@@ -134,7 +134,8 @@ def test_weighted_vs_oracle(hiplib, oracle, precision):
     print("precision", precision, "nops", r.info["nops"], o.info["nops"], "hip", r.hip_info)
 
 
-@pytest.mark.parametrize("mode,mfma,prec", [("0", "0", 0), ("1", "0", 0), ("1", "1", 0), ("1", "0", 1), ("1", "1", 1)])
+@pytest.mark.parametrize("mode,mfma,prec", [("0", "0", 0), ("1", "0", 0), ("1", "1", 0), ("1", "0", 1), ("1", "1", 1),
+                                            ("2", "1", 0), ("2", "1", 1)])
 def test_kernel_variants_agree(hiplib, monkeypatch, mode, mfma, prec):
     """one-workgroup-per-front vs blocked multi-workgroup solves, FMA vs MFMA
     Schur updates: the same solution to rounding (46^3 has fronts of ~2000)."""
@@ -152,3 +153,63 @@ def test_kernel_variants_agree(hiplib, monkeypatch, mode, mfma, prec):
     print("mode", mode, "mfma", mfma, "residual", rel)
     assert rel <= 1e-12
     hiplib.speigs_hip_op_destroy(op)
+
+
+def test_inverse_solve_weighted_100(hiplib, monkeypatch):
+    """inverse-based (chain-free) FP64 solve on a 40^3 C3-family matrix: residual at rounding level."""
+    monkeypatch.setenv("SPEIGS_HIP_SOLVE_MODE", "2")
+    A = gen.weighted_lap3d(40, seed=3, upper=True)
+    n = len(A[0]) - 1
+    rc_, op, hi = _op(hiplib, A, 1, 0, 0.0)
+    assert rc_ == 0
+    B = np.asfortranarray(np.random.default_rng(8).standard_normal((n, 1)))
+    X = np.zeros((n, 1), order="F")
+    assert hiplib.speigs_hip_op_apply(op, B.ctypes.data, X.ctypes.data, 1) == 0
+    rel = np.linalg.norm(gen.spmv_upper(*A, X) - B) / np.linalg.norm(B)
+    print("inverse-based FP64 solve residual", rel)
+    assert rel <= 1e-11
+    hiplib.speigs_hip_op_destroy(op)
+
+
+def test_block_fp32_path_matches_fp64(hiplib):
+    """FP32-factor block eigensolver: same eigenvalues as the FP64 Lanczos path (1e-10 relative)."""
+    A = gen.weighted_lap3d(20, seed=13)
+    r64 = _solve(A, 12, "sm", opts={"tol": 1e-11})
+    rb = _solve(A, 12, "sm", opts={"tol": 1e-11}, hip_opts={"precision": 2})
+    print("fp64 nops", r64.info["nops"], "block", rb.info, rb.hip_info)
+    assert r64.rc == 0 and rb.rc == 0 and rb.info["flag"] == 0
+    assert np.abs(rb.lam - r64.lam).max() <= 1e-10 * np.abs(r64.lam).max()
+    assert rb.resid.max() <= 1e-11 * rb.info["anorm"]
+    R = gen.spmv_upper(*A, rb.V) - rb.V * rb.lam
+    assert np.linalg.norm(R, axis=0).max() <= 1e-11 * rb.info["anorm"] * 1.01
+
+
+@pytest.mark.parametrize("N,nrhs", [(12, 7), (24, 20), (40, 32), (40, 45)])
+def test_block_solve_fp32(hiplib, monkeypatch, N, nrhs):
+    """unrefined FP32 multi-right-hand-side solve (precision 2): the MFMA split-K
+    kernels with FP32 vectors and the generic FMA tiles with FP64 vectors agree
+    to FP32 rounding, and both solve A X = B to ~kappa*u32."""
+    A = gen.weighted_lap3d(N, seed=3, upper=True)
+    n = len(A[0]) - 1
+    B = np.asfortranarray(np.random.default_rng(N).standard_normal((n, nrhs)))
+    rc_, op64, _ = _op(hiplib, A, 1, 0, 0.0)
+    assert rc_ == 0
+    X64 = np.zeros((n, nrhs), order="F")
+    assert hiplib.speigs_hip_op_apply(op64, B.ctypes.data, X64.ctypes.data, nrhs) == 0
+    hiplib.speigs_hip_op_destroy(op64)
+    rc_, op, _ = _op(hiplib, A, 1, 2, 0.0)
+    assert rc_ == 0
+    Xf = np.zeros((n, nrhs), order="F")
+    Xg = np.zeros((n, nrhs), order="F")
+    monkeypatch.setenv("SPEIGS_HIP_BLOCK_GENERIC", "0")
+    assert hiplib.speigs_hip_op_apply(op, B.ctypes.data, Xf.ctypes.data, nrhs) == 0
+    monkeypatch.setenv("SPEIGS_HIP_BLOCK_GENERIC", "1")
+    assert hiplib.speigs_hip_op_apply(op, B.ctypes.data, Xg.ctypes.data, nrhs) == 0
+    hiplib.speigs_hip_op_destroy(op)
+    col = lambda D, R: (np.linalg.norm(D, axis=0) / np.linalg.norm(R, axis=0)).max()
+    e_fg, e_g64, e_f64 = col(Xf - Xg, Xg), col(Xg - X64, X64), col(Xf - X64, X64)
+    print(f"[block] N={N} nrhs={nrhs} fast-vs-generic {e_fg:.2e} generic-vs-fp64 {e_g64:.2e} "
+          f"fast-vs-fp64 {e_f64:.2e}")
+    assert e_g64 <= 1e-3
+    assert e_f64 <= 1e-3
+    assert e_fg <= 1e-4
