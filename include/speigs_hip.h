@@ -59,13 +59,15 @@ typedef struct {
     int64_t nsuper;         /* supernodes (fronts)                                     */
     int64_t nlevels;        /* levels of the supernodal elimination tree               */
     int64_t ir_steps;       /* refinement corrections over all applies                 */
-    int64_t solves;         /* triangular solve pairs executed                         */
+    int64_t solves;         /* triangular solve pairs executed (block path: multi-
+                             * right-hand-side solves, each over <= 32 columns)       */
     int     precision_used; /* factor precision actually used                          */
     int     fallback;       /* 1 if FP32 factor/refinement failed and FP64 took over    */
     double  t_order, t_symbolic, t_upload, t_numeric, t_solve, t_spmv;
     double  bytes_solve;    /* algorithmic bytes of one solve pair (factor + vectors)  */
     double  bytes_spmv;     /* algorithmic bytes of one SpMV                           */
-    double  reserved[8];
+    double  t_invert;       /* in-place inversion of the diagonal blocks (block path)  */
+    double  reserved[7];
 } speigs_hip_info;
 
 void speigs_hip_default_opts(speigs_hip_opts *o);

@@ -11,6 +11,7 @@
 // There is no CPU fallback for the GPU paths: without a HIP device they fail
 // with SPEIGS_ERR_INTERNAL.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -19,6 +20,7 @@
 
 #include "dev.h"
 #include "factor_impl.h"
+#include "host_par.h"
 
 using namespace sehip;
 
@@ -118,33 +120,45 @@ int dense_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax
     return SPEIGS_OK;
 }
 
-int64_t uf_root(std::vector<int64_t> &p, int64_t x)
-{
-    while (p[x] != x) {
-        p[x] = p[p[x]];
-        x = p[x];
-    }
-    return x;
-}
-
-// connected components over numerically nonzero entries (src/speigs_core.c:436-466)
+// Connected components of the nonzero pattern (reference splits them at
+// src/speigs_core.c:577-607), numbered by their lowest node.  Lock-free
+// union-find over the columns in parallel: a union links the larger root
+// under the smaller, so every root is its set's minimum node, and components
+// numbered in the order of their roots are numbered by first appearance.
 int64_t components(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax, std::vector<int64_t> &comp)
 {
-    std::vector<int64_t> p(n), lab(n, -1);
-    for (int64_t i = 0; i < n; i++) p[i] = i;
-    for (int64_t c = 0; c < n; c++)
-        for (int64_t q = Ap[c]; q < Ap[c + 1]; q++)
-            if (Ax[q] != 0.0) {
-                int64_t a = uf_root(p, Ai[q]), b = uf_root(p, c);
-                if (a != b) p[a] = b;
+    std::unique_ptr<std::atomic<int64_t>[]> p(new std::atomic<int64_t>[n]);
+    const int nt = host_threads();
+    parallel_for(n, nt, 65536, [&](int64_t i, int) { p[i].store(i, std::memory_order_relaxed); });
+    auto find = [&](int64_t x) {
+        for (;;) {
+            int64_t px = p[x].load(std::memory_order_relaxed);
+            if (px == x) return x;
+            const int64_t ppx = p[px].load(std::memory_order_relaxed);
+            if (ppx != px) p[x].compare_exchange_weak(px, ppx, std::memory_order_relaxed);
+            x = ppx;
+        }
+    };
+    parallel_for(n, nt, 4096, [&](int64_t c, int) {
+        for (int64_t q = Ap[c]; q < Ap[c + 1]; q++) {
+            if (Ax[q] == 0.0) continue;
+            int64_t a = Ai[q], b = c;
+            for (;;) {
+                a = find(a);
+                b = find(b);
+                if (a == b) break;
+                if (a < b) std::swap(a, b);   // link root a under the smaller root b
+                int64_t expect = a;
+                if (p[a].compare_exchange_strong(expect, b, std::memory_order_relaxed)) break;
             }
-    int64_t nc = 0;
+        }
+    });
     comp.resize(n);
-    for (int64_t i = 0; i < n; i++) {
-        int64_t r = uf_root(p, i);
-        if (lab[r] < 0) lab[r] = nc++;
-        comp[i] = lab[r];
-    }
+    parallel_for(n, nt, 65536, [&](int64_t i, int) { comp[i] = find(i); });
+    std::vector<int64_t> lab(n);
+    int64_t nc = 0;
+    for (int64_t i = 0; i < n; i++) lab[i] = comp[i] == i ? nc++ : -1;
+    parallel_for(n, nt, 65536, [&](int64_t i, int) { comp[i] = lab[comp[i]]; });
     return nc;
 }
 
@@ -398,6 +412,7 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
     if (opts->check_sym != 0 && symmetry_kind(n, Ap, Ai, Ax) == 0) return SPEIGS_ERR_NOTSYM;
 
     const double t0 = wall();
+    double tp = t0;
     const int dmax = opts->dense_max < 0 ? 800 : opts->dense_max;
     if (opts->detect) {
         bool diag = true;
@@ -420,6 +435,7 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
             return rc;
         }
     }
+    trace_phase("components", tp);
     double anorm = norm1_upper(n, Ap, Ai, Ax);
     if (anorm <= 0) anorm = 1.0;
     info->anorm = anorm;
@@ -438,7 +454,10 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
         info->path = SPEIGS_PATH_DIRECT;
     } else {
         auto *hs = new HostSolveOp();
-        if (try_banded(n, Ap, Ai, Ax, opts, anorm, hs, info)) {
+        trace_phase("device, stream, norm", tp);
+        const bool banded = try_banded(n, Ap, Ai, Ax, opts, anorm, hs, info);
+        trace_phase("banded gate", tp);
+        if (banded) {
             op.reset(hs);
         } else {
             delete hs;
@@ -494,6 +513,7 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
             }
             info->t_factor = wall() - tf;
             hinfo->t_numeric = info->t_factor;
+            tp = wall();
         }
     }
     op->n = n;
@@ -501,6 +521,7 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
     double tu = wall();
     if ((rc = sell_build(n, Ap, Ai, Ax, &op->A, mem, sg.s))) return rc;
     hinfo->t_upload += wall() - tu;
+    trace_phase("SELL build + upload", tp);
     hinfo->bytes_spmv = spmv_bytes(op->A);
     if (auto *fo = dynamic_cast<FactorOp *>(op.get())) {
         if (fo->F) factor_attach_matrix(fo->F, &op->A);
@@ -513,27 +534,29 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
     else
         rc = lanczos_dev(op.get(), n, k, opts, hopts, anorm, lambda, V, resid, info);
     info->t_iter = wall() - ti;
+    tp = wall();
     if (auto *fo = dynamic_cast<FactorOp *>(op.get())) {
-        if (fo->F) {
-            const int64_t bsolves = hinfo->solves;
-            factor_stats(fo->F, hinfo);
-            if (hopts->precision == SPEIGS_HIP_FP32_BLOCK) hinfo->solves = bsolves;   // block solves
-        }
+        if (fo->F) factor_stats(fo->F, hinfo);
     }
     if (rc == SPEIGS_OK) {
         std::vector<KeyIdx> ord(k);
-        std::vector<double> lc(lambda, lambda + k), rc2(k), Vc;
-        if (resid) rc2.assign(resid, resid + k);
-        if (V) Vc.assign(V, V + (size_t)n * k);
         for (int64_t i = 0; i < k; i++) ord[i] = {std::fabs(lambda[i]), i};
         qsort(ord.data(), (size_t)k, sizeof(KeyIdx), which == SPEIGS_SM ? key_asc : key_desc);
-        for (int64_t t = 0; t < k; t++) {
-            int64_t i = ord[t].idx;
-            lambda[t] = lc[i];
-            if (resid) resid[t] = rc2[i];
-            if (V) std::memcpy(V + t * n, Vc.data() + i * n, (size_t)n * sizeof(double));
+        bool ident = true;
+        for (int64_t t = 0; t < k; t++) ident = ident && ord[t].idx == t;
+        if (!ident) {
+            std::vector<double> lc(lambda, lambda + k), rc2(k), Vc;
+            if (resid) rc2.assign(resid, resid + k);
+            if (V) Vc.assign(V, V + (size_t)n * k);
+            for (int64_t t = 0; t < k; t++) {
+                int64_t i = ord[t].idx;
+                lambda[t] = lc[i];
+                if (resid) resid[t] = rc2[i];
+                if (V) std::memcpy(V + t * n, Vc.data() + i * n, (size_t)n * sizeof(double));
+            }
         }
     }
+    trace_phase("final order", tp);
     return rc;
 }
 

@@ -20,6 +20,7 @@
 #include <cstring>
 
 #include "factor_impl.h"
+#include "mfma.h"
 
 namespace sehip {
 
@@ -63,103 +64,98 @@ __global__ void __launch_bounds__(256) k_inv_diag(T *pool, const InvTask *__rest
     }
 }
 
-// scratch_s[I] (64 x w) = sum_{J<K<=I} X_IK L_KJ   (X_II lower triangular)
+// Split-K products of one inversion step J (MFMA): part task (s, J, I, k0, k1)
+// writes P = sum_{k0<=K<k1} X_IK L_KJ (64 x 64, column-major) to its scratch
+// slot, with J < k0 < k1 <= I + 1 and X_II lower triangular.
+struct InvPart {
+    int32_t s, J, I, k0, k1, pad;
+};
+// finish task: L_IJ <- -(sum of its np partials, fixed order) X_JJ
+struct InvFin {
+    int32_t s, J, I, np;
+    int64_t p0;
+};
+
+constexpr int IP = TB + 4;   // LDS row pitch of the inverse kernels
+
 template <class T>
-__global__ void __launch_bounds__(256) k_inv_trmm(const T *__restrict__ pool, const InvTask *__restrict__ tasks,
+__global__ void __launch_bounds__(256) k_inv_part(const T *__restrict__ pool, const InvPart *__restrict__ tasks,
                                                   const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
-                                                  const int32_t *__restrict__ ncol, T *scratch,
-                                                  const int64_t *__restrict__ soff)
+                                                  const int32_t *__restrict__ ncol, T *__restrict__ part)
 {
-    __shared__ T Xt[TB][TB + 4];   // [kk][row]
-    __shared__ T Lt[TB][TB + 4];   // [kk][col]
-    const InvTask t = tasks[blockIdx.x];
+    __shared__ T Xt[TB][IP];   // Xt[kk][row] = X_IK(row, kk)
+    __shared__ T Lt[TB][IP];   // Lt[kk][col] = L_KJ(kk, col)
+    typedef typename MfmaAcc<T>::V V;
+    const InvPart t = tasks[blockIdx.x];
     const int32_t F = Fs[t.s], c = ncol[t.s];
     const int J = t.J, I = t.I;
     const int wJ = tile_w(c, J), wI = tile_w(c, I);
     const T *fr = pool + foff[t.s];
-    const int ri = 4 * (threadIdx.x & 15), qj = 4 * (threadIdx.x >> 4);
-    T acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = T(0);
-    for (int K = J + 1; K <= I; K++) {
+    V acc[2][2];
+    zero_acc<T>(acc);
+    for (int K = t.k0; K < t.k1; K++) {
         const int wK = tile_w(c, K);
         __syncthreads();
         for (int e = threadIdx.x; e < TB * TB; e += 256) {
-            const int r = e % TB, kk = e / TB;
-            // X_IK[r][kk] = entry (I*64 + r, K*64 + kk); lower triangular on the diagonal tile
+            const int r = e & 63, q = e >> 6;
             T xv = T(0);
-            if (r < wI && kk < wK && (K < I || r >= kk)) xv = fr[(int64_t)(K * TB + kk) * F + I * TB + r];
-            Xt[kk][r] = xv;
-            // L_KJ[kk][q] = entry (K*64 + kk, J*64 + q), stored as Lt[kk][q]
-            const int q = e % TB, k2 = e / TB;
-            Lt[k2][q] = (k2 < wK && q < wJ) ? fr[(int64_t)(J * TB + q) * F + K * TB + k2] : T(0);
+            if (r < wI && q < wK && (K < I || r >= q)) xv = fr[(int64_t)(K * TB + q) * F + I * TB + r];
+            Xt[q][r] = xv;
+            Lt[r][q] = (r < wK && q < wJ) ? fr[(int64_t)(J * TB + q) * F + K * TB + r] : T(0);
         }
         __syncthreads();
-        for (int kk = 0; kk < wK; kk++) {
-            T xa[4], lb[4];
-#pragma unroll
-            for (int a = 0; a < 4; a++) { xa[a] = Xt[kk][ri + a]; lb[a] = Lt[kk][qj + a]; }
-#pragma unroll
-            for (int a = 0; a < 4; a++)
-#pragma unroll
-                for (int b = 0; b < 4; b++) acc[a][b] = fma(xa[a], lb[b], acc[a][b]);
-        }
+        tile_mma<T, IP>(Xt, Lt, (wK + 3) & ~3, acc);
     }
-    T *out = scratch + soff[blockIdx.x];
+    T *out = part + (int64_t)blockIdx.x * (TB * TB);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int rb = (wv & 1) * 32, cb = (wv >> 1) * 32, lr = lane & 15, lk = lane >> 4;
 #pragma unroll
-    for (int b = 0; b < 4; b++)
+    for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int a = 0; a < 4; a++) out[(qj + b) * TB + ri + a] = acc[a][b];
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                out[(cb + 16 * b + MfmaAcc<T>::colof(lk, v)) * TB + rb + 16 * a + lr] = acc[a][b][v];
 }
 
-// L_IJ = -T_I X_JJ
 template <class T>
-__global__ void __launch_bounds__(256) k_inv_scale(T *pool, const InvTask *__restrict__ tasks,
-                                                   const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
-                                                   const int32_t *__restrict__ ncol, const T *__restrict__ scratch,
-                                                   const int64_t *__restrict__ soff)
+__global__ void __launch_bounds__(256) k_inv_fin(T *pool, const InvFin *__restrict__ tasks,
+                                                 const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
+                                                 const int32_t *__restrict__ ncol, const T *__restrict__ part)
 {
-    __shared__ T Tt[TB][TB + 4];   // [kk][row]
-    __shared__ T Xd[TB][TB + 4];   // [kk][col] = X_JJ[kk][col]
-    const InvTask t = tasks[blockIdx.x];
+    __shared__ T Tt[TB][IP];   // Tt[kk][row] = P(row, kk)
+    __shared__ T Xd[TB][IP];   // Xd[kk][col] = X_JJ(kk, col)
+    typedef typename MfmaAcc<T>::V V;
+    const InvFin t = tasks[blockIdx.x];
     const int32_t F = Fs[t.s], c = ncol[t.s];
     const int J = t.J, I = t.I;
     const int wJ = tile_w(c, J), wI = tile_w(c, I);
     T *fr = pool + foff[t.s];
-    const T *in = scratch + soff[blockIdx.x];
     for (int e = threadIdx.x; e < TB * TB; e += 256) {
-        const int r = e % TB, kk = e / TB;
-        Tt[kk][r] = in[kk * TB + r];
-        const int q = e % TB, k2 = e / TB;
-        Xd[k2][q] = (k2 < wJ && q < wJ && k2 >= q) ? fr[(int64_t)(J * TB + q) * F + J * TB + k2] : T(0);
+        const int r = e & 63, q = e >> 6;
+        T sum = T(0);
+        for (int p = 0; p < t.np; p++) sum += part[(t.p0 + p) * (TB * TB) + e];
+        Tt[q][r] = sum;
+        Xd[r][q] = (r < wJ && q < wJ && r >= q) ? fr[(int64_t)(J * TB + q) * F + J * TB + r] : T(0);
     }
     __syncthreads();
-    const int ri = 4 * (threadIdx.x & 15), qj = 4 * (threadIdx.x >> 4);
-    T acc[4][4];
+    V acc[2][2];
+    zero_acc<T>(acc);
+    tile_mma<T, IP>(Tt, Xd, (wJ + 3) & ~3, acc);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int rb = (wv & 1) * 32, cb = (wv >> 1) * 32, lr = lane & 15, lk = lane >> 4;
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) acc[a][b] = T(0);
-    for (int kk = 0; kk < wJ; kk++) {
-        T ta[4], xb[4];
+        for (int b = 0; b < 2; b++)
 #pragma unroll
-        for (int a = 0; a < 4; a++) { ta[a] = Tt[kk][ri + a]; xb[a] = Xd[kk][qj + a]; }
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int b = 0; b < 4; b++) acc[a][b] = fma(ta[a], xb[b], acc[a][b]);
-    }
-#pragma unroll
-    for (int b = 0; b < 4; b++) {
-        const int q = qj + b;
-        if (q >= wJ) continue;
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-            if (ri + a < wI) fr[(int64_t)(J * TB + q) * F + I * TB + ri + a] = -acc[a][b];
-    }
+            for (int v = 0; v < 4; v++) {
+                const int row = rb + 16 * a + lr, col = cb + 16 * b + MfmaAcc<T>::colof(lk, v);
+                if (row < wI && col < wJ) fr[(int64_t)(J * TB + col) * F + I * TB + row] = -acc[a][b][v];
+            }
 }
+
+constexpr int INV_KCHUNK = 8;   // K tiles per part task
 
 template <class T>
 static int invert_impl(Factor *F)
@@ -170,61 +166,69 @@ static int invert_impl(Factor *F)
     std::vector<std::vector<int64_t>> bylevel(S.nlevels);
     for (int64_t s = 0; s < S.ns; s++) bylevel[S.level[s]].push_back(s);
     // all task lists of all steps, uploaded once
-    std::vector<InvTask> diag, prod;
-    std::vector<int64_t> soff;
-    struct Step { int64_t d_off, d_cnt, p_off, p_cnt; };
-    std::vector<std::vector<Step>> steps(S.nlevels);
-    int64_t smax = 1;
+    std::vector<InvTask> diag;
+    std::vector<InvPart> parts;
+    std::vector<InvFin> fins;
+    struct Step { int64_t d_off, d_cnt, p_off, p_cnt, f_off, f_cnt; };
+    std::vector<Step> steps;
+    int64_t pmax = 1;
     for (int32_t L = 0; L < S.nlevels; L++) {
         int64_t nbmax = 0;
         for (int64_t s : bylevel[L]) nbmax = std::max<int64_t>(nbmax, (S.c1[s] - S.c0[s] + TB - 1) / TB);
         for (int64_t st2 = 0; st2 < nbmax; st2++) {
             Step sp;
             sp.d_off = (int64_t)diag.size();
-            sp.p_off = (int64_t)prod.size();
-            int64_t so = 0;
+            sp.p_off = (int64_t)parts.size();
+            sp.f_off = (int64_t)fins.size();
             for (int64_t s : bylevel[L]) {
                 const int64_t nb = (S.c1[s] - S.c0[s] + TB - 1) / TB;
                 if (st2 >= nb) continue;
                 const int32_t J = (int32_t)(nb - 1 - st2);
                 diag.push_back({(int32_t)s, J, J, 0});
-                for (int64_t I = J + 1; I < nb; I++) {
-                    prod.push_back({(int32_t)s, J, (int32_t)I, 0});
-                    soff.push_back(so);
-                    so += TB * TB;
+                for (int32_t I = J + 1; I < nb; I++) {
+                    InvFin f;
+                    f.s = (int32_t)s;
+                    f.J = J;
+                    f.I = I;
+                    f.p0 = (int64_t)parts.size() - sp.p_off;
+                    for (int32_t k0 = J + 1; k0 <= I; k0 += INV_KCHUNK)
+                        parts.push_back({(int32_t)s, J, I, k0, std::min<int32_t>(I + 1, k0 + INV_KCHUNK), 0});
+                    f.np = (int32_t)((int64_t)parts.size() - sp.p_off - f.p0);
+                    fins.push_back(f);
                 }
             }
-            smax = std::max(smax, so);
             sp.d_cnt = (int64_t)diag.size() - sp.d_off;
-            sp.p_cnt = (int64_t)prod.size() - sp.p_off;
-            steps[L].push_back(sp);
+            sp.p_cnt = (int64_t)parts.size() - sp.p_off;
+            sp.f_cnt = (int64_t)fins.size() - sp.f_off;
+            pmax = std::max(pmax, sp.p_cnt);
+            steps.push_back(sp);
         }
     }
     if (diag.empty()) return 0;
-    if (prod.empty()) { prod.push_back({0, 0, 0, 0}); soff.push_back(0); }
+    if (parts.empty()) parts.push_back({0, 0, 0, 0, 0, 0});
+    if (fins.empty()) fins.push_back({0, 0, 0, 0, 0});
     DevMem tmp;
-    InvTask *d_diag = nullptr, *d_prod = nullptr;
-    int64_t *d_soff = nullptr;
+    InvTask *d_diag = nullptr;
+    InvPart *d_parts = nullptr;
+    InvFin *d_fins = nullptr;
     T *scratch = nullptr;
     int rc;
     if ((rc = tmp.alloc(&d_diag, diag.size()))) return rc;
-    if ((rc = tmp.alloc(&d_prod, prod.size()))) return rc;
-    if ((rc = tmp.alloc(&d_soff, soff.size()))) return rc;
-    if ((rc = tmp.alloc(&scratch, (size_t)smax))) return rc;
+    if ((rc = tmp.alloc(&d_parts, parts.size()))) return rc;
+    if ((rc = tmp.alloc(&d_fins, fins.size()))) return rc;
+    if ((rc = tmp.alloc(&scratch, (size_t)pmax * TB * TB))) return rc;
     SEHIP_CHECK(hipMemcpyAsync(d_diag, diag.data(), diag.size() * sizeof(InvTask), hipMemcpyHostToDevice, st));
-    SEHIP_CHECK(hipMemcpyAsync(d_prod, prod.data(), prod.size() * sizeof(InvTask), hipMemcpyHostToDevice, st));
-    SEHIP_CHECK(hipMemcpyAsync(d_soff, soff.data(), soff.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
-    for (int32_t L = 0; L < S.nlevels; L++)
-        for (const Step &sp : steps[L]) {
-            k_inv_diag<T><<<dim3((unsigned)sp.d_cnt), dim3(256), 0, st>>>(pool, d_diag + sp.d_off, F->foff, F->Fs,
-                                                                         F->ncol);
-            if (sp.p_cnt) {
-                k_inv_trmm<T><<<dim3((unsigned)sp.p_cnt), dim3(256), 0, st>>>(pool, d_prod + sp.p_off, F->foff, F->Fs,
-                                                                             F->ncol, scratch, d_soff + sp.p_off);
-                k_inv_scale<T><<<dim3((unsigned)sp.p_cnt), dim3(256), 0, st>>>(pool, d_prod + sp.p_off, F->foff, F->Fs,
-                                                                              F->ncol, scratch, d_soff + sp.p_off);
-            }
+    SEHIP_CHECK(hipMemcpyAsync(d_parts, parts.data(), parts.size() * sizeof(InvPart), hipMemcpyHostToDevice, st));
+    SEHIP_CHECK(hipMemcpyAsync(d_fins, fins.data(), fins.size() * sizeof(InvFin), hipMemcpyHostToDevice, st));
+    for (const Step &sp : steps) {
+        k_inv_diag<T><<<dim3((unsigned)sp.d_cnt), dim3(256), 0, st>>>(pool, d_diag + sp.d_off, F->foff, F->Fs, F->ncol);
+        if (sp.f_cnt) {
+            k_inv_part<T><<<dim3((unsigned)sp.p_cnt), dim3(256), 0, st>>>(pool, d_parts + sp.p_off, F->foff, F->Fs,
+                                                                         F->ncol, scratch);
+            k_inv_fin<T><<<dim3((unsigned)sp.f_cnt), dim3(256), 0, st>>>(pool, d_fins + sp.f_off, F->foff, F->Fs,
+                                                                        F->ncol, scratch);
         }
+    }
     SEHIP_CHECK(hipStreamSynchronize(st));
     return 0;
 }
@@ -233,7 +237,9 @@ int factor_invert_l11(Factor *F)
 {
     if (!F->factored) return SPEIGS_ERR_FACTOR;
     if (F->inverted) return 0;
+    const double t0 = wall();
     int rc = F->fp32 ? invert_impl<float>(F) : invert_impl<double>(F);
+    F->t_invert += wall() - t0;
     if (rc == 0) F->inverted = true;
     return rc;
 }
@@ -727,7 +733,12 @@ int factor_block_solve(Factor *F, const float *Bin, float *Xout, int nb)
         if (rc) return rc;
         if ((rc = bsf_prepare(F, B))) return rc;
     }
-    return F->fp32 ? bsf_run<float>(F, B, Bin, Xout, nb) : bsf_run<double>(F, B, Bin, Xout, nb);
+    const int e = ev_begin(F);
+    const int rc = F->fp32 ? bsf_run<float>(F, B, Bin, Xout, nb) : bsf_run<double>(F, B, Bin, Xout, nb);
+    ev_end(F, e);
+    F->block_solves++;
+    F->block_rhs += nb;
+    return rc;
 }
 
 int factor_block_solve_d(Factor *F, const double *Bin, double *Xout, int nb)

@@ -30,6 +30,10 @@ namespace sehip {
 
 int hip_error_code(hipError_t e);
 double wall();
+// SPEIGS_HIP_TRACE=1: phase times of the set-up on stderr; trace_phase prints
+// the time since t and resets t to now
+bool trace_on();
+void trace_phase(const char *what, double &t);
 
 constexpr int kBlock = 256;          // 4 wave64s
 constexpr int kRowsPerThread = 4;    // reduction kernels: rows per thread

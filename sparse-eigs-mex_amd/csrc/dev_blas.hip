@@ -8,8 +8,11 @@
 // column, then a fixed-order tree sums the partials.  Blocks are 256 threads
 // (4 wave64s); wave reductions use cross-lane shuffles, block reductions LDS.
 #include "dev.h"
+#include "host_par.h"
 
 #include <algorithm>
+#include <atomic>
+#include <memory>
 #include <cstring>
 #include <ctime>
 
@@ -26,6 +29,22 @@ double wall()
     timespec t;
     clock_gettime(CLOCK_MONOTONIC, &t);
     return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+
+bool trace_on()
+{
+    static const bool on = [] {
+        const char *e = getenv("SPEIGS_HIP_TRACE");
+        return e && atoi(e) > 0;
+    }();
+    return on;
+}
+
+void trace_phase(const char *what, double &t)
+{
+    const double now = wall();
+    if (trace_on()) fprintf(stderr, "[speigs_hip] %-22s %8.2f ms\n", what, 1e3 * (now - t));
+    t = now;
 }
 
 // ------------------------------------------------------------------ helpers
@@ -107,50 +126,75 @@ __global__ void __launch_bounds__(256) k_sell_spmm(int64_t nslices, const int64_
 int sell_build(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax, SellA *S,
                DevMem &mem, hipStream_t st)
 {
-    // row lengths of the full symmetric matrix from the stored upper triangle
-    std::vector<int64_t> rlen(n, 0);
-    for (int64_t j = 0; j < n; j++)
+    // Host construction in parallel over the stored columns.  Each row of the
+    // full symmetric matrix lists its entries in increasing column order (ties
+    // by value), whatever the order of the input, so the SpMV's summation
+    // order is a function of the matrix alone.
+    const int nt = host_threads();
+    std::unique_ptr<std::atomic<int32_t>[]> rl(new std::atomic<int32_t>[n]);
+    parallel_for(n, nt, 65536, [&](int64_t i, int) { rl[i].store(0, std::memory_order_relaxed); });
+    parallel_for(n, nt, 4096, [&](int64_t j, int) {
         for (int64_t q = Ap[j]; q < Ap[j + 1]; q++) {
-            int64_t i = Ai[q];
+            const int64_t i = Ai[q];
             if (i > j) continue;
-            rlen[i]++;
-            if (i != j) rlen[j]++;
+            rl[i].fetch_add(1, std::memory_order_relaxed);
+            if (i != j) rl[j].fetch_add(1, std::memory_order_relaxed);
         }
-    int64_t nsl = (n + 63) / 64;
+    });
+    const int64_t nsl = (n + 63) / 64;
     std::vector<int64_t> sptr(nsl + 1, 0);
     std::vector<int32_t> swid(nsl, 0);
-    for (int64_t s = 0; s < nsl; s++) {
-        int64_t w = 0;
-        for (int64_t r = s * 64; r < std::min(n, s * 64 + 64); r++) w = std::max(w, rlen[r]);
-        swid[s] = (int32_t)w;
-        sptr[s + 1] = sptr[s] + 64 * w;
-    }
-    int64_t tot = sptr[nsl];
+    parallel_for(nsl, nt, 1024, [&](int64_t s, int) {
+        int32_t w = 0;
+        for (int64_t r = s * 64; r < std::min(n, s * 64 + 64); r++) w = std::max(w, rl[r].load(std::memory_order_relaxed));
+        swid[s] = w;
+    });
+    for (int64_t s = 0; s < nsl; s++) sptr[s + 1] = sptr[s] + 64 * (int64_t)swid[s];
+    const int64_t tot = sptr[nsl];
     std::vector<int32_t> col(tot);
-    std::vector<double> val(tot, 0.0);
-    std::vector<int64_t> fill(n, 0);
-    for (int64_t r = 0; r < n; r++) {            // padding points at the row itself (value 0)
-        int64_t s = r >> 6, lane = r & 63;
-        for (int64_t k = 0; k < swid[s]; k++) col[sptr[s] + k * 64 + lane] = (int32_t)r;
-    }
-    int64_t nnz_full = 0;
+    std::vector<double> val(tot);
+    // padding points at the row itself with value 0
+    parallel_for(nsl, nt, 256, [&](int64_t s, int) {
+        for (int64_t k = 0; k < swid[s]; k++)
+            for (int64_t lane = 0; lane < 64; lane++) {
+                col[sptr[s] + k * 64 + lane] = (int32_t)std::min(n - 1, s * 64 + lane);
+                val[sptr[s] + k * 64 + lane] = 0.0;
+            }
+    });
+    std::unique_ptr<std::atomic<int32_t>[]> cur(new std::atomic<int32_t>[n]);
+    parallel_for(n, nt, 65536, [&](int64_t i, int) { cur[i].store(0, std::memory_order_relaxed); });
     auto put = [&](int64_t r, int64_t c, double v) {
-        int64_t s = r >> 6, lane = r & 63;
-        int64_t pos = sptr[s] + fill[r] * 64 + lane;
+        const int64_t k = cur[r].fetch_add(1, std::memory_order_relaxed);
+        const int64_t pos = sptr[r >> 6] + k * 64 + (r & 63);
         col[pos] = (int32_t)c;
         val[pos] = v;
-        fill[r]++;
-        nnz_full++;
     };
-    // column j of the upper triangle gives row j's entries left of/at the diagonal
-    // (by symmetry) and the entries of rows i<j at column j
-    for (int64_t j = 0; j < n; j++)
+    parallel_for(n, nt, 4096, [&](int64_t j, int) {
         for (int64_t q = Ap[j]; q < Ap[j + 1]; q++) {
-            int64_t i = Ai[q];
+            const int64_t i = Ai[q];
             if (i > j) continue;
             put(i, j, Ax[q]);
             if (i != j) put(j, i, Ax[q]);
         }
+    });
+    std::atomic<int64_t> nnz_sum{0};
+    parallel_for(n, nt, 4096, [&](int64_t r, int) {
+        const int64_t base = sptr[r >> 6] + (r & 63), len = rl[r].load(std::memory_order_relaxed);
+        for (int64_t a = 1; a < len; a++) {       // insertion sort of the row by (column, value)
+            const int32_t c = col[base + a * 64];
+            const double v = val[base + a * 64];
+            int64_t b = a - 1;
+            while (b >= 0 && (col[base + b * 64] > c || (col[base + b * 64] == c && val[base + b * 64] > v))) {
+                col[base + (b + 1) * 64] = col[base + b * 64];
+                val[base + (b + 1) * 64] = val[base + b * 64];
+                b--;
+            }
+            col[base + (b + 1) * 64] = c;
+            val[base + (b + 1) * 64] = v;
+        }
+        nnz_sum.fetch_add(len, std::memory_order_relaxed);
+    });
+    const int64_t nnz_full = nnz_sum.load();
     S->n = n;
     S->nslices = nsl;
     S->stored = tot;

@@ -15,12 +15,15 @@
 // and the Schur update of the trailing tiles -- each a batched launch over a
 // task list that spans every front of the level.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
 
 #include "factor_impl.h"
+#include "mfma.h"
+#include "host_par.h"
 
 namespace sehip {
 
@@ -241,33 +244,6 @@ __global__ void __launch_bounds__(256) k_update(T *pool, const TileTask *__restr
             if (ri + a < nr) col[ri + a] -= acc[a][b];
     }
 }
-
-typedef double dbl4 __attribute__((ext_vector_type(4)));
-typedef float flt4 __attribute__((ext_vector_type(4)));
-
-// Operand layout (both types, wave64, K = 4): lane l supplies A[l%16][l/16] and
-// B[l/16][l%16].  Result layout (measured on gfx950 with an asymmetric lane probe):
-//   f64 16x16x4: lane l, value v holds D[l/16 + 4v][l%16]
-//   f32 16x16x4: lane l, value v holds D[4(l/16) + v][l%16]
-// blk(xi, xj) computes D = L_j L_i^T = C^T, so value v of lane l is
-// C[row = l%16][col = colof(l/16, v)]: 16 lanes cover 16 consecutive rows.
-template <class T> struct MfmaAcc;
-template <> struct MfmaAcc<double> {
-    typedef dbl4 V;
-    static __device__ __forceinline__ V blk(double xi, double xj, V c)
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(xj, xi, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int colof(int lk, int v) { return lk + 4 * v; }
-};
-template <> struct MfmaAcc<float> {
-    typedef flt4 V;
-    static __device__ __forceinline__ V blk(float xi, float xj, V c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(xj, xi, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int colof(int lk, int v) { return 4 * lk + v; }
-};
 
 // Schur update of tile (i, j) on the matrix cores: each wave owns a 32 x 32
 // quarter of the 64 x 64 tile as 2 x 2 blocks of 16 x 16, K = 4 per MFMA.
@@ -772,8 +748,6 @@ __global__ void k_norm2_partial(const double *x, int64_t n, double *partial)
 
 
 
-namespace {
-
 constexpr int kEvPairs = 32;
 
 void ev_harvest(Factor *F, bool wait)
@@ -808,6 +782,8 @@ void ev_end(Factor *F, int i)
     (void)hipEventRecord(F->ev[2 * i + 1], F->st);
     F->ev_pending.push_back(i);
 }
+
+namespace {
 
 template <class T>
 int run_numeric(Factor *F, double delta)
@@ -935,7 +911,7 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     int rc = symbolic_analyze(n, Ap, Ai, so, &F->S);
     if (rc) return rc;
     const Symbolic &S = F->S;
-    double tu = wall();
+    double tu = wall(), tp = tu;
     // per-supernode arrays
     const int64_t ns = S.ns;
     std::vector<int32_t> Fs(ns), nc(ns), rows32(std::max<size_t>(S.rows.size(), 1)), chl32(std::max<size_t>(S.chl.size(), 1));
@@ -952,7 +928,17 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     std::vector<int64_t> snode(n);
     for (int64_t s = 0; s < ns; s++)
         for (int64_t j = S.c0[s]; j < S.c1[s]; j++) snode[j] = s;
-    std::vector<int64_t> dst, src;
+    // columns in parallel, each into its own range of the map (map order =
+    // column order, as a serial scan would give)
+    const int nt = host_threads();
+    std::vector<int64_t> cnt(n + 1, 0);
+    parallel_for(n, nt, 4096, [&](int64_t j, int) {
+        int64_t c = 0;
+        for (int64_t q = Ap[j]; q < Ap[j + 1]; q++) c += Ai[q] <= j;
+        cnt[j + 1] = c;
+    });
+    for (int64_t j = 0; j < n; j++) cnt[j + 1] += cnt[j];
+    std::vector<int64_t> dst(cnt[n]), src(cnt[n]);
     std::vector<char> hasdiag(n, 0);
     std::vector<double> diagA(n, 0.0);
     auto local_row = [&](int64_t s, int64_t row) -> int64_t {
@@ -962,7 +948,9 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
         if (pos == R + nr || *pos != row) return -1;
         return (S.c1[s] - S.c0[s]) + (pos - R);
     };
-    for (int64_t j = 0; j < n; j++)
+    std::atomic<int> bad{0};
+    parallel_for(n, nt, 4096, [&](int64_t j, int) {
+        int64_t o = cnt[j];
         for (int64_t q = Ap[j]; q < Ap[j + 1]; q++) {
             const int64_t i = Ai[q];
             if (i > j) continue;
@@ -970,16 +958,19 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
             const int64_t col = std::min(ni, nj), row = std::max(ni, nj);
             const int64_t s = snode[col];
             const int64_t lr = local_row(s, row);
-            if (lr < 0) return SPEIGS_ERR_INTERNAL;
-            dst.push_back(S.foff[s] + lr + (col - S.c0[s]) * S.F[s]);
+            if (lr < 0) { bad.store(1); return; }
+            dst[o] = S.foff[s] + lr + (col - S.c0[s]) * S.F[s];
             if (i == j) {
-                src.push_back(~q);
+                src[o] = ~q;
                 hasdiag[col] = 1;
                 diagA[col] = Ax[q];
             } else {
-                src.push_back(q);
+                src[o] = q;
             }
+            o++;
         }
+    });
+    if (bad.load()) return SPEIGS_ERR_INTERNAL;
     for (int64_t j = 0; j < n; j++)
         if (!hasdiag[j]) {
             const int64_t s = snode[j], l = j - S.c0[s];
@@ -988,6 +979,7 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
         }
     F->nasm = (int64_t)dst.size();
     F->nnzA = Ap[n];
+    trace_phase("assembly map", tp);
     // plans: extend-add tasks, tile tasks, fronts per level
     std::vector<EaTask> ea;
     std::vector<TileTask> tiles;
@@ -1046,9 +1038,7 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
             P.k_upd_cnt.push_back((int64_t)tiles.size() - P.k_upd_off.back());
         }
     }
-    for (auto &P : F->plan) {
-        (void)P;
-    }
+    trace_phase("factor task lists", tp);
     // blocked solve plans: forward row blocks in increasing order per front,
     // backward panels in decreasing order per front (ticket = dependency order)
     std::vector<EaTask> sea;
@@ -1107,6 +1097,7 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
         partmax = std::max(partmax, off);
         P.ub_cnt = (int64_t)updv.size() - P.ub_off;
     }
+    trace_phase("solve task lists", tp);
     if (updv.empty()) updv.push_back({0, 0, 0, 0});
     if ((rc = upload(F.get(), &F->upd, updv))) return rc;
     if ((rc = upload(F.get(), &F->partoff, partoff))) return rc;
@@ -1138,6 +1129,7 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     F->ev.resize(2 * kEvPairs);
     for (auto &e : F->ev)
         if (hipEventCreate(&e) != hipSuccess) return SPEIGS_ERR_INTERNAL;
+    trace_phase("small uploads, events", tp);
     // uploads
     const size_t esz = F->fp32 ? sizeof(float) : sizeof(double);
     if (F->fp32) {
@@ -1146,6 +1138,7 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
         if ((rc = F->mem.alloc((double **)&F->pool, (size_t)S.pool))) return rc;
     }
     (void)esz;
+    trace_phase("pool allocation", tp);
     std::vector<double> axv(Ax, Ax + std::max<int64_t>(Ap[n], 1));
     if (Ap[n] == 0) axv.assign(1, 0.0);
     if ((rc = upload(F.get(), &F->Ax, axv))) return rc;
@@ -1178,6 +1171,7 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     if ((rc = F->mem.alloc(&F->partial, 256))) return rc;
     if (hipHostMalloc((void **)&F->hpart, 256 * sizeof(double)) != hipSuccess) return SPEIGS_ERR_MEM;
     SEHIP_CHECK(hipStreamSynchronize(st));
+    trace_phase("uploads", tp);
     if (hinfo) {
         hinfo->t_order = S.t_order;
         hinfo->t_symbolic = S.t_symbolic;
@@ -1253,6 +1247,13 @@ void factor_stats(const Factor *Fc, speigs_hip_info *h)
     h->precision_used = F->opt.precision == SPEIGS_HIP_FP32_BLOCK ? SPEIGS_HIP_FP32_BLOCK : F->fp32 ? SPEIGS_HIP_FP32_IR : SPEIGS_HIP_FP64;
     const double es = F->fp32 ? 4.0 : 8.0;
     h->bytes_solve = 2.0 * (double)F->S.nnzL * es + 2.0 * 2.0 * (double)F->n * 8.0;
+    h->t_invert = F->t_invert;
+    if (F->block_solves > 0) {
+        // multi-right-hand-side solves: factor read once per sweep, FP32 vectors in and out
+        const double nb = (double)F->block_rhs / (double)F->block_solves;
+        h->solves = F->block_solves;
+        h->bytes_solve = 2.0 * (double)F->S.nnzL * es + 2.0 * nb * (double)F->n * 4.0;
+    }
 }
 
 void factor_destroy(Factor *F)

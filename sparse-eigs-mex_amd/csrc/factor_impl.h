@@ -90,7 +90,8 @@ struct Factor {
     std::vector<hipEvent_t> ev;
     std::vector<int> ev_pending;      // indices of recorded pairs not yet read
     int ev_next = 0;
-    double t_solve = 0.0;
+    double t_solve = 0.0, t_invert = 0.0;
+    int64_t block_solves = 0, block_rhs = 0;   // multi-right-hand-side solves (FP32 vectors)
     bool inverted = false;            // L11 of every front holds its inverse
     void *bs_f = nullptr, *bs_d = nullptr;   // block-solve state (FP32 / FP64 vectors)
     ~Factor()
@@ -100,6 +101,10 @@ struct Factor {
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     }
 };
+
+// HIP event pair around one solve on F->st, accumulated into t_solve
+int ev_begin(Factor *F);
+void ev_end(Factor *F, int i);
 
 // in-place inverse of every front's L11 (chain-free solves); 0 ok
 int factor_invert_l11(Factor *F);

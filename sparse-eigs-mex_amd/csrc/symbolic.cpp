@@ -19,10 +19,14 @@
 // nonzero rows of the region's last column in L, and every child's rows are a
 // subset of its parent's front -- the extend-add of the multifrontal method.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <memory>
 #include <numeric>
+#include <thread>
 
 #include "factor.h"
+#include "host_par.h"
 
 namespace sehip {
 
@@ -57,10 +61,24 @@ void build_graph(int64_t n, const int64_t *Ap, const int64_t *Ai, Graph &g)
         }
 }
 
+// Region ids live in one array shared by the threads that dissect disjoint
+// regions; a thread reads the id of a neighbour that may belong to another
+// thread's region (only to see that it is not its own), so the accesses are
+// relaxed atomics.
+inline int32_t rid_ld(const int32_t *a, int64_t i) { return __atomic_load_n(a + i, __ATOMIC_RELAXED); }
+inline void rid_st(int32_t *a, int64_t i, int32_t v) { __atomic_store_n(a + i, v, __ATOMIC_RELAXED); }
+
+// dissection tree before numbering: a separator (or leaf region) and its children
+struct NdNode {
+    std::vector<int32_t> cols;
+    std::vector<std::unique_ptr<NdNode>> kids;
+};
+typedef std::vector<std::unique_ptr<NdNode>> NdList;
+
 class Dissection {
 public:
-    Dissection(const Graph &g, int64_t leaf, std::vector<int64_t> &perm)
-        : g_(g), leaf_(leaf), perm_(perm), rid_(g.n, 1), lev_(g.n, 0), q_(g.n) {}
+    Dissection(const Graph &g, int64_t leaf, std::vector<int64_t> &perm, int threads)
+        : g_(g), leaf_(leaf), perm_(perm), rid_(g.n, 1), lev_(g.n, 0), free_threads_(threads - 1) {}
 
     std::vector<int64_t> c0, c1;
     std::vector<std::vector<int64_t>> kids;
@@ -69,127 +87,167 @@ public:
     {
         std::vector<int32_t> all(g_.n);
         std::iota(all.begin(), all.end(), 0);
-        std::vector<int64_t> roots;
+        NdList roots;
         split_components(all, 1, roots);
+        // number the tree in postorder (children in order, then the node): the
+        // columns of a node follow all of its descendants'
+        for (auto &r : roots) number(*r);
     }
 
 private:
+    // region sizes below which a split is not handed to another thread
+    static constexpr int64_t kSpawnMin = 16384;
+    struct Scratch {
+        std::vector<int32_t> q, qs, ls;
+    };
+
     const Graph &g_;
     int64_t leaf_;
     std::vector<int64_t> &perm_;
-    std::vector<int32_t> rid_, lev_, q_;
+    std::vector<int32_t> rid_, lev_;
+    std::atomic<int32_t> next_rid_{2};
+    std::atomic<int> free_threads_;
     int64_t next_pos_ = 0;
-    int32_t next_rid_ = 2;
 
     int64_t deg(int32_t v) const { return g_.xadj[v + 1] - g_.xadj[v]; }
 
-    // BFS inside region r from s; fills q_[0..cnt), lev_, returns cnt and height
-    int64_t bfs(int32_t r, int32_t s, int32_t &h)
+    static Scratch &scratch()
     {
+        thread_local Scratch sc;
+        return sc;
+    }
+
+    // BFS inside region r from s; fills q[0..cnt), lev_, returns cnt and height
+    int64_t bfs(int32_t r, int32_t s, int32_t &h, std::vector<int32_t> &q)
+    {
+        int32_t *rid = rid_.data();
         int64_t head = 0, tail = 0;
         const int32_t vis = -r;       // temporary mark
-        q_[tail++] = s;
+        q[tail++] = s;
         lev_[s] = 0;
-        rid_[s] = vis;
+        rid_st(rid, s, vis);
         h = 0;
         while (head < tail) {
-            const int32_t v = q_[head++];
+            const int32_t v = q[head++];
             const int32_t lv = lev_[v] + 1;
             for (int64_t p = g_.xadj[v]; p < g_.xadj[v + 1]; p++) {
                 const int32_t u = g_.adj[p];
-                if (rid_[u] == r) {
-                    rid_[u] = vis;
+                if (rid_ld(rid, u) == r) {
+                    rid_st(rid, u, vis);
                     lev_[u] = lv;
                     if (lv > h) h = lv;
-                    q_[tail++] = u;
+                    q[tail++] = u;
                 }
             }
         }
-        for (int64_t i = 0; i < tail; i++) rid_[q_[i]] = r;
+        for (int64_t i = 0; i < tail; i++) rid_st(rid, q[i], r);
         return tail;
     }
 
-    int64_t new_node(const int32_t *cols, int64_t nc, std::vector<int64_t> &&children)
+    void number(NdNode &nd)
     {
-        const int64_t id = (int64_t)c0.size();
+        std::vector<int64_t> ch;
+        for (auto &k : nd.kids) {
+            number(*k);
+            ch.push_back((int64_t)c0.size() - 1);
+        }
         c0.push_back(next_pos_);
-        for (int64_t i = 0; i < nc; i++) {
-            perm_[next_pos_++] = cols[i];
-            rid_[cols[i]] = 0;
+        for (int32_t v : nd.cols) {
+            perm_[next_pos_++] = v;
+            rid_[v] = 0;
         }
         c1.push_back(next_pos_);
-        kids.push_back(std::move(children));
-        return id;
+        kids.push_back(std::move(ch));
+        NdList().swap(nd.kids);
+        std::vector<int32_t>().swap(nd.cols);
+    }
+
+    static std::unique_ptr<NdNode> leaf_node(const int32_t *cols, int64_t nc)
+    {
+        std::unique_ptr<NdNode> nd(new NdNode());
+        nd->cols.assign(cols, cols + nc);
+        return nd;
     }
 
     // region r (nodes) may be disconnected: dissect each component
-    void split_components(std::vector<int32_t> &nodes, int32_t r, std::vector<int64_t> &roots)
+    void split_components(std::vector<int32_t> &nodes, int32_t r, NdList &roots)
     {
         if ((int64_t)nodes.size() <= leaf_) {
-            roots.push_back(new_node(nodes.data(), (int64_t)nodes.size(), {}));
+            roots.push_back(leaf_node(nodes.data(), (int64_t)nodes.size()));
             return;
         }
+        std::vector<int32_t> &q = scratch().q;
+        if (q.size() < nodes.size()) q.resize(nodes.size());
         int32_t h = 0;
-        int64_t cnt = bfs(r, nodes[0], h);
+        int64_t cnt = bfs(r, nodes[0], h, q);
         if (cnt == (int64_t)nodes.size()) {
-            dissect(nodes, r, roots);
+            dissect(nodes, r, roots, h);    // q and lev_ hold the BFS from nodes[0]
             return;
         }
         // peel components off one at a time
         std::vector<std::vector<int32_t>> comps;
         std::vector<int32_t> rids;
         size_t start = 0;
+        int32_t *rid = rid_.data();
         for (;;) {
-            while (start < nodes.size() && rid_[nodes[start]] != r) start++;
+            while (start < nodes.size() && rid_ld(rid, nodes[start]) != r) start++;
             if (start == nodes.size()) break;
-            cnt = bfs(r, nodes[start], h);
-            const int32_t nr = next_rid_++;
-            comps.emplace_back(q_.begin(), q_.begin() + cnt);
-            for (int32_t v : comps.back()) rid_[v] = nr;
+            cnt = bfs(r, nodes[start], h, q);
+            const int32_t nr = next_rid_.fetch_add(1);
+            comps.emplace_back(q.begin(), q.begin() + cnt);
+            for (int32_t v : comps.back()) rid_st(rid, v, nr);
             rids.push_back(nr);
         }
-        nodes.clear();
-        nodes.shrink_to_fit();
+        std::vector<int32_t>().swap(nodes);
         for (size_t c = 0; c < comps.size(); c++) {
             if ((int64_t)comps[c].size() <= leaf_)
-                roots.push_back(new_node(comps[c].data(), (int64_t)comps[c].size(), {}));
+                roots.push_back(leaf_node(comps[c].data(), (int64_t)comps[c].size()));
             else
-                dissect(comps[c], rids[c], roots);
+                dissect(comps[c], rids[c], roots, -1);
             std::vector<int32_t>().swap(comps[c]);
         }
     }
 
-    // connected region r with nodes.size() > leaf
-    void dissect(std::vector<int32_t> &nodes, int32_t r, std::vector<int64_t> &roots)
+    // connected region r with nodes.size() > leaf; h0 >= 0: the thread's queue
+    // and lev_ already hold the BFS from nodes[0], of height h0
+    void dissect(std::vector<int32_t> &nodes, int32_t r, NdList &roots, int32_t h0)
     {
+        Scratch &sc = scratch();
         const int64_t cnt = (int64_t)nodes.size();
-        int32_t h = 0;
+        if ((int64_t)sc.q.size() < cnt) sc.q.resize(cnt);
+        std::vector<int32_t> &q = sc.q;
+        int32_t h = h0;
         int32_t s = nodes[0];
-        bfs(r, s, h);
+        if (h0 < 0) bfs(r, s, h, q);
         for (int it = 0; it < 3; it++) {         // pseudo-peripheral root
             int32_t best = -1;
             int64_t bd = 0;
-            for (int64_t i = cnt - 1; i >= 0 && lev_[q_[i]] == h; i--) {
-                const int32_t v = q_[i];
+            for (int64_t i = cnt - 1; i >= 0 && lev_[q[i]] == h; i--) {
+                const int32_t v = q[i];
                 if (best < 0 || deg(v) < bd) { best = v; bd = deg(v); }
             }
+            // keep the current level structure in case the candidate is no better
+            sc.qs.assign(q.begin(), q.begin() + cnt);
+            sc.ls.resize(cnt);
+            for (int64_t i = 0; i < cnt; i++) sc.ls[i] = lev_[q[i]];
             int32_t h2 = 0;
-            bfs(r, best, h2);
+            bfs(r, best, h2, q);
             if (h2 > h) {
                 s = best;
                 h = h2;
             } else {
-                bfs(r, s, h);
+                std::copy(sc.qs.begin(), sc.qs.end(), q.begin());
+                for (int64_t i = 0; i < cnt; i++) lev_[q[i]] = sc.ls[i];
                 break;
             }
         }
         if (h < 2) {
-            roots.push_back(new_node(q_.data(), cnt, {}));
+            roots.push_back(leaf_node(q.data(), cnt));
             return;
         }
         // choose the separator level: smallest level with both sides >= 30%
         std::vector<int64_t> lsz(h + 1, 0);
-        for (int64_t i = 0; i < cnt; i++) lsz[lev_[q_[i]]]++;
+        for (int64_t i = 0; i < cnt; i++) lsz[lev_[q[i]]]++;
         int32_t sep = -1, med = -1;
         int64_t cum = 0, best = 0;
         for (int32_t l = 0; l <= h; l++) {
@@ -202,14 +260,15 @@ private:
         if (sep < 0) sep = med;
         sep = std::min(std::max(sep, 1), h - 1);
         // thin the separator: no neighbour after it -> before it; then the reverse
+        const int32_t *rid = rid_.data();
         std::vector<int32_t> S;
         for (int64_t i = 0; i < cnt; i++)
-            if (lev_[q_[i]] == sep) S.push_back(q_[i]);
+            if (lev_[q[i]] == sep) S.push_back(q[i]);
         for (int32_t v : S) {
             bool touches_b = false;
             for (int64_t p = g_.xadj[v]; p < g_.xadj[v + 1] && !touches_b; p++) {
                 const int32_t u = g_.adj[p];
-                touches_b = rid_[u] == r && lev_[u] > sep;
+                touches_b = rid_ld(rid, u) == r && lev_[u] > sep;
             }
             if (!touches_b) lev_[v] = sep - 1;
         }
@@ -218,30 +277,46 @@ private:
             bool touches_a = false;
             for (int64_t p = g_.xadj[v]; p < g_.xadj[v + 1] && !touches_a; p++) {
                 const int32_t u = g_.adj[p];
-                touches_a = rid_[u] == r && lev_[u] < sep;
+                touches_a = rid_ld(rid, u) == r && lev_[u] < sep;
             }
             if (!touches_a) lev_[v] = sep + 1;
         }
-        std::vector<int32_t> A, B, Sep;
+        std::vector<int32_t> A, B;
+        std::unique_ptr<NdNode> nd(new NdNode());
         A.reserve(cnt / 2 + 1);
         B.reserve(cnt / 2 + 1);
-        const int32_t ra = next_rid_++, rb = next_rid_++;
+        const int32_t ra = next_rid_.fetch_add(2), rb = ra + 1;
         for (int64_t i = 0; i < cnt; i++) {
-            const int32_t v = q_[i], l = lev_[v];
+            const int32_t v = q[i], l = lev_[v];
             if (l < sep) A.push_back(v);
             else if (l > sep) B.push_back(v);
-            else Sep.push_back(v);
+            else nd->cols.push_back(v);
         }
-        for (int32_t v : A) rid_[v] = ra;
-        for (int32_t v : B) rid_[v] = rb;
-        nodes.clear();
-        nodes.shrink_to_fit();
-        std::vector<int64_t> children;
-        split_components(A, ra, children);
+        for (int32_t v : A) rid_st(rid_.data(), v, ra);
+        for (int32_t v : B) rid_st(rid_.data(), v, rb);
+        std::vector<int32_t>().swap(nodes);
+        NdList ka, kb;
+        bool spawned = false;
+        std::thread th;
+        if ((int64_t)A.size() >= kSpawnMin && (int64_t)B.size() >= kSpawnMin) {
+            int f = free_threads_.load();
+            while (f > 0 && !free_threads_.compare_exchange_weak(f, f - 1)) {}
+            if (f > 0) {
+                th = std::thread([this, &A, &ka, ra] {
+                    split_components(A, ra, ka);
+                    free_threads_.fetch_add(1);
+                });
+                spawned = true;
+            }
+        }
+        if (!spawned) split_components(A, ra, ka);
+        split_components(B, rb, kb);
+        if (spawned) th.join();
         std::vector<int32_t>().swap(A);
-        split_components(B, rb, children);
         std::vector<int32_t>().swap(B);
-        roots.push_back(new_node(Sep.data(), (int64_t)Sep.size(), std::move(children)));
+        for (auto &k : ka) nd->kids.push_back(std::move(k));
+        for (auto &k : kb) nd->kids.push_back(std::move(k));
+        roots.push_back(std::move(nd));
     }
 };
 
@@ -257,7 +332,8 @@ int symbolic_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, const Symb
     build_graph(n, Ap, Ai, g);
     S->n = n;
     S->perm.assign(n, 0);
-    Dissection nd(g, std::max<int64_t>(o.leaf, 1), S->perm);
+    const int nt = host_threads();
+    Dissection nd(g, std::max<int64_t>(o.leaf, 1), S->perm, nt);
     nd.run();
     S->iperm.assign(n, 0);
     for (int64_t i = 0; i < n; i++) S->iperm[S->perm[i]] = i;
@@ -277,44 +353,56 @@ int symbolic_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, const Symb
             S->chl[S->chptr[s] + i] = nd.kids[s][i];
             S->parent[nd.kids[s][i]] = s;
         }
-    // boundary rows, children before parents (ids are a postorder)
-    std::vector<int64_t> mark(n, -1), cand;
-    S->rptr.assign(ns + 1, 0);
-    S->rows.clear();
-    for (int64_t s = 0; s < ns; s++) {
-        cand.clear();
-        const int64_t hi = S->c1[s];
-        for (int64_t p = S->chptr[s]; p < S->chptr[s + 1]; p++) {
-            const int64_t c = S->chl[p];
-            for (int64_t q = S->rptr[c]; q < S->rptr[c + 1]; q++) {
-                const int64_t v = S->rows[q];
-                if (v >= hi && mark[v] != s) { mark[v] = s; cand.push_back(v); }
-            }
-        }
-        for (int64_t col = S->c0[s]; col < hi; col++) {
-            const int32_t old = (int32_t)S->perm[col];
-            for (int64_t p = g.xadj[old]; p < g.xadj[old + 1]; p++) {
-                const int64_t v = S->iperm[g.adj[p]];
-                if (v >= hi && mark[v] != s) { mark[v] = s; cand.push_back(v); }
-            }
-        }
-        std::sort(cand.begin(), cand.end());
-        S->rows.insert(S->rows.end(), cand.begin(), cand.end());
-        S->rptr[s + 1] = (int64_t)S->rows.size();
-    }
-    // levels, front sizes, offsets, counts
+    // levels (a node's level exceeds its children's)
     S->level.assign(ns, 0);
     S->nlevels = 0;
+    for (int64_t s = 0; s < ns; s++) {
+        int32_t lv = 0;
+        for (int64_t p = S->chptr[s]; p < S->chptr[s + 1]; p++) lv = std::max(lv, S->level[S->chl[p]] + 1);
+        S->level[s] = lv;
+        S->nlevels = std::max(S->nlevels, lv + 1);
+    }
+    std::vector<std::vector<int64_t>> bylevel(S->nlevels);
+    for (int64_t s = 0; s < ns; s++) bylevel[S->level[s]].push_back(s);
+    // boundary rows, one level at a time (children's rows are complete before
+    // their parent's level starts), nodes of a level in parallel
+    std::vector<std::vector<int64_t>> rowset(ns);
+    std::vector<std::vector<int32_t>> mark(nt);
+    for (int32_t L = 0; L < S->nlevels; L++) {
+        const std::vector<int64_t> &nodes = bylevel[L];
+        parallel_for((int64_t)nodes.size(), nt, 16, [&](int64_t i, int tid) {
+            const int64_t s = nodes[i];
+            std::vector<int32_t> &mk = mark[tid];
+            if (mk.empty()) mk.assign(n, -1);
+            std::vector<int64_t> &cand = rowset[s];
+            const int64_t hi = S->c1[s];
+            for (int64_t p = S->chptr[s]; p < S->chptr[s + 1]; p++)
+                for (int64_t v : rowset[S->chl[p]])
+                    if (v >= hi && mk[v] != (int32_t)s) { mk[v] = (int32_t)s; cand.push_back(v); }
+            for (int64_t col = S->c0[s]; col < hi; col++) {
+                const int32_t old = (int32_t)S->perm[col];
+                for (int64_t p = g.xadj[old]; p < g.xadj[old + 1]; p++) {
+                    const int64_t v = S->iperm[g.adj[p]];
+                    if (v >= hi && mk[v] != (int32_t)s) { mk[v] = (int32_t)s; cand.push_back(v); }
+                }
+            }
+            std::sort(cand.begin(), cand.end());
+        });
+    }
+    S->rptr.assign(ns + 1, 0);
+    for (int64_t s = 0; s < ns; s++) S->rptr[s + 1] = S->rptr[s] + (int64_t)rowset[s].size();
+    S->rows.resize(S->rptr[ns]);
+    parallel_for(ns, nt, 256, [&](int64_t s, int) {
+        std::copy(rowset[s].begin(), rowset[s].end(), S->rows.begin() + S->rptr[s]);
+    });
+    std::vector<std::vector<int64_t>>().swap(rowset);
+    // front sizes, offsets, counts
     S->F.assign(ns, 0);
     S->foff.assign(ns, 0);
     S->pool = 0;
     S->flops = 0.0;
     S->nnzL = 0;
     for (int64_t s = 0; s < ns; s++) {
-        int32_t lv = 0;
-        for (int64_t p = S->chptr[s]; p < S->chptr[s + 1]; p++) lv = std::max(lv, S->level[S->chl[p]] + 1);
-        S->level[s] = lv;
-        S->nlevels = std::max(S->nlevels, lv + 1);
         const int64_t c = S->c1[s] - S->c0[s], r = S->rptr[s + 1] - S->rptr[s], F = c + r;
         S->F[s] = F;
         S->foff[s] = S->pool;
