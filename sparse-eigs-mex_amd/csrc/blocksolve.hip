@@ -268,10 +268,13 @@ __device__ __forceinline__ int lb_i32(const int32_t *a, int n, int v)
     return lo;
 }
 
-// W_s rows [ck*256, +256) = [B(c0..c0+c) ; 0] + children's U (children in order)
+// W_s rows [ck*256, +256) = [B(c0..c0+c) ; 0] + children's U (children in order).
+// Each thread batches its nb loads into registers before the adds and stores
+// (the workspace regions of parent and children are distinct, so the loads of
+// one child may all be in flight at once).
 template <class V>
 __global__ void __launch_bounds__(256) k_bs_init(const RowTask *__restrict__ tasks, BsArgs a, const V *__restrict__ B,
-                                                 int64_t n, int nb, V *ws)
+                                                 int64_t n, int nb, V *__restrict__ ws)
 {
     const RowTask t = tasks[blockIdx.x];
     const int s = t.s;
@@ -280,8 +283,14 @@ __global__ void __launch_bounds__(256) k_bs_init(const RowTask *__restrict__ tas
     const int64_t c0 = a.col0[s];
     const int r0 = t.I * 256, r1 = min(F, r0 + 256);
     const int i = r0 + threadIdx.x;
-    if (i < r1)
-        for (int v = 0; v < nb; v++) w[(int64_t)v * F + i] = i < c ? B[(int64_t)v * n + c0 + i] : V(0);
+    if (i < r1) {
+        V x[NBMAX];
+#pragma unroll
+        for (int v = 0; v < NBMAX; v++) x[v] = (v < nb && i < c) ? B[(int64_t)v * n + c0 + i] : V(0);
+#pragma unroll
+        for (int v = 0; v < NBMAX; v++)
+            if (v < nb) w[(int64_t)v * F + i] = x[v];
+    }
     for (int64_t q = a.chptr[s]; q < a.chptr[s + 1]; q++) {
         __syncthreads();
         const int ch = a.chl[q];
@@ -289,8 +298,18 @@ __global__ void __launch_bounds__(256) k_bs_init(const RowTask *__restrict__ tas
         const int32_t *rl = a.rel + a.relptr[ch];
         const int lo = lb_i32(rl, rc, r0), hi = lb_i32(rl, rc, r1);
         const V *u = ws + a.wsoff[ch] * NBMAX;
-        for (int e = lo + threadIdx.x; e < hi; e += 256)
-            for (int v = 0; v < nb; v++) w[(int64_t)v * F + rl[e]] += u[(int64_t)v * Fc + cc + e];
+        for (int e = lo + threadIdx.x; e < hi; e += 256) {
+            const int dst = rl[e];
+            V x[NBMAX], y[NBMAX];
+#pragma unroll
+            for (int v = 0; v < NBMAX; v++) {
+                x[v] = v < nb ? u[(int64_t)v * Fc + cc + e] : V(0);
+                y[v] = v < nb ? w[(int64_t)v * F + dst] : V(0);
+            }
+#pragma unroll
+            for (int v = 0; v < NBMAX; v++)
+                if (v < nb) w[(int64_t)v * F + dst] = y[v] + x[v];
+        }
     }
 }
 
@@ -378,13 +397,17 @@ __global__ void __launch_bounds__(256) k_bs_tile(const T *__restrict__ pool, con
 }
 
 // ---- fast path: FP32 vectors, split-K MFMA tile products + fixed-order epilogue
-// Part task (front s, output tile I, K tiles [k0, k1)): partial[task] (64 x 32,
-// row-major) = sum_K op(A)_IK X_K, on v_mfma_f32_16x16x4f32: wave w owns rows
-// 16w..16w+15, two 16-column blocks of right-hand sides; lane l supplies
-// A[16w + l%16][k + l/16] and X[k + l/16][16h + l%16] and receives
-// out[16w + 4(l/16) + v][16h + l%16].
+// Part task (front s, output tile I, K tiles [k0, k1)): P (64 rows x 32
+// right-hand sides) = sum_K op(A)_IK X_K on v_mfma_f32_16x16x4f32, formed as
+// P^T = X^T op(A)^T: wave w owns rows 16w..16w+15 and two 16-column blocks h of
+// right-hand sides; lane l supplies X[k + l/16][16h + l%16] and
+// op(A)[16w + l%16][k + l/16] and receives P[16w + l%16][16h + 4(l/16) + v], so
+// lanes 0..15 of a result register are 16 consecutive rows of one column.
+// A task that is the only part of its output tile (direct = 1) applies the
+// epilogue itself; otherwise P goes to its partial slot (column-major 64 x 32)
+// and a finish task sums the slots in a fixed order.
 struct PartTask {
-    int32_t s, I, k0, k1;
+    int32_t s, I, k0, k1, direct, pad;
 };
 struct FinTask {
     int32_t s, I;
@@ -394,10 +417,22 @@ struct FinTask {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
+// epilogue of output rows orow0 + r of front s for right-hand side v
+template <int MODE>
+__device__ __forceinline__ void bsf_store(float val, int r, int v, int32_t F, int64_t c0, int orow0, int64_t n,
+                                          float *__restrict__ w, float *__restrict__ Yg, float *__restrict__ Xg)
+{
+    if (MODE == 0) Yg[(int64_t)v * n + c0 + orow0 + r] = val;
+    else if (MODE == 1) w[(int64_t)v * F + orow0 + r] -= val;
+    else if (MODE == 2) w[(int64_t)v * F + orow0 + r] = Yg[(int64_t)v * n + c0 + orow0 + r] - val;
+    else Xg[(int64_t)v * n + c0 + orow0 + r] = val;
+}
+
 template <class T, int MODE>
 __global__ void __launch_bounds__(256) k_bsf_part(const T *__restrict__ pool, const PartTask *__restrict__ tasks,
-                                                  BsArgs a, int64_t n, int nb, const float *__restrict__ ws,
-                                                  const float *__restrict__ Yg, float *__restrict__ part)
+                                                  BsArgs a, int64_t n, int nb, float *__restrict__ ws,
+                                                  float *__restrict__ Yg, float *__restrict__ Xg,
+                                                  float *__restrict__ part)
 {
     __shared__ float At[TB][TB + 4];
     __shared__ float Xs[TB][48];
@@ -405,7 +440,7 @@ __global__ void __launch_bounds__(256) k_bsf_part(const T *__restrict__ pool, co
     const int s = t.s, I = t.I;
     const int32_t F = a.Fs[s], c = a.ncol[s];
     const T *L = pool + a.foff[s];
-    const float *w = ws + a.wsoff[s] * NBMAX;
+    float *w = ws + a.wsoff[s] * NBMAX;
     const int64_t c0 = a.col0[s];
     int orow0, on;
     if (MODE == 1) { orow0 = c + I * TB; on = min(TB, F - orow0); }
@@ -444,16 +479,27 @@ __global__ void __launch_bounds__(256) k_bsf_part(const T *__restrict__ pool, co
         for (int k = 0; k < TB; k += 4) {
             const float av = At[16 * wv + lr][k + lk];
             const float b0 = Xs[k + lk][lr], b1 = Xs[k + lk][16 + lr];
-            acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b0, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(av, b1, acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, av, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, av, acc1, 0, 0, 0);
         }
+    }
+    const int row = 16 * wv + lr;
+    if (t.direct) {
+        if (row < on) {
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int v0 = 4 * lk + v, v1 = 16 + 4 * lk + v;
+                if (v0 < nb) bsf_store<MODE>(acc0[v], row, v0, F, c0, orow0, n, w, Yg, Xg);
+                if (v1 < nb) bsf_store<MODE>(acc1[v], row, v1, F, c0, orow0, n, w, Yg, Xg);
+            }
+        }
+        return;
     }
     float *out = part + (int64_t)blockIdx.x * (TB * 32);
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-        const int row = 16 * wv + 4 * lk + v;
-        out[row * 32 + lr] = acc0[v];
-        out[row * 32 + 16 + lr] = acc1[v];
+        out[(4 * lk + v) * TB + row] = acc0[v];
+        out[(16 + 4 * lk + v) * TB + row] = acc1[v];
     }
 }
 
@@ -474,11 +520,8 @@ __global__ void __launch_bounds__(256) k_bsf_fin(const FinTask *__restrict__ tas
         const int r = e & 63, v = e >> 6;
         if (r >= on || v >= nb) continue;
         float sum = 0.f;
-        for (int q = 0; q < t.np; q++) sum += part[(t.p0 + q) * (TB * 32) + r * 32 + v];
-        if (MODE == 0) Yg[(int64_t)v * n + c0 + orow0 + r] = sum;
-        else if (MODE == 1) w[(int64_t)v * F + orow0 + r] -= sum;
-        else if (MODE == 2) w[(int64_t)v * F + orow0 + r] = Yg[(int64_t)v * n + c0 + orow0 + r] - sum;
-        else Xg[(int64_t)v * n + c0 + orow0 + r] = sum;
+        for (int q = 0; q < t.np; q++) sum += part[(t.p0 + q) * (TB * 32) + v * TB + r];
+        bsf_store<MODE>(sum, r, v, F, c0, orow0, n, w, Yg, Xg);
     }
 }
 
@@ -535,7 +578,7 @@ constexpr int KCHUNK = 6;   // K tiles per part task
 
 // part/fin task lists of one mode for the fronts of a level
 void plan_mode(const Symbolic &S, const std::vector<int64_t> &fronts, int mode, std::vector<PartTask> &pt,
-               std::vector<FinTask> &ft)
+               std::vector<FinTask> &ft, int64_t pbase)
 {
     for (int64_t s : fronts) {
         const int64_t c = S.c1[s] - S.c0[s], F = S.F[s];
@@ -549,15 +592,17 @@ void plan_mode(const Symbolic &S, const std::vector<int64_t> &fronts, int mode, 
             else { kb = I; ke = nt11; }
             // a front without update rows still needs Z = Y (a finish with no parts)
             if (ke <= kb && mode != 2) continue;
+            const int64_t np = (ke - kb + KCHUNK - 1) / KCHUNK;
             FinTask f;
             f.s = (int32_t)s;
             f.I = (int32_t)I;
-            f.p0 = (int64_t)pt.size();
+            f.p0 = (int64_t)pt.size() - pbase;
             f.pad = 0;
             for (int64_t k0 = kb; k0 < ke; k0 += KCHUNK)
-                pt.push_back({(int32_t)s, (int32_t)I, (int32_t)k0, (int32_t)std::min<int64_t>(ke, k0 + KCHUNK)});
-            f.np = (int32_t)((int64_t)pt.size() - f.p0);
-            ft.push_back(f);
+                pt.push_back({(int32_t)s, (int32_t)I, (int32_t)k0, (int32_t)std::min<int64_t>(ke, k0 + KCHUNK),
+                              np == 1 ? 1 : 0, 0});
+            f.np = (int32_t)np;
+            if (np != 1) ft.push_back(f);
         }
     }
 }
@@ -574,13 +619,12 @@ int bsf_prepare(Factor *F, BsState<float> &B)
     for (int32_t L = 0; L < S.nlevels; L++) {
         for (int mode = 0; mode < 4; mode++) {
             const int64_t p0 = (int64_t)pt.size(), f0 = (int64_t)ft.size();
-            plan_mode(S, bylevel[L], mode, pt, ft);
+            // part indices of fin tasks are relative to the mode's launch
+            plan_mode(S, bylevel[L], mode, pt, ft, p0);
             B.pt_off[mode].push_back(p0);
             B.pt_cnt[mode].push_back((int64_t)pt.size() - p0);
             B.ft_off[mode].push_back(f0);
             B.ft_cnt[mode].push_back((int64_t)ft.size() - f0);
-            // part indices of fin tasks are relative to the mode's launch
-            for (int64_t q = f0; q < (int64_t)ft.size(); q++) ft[q].p0 -= p0;
             maxpart = std::max<int64_t>(maxpart, (int64_t)pt.size() - p0);
         }
         B.gt_off.push_back((int64_t)gt.size());
@@ -590,7 +634,7 @@ int bsf_prepare(Factor *F, BsState<float> &B)
         }
         B.gt_cnt.push_back((int64_t)gt.size() - B.gt_off.back());
     }
-    if (pt.empty()) pt.push_back({0, 0, 0, 0});
+    if (pt.empty()) pt.push_back({0, 0, 0, 0, 0, 0});
     if (ft.empty()) ft.push_back({0, 0, 0, 0, 0});
     if (gt.empty()) gt.push_back({0, 0});
     int rc;
@@ -616,19 +660,18 @@ int bsf_run(Factor *F, BsState<float> &B, const float *Bin, float *Xout, int nb)
     k_bs_perm<float><<<dim3(nbk), dim3(256), 0, st>>>(Bin, F->perm, B.Pg, F->n, nb, 1);
     auto mode_launch = [&](int mode, int32_t L) {
         const int64_t pc = B.pt_cnt[mode][L], fc = B.ft_cnt[mode][L];
-        if (!fc) return;
         const PartTask *pt = B.pt + B.pt_off[mode][L];
         const FinTask *ft = B.ft + B.ft_off[mode][L];
         const dim3 gp((unsigned)pc), gf((unsigned)fc), blk(256);
         switch (mode) {
-        case 0: if (pc) k_bsf_part<T, 0><<<gp, blk, 0, st>>>(pool, pt, a, F->n, nb, B.ws, B.Yg, B.part);
-                k_bsf_fin<0><<<gf, blk, 0, st>>>(ft, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part); break;
-        case 1: if (pc) k_bsf_part<T, 1><<<gp, blk, 0, st>>>(pool, pt, a, F->n, nb, B.ws, B.Yg, B.part);
-                k_bsf_fin<1><<<gf, blk, 0, st>>>(ft, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part); break;
-        case 2: if (pc) k_bsf_part<T, 2><<<gp, blk, 0, st>>>(pool, pt, a, F->n, nb, B.ws, B.Yg, B.part);
-                k_bsf_fin<2><<<gf, blk, 0, st>>>(ft, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part); break;
-        default: if (pc) k_bsf_part<T, 3><<<gp, blk, 0, st>>>(pool, pt, a, F->n, nb, B.ws, B.Yg, B.part);
-                 k_bsf_fin<3><<<gf, blk, 0, st>>>(ft, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part);
+        case 0: if (pc) k_bsf_part<T, 0><<<gp, blk, 0, st>>>(pool, pt, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part);
+                if (fc) k_bsf_fin<0><<<gf, blk, 0, st>>>(ft, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part); break;
+        case 1: if (pc) k_bsf_part<T, 1><<<gp, blk, 0, st>>>(pool, pt, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part);
+                if (fc) k_bsf_fin<1><<<gf, blk, 0, st>>>(ft, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part); break;
+        case 2: if (pc) k_bsf_part<T, 2><<<gp, blk, 0, st>>>(pool, pt, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part);
+                if (fc) k_bsf_fin<2><<<gf, blk, 0, st>>>(ft, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part); break;
+        default: if (pc) k_bsf_part<T, 3><<<gp, blk, 0, st>>>(pool, pt, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part);
+                 if (fc) k_bsf_fin<3><<<gf, blk, 0, st>>>(ft, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part);
         }
     };
     for (int32_t L = 0; L < S.nlevels; L++) {

@@ -89,9 +89,6 @@ void cgs_update(const double *V, int64_t ldv, int m, const double *h, double *w,
 // X[:, t] = V[:, 0:m] S[:, t], t < p (S column-major m x p, device)
 void tsgemm(const double *V, int64_t ldv, int m, const double *S, int p, double *X, int64_t ldx,
             int64_t n, hipStream_t st);
-// X = V S + beta X (X must not alias V)
-void tsgemm_beta(const double *V, int64_t ldv, int m, const double *S, int p, double *X, int64_t ldx,
-                 int64_t n, double beta, hipStream_t st);
 // partial[b*p + t] = sum X[:,t] . Y[:,t]
 void pairdots_partial(const double *X, int64_t ldx, const double *Y, int64_t ldy, int p, int64_t n,
                       double *partial, hipStream_t st);
@@ -109,15 +106,32 @@ void axpby_residual(const double *b, const double *Ax, const double *x, double d
 void vadd(double *x, const double *d, int64_t n, hipStream_t st);
 
 // ------------------------------------------------------------ block kernels
-// G (p x q, column-major, ld p) = A^T B for p, q <= 96; partial must hold
-// gram_blocks(n) * p * q doubles
-int64_t gram_blocks(int64_t n);
-void gram(const double *A, int64_t lda, int p, const double *B, int64_t ldb, int q, int64_t n, double *partial,
-          double *G, hipStream_t st);
-void block_resid(const double *AX, const double *X, int64_t ld, const double *lam, int m, int64_t n, double *R,
-                 hipStream_t st);
 void block_to_f32(const double *X, int64_t ld, float *Y, int m, int64_t n, hipStream_t st);
 void block_to_f64(const float *Y, double *X, int64_t ld, int m, int64_t n, hipStream_t st);
+
+// ------------------------------------------------------------ row-major multivectors
+// n x w blocks with element (r, c) at X[r * ld + c] (blockops.hip)
+int64_t rgram_blocks(int64_t n);
+// G (p x q, column-major, ld p) = A^T B for p, q <= 96; partial must hold
+// rgram_blocks(n) * p * q doubles
+void rgram(const double *A, int64_t lda, int p, const double *B, int64_t ldb, int q, int64_t n, double *partial,
+           double *G, hipStream_t st);
+// X = V C + beta X, V n x m, C m x p column-major on the device, m, p <= 96; X must not alias V
+void rgemm(const double *V, int64_t ldv, int m, const double *C, int p, double *X, int64_t ldx, int64_t n,
+           double beta, hipStream_t st);
+// Y = A X (nv <= 96)
+void rspmm(const SellA &S, const double *X, int64_t ldx, double *Y, int64_t ldy, int nv, hipStream_t st);
+// R[:, c] = AX[:, c] - lam[c] X[:, c]
+void rresid(const double *AX, int64_t ldax, const double *X, int64_t ldx, const double *lam, int m, int64_t n,
+            double *R, int64_t ldr, hipStream_t st);
+void rcopy(const double *X, int64_t ldx, int m, int64_t n, double *Y, int64_t ldy, hipStream_t st);
+// layout changes against column-major (ld n) blocks, m <= 96
+void rto_f32cols(const double *X, int64_t ldx, int m, int64_t n, float *Y, hipStream_t st);
+void rfrom_f32cols(const float *Y, int m, int64_t n, double *X, int64_t ldx, hipStream_t st);
+void rto_cols(const double *X, int64_t ldx, int m, int64_t n, double *Y, hipStream_t st);
+void rfrom_cols(const double *Y, int m, int64_t n, double *X, int64_t ldx, hipStream_t st);
+// columns [c0, c1): uniform [-0.5, 0.5) from a counter-based generator
+void rrandom(double *X, int64_t ldx, int c0, int c1, int64_t n, uint64_t seed, hipStream_t st);
 
 // ------------------------------------------------------------ operator seam
 struct DevOperator {

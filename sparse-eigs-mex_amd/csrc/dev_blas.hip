@@ -363,42 +363,62 @@ void cgs_update(const double *V, int64_t ldv, int m, const double *h, double *w,
 }
 
 // ------------------------------------------------------------------ Ritz GEMM
+// X[:, t] = V S[:, t] (+ beta X[:, t]): one row per thread, up to 32 outputs
+// per pass (V re-read per pass), 4 rows of V in flight per step
+template <int PT>
 __global__ void __launch_bounds__(256) k_tsgemm(const double *__restrict__ V, int64_t ldv, int m,
-                                                const double *__restrict__ S, int p,
+                                                const double *__restrict__ S, int p, int t0,
                                                 double *__restrict__ X, int64_t ldx, int64_t n, double beta)
 {
     const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (row >= n) return;
-    for (int t0 = 0; t0 < p; t0 += 16) {
-        double acc[16];
+    double acc[PT];
 #pragma unroll
-        for (int t = 0; t < 16; t++) acc[t] = 0.0;
-        for (int c = 0; c < m; c++) {
-            const double v = V[row + (int64_t)c * ldv];
+    for (int t = 0; t < PT; t++) acc[t] = 0.0;
+    const double *vr = V + row;
+    int c = 0;
+    for (; c + 4 <= m; c += 4) {
+        double v[4];
 #pragma unroll
-            for (int t = 0; t < 16; t++)
-                if (t0 + t < p) acc[t] = fma(v, S[c + (int64_t)(t0 + t) * m], acc[t]);
+        for (int u = 0; u < 4; u++) v[u] = vr[(int64_t)(c + u) * ldv];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+            for (int t = 0; t < PT; t++)
+                if (t0 + t < p) acc[t] = fma(v[u], S[c + u + (int64_t)(t0 + t) * m], acc[t]);
+    }
+    for (; c < m; c++) {
+        const double v = vr[(int64_t)c * ldv];
+#pragma unroll
+        for (int t = 0; t < PT; t++)
+            if (t0 + t < p) acc[t] = fma(v, S[c + (int64_t)(t0 + t) * m], acc[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < PT; t++)
+        if (t0 + t < p) {
+            double *xp = X + row + (int64_t)(t0 + t) * ldx;
+            *xp = beta == 0.0 ? acc[t] : acc[t] + beta * *xp;
         }
-#pragma unroll
-        for (int t = 0; t < 16; t++)
-            if (t0 + t < p) {
-                double *xp = X + row + (int64_t)(t0 + t) * ldx;
-                *xp = beta == 0.0 ? acc[t] : acc[t] + beta * *xp;
-            }
+}
+
+static void tsgemm_launch(const double *V, int64_t ldv, int m, const double *S, int p, double *X, int64_t ldx,
+                          int64_t n, double beta, hipStream_t st)
+{
+    const dim3 g((unsigned)((n + 255) / 256)), b(256);
+    for (int t0 = 0; t0 < p; t0 += 32) {
+        if (p - t0 <= 16)
+            k_tsgemm<16><<<g, b, 0, st>>>(V, ldv, m, S, p, t0, X, ldx, n, beta);
+        else
+            k_tsgemm<32><<<g, b, 0, st>>>(V, ldv, m, S, p, t0, X, ldx, n, beta);
     }
 }
 
 void tsgemm(const double *V, int64_t ldv, int m, const double *S, int p, double *X, int64_t ldx, int64_t n,
             hipStream_t st)
 {
-    k_tsgemm<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st>>>(V, ldv, m, S, p, X, ldx, n, 0.0);
+    tsgemm_launch(V, ldv, m, S, p, X, ldx, n, 0.0, st);
 }
 
-void tsgemm_beta(const double *V, int64_t ldv, int m, const double *S, int p, double *X, int64_t ldx, int64_t n,
-                 double beta, hipStream_t st)
-{
-    k_tsgemm<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st>>>(V, ldv, m, S, p, X, ldx, n, beta);
-}
 
 // partial dots / residual norms over p column pairs
 __global__ void __launch_bounds__(256) k_pair_partial(const double *__restrict__ X, int64_t ldx,
@@ -493,105 +513,6 @@ void vadd(double *x, const double *d, int64_t n, hipStream_t st)
 }
 
 // ------------------------------------------------------------------ block kernels
-// Gram matrix G = A^T B (p x q, p, q <= 96): each block walks its rows in
-// 32-row chunks staged in LDS; thread (tx, ty) of a 16 x 16 grid owns the 6 x 6
-// outputs (6 tx + a, 6 ty + b).  Partials per block, then a fixed-order sum.
-constexpr int GRAM_MAX = 96;
-constexpr int GRAM_ROWS = 32;
-
-__global__ void __launch_bounds__(256) k_gram_partial(const double *__restrict__ A, int64_t lda, int p,
-                                                      const double *__restrict__ B, int64_t ldb, int q, int64_t n,
-                                                      int64_t rows_per_block, double *__restrict__ partial)
-{
-    __shared__ double As[GRAM_ROWS][GRAM_MAX + 1];
-    __shared__ double Bs[GRAM_ROWS][GRAM_MAX + 1];
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    double acc[6][6];
-#pragma unroll
-    for (int a = 0; a < 6; a++)
-#pragma unroll
-        for (int b = 0; b < 6; b++) acc[a][b] = 0.0;
-    const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
-    const int64_t r1 = min(n, r0 + rows_per_block);
-    for (int64_t rc = r0; rc < r1; rc += GRAM_ROWS) {
-        const int nr = (int)min((int64_t)GRAM_ROWS, r1 - rc);
-        __syncthreads();
-        for (int e = threadIdx.x; e < GRAM_ROWS * p; e += 256) {
-            const int r = e % GRAM_ROWS, c = e / GRAM_ROWS;
-            As[r][c] = r < nr ? A[(int64_t)c * lda + rc + r] : 0.0;
-        }
-        for (int e = threadIdx.x; e < GRAM_ROWS * q; e += 256) {
-            const int r = e % GRAM_ROWS, c = e / GRAM_ROWS;
-            Bs[r][c] = r < nr ? B[(int64_t)c * ldb + rc + r] : 0.0;
-        }
-        __syncthreads();
-        for (int r = 0; r < nr; r++) {
-            double x[6], y[6];
-#pragma unroll
-            for (int a = 0; a < 6; a++) {
-                const int i = 6 * tx + a, j = 6 * ty + a;
-                x[a] = i < p ? As[r][i] : 0.0;
-                y[a] = j < q ? Bs[r][j] : 0.0;
-            }
-#pragma unroll
-            for (int a = 0; a < 6; a++)
-#pragma unroll
-                for (int b = 0; b < 6; b++) acc[a][b] = fma(x[a], y[b], acc[a][b]);
-        }
-    }
-    double *out = partial + (int64_t)blockIdx.x * p * q;
-#pragma unroll
-    for (int a = 0; a < 6; a++)
-#pragma unroll
-        for (int b = 0; b < 6; b++) {
-            const int i = 6 * tx + a, j = 6 * ty + b;
-            if (i < p && j < q) out[(int64_t)j * p + i] = acc[a][b];
-        }
-}
-
-__global__ void k_gram_reduce(const double *__restrict__ partial, int64_t nblk, int pq, double *__restrict__ G)
-{
-    const int idx = blockIdx.x * 256 + threadIdx.x;
-    if (idx >= pq) return;
-    double s = 0.0;
-    for (int64_t b = 0; b < nblk; b++) s += partial[b * pq + idx];
-    G[idx] = s;
-}
-
-int64_t gram_blocks(int64_t n)
-{
-    const int64_t target = 512;
-    int64_t rpb = (n + target - 1) / target;
-    rpb = (rpb + GRAM_ROWS - 1) / GRAM_ROWS * GRAM_ROWS;
-    return (n + rpb - 1) / rpb;
-}
-
-void gram(const double *A, int64_t lda, int p, const double *B, int64_t ldb, int q, int64_t n, double *partial,
-          double *G, hipStream_t st)
-{
-    const int64_t target = 512;
-    int64_t rpb = (n + target - 1) / target;
-    rpb = (rpb + GRAM_ROWS - 1) / GRAM_ROWS * GRAM_ROWS;
-    const int64_t nblk = (n + rpb - 1) / rpb;
-    k_gram_partial<<<dim3((unsigned)nblk), dim3(256), 0, st>>>(A, lda, p, B, ldb, q, n, rpb, partial);
-    k_gram_reduce<<<dim3((unsigned)((p * q + 255) / 256)), dim3(256), 0, st>>>(partial, nblk, p * q, G);
-}
-
-// R[:, t] = AX[:, t] - lam[t] X[:, t]
-__global__ void k_block_resid(const double *AX, const double *X, int64_t ld, const double *lam, int m, int64_t n,
-                              double *R)
-{
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    for (int t = 0; t < m; t++) R[(int64_t)t * ld + i] = AX[(int64_t)t * ld + i] - lam[t] * X[(int64_t)t * ld + i];
-}
-
-void block_resid(const double *AX, const double *X, int64_t ld, const double *lam, int m, int64_t n, double *R,
-                 hipStream_t st)
-{
-    k_block_resid<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st>>>(AX, X, ld, lam, m, n, R);
-}
-
 // FP64 (ld) <-> FP32 (ld n) block conversion
 __global__ void k_to_f32(const double *X, int64_t ld, float *Y, int m, int64_t n)
 {

@@ -11,10 +11,11 @@
 // into BLAS-3), as LOBPCG (Knyazev 2001):
 //   S = [X, W = T R, P],  R = A X - X Theta,  Rayleigh-Ritz of A on span(S),
 //   X <- smallest m Ritz vectors, P <- their component outside X.
-// Robustness: W and P are projected out of X twice and [W P] is made
-// orthonormal by SVQB (Stathopoulos and Wu 2002: scale, eigendecompose the Gram
-// matrix, drop directions below 1e-12), twice, so S stays orthonormal to
-// rounding and the projected problem is a standard symmetric one.
+// Robustness: [W P] is projected out of X and made orthonormal by SVQB
+// (Stathopoulos and Wu 2002: scale, eigendecompose the Gram matrix, drop
+// directions below 1e-12), and the pair of steps is repeated, so S stays
+// orthonormal to rounding and the projected problem is a standard symmetric one.
+// Blocks are row-major in HBM (see LWork).
 // Convergence is the reference's test: ||A x - lambda x||_2 <= tol*max(||A||_1,1)
 // for the k wanted pairs (src/speigs_core.c:243-265).
 #include <algorithm>
@@ -28,16 +29,20 @@ namespace sehip {
 
 namespace {
 
+// Blocks live in HBM row-major (element (r, c) at X[r * ld + c]): S = [X | W | P]
+// and AS = A S share the row stride ldS, so [X Z] (Z = [W P]) is one block and
+// its Gram and products stream contiguous rows.
 struct LWork {
-    int64_t n = 0, ld = 0;
+    int64_t n = 0;
     hipStream_t st = nullptr;
     DevMem mem;
     double *partial = nullptr, *G = nullptr, *C = nullptr, *red = nullptr;
 };
 
-int dgram(LWork &W, const double *A, int p, const double *B, int q, std::vector<double> &out)
+int dgram(LWork &W, const double *A, int64_t lda, int p, const double *B, int64_t ldb, int q,
+          std::vector<double> &out)
 {
-    gram(A, W.ld, p, B, W.ld, q, W.n, W.partial, W.G, W.st);
+    rgram(A, lda, p, B, ldb, q, W.n, W.partial, W.G, W.st);
     out.resize((size_t)p * q);
     SEHIP_CHECK(hipMemcpyAsync(out.data(), W.G, (size_t)p * q * sizeof(double), hipMemcpyDeviceToHost, W.st));
     SEHIP_CHECK(hipStreamSynchronize(W.st));
@@ -45,18 +50,20 @@ int dgram(LWork &W, const double *A, int p, const double *B, int q, std::vector<
 }
 
 // Y = V Cm + beta Y  (V n x m, Cm m x p on the host, column-major; Y must not alias V)
-int dgemm_tall(LWork &W, const double *V, int m, const std::vector<double> &Cm, int p, double *Y, double beta)
+int dgemm_tall(LWork &W, const double *V, int64_t ldv, int m, const std::vector<double> &Cm, int p, double *Y,
+               int64_t ldy, double beta)
 {
     SEHIP_CHECK(hipMemcpyAsync(W.C, Cm.data(), (size_t)m * p * sizeof(double), hipMemcpyHostToDevice, W.st));
-    tsgemm_beta(V, W.ld, m, W.C, p, Y, W.ld, W.n, beta, W.st);
+    rgemm(V, ldv, m, W.C, p, Y, ldy, W.n, beta, W.st);
     return 0;
 }
 
 // Z (columns m .. m+nz of S) <- Z - X (X^T Z); returns Z^T Z after the update
-int project_out(LWork &W, const double *S, int m, double *Z, int nz, std::vector<double> &zz)
+int project_out(LWork &W, double *S, int64_t ldS, int m, int nz, std::vector<double> &zz)
 {
+    double *Z = S + m;
     std::vector<double> g;
-    int rc = dgram(W, S, m + nz, Z, nz, g);           // rows 0..m-1: X^T Z, rows m..: Z^T Z
+    int rc = dgram(W, S, ldS, m + nz, Z, ldS, nz, g);   // rows 0..m-1: X^T Z, rows m..: Z^T Z
     if (rc) return rc;
     const int ld = m + nz;
     std::vector<double> gx((size_t)m * nz);
@@ -72,11 +79,12 @@ int project_out(LWork &W, const double *S, int m, double *Z, int nz, std::vector
             for (int q = 0; q < m; q++) acc += gx[(size_t)i * m + q] * gx[(size_t)j * m + q];
             zz[(size_t)j * nz + i] -= acc;
         }
-    return dgemm_tall(W, S, m, gx, nz, Z, 1.0);
+    return dgemm_tall(W, S, ldS, m, gx, nz, Z, ldS, 1.0);
 }
 
 // SVQB from the Gram matrix zz of Z: out = Z D U S^-1/2 (kept directions); returns r
-int svqb(LWork &W, const double *Z, int nz, const std::vector<double> &zz, double *out, int *r_out)
+int svqb(LWork &W, const double *Z, int64_t ldz, int nz, const std::vector<double> &zz, double *out, int64_t ldo,
+         int *r_out)
 {
     std::vector<double> d(nz);
     for (int i = 0; i < nz; i++) {
@@ -97,11 +105,20 @@ int svqb(LWork &W, const double *Z, int nz, const std::vector<double> &zz, doubl
         for (int i = 0; i < nz; i++) T[(size_t)c * nz + i] = d[i] * gs[(size_t)j * nz + i] * sc;
     }
     if (r > 0) {
-        int rc = dgemm_tall(W, Z, nz, T, r, out, 0.0);
+        int rc = dgemm_tall(W, Z, ldz, nz, T, r, out, ldo, 0.0);
         if (rc) return rc;
     }
     *r_out = r;
     return 0;
+}
+
+void symmetrize(std::vector<double> &H, int d)
+{
+    for (int j = 0; j < d; j++)
+        for (int i = 0; i < j; i++) {
+            const double v = 0.5 * (H[(size_t)j * d + i] + H[(size_t)i * d + j]);
+            H[(size_t)j * d + i] = H[(size_t)i * d + j] = v;
+        }
 }
 
 }  // namespace
@@ -121,110 +138,100 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
 
     LWork W;
     W.n = n;
-    W.ld = pad_ld(n);
     W.st = st;
-    const size_t col = (size_t)W.ld;
-    double *S, *AS, *Sn, *ASn, *Pb, *T2;
+    const int64_t ldS = (3 * m + 3) / 4 * 4, ldT = (2 * m + 3) / 4 * 4;
+    double *S, *AS, *Sn, *ASn, *Pb, *T2, *v0;
     float *Rf, *Wf;
-    if ((rc = W.mem.alloc(&S, col * 3 * m))) return rc;     // [X | W | P]
-    if ((rc = W.mem.alloc(&AS, col * 3 * m))) return rc;
-    if ((rc = W.mem.alloc(&Sn, col * 3 * m))) return rc;
-    if ((rc = W.mem.alloc(&ASn, col * 3 * m))) return rc;
-    if ((rc = W.mem.alloc(&Pb, col * m))) return rc;
-    if ((rc = W.mem.alloc(&T2, col * 2 * m))) return rc;
+    if ((rc = W.mem.alloc(&S, (size_t)n * ldS))) return rc;     // [X | W | P]
+    if ((rc = W.mem.alloc(&AS, (size_t)n * ldS))) return rc;
+    if ((rc = W.mem.alloc(&Sn, (size_t)n * ldS))) return rc;
+    if ((rc = W.mem.alloc(&ASn, (size_t)n * ldS))) return rc;
+    if ((rc = W.mem.alloc(&Pb, (size_t)n * m))) return rc;
+    if ((rc = W.mem.alloc(&T2, (size_t)n * ldT))) return rc;
+    if ((rc = W.mem.alloc(&v0, (size_t)n))) return rc;
     if ((rc = W.mem.alloc(&Rf, (size_t)n * m))) return rc;
     if ((rc = W.mem.alloc(&Wf, (size_t)n * m))) return rc;
-    if ((rc = W.mem.alloc(&W.partial, (size_t)gram_blocks(n) * 96 * 96))) return rc;
+    if ((rc = W.mem.alloc(&W.partial, (size_t)rgram_blocks(n) * 96 * 96))) return rc;
     if ((rc = W.mem.alloc(&W.G, 96 * 96))) return rc;
     if ((rc = W.mem.alloc(&W.C, 96 * 96))) return rc;
     if ((rc = W.mem.alloc(&W.red, 4 * 96))) return rc;
 
-    // start block: the reference's xorshift64* stream (column 0 = its start vector)
+    // start block: column 0 = the reference's xorshift64* start vector (or the
+    // caller's), the other columns from a counter-based generator on the device
     {
         uint64_t seed = o->seed ? o->seed : 88172645463325252ULL;
         std::vector<double> h((size_t)n);
-        for (int t = 0; t < m; t++) {
-            if (t == 0 && ho && ho->start == SPEIGS_HIP_START_VECTOR && ho->start_vec)
-                std::memcpy(h.data(), ho->start_vec, (size_t)n * sizeof(double));
-            else
-                uniform_vec(&seed, n, h.data());
-            SEHIP_CHECK(hipMemcpyAsync(T2 + t * col, h.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
-            SEHIP_CHECK(hipStreamSynchronize(st));
-        }
+        if (ho && ho->start == SPEIGS_HIP_START_VECTOR && ho->start_vec)
+            std::memcpy(h.data(), ho->start_vec, (size_t)n * sizeof(double));
+        else
+            uniform_vec(&seed, n, h.data());
+        SEHIP_CHECK(hipMemcpyAsync(v0, h.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+        rfrom_cols(v0, 1, n, T2, ldT, st);
+        rrandom(T2, ldT, 1, m, n, seed, st);
         for (int pass = 0; pass < 2; pass++) {
             std::vector<double> g;
-            if ((rc = dgram(W, T2, m, T2, m, g))) return rc;
+            if ((rc = dgram(W, T2, ldT, m, T2, ldT, m, g))) return rc;
             int r = 0;
-            if ((rc = svqb(W, T2, m, g, S, &r))) return rc;
+            if ((rc = svqb(W, T2, ldT, m, g, S, ldS, &r))) return rc;
             if (r < m) return SPEIGS_ERR_INTERNAL;
-            if (pass == 0) SEHIP_CHECK(hipMemcpyAsync(T2, S, col * m * sizeof(double), hipMemcpyDeviceToDevice, st));
+            if (pass == 0) rcopy(S, ldS, m, n, T2, ldT, st);
         }
     }
-    double *X = S, *AX = AS;
-    spmv_multi(A, X, W.ld, AX, W.ld, m, st);
+    rspmm(A, S, ldS, AS, ldS, m, st);
     std::vector<double> theta(m);
     {
         std::vector<double> g;
-        if ((rc = dgram(W, X, m, AX, m, g))) return rc;
-        for (int j = 0; j < m; j++)
-            for (int i = 0; i < j; i++) g[(size_t)j * m + i] = g[(size_t)i * m + j] = 0.5 * (g[(size_t)j * m + i] + g[(size_t)i * m + j]);
+        if ((rc = dgram(W, S, ldS, m, AS, ldS, m, g))) return rc;
+        symmetrize(g, m);
         if (dense_syev(m, g.data(), theta.data())) return SPEIGS_ERR_LAPACK;
-        if ((rc = dgemm_tall(W, X, m, g, m, Sn, 0.0))) return rc;
-        if ((rc = dgemm_tall(W, AX, m, g, m, ASn, 0.0))) return rc;
+        if ((rc = dgemm_tall(W, S, ldS, m, g, m, Sn, ldS, 0.0))) return rc;
+        if ((rc = dgemm_tall(W, AS, ldS, m, g, m, ASn, ldS, 0.0))) return rc;
         std::swap(S, Sn);
         std::swap(AS, ASn);
     }
     int np = 0;
     int64_t it = 0, nconv = 0, solves = 0;
     std::vector<double> res(m);
-    double *lam_dev = W.red, *nrm_dev = W.red + 96;
+    double *lam_dev = W.red;
     for (;; it++) {
-        X = S;
-        AX = AS;
-        double *Z = S + col * m, *AZ = AS + col * m;
+        double *Z = S + m, *AZ = AS + m;
         SEHIP_CHECK(hipMemcpyAsync(lam_dev, theta.data(), m * sizeof(double), hipMemcpyHostToDevice, st));
-        block_resid(AX, X, W.ld, lam_dev, m, n, Z, st);
-        pairdots_partial(Z, W.ld, Z, W.ld, m, n, W.partial, st);
-        reduce_partial(W.partial, nblocks_rows(n), m, nrm_dev, st);
-        SEHIP_CHECK(hipMemcpyAsync(res.data(), nrm_dev, m * sizeof(double), hipMemcpyDeviceToHost, st));
-        SEHIP_CHECK(hipStreamSynchronize(st));
-        nconv = 0;
-        for (int i = 0; i < k; i++) {
-            res[i] = std::sqrt(res[i]);
-            if (res[i] <= rtol) nconv++;
+        rresid(AS, ldS, S, ldS, lam_dev, m, n, Z, ldS, st);
+        {
+            std::vector<double> g;
+            if ((rc = dgram(W, Z, ldS, k, Z, ldS, k, g))) return rc;
+            nconv = 0;
+            for (int i = 0; i < k; i++) {
+                res[i] = std::sqrt(std::max(g[(size_t)i * k + i], 0.0));
+                if (res[i] <= rtol) nconv++;
+            }
         }
         if (nconv >= k || it >= maxit) break;
         // W = T R: one FP32 multi-right-hand-side solve
-        block_to_f32(Z, W.ld, Rf, m, n, st);
+        rto_f32cols(Z, ldS, m, n, Rf, st);
         if ((rc = factor_block_solve(F, Rf, Wf, m))) return rc;
-        block_to_f64(Wf, Z, W.ld, m, n, st);
+        rfrom_f32cols(Wf, m, n, Z, ldS, st);
         solves++;
-        if (np > 0)
-            SEHIP_CHECK(hipMemcpyAsync(Z + col * m, Pb, col * np * sizeof(double), hipMemcpyDeviceToDevice, st));
+        if (np > 0) rcopy(Pb, m, np, n, Z + m, ldS, st);
         int nz = m + np;
         std::vector<double> zz;
-        if ((rc = project_out(W, S, m, Z, nz, zz))) return rc;
-        if ((rc = project_out(W, S, m, Z, nz, zz))) return rc;
+        if ((rc = project_out(W, S, ldS, m, nz, zz))) return rc;
         int r = 0;
-        if ((rc = svqb(W, Z, nz, zz, T2, &r))) return rc;
+        if ((rc = svqb(W, Z, ldS, nz, zz, T2, ldT, &r))) return rc;
         if (r == 0) break;
-        SEHIP_CHECK(hipMemcpyAsync(Z, T2, col * r * sizeof(double), hipMemcpyDeviceToDevice, st));
+        rcopy(T2, ldT, r, n, Z, ldS, st);
         nz = r;
-        if ((rc = project_out(W, S, m, Z, nz, zz))) return rc;
-        if ((rc = svqb(W, Z, nz, zz, T2, &r))) return rc;
+        if ((rc = project_out(W, S, ldS, m, nz, zz))) return rc;
+        if ((rc = svqb(W, Z, ldS, nz, zz, T2, ldT, &r))) return rc;
         if (r == 0) break;
-        SEHIP_CHECK(hipMemcpyAsync(Z, T2, col * r * sizeof(double), hipMemcpyDeviceToDevice, st));
+        rcopy(T2, ldT, r, n, Z, ldS, st);
         nz = r;
-        spmv_multi(A, Z, W.ld, AZ, W.ld, nz, st);
+        rspmm(A, Z, ldS, AZ, ldS, nz, st);
         // Rayleigh-Ritz on S = [X Z] (orthonormal): H = S^T A S
         const int d = m + nz;
         std::vector<double> H;
-        if ((rc = dgram(W, S, d, AS, d, H))) return rc;
-        for (int j = 0; j < d; j++)
-            for (int i = 0; i < j; i++) {
-                const double v = 0.5 * (H[(size_t)j * d + i] + H[(size_t)i * d + j]);
-                H[(size_t)j * d + i] = H[(size_t)i * d + j] = v;
-            }
+        if ((rc = dgram(W, S, ldS, d, AS, ldS, d, H))) return rc;
+        symmetrize(H, d);
         std::vector<double> ev(d);
         if (dense_syev(d, H.data(), ev.data())) return SPEIGS_ERR_LAPACK;
         for (int t = 0; t < m; t++) theta[t] = ev[t];
@@ -233,42 +240,35 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
             for (int i = 0; i < d; i++) C[(size_t)t * d + i] = H[(size_t)t * d + i];
             for (int i = 0; i < nz; i++) Cz[(size_t)t * nz + i] = H[(size_t)t * d + m + i];
         }
-        if ((rc = dgemm_tall(W, S, d, C, m, Sn, 0.0))) return rc;      // new X
-        if ((rc = dgemm_tall(W, AS, d, C, m, ASn, 0.0))) return rc;    // new A X
-        if ((rc = dgemm_tall(W, Z, nz, Cz, m, Pb, 0.0))) return rc;    // new P
+        if ((rc = dgemm_tall(W, S, ldS, d, C, m, Sn, ldS, 0.0))) return rc;      // new X
+        if ((rc = dgemm_tall(W, AS, ldS, d, C, m, ASn, ldS, 0.0))) return rc;    // new A X
+        if ((rc = dgemm_tall(W, Z, ldS, nz, Cz, m, Pb, m, 0.0))) return rc;      // new P
         std::swap(S, Sn);
         std::swap(AS, ASn);
         np = m;
-        if ((it + 1) % 8 == 0) spmv_multi(A, S, W.ld, AS, W.ld, m, st);   // refresh A X against drift
+        if ((it + 1) % 8 == 0) rspmm(A, S, ldS, AS, ldS, m, st);   // refresh A X against drift
     }
-    X = S;
-    AX = AS;
     // final pairs: exact A X, Rayleigh quotients, residuals
-    spmv_multi(A, X, W.ld, AX, W.ld, m, st);
-    pairdots_partial(X, W.ld, AX, W.ld, m, n, W.partial, st);
-    reduce_partial(W.partial, nblocks_rows(n), m, lam_dev, st);
-    pairdots_partial(X, W.ld, X, W.ld, m, n, W.partial, st);
-    reduce_partial(W.partial, nblocks_rows(n), m, nrm_dev, st);
-    std::vector<double> lam(m), xx(m);
-    SEHIP_CHECK(hipMemcpyAsync(lam.data(), lam_dev, m * sizeof(double), hipMemcpyDeviceToHost, st));
-    SEHIP_CHECK(hipMemcpyAsync(xx.data(), nrm_dev, m * sizeof(double), hipMemcpyDeviceToHost, st));
-    SEHIP_CHECK(hipStreamSynchronize(st));
-    for (int t = 0; t < m; t++) lam[t] /= xx[t];
+    rspmm(A, S, ldS, AS, ldS, m, st);
+    std::vector<double> xax, xx, rr;
+    if ((rc = dgram(W, S, ldS, m, AS, ldS, m, xax))) return rc;
+    if ((rc = dgram(W, S, ldS, m, S, ldS, m, xx))) return rc;
+    std::vector<double> lam(m);
+    for (int t = 0; t < m; t++) lam[t] = xax[(size_t)t * m + t] / xx[(size_t)t * m + t];
     SEHIP_CHECK(hipMemcpyAsync(lam_dev, lam.data(), m * sizeof(double), hipMemcpyHostToDevice, st));
-    residual_partial(X, W.ld, AX, W.ld, lam_dev, m, n, W.partial, st);
-    reduce_partial(W.partial, nblocks_rows(n), m, nrm_dev, st);
-    SEHIP_CHECK(hipMemcpyAsync(res.data(), nrm_dev, m * sizeof(double), hipMemcpyDeviceToHost, st));
-    SEHIP_CHECK(hipStreamSynchronize(st));
+    rresid(AS, ldS, S, ldS, lam_dev, m, n, S + m, ldS, st);
+    if ((rc = dgram(W, S + m, ldS, k, S + m, ldS, k, rr))) return rc;
     nconv = 0;
     for (int t = 0; t < k; t++) {
-        const double r = std::sqrt(res[t]) / std::sqrt(xx[t]);
+        const double r = std::sqrt(std::max(rr[(size_t)t * k + t], 0.0)) / std::sqrt(xx[(size_t)t * m + t]);
         lambda[t] = lam[t];
         if (resid) resid[t] = r;
         if (r <= rtol) nconv++;
     }
-    if (Vout)
-        SEHIP_CHECK(hipMemcpy2DAsync(Vout, (size_t)n * sizeof(double), X, col * sizeof(double), (size_t)n * sizeof(double),
-                                     (size_t)k, hipMemcpyDeviceToHost, st));
+    if (Vout) {
+        rto_cols(S, ldS, (int)k, n, T2, st);   // T2 holds n x 2m >= n x k doubles
+        SEHIP_CHECK(hipMemcpyAsync(Vout, T2, (size_t)n * k * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
     SEHIP_CHECK(hipStreamSynchronize(st));
     info->nconv = nconv;
     info->restarts = it;
