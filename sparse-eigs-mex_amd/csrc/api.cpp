@@ -21,6 +21,7 @@
 #include "dev.h"
 #include "factor_impl.h"
 #include "host_par.h"
+#include "batch.h"
 
 using namespace sehip;
 
@@ -342,7 +343,7 @@ int try_banded(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax
 
 int blocks_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax, int64_t k, speigs_which which,
                 const speigs_opts *opts, const speigs_hip_opts *ho, const std::vector<int64_t> &comp, int64_t ncomp,
-                double *lambda, double *V, double *resid, speigs_info *info);
+                double *lambda, double *V, double *resid, speigs_info *info, speigs_hip_info *hinfo);
 
 }  // namespace
 
@@ -430,7 +431,7 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
         std::vector<int64_t> comp;
         int64_t nc = components(n, Ap, Ai, Ax, comp);
         if (nc > 1) {
-            int rc = blocks_path(n, Ap, Ai, Ax, k, which, opts, hopts, comp, nc, lambda, V, resid, info);
+            int rc = blocks_path(n, Ap, Ai, Ax, k, which, opts, hopts, comp, nc, lambda, V, resid, info, hinfo);
             info->t_iter = wall() - t0;
             return rc;
         }
@@ -644,9 +645,40 @@ int speigs_z(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const dou
 
 namespace {
 
+// Components of the pattern (reference src/speigs_core.c:532-632): every
+// component is solved for min(k, n_c) pairs and the k best candidates over all
+// components are returned.  Components that the reference's recursive call
+// would send to its diagonal or dense short-circuits are solved on the host,
+// spread over threads; all the others are solved together on the device: one
+// block-diagonal operator (one factorization of all of them for 'sm') and
+// lockstep Lanczos recurrences (batched.cpp).  SPEIGS_HIP_BATCH=0 solves the
+// device components one recursive call at a time instead.
+struct CompMatrix {
+    std::vector<int64_t> Ap, Ai;
+    std::vector<double> Ax;
+};
+
+void extract_comp(int64_t c, int64_t nc, const std::vector<int64_t> &gidx, int64_t off, const std::vector<int64_t> &comp,
+                  const std::vector<int64_t> &loc, const int64_t *Ap, const int64_t *Ai, const double *Ax, CompMatrix &M)
+{
+    M.Ap.assign(nc + 1, 0);
+    M.Ai.clear();
+    M.Ax.clear();
+    for (int64_t jj = 0; jj < nc; jj++) {
+        const int64_t g = gidx[off + jj];
+        M.Ap[jj] = (int64_t)M.Ai.size();
+        for (int64_t q = Ap[g]; q < Ap[g + 1]; q++)
+            if (comp[Ai[q]] == c) {
+                M.Ai.push_back(loc[Ai[q]]);
+                M.Ax.push_back(Ax[q]);
+            }
+    }
+    M.Ap[nc] = (int64_t)M.Ai.size();
+}
+
 int blocks_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax, int64_t k, speigs_which which,
                 const speigs_opts *opts, const speigs_hip_opts *ho, const std::vector<int64_t> &comp, int64_t ncomp,
-                double *lambda, double *V, double *resid, speigs_info *info)
+                double *lambda, double *V, double *resid, speigs_info *info, speigs_hip_info *hinfo)
 {
     std::vector<int64_t> sz(ncomp, 0), off(ncomp + 1, 0), gidx(n), loc(n), fill(ncomp, 0);
     for (int64_t i = 0; i < n; i++) sz[comp[i]]++;
@@ -656,40 +688,155 @@ int blocks_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *A
         gidx[slot] = i;
         loc[i] = slot - off[c];
     }
-    std::vector<std::vector<double>> cv(ncomp);
-    std::vector<double> clam, cres;
-    std::vector<int64_t> ccmp, cslt;
-    std::vector<int64_t> sAp(n + 1), sAi(std::max<int64_t>(Ap[n], 1));
-    std::vector<double> sAx(std::max<int64_t>(Ap[n], 1));
+    const int dmax = opts->dense_max < 0 ? 800 : opts->dense_max;
+    const char *be = getenv("SPEIGS_HIP_BATCH");
+    const bool batch = !(be && atoi(be) == 0);
+    // per component: its k, whether it is solved by the batch, its candidates
+    std::vector<int64_t> kc(ncomp);
+    std::vector<char> onhost(ncomp, 1);
     for (int64_t c = 0; c < ncomp; c++) {
-        int64_t nc = sz[c], cnt = 0;
-        for (int64_t jj = 0; jj < nc; jj++) {
-            int64_t g = gidx[off[c] + jj];
-            sAp[jj] = cnt;
-            for (int64_t q = Ap[g]; q < Ap[g + 1]; q++)
-                if (comp[Ai[q]] == c) {
-                    sAi[cnt] = loc[Ai[q]];
-                    sAx[cnt] = Ax[q];
-                    cnt++;
-                }
-        }
-        sAp[nc] = cnt;
-        int64_t kc = std::min(k, nc);
-        std::vector<double> lv(kc), rv(kc);
-        if (V) cv[c].assign((size_t)nc * kc, 0.0);
-        speigs_info si;
-        int r2 = speigs_ex(nc, sAp.data(), sAi.data(), sAx.data(), kc, which, opts, ho, lv.data(),
-                           V ? cv[c].data() : nullptr, rv.data(), &si, nullptr);
-        if (r2) return r2;
-        info->nops += si.nops;
-        if (si.flag) info->flag = 1;
-        for (int64_t t = 0; t < kc; t++) {
-            clam.push_back(lv[t]);
-            cres.push_back(rv[t]);
-            ccmp.push_back(c);
-            cslt.push_back(t);
+        kc[c] = std::min(k, sz[c]);
+        const bool shortcut = (dmax > 0 && (sz[c] <= dmax || 4 * kc[c] > sz[c])) || sz[c] == 1;
+        onhost[c] = !batch || shortcut;
+    }
+    std::vector<std::vector<double>> cv(ncomp), cl(ncomp), cr(ncomp);
+    std::vector<int> crc(ncomp, 0);
+    std::vector<speigs_info> cinfo(ncomp);
+    // host components (each a recursive call: diagonal, dense, or -- when
+    // batching is off -- the full single-component device path), in parallel
+    // when they stay on the host
+    {
+        std::vector<int64_t> hostc;
+        for (int64_t c = 0; c < ncomp; c++)
+            if (onhost[c]) hostc.push_back(c);
+        const int nt = batch ? host_threads() : 1;
+        std::vector<CompMatrix> mats(nt);
+        parallel_for((int64_t)hostc.size(), nt, 1, [&](int64_t i, int tid) {
+            const int64_t c = hostc[i], nc = sz[c];
+            CompMatrix &M = mats[tid];
+            extract_comp(c, nc, gidx, off[c], comp, loc, Ap, Ai, Ax, M);
+            cl[c].assign(kc[c], 0.0);
+            cr[c].assign(kc[c], 0.0);
+            if (V) cv[c].assign((size_t)nc * kc[c], 0.0);
+            crc[c] = speigs_ex(nc, M.Ap.data(), M.Ai.data(), M.Ax.data(), kc[c], which, opts, ho, cl[c].data(),
+                               V ? cv[c].data() : nullptr, cr[c].data(), &cinfo[c], nullptr);
+        });
+        for (int64_t c : hostc) {
+            if (crc[c]) return crc[c];
+            info->nops += cinfo[c].nops;
+            if (cinfo[c].flag) info->flag = 1;
         }
     }
+    // device components: one block-diagonal matrix
+    std::vector<int64_t> devc;
+    for (int64_t c = 0; c < ncomp; c++)
+        if (!onhost[c]) devc.push_back(c);
+    if (!devc.empty()) {
+        if (speigs_hip_device_count() <= 0) return SPEIGS_ERR_INTERNAL;
+        if (hipSetDevice(ho->device) != hipSuccess) return SPEIGS_ERR_ARG;
+        std::vector<BatchComp> bc(devc.size());
+        int64_t nb = 0, nnzb = 0;
+        for (size_t i = 0; i < devc.size(); i++) {
+            bc[i].off = nb;
+            bc[i].n = sz[devc[i]];
+            bc[i].k = kc[devc[i]];
+            nb += bc[i].n;
+        }
+        std::vector<int64_t> bAp(nb + 1, 0), bAi;
+        std::vector<double> bAx;
+        bAi.reserve(Ap[n]);
+        bAx.reserve(Ap[n]);
+        {
+            CompMatrix M;
+            for (size_t i = 0; i < devc.size(); i++) {
+                const int64_t c = devc[i], nc = sz[c];
+                extract_comp(c, nc, gidx, off[c], comp, loc, Ap, Ai, Ax, M);
+                bc[i].anorm = norm1_upper(nc, M.Ap.data(), M.Ai.data(), M.Ax.data());
+                if (bc[i].anorm <= 0) bc[i].anorm = 1.0;
+                for (int64_t jj = 0; jj < nc; jj++) {
+                    bAp[bc[i].off + jj] = nnzb + M.Ap[jj];
+                }
+                for (size_t q = 0; q < M.Ai.size(); q++) {
+                    bAi.push_back(M.Ai[q] + bc[i].off);
+                    bAx.push_back(M.Ax[q]);
+                }
+                nnzb += (int64_t)M.Ai.size();
+            }
+            bAp[nb] = nnzb;
+        }
+        StreamGuard sg;
+        if (hipStreamCreate(&sg.s) != hipSuccess) return SPEIGS_ERR_INTERNAL;
+        DevMem mem;
+        std::unique_ptr<MultOp> op;
+        int rc = SPEIGS_OK;
+        bool fallback = false;
+        if (which == SPEIGS_LM) {
+            op.reset(new MultOp());
+        } else {
+            auto *fo = new FactorOp();
+            op.reset(fo);
+            FactorOptions fopt;
+            fopt.precision = ho->precision == SPEIGS_HIP_FP32_BLOCK ? SPEIGS_HIP_FP32_IR : ho->precision;
+            fopt.ir_tol = ho->ir_tol > 0 ? ho->ir_tol : 1e-13;
+            fopt.ir_maxit = ho->ir_maxit > 0 ? ho->ir_maxit : 12;
+            fopt.ordering = opts->ordering;
+            double ta = wall();
+            if ((rc = factor_create(nb, bAp.data(), bAi.data(), bAx.data(), fopt, sg.s, &fo->F, hinfo))) return rc;
+            info->t_analyze += wall() - ta;
+            double tf = wall();
+            const int st = factor_numeric(fo->F, 0.0);
+            if (st < 0) return st;
+            info->t_factor += wall() - tf;
+            if (hinfo) hinfo->t_numeric = info->t_factor;
+            // a component that is not positive definite needs its own shift
+            // (or LU): solve the device components one call at a time
+            fallback = st == 1;
+        }
+        if (!fallback) {
+            op->n = nb;
+            op->stream = sg.s;
+            if ((rc = sell_build(nb, bAp.data(), bAi.data(), bAx.data(), &op->A, mem, sg.s))) return rc;
+            if (auto *fo = dynamic_cast<FactorOp *>(op.get())) factor_attach_matrix(fo->F, &op->A);
+            double ti = wall();
+            if ((rc = lanczos_batched(op.get(), nb, bc, opts, V != nullptr))) return rc;
+            info->t_iter += wall() - ti;
+            if (auto *fo = dynamic_cast<FactorOp *>(op.get())) factor_stats(fo->F, hinfo);
+            for (size_t i = 0; i < devc.size(); i++) {
+                const int64_t c = devc[i];
+                cl[c] = bc[i].lambda;
+                cr[c] = bc[i].resid;
+                if (V) cv[c] = std::move(bc[i].V);
+                info->nops += bc[i].nops;
+                if (bc[i].flag) info->flag = 1;
+            }
+        } else {
+            op.reset();
+            CompMatrix M;
+            for (int64_t c : devc) {
+                const int64_t nc = sz[c];
+                extract_comp(c, nc, gidx, off[c], comp, loc, Ap, Ai, Ax, M);
+                cl[c].assign(kc[c], 0.0);
+                cr[c].assign(kc[c], 0.0);
+                if (V) cv[c].assign((size_t)nc * kc[c], 0.0);
+                speigs_info si;
+                const int r2 = speigs_ex(nc, M.Ap.data(), M.Ai.data(), M.Ax.data(), kc[c], which, opts, ho,
+                                         cl[c].data(), V ? cv[c].data() : nullptr, cr[c].data(), &si, nullptr);
+                if (r2) return r2;
+                info->nops += si.nops;
+                if (si.flag) info->flag = 1;
+            }
+        }
+    }
+    // candidates in component order, then the k best by |lambda|
+    std::vector<double> clam, cres;
+    std::vector<int64_t> ccmp, cslt;
+    for (int64_t c = 0; c < ncomp; c++)
+        for (size_t t = 0; t < cl[c].size(); t++) {
+            clam.push_back(cl[c][t]);
+            cres.push_back(cr[c][t]);
+            ccmp.push_back(c);
+            cslt.push_back((int64_t)t);
+        }
     const int64_t ncand = (int64_t)clam.size();
     std::vector<KeyIdx> ord(ncand);
     for (int64_t i = 0; i < ncand; i++) ord[i] = {std::fabs(clam[i]), i};

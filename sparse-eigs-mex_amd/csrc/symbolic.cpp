@@ -199,13 +199,35 @@ private:
             rids.push_back(nr);
         }
         std::vector<int32_t>().swap(nodes);
-        for (size_t c = 0; c < comps.size(); c++) {
-            if ((int64_t)comps[c].size() <= leaf_)
-                roots.push_back(leaf_node(comps[c].data(), (int64_t)comps[c].size()));
-            else
-                dissect(comps[c], rids[c], roots, -1);
-            std::vector<int32_t>().swap(comps[c]);
+        // components are independent: dissect them on the free threads, each
+        // into its own slot, and collect the slots in component order
+        std::vector<NdList> slot(comps.size());
+        std::atomic<size_t> next{0};
+        auto work = [&] {
+            for (;;) {
+                const size_t c = next.fetch_add(1);
+                if (c >= comps.size()) break;
+                if ((int64_t)comps[c].size() <= leaf_)
+                    slot[c].push_back(leaf_node(comps[c].data(), (int64_t)comps[c].size()));
+                else
+                    dissect(comps[c], rids[c], slot[c], -1);
+                std::vector<int32_t>().swap(comps[c]);
+            }
+        };
+        std::vector<std::thread> th;
+        for (size_t t = 1; t < comps.size(); t++) {
+            int f = free_threads_.load();
+            while (f > 0 && !free_threads_.compare_exchange_weak(f, f - 1)) {}
+            if (f <= 0) break;
+            th.emplace_back([&] {
+                work();
+                free_threads_.fetch_add(1);
+            });
         }
+        work();
+        for (auto &t : th) t.join();
+        for (auto &sl : slot)
+            for (auto &r : sl) roots.push_back(std::move(r));
     }
 
     // connected region r with nodes.size() > leaf; h0 >= 0: the thread's queue
