@@ -99,20 +99,24 @@ int diag_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax,
 int dense_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax, int64_t k, speigs_which which,
                double *lambda, double *V, double *resid, speigs_info *info)
 {
-    std::vector<double> D((size_t)n * n, 0.0), ev(n);
+    std::vector<double> D((size_t)n * n, 0.0);
     for (int64_t c = 0; c < n; c++)
         for (int64_t p = Ap[c]; p < Ap[c + 1]; p++) {
             D[Ai[p] + c * n] = Ax[p];
             D[c + Ai[p] * n] = Ax[p];
         }
-    if (dense_syev(n, D.data(), ev.data())) return SPEIGS_ERR_LAPACK;
+    // eigenvalues first, the k wanted by |lambda| (the reference sorts all of
+    // them, src/speigs_core.c:336-350), then only those k vectors
+    DenseTri T;
+    if (dense_tri_values(n, D.data(), T)) return SPEIGS_ERR_LAPACK;
     std::vector<KeyIdx> ord(n);
-    for (int64_t i = 0; i < n; i++) ord[i] = {std::fabs(ev[i]), i};
+    for (int64_t i = 0; i < n; i++) ord[i] = {std::fabs(T.w[i]), i};
     qsort(ord.data(), (size_t)n, sizeof(KeyIdx), which == SPEIGS_SM ? key_asc : key_desc);
+    std::vector<int64_t> sel(k);
+    for (int64_t t = 0; t < k; t++) sel[t] = ord[t].idx;
+    if (V && dense_tri_vectors(T, k, sel.data(), V, n)) return SPEIGS_ERR_LAPACK;
     for (int64_t t = 0; t < k; t++) {
-        int64_t i = ord[t].idx;
-        lambda[t] = ev[i];
-        if (V) std::memcpy(V + t * n, D.data() + i * n, (size_t)n * sizeof(double));
+        lambda[t] = T.w[sel[t]];
         if (resid) resid[t] = 0.0;
     }
     info->nconv = k;
@@ -680,6 +684,7 @@ int blocks_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *A
                 const speigs_opts *opts, const speigs_hip_opts *ho, const std::vector<int64_t> &comp, int64_t ncomp,
                 double *lambda, double *V, double *resid, speigs_info *info, speigs_hip_info *hinfo)
 {
+    double tp = wall();
     std::vector<int64_t> sz(ncomp, 0), off(ncomp + 1, 0), gidx(n), loc(n), fill(ncomp, 0);
     for (int64_t i = 0; i < n; i++) sz[comp[i]]++;
     for (int64_t c = 0; c < ncomp; c++) off[c + 1] = off[c] + sz[c];
@@ -726,6 +731,7 @@ int blocks_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *A
             info->nops += cinfo[c].nops;
             if (cinfo[c].flag) info->flag = 1;
         }
+        trace_phase("host components", tp);
     }
     // device components: one block-diagonal matrix
     std::vector<int64_t> devc;
@@ -744,26 +750,32 @@ int blocks_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *A
         }
         std::vector<int64_t> bAp(nb + 1, 0), bAi;
         std::vector<double> bAx;
-        bAi.reserve(Ap[n]);
-        bAx.reserve(Ap[n]);
         {
-            CompMatrix M;
-            for (size_t i = 0; i < devc.size(); i++) {
+            // components in parallel: extract, norm, then copy into place
+            const int nt = host_threads();
+            std::vector<CompMatrix> mats(devc.size());
+            parallel_for((int64_t)devc.size(), nt, 1, [&](int64_t i, int) {
                 const int64_t c = devc[i], nc = sz[c];
-                extract_comp(c, nc, gidx, off[c], comp, loc, Ap, Ai, Ax, M);
-                bc[i].anorm = norm1_upper(nc, M.Ap.data(), M.Ai.data(), M.Ax.data());
+                extract_comp(c, nc, gidx, off[c], comp, loc, Ap, Ai, Ax, mats[i]);
+                bc[i].anorm = norm1_upper(nc, mats[i].Ap.data(), mats[i].Ai.data(), mats[i].Ax.data());
                 if (bc[i].anorm <= 0) bc[i].anorm = 1.0;
-                for (int64_t jj = 0; jj < nc; jj++) {
-                    bAp[bc[i].off + jj] = nnzb + M.Ap[jj];
-                }
+            });
+            std::vector<int64_t> zoff(devc.size() + 1, 0);
+            for (size_t i = 0; i < devc.size(); i++) zoff[i + 1] = zoff[i] + (int64_t)mats[i].Ai.size();
+            nnzb = zoff[devc.size()];
+            bAi.resize(std::max<int64_t>(nnzb, 1));
+            bAx.resize(std::max<int64_t>(nnzb, 1));
+            parallel_for((int64_t)devc.size(), nt, 1, [&](int64_t i, int) {
+                const CompMatrix &M = mats[i];
+                for (int64_t jj = 0; jj < bc[i].n; jj++) bAp[bc[i].off + jj] = zoff[i] + M.Ap[jj];
                 for (size_t q = 0; q < M.Ai.size(); q++) {
-                    bAi.push_back(M.Ai[q] + bc[i].off);
-                    bAx.push_back(M.Ax[q]);
+                    bAi[zoff[i] + q] = M.Ai[q] + bc[i].off;
+                    bAx[zoff[i] + q] = M.Ax[q];
                 }
-                nnzb += (int64_t)M.Ai.size();
-            }
+            });
             bAp[nb] = nnzb;
         }
+        trace_phase("device block matrix", tp);
         StreamGuard sg;
         if (hipStreamCreate(&sg.s) != hipSuccess) return SPEIGS_ERR_INTERNAL;
         DevMem mem;
@@ -797,9 +809,11 @@ int blocks_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *A
             op->stream = sg.s;
             if ((rc = sell_build(nb, bAp.data(), bAi.data(), bAx.data(), &op->A, mem, sg.s))) return rc;
             if (auto *fo = dynamic_cast<FactorOp *>(op.get())) factor_attach_matrix(fo->F, &op->A);
+            trace_phase("factor, SELL build", tp);
             double ti = wall();
             if ((rc = lanczos_batched(op.get(), nb, bc, opts, V != nullptr))) return rc;
             info->t_iter += wall() - ti;
+            trace_phase("batched Lanczos", tp);
             if (auto *fo = dynamic_cast<FactorOp *>(op.get())) factor_stats(fo->F, hinfo);
             for (size_t i = 0; i < devc.size(); i++) {
                 const int64_t c = devc[i];

@@ -226,9 +226,13 @@ int lanczos_batched(DevOperator *op, int64_t n, std::vector<BatchComp> &comps, c
             if (cs[c].state == st2) return true;
         return false;
     };
+    double t_apply = 0.0, t_orth = 0.0, t_eval = 0.0, t_rest = 0.0;
+    int64_t steps = 0, evals = 0;
     while (active(EXTEND) || active(EVAL)) {
         // ---- one Krylov step of every extending component
         if (active(EXTEND)) {
+            steps++;
+            double ta = wall();
             for (int64_t c = 0; c < nc; c++) {
                 const bool e = cs[c].state == EXTEND;
                 W.ij[c] = e ? (int32_t)cs[c].j : -1;
@@ -240,6 +244,11 @@ int lanczos_batched(DevOperator *op, int64_t n, std::vector<BatchComp> &comps, c
             if ((rc = op->apply(W.x, W.w))) return rc;
             for (int64_t c = 0; c < nc; c++)
                 if (W.iact[c]) comps[c].nops++;
+            if (trace_on()) {
+                SEHIP_CHECK(hipStreamSynchronize(W.st));
+                t_apply += wall() - ta;
+                ta = wall();
+            }
             if ((rc = seg_cgs2(W, W.V, W.w))) return rc;
             std::vector<int32_t> brk(nc, 0);
             bool anybrk = false;
@@ -296,8 +305,11 @@ int lanczos_batched(DevOperator *op, int64_t n, std::vector<BatchComp> &comps, c
                 s.j++;
                 if (s.j >= s.ncv) s.state = EVAL;
             }
+            t_orth += wall() - ta;
             continue;   // extend until no component is extending
         }
+        evals++;
+        double te = wall();
 
         // ---- projected problems and residuals of every component at a restart point
         for (int64_t c = 0; c < nc; c++) {
@@ -371,6 +383,8 @@ int lanczos_batched(DevOperator *op, int64_t n, std::vector<BatchComp> &comps, c
             s.l = std::max<int64_t>(1, std::min<int64_t>(b.k + 5, s.ncv - 1));
             s.nret = s.computed = s.base = 0;
         }
+        t_eval += wall() - te;
+        te = wall();
         if (!anyrestart) continue;
 
         // ---- thick restart: l dominant Ritz vectors per component, orthonormalised one by one
@@ -478,7 +492,12 @@ int lanczos_batched(DevOperator *op, int64_t n, std::vector<BatchComp> &comps, c
         seg_store_col(W.L, W.w, W.sc, W.jc, W.Vn, W.ld, W.st);
         if ((rc = up_i32(W, W.mc, mm))) return rc;
         seg_copy_cols(W.L, W.Vn, W.V, W.ld, W.mc, W.st);
+        t_rest += wall() - te;
     }
+    if (trace_on())
+        fprintf(stderr, "[speigs_hip] batched Lanczos: %lld steps (apply %.1f ms, CGS2 %.1f ms), %lld evaluations "
+                        "(%.1f ms), restarts %.1f ms\n",
+                (long long)steps, 1e3 * t_apply, 1e3 * t_orth, (long long)evals, 1e3 * t_eval, 1e3 * t_rest);
     for (int64_t c = 0; c < nc; c++) {
         BatchComp &b = comps[c];
         b.flag = b.nconv >= std::min<int64_t>(b.k, cs[c].ncv) ? 0 : 1;

@@ -153,6 +153,14 @@ int lanczos_dev(DevOperator *op, int64_t n, int64_t k, const speigs_opts *o,
 // host dense kernels (host_dense.cpp)
 int jacobi_eig(int64_t m, const double *Hupper, int64_t ldh, double *Z, double *w);
 int dense_syev(int64_t n, double *A, double *w);   // Householder + implicit QL, ascending
+// dense symmetric eigenproblem in two steps: all eigenvalues (ascending, T.w),
+// then the eigenvectors of the ascending positions sel[0..k) into Vsel (ld ldv)
+struct DenseTri {
+    int64_t n = 0;
+    std::vector<double> A, d0, e0, w;   // reflectors, tridiagonal, eigenvalues
+};
+int dense_tri_values(int64_t n, const double *Afull, DenseTri &T);
+int dense_tri_vectors(const DenseTri &T, int64_t k, const int64_t *sel, double *Vsel, int64_t ldv);
 int band_chol_u(int64_t n, int64_t kd, double *AB, int64_t ld);
 void band_solve_u(int64_t n, int64_t kd, const double *AB, int64_t ld, double *b);
 int lu_factor(int64_t n, double *A, int64_t *piv);

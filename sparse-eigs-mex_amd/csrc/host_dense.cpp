@@ -4,8 +4,10 @@
 //     with the relative stopping test |a_pq| <= eps sqrt(|a_pp a_qq|) keeps the
 //     small Ritz values of a graded positive definite projection to relative
 //     accuracy (Demmel and Veselic 1992); m <= ncv is at most a few hundred.
-//   * dense_syev: the dense short-circuit (src/speigs_core.c:318-353):
-//     Householder tridiagonalisation, implicit QL with shifts, ascending order.
+//   * dense_syev: Householder tridiagonalisation, implicit QL with shifts,
+//     ascending order, all vectors (small projected problems);
+//     dense_tri_values / dense_tri_vectors: the dense short-circuit
+//     (src/speigs_core.c:318-353), all eigenvalues but only the k returned vectors.
 //   * band_chol_u / band_solve_u: the banded short-circuit (dpbtrf/dpbtrs "U",
 //     src/speigs_core.c:869-931, :101-110), LAPACK's upper band storage.
 //   * lu_factor / lu_solve: dense partial-pivoting LU for the indefinite
@@ -13,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 #include <vector>
 
 #include "dev.h"
@@ -170,6 +173,241 @@ int dense_syev(int64_t n, double *A, double *d)
     }
     std::memcpy(A, Q.data(), (size_t)n * n * sizeof(double));
     sort_ascending(n, d, A, n);
+    return 0;
+}
+
+// ---- selected eigenpairs of a dense symmetric matrix (the dense short-circuit
+// needs k of n): Householder tridiagonalisation with the reflectors kept in
+// place, implicit QL for all eigenvalues (the same iteration as dense_syev, so
+// the same values), inverse iteration on the tridiagonal for the selected
+// ones (Wilkinson; vectors of a cluster are reorthogonalised against each
+// other, modified Gram-Schmidt), and the back-transformation of just those
+// vectors: O(n^3 * 4/3) + O(n^2 k) instead of O(n^3) per accumulated sweep.
+namespace {
+
+// A (n x n, full symmetric) -> tridiagonal d, e; reflector k (unit vector,
+// H_k = I - 2 v v^T acting on rows k+1..n-1) left in A[k+1.., k]
+void tridiag_inplace(int64_t n, double *A, double *d, double *e)
+{
+    std::vector<double> v(n), p(n);
+    for (int64_t k = 0; k + 2 < n; k++) {
+        double xn = 0.0;
+        for (int64_t i = k + 1; i < n; i++) xn += A[i + k * n] * A[i + k * n];
+        xn = std::sqrt(xn);
+        const double x0 = A[(k + 1) + k * n];
+        for (int64_t i = k + 1; i < n; i++) v[i] = 0.0;
+        if (xn == 0.0) {
+            e[k] = 0.0;
+            for (int64_t i = k + 1; i < n; i++) A[i + k * n] = 0.0;
+            continue;
+        }
+        const double alpha = x0 > 0 ? -xn : xn;
+        for (int64_t i = k + 1; i < n; i++) v[i] = A[i + k * n];
+        v[k + 1] -= alpha;
+        double vn = 0.0;
+        for (int64_t i = k + 1; i < n; i++) vn += v[i] * v[i];
+        vn = std::sqrt(vn);
+        if (vn == 0.0) {
+            e[k] = x0;
+            for (int64_t i = k + 1; i < n; i++) A[i + k * n] = 0.0;
+            continue;
+        }
+        for (int64_t i = k + 1; i < n; i++) v[i] /= vn;
+        double K = 0.0;
+        for (int64_t i = k + 1; i < n; i++) p[i] = 0.0;
+        for (int64_t j = k + 1; j < n; j++) {
+            const double vj = v[j];
+            const double *aj = A + j * n;
+            for (int64_t i = k + 1; i < n; i++) p[i] += aj[i] * vj;
+        }
+        for (int64_t i = k + 1; i < n; i++) K += v[i] * p[i];
+        for (int64_t i = k + 1; i < n; i++) p[i] = 2.0 * p[i] - 2.0 * K * v[i];
+        for (int64_t j = k + 1; j < n; j++) {
+            double *aj = A + j * n;
+            const double vj = v[j], pj = p[j];
+            for (int64_t i = k + 1; i < n; i++) aj[i] -= v[i] * pj + p[i] * vj;
+        }
+        e[k] = alpha;
+        for (int64_t i = k + 1; i < n; i++) A[i + k * n] = v[i];
+    }
+    for (int64_t i = 0; i < n; i++) d[i] = A[i + i * n];
+    if (n >= 2) e[n - 2] = A[(n - 1) + (n - 2) * n];
+    e[n - 1] = 0.0;
+}
+
+// eigenvalues of the tridiagonal (d, e) by implicit QL, no vectors; ascending
+int ql_values(int64_t n, double *d, double *e)
+{
+    for (int64_t l = 0; l < n; l++) {
+        int iter = 0;
+        for (;;) {
+            int64_t mm = l;
+            for (; mm < n - 1; mm++)
+                if (std::fabs(e[mm]) <= kEps * (std::fabs(d[mm]) + std::fabs(d[mm + 1]))) break;
+            if (mm == l) break;
+            if (++iter > 60) return (int)(l + 1);
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = std::hypot(g, 1.0);
+            g = d[mm] - d[l] + e[l] / (g + (g >= 0 ? r : -r));
+            double s = 1.0, c = 1.0, pp = 0.0;
+            bool under = false;
+            for (int64_t i = mm - 1; i >= l; i--) {
+                const double f = s * e[i], b = c * e[i];
+                r = std::hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) { d[i + 1] -= pp; e[mm] = 0.0; under = true; break; }
+                s = f / r;
+                c = g / r;
+                g = d[i + 1] - pp;
+                r = (d[i] - g) * s + 2.0 * c * b;
+                pp = s * r;
+                d[i + 1] = g + pp;
+                g = c * r - b;
+            }
+            if (under) continue;
+            d[l] -= pp;
+            e[l] = g;
+            e[mm] = 0.0;
+        }
+    }
+    std::sort(d, d + n);
+    return 0;
+}
+
+// solve (T - sigma I) x = b in place, T = tridiag(e, d, e), Gaussian
+// elimination with partial pivoting; tiny pivots replaced by eps3
+void tri_shift_solve(int64_t n, const double *d, const double *e, double sigma, double eps3, double *x)
+{
+    std::vector<double> u0(n), u1(n), u2(n), l(n);
+    std::vector<char> sw(n, 0);
+    // row i of the working matrix: (a_i,i, a_i,i+1, a_i,i+2)
+    double a0 = d[0] - sigma, a1 = n > 1 ? e[0] : 0.0, a2 = 0.0;
+    for (int64_t i = 0; i + 1 < n; i++) {
+        // next row: (e_i, d_{i+1} - sigma, e_{i+1})
+        const double b0 = e[i], b1 = d[i + 1] - sigma, b2 = i + 2 < n ? e[i + 1] : 0.0;
+        if (std::fabs(b0) > std::fabs(a0)) {      // swap rows
+            sw[i] = 1;
+            u0[i] = b0; u1[i] = b1; u2[i] = b2;
+            const double m = a0 / b0;
+            l[i] = m;
+            a0 = a1 - m * b1;
+            a1 = a2 - m * b2;
+            a2 = 0.0;
+        } else {
+            if (a0 == 0.0) a0 = eps3;
+            u0[i] = a0; u1[i] = a1; u2[i] = a2;
+            const double m = b0 / a0;
+            l[i] = m;
+            a0 = b1 - m * a1;
+            a1 = b2 - m * a2;
+            a2 = 0.0;
+        }
+    }
+    if (a0 == 0.0) a0 = eps3;
+    u0[n - 1] = a0; u1[n - 1] = 0.0; u2[n - 1] = 0.0;
+    for (int64_t i = 0; i + 1 < n; i++) {       // forward: apply the row operations to x
+        if (sw[i]) {
+            const double t = x[i];
+            x[i] = x[i + 1];
+            x[i + 1] = t - l[i] * x[i];
+        } else {
+            x[i + 1] -= l[i] * x[i];
+        }
+    }
+    for (int64_t i = n - 1; i >= 0; i--) {      // back substitution
+        double s = x[i];
+        if (i + 1 < n) s -= u1[i] * x[i + 1];
+        if (i + 2 < n) s -= u2[i] * x[i + 2];
+        double piv = u0[i];
+        if (std::fabs(piv) < eps3) piv = piv < 0 ? -eps3 : eps3;
+        x[i] = s / piv;
+    }
+}
+
+}  // namespace
+
+int dense_tri_values(int64_t n, const double *Afull, DenseTri &T)
+{
+    T.n = n;
+    T.A.assign(Afull, Afull + (size_t)n * n);
+    T.w.assign(n, 0.0);
+    if (n <= 0) return 0;
+    if (n == 1) {
+        T.w[0] = T.A[0];
+        return 0;
+    }
+    for (int64_t c = 0; c < n; c++)
+        for (int64_t r = c + 1; r < n; r++) T.A[r + c * n] = T.A[c + r * n];
+    T.d0.assign(n, 0.0);
+    T.e0.assign(n, 0.0);
+    tridiag_inplace(n, T.A.data(), T.d0.data(), T.e0.data());
+    std::vector<double> e = T.e0;
+    T.w = T.d0;
+    return ql_values(n, T.w.data(), e.data());
+}
+
+int dense_tri_vectors(const DenseTri &T, int64_t k, const int64_t *sel, double *Vsel, int64_t ldv)
+{
+    const int64_t n = T.n;
+    if (k <= 0) return 0;
+    if (n == 1) {
+        Vsel[0] = 1.0;
+        return 0;
+    }
+    const std::vector<double> &d0 = T.d0, &e0 = T.e0, &w = T.w;
+    double tnorm = 0.0;
+    for (int64_t i = 0; i < n; i++)
+        tnorm = std::max(tnorm, std::fabs(d0[i]) + (i > 0 ? std::fabs(e0[i - 1]) : 0.0) + std::fabs(e0[i]));
+    if (tnorm == 0.0) tnorm = 1.0;
+    const double eps3 = kEps * tnorm, gaptol = 1e-3 * tnorm;
+    // selected eigenvalues in ascending order, so clusters are contiguous
+    std::vector<int64_t> pos(k);
+    std::iota(pos.begin(), pos.end(), 0);
+    std::sort(pos.begin(), pos.end(), [&](int64_t a, int64_t b) { return sel[a] < sel[b]; });
+    std::vector<std::vector<double>> X(k, std::vector<double>(n));
+    int64_t cstart = 0;
+    double prev_sigma = 0.0;
+    for (int64_t q = 0; q < k; q++) {
+        const int64_t t = pos[q];
+        const double lam = w[sel[t]];
+        if (q > 0 && lam - w[sel[pos[q - 1]]] > gaptol) cstart = q;
+        double sigma = lam;
+        if (q > 0 && sigma <= prev_sigma) sigma = prev_sigma + eps3;   // separate equal shifts
+        prev_sigma = sigma;
+        std::vector<double> &x = X[t];
+        uint64_t z = 0x9E3779B97F4A7C15ULL * (uint64_t)(q + 1);
+        for (int64_t i = 0; i < n; i++) {          // deterministic start vector
+            z ^= z >> 12; z ^= z << 25; z ^= z >> 27;
+            x[i] = (double)((z * 0x2545F4914F6CDD1DULL) >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+        }
+        for (int it = 0; it < 4; it++) {
+            tri_shift_solve(n, d0.data(), e0.data(), sigma, eps3, x.data());
+            for (int64_t p2 = cstart; p2 < q; p2++) {     // reorthogonalise within the cluster
+                const std::vector<double> &y = X[pos[p2]];
+                double dot = 0.0;
+                for (int64_t i = 0; i < n; i++) dot += x[i] * y[i];
+                for (int64_t i = 0; i < n; i++) x[i] -= dot * y[i];
+            }
+            double nr = 0.0;
+            for (int64_t i = 0; i < n; i++) nr += x[i] * x[i];
+            nr = std::sqrt(nr);
+            if (nr == 0.0) return 2;
+            for (int64_t i = 0; i < n; i++) x[i] /= nr;
+        }
+    }
+    // back-transformation: x <- H_0 H_1 ... H_{n-3} x
+    for (int64_t t = 0; t < k; t++) {
+        std::vector<double> &x = X[t];
+        for (int64_t kk = n - 3; kk >= 0; kk--) {
+            const double *v = T.A.data() + kk * n;
+            double s = 0.0;
+            for (int64_t i = kk + 1; i < n; i++) s += v[i] * x[i];
+            s *= 2.0;
+            if (s != 0.0)
+                for (int64_t i = kk + 1; i < n; i++) x[i] -= s * v[i];
+        }
+        std::memcpy(Vsel + (size_t)t * ldv, x.data(), (size_t)n * sizeof(double));
+    }
     return 0;
 }
 
