@@ -50,7 +50,9 @@ typedef struct {
     int           device;       /* HIP device ordinal; default 0                             */
     int           verbose;      /* 0 silent                                                  */
     const double *start_vec;    /* n values for SPEIGS_HIP_START_VECTOR                      */
-    int64_t       reserved[8];
+    const struct speigs_hip_model *model;   /* trained start network for
+                                             * SPEIGS_HIP_START_LEARNED (NULL -> uniform) */
+    int64_t       reserved[7];
 } speigs_hip_opts;
 
 typedef struct {
@@ -115,6 +117,38 @@ int  speigs_hip_op_apply(speigs_hip_op *op, const double *x, double *y, speigs_i
 /* y = A x (always A, never the shifted matrix) */
 int  speigs_hip_op_multA(speigs_hip_op *op, const double *x, double *y, speigs_int nvec);
 void speigs_hip_op_destroy(speigs_hip_op *op);
+
+/* ---- learned start: a message-passing network over the graph of A --------
+ * No counterpart in the reference (its plan, docs/REQUIREMENTS.md, names a
+ * learned starting block as future work).  Three layers of width 32 over the
+ * graph with edge weights |a_ij| / a_ii; node features from the diagonal, the
+ * off-diagonal row mass and 16 seeded noise channels; the 32 output channels
+ * are the proposed start block (BF16 weights, FP32 arithmetic on the matrix
+ * cores).  Trained on the GPU against eigenvectors of the plain solver; the
+ * eigensolver's residual test still decides convergence, so the network can
+ * change the iteration count but not the answer. */
+typedef struct speigs_hip_model speigs_hip_model;
+
+/* new network with Glorot-uniform weights from `seed` (layers 1..8, usually 3) */
+int     speigs_hip_model_create(int layers, uint64_t seed, speigs_hip_model **model);
+int64_t speigs_hip_model_nparams(const speigs_hip_model *model);
+/* parameters: per layer W (32 x 64 row-major, [output][input], input = [H | A_hat H])
+ * then b (32), FP32 */
+int     speigs_hip_model_set_params(speigs_hip_model *model, const float *params, int64_t nparams);
+int     speigs_hip_model_get_params(const speigs_hip_model *model, float *params, int64_t nparams);
+/* X = the n x 32 output block (row-major FP32) for the graph of A */
+int     speigs_hip_model_infer(speigs_hip_model *model, speigs_int n, const speigs_int *Ap, const speigs_int *Ai,
+                               const double *Ax, uint64_t seed, float *X);
+/* loss 1 - tr(S^-1 X^T U U^T X)/k (S = X^T X) against the k orthonormal columns
+ * of U (n x k column-major) and its gradient (nparams floats; NULL skips it) */
+int     speigs_hip_model_grad(speigs_hip_model *model, speigs_int n, const speigs_int *Ap, const speigs_int *Ai,
+                              const double *Ax, const double *U, speigs_int k, uint64_t seed, double *loss,
+                              float *grad);
+/* one Adam step on that loss (learning rate lr) */
+int     speigs_hip_model_train_step(speigs_hip_model *model, speigs_int n, const speigs_int *Ap,
+                                    const speigs_int *Ai, const double *Ax, const double *U, speigs_int k,
+                                    uint64_t seed, float lr, double *loss);
+void    speigs_hip_model_destroy(speigs_hip_model *model);
 
 #ifdef __cplusplus
 }

@@ -52,7 +52,7 @@ class HipOpts(ctypes.Structure):
     _fields_ = [("precision", c_int), ("ir_tol", c_double), ("ir_maxit", c_int),
                 ("start", c_int), ("start_block", c_int), ("use_graph", c_int),
                 ("device", c_int), ("verbose", c_int), ("start_vec", POINTER(c_double)),
-                ("reserved", c_int64 * 8)]
+                ("model", ctypes.c_void_p), ("reserved", c_int64 * 7)]
 
 
 class HipInfo(ctypes.Structure):
@@ -122,6 +122,105 @@ def _bind_ext(lib: ctypes.CDLL) -> None:
                               POINTER(HipOpts), _F64P, _F64P, _F64P, POINTER(Info), POINTER(HipInfo)]
     lib.speigs_hip_device_count.restype = c_int
     lib.speigs_hip_device_count.argtypes = []
+    vp, u64, f32p = ctypes.c_void_p, ctypes.c_uint64, POINTER(ctypes.c_float)
+    lib.speigs_hip_model_create.restype = c_int
+    lib.speigs_hip_model_create.argtypes = [c_int, u64, POINTER(vp)]
+    lib.speigs_hip_model_nparams.restype = c_int64
+    lib.speigs_hip_model_nparams.argtypes = [vp]
+    lib.speigs_hip_model_set_params.restype = c_int
+    lib.speigs_hip_model_set_params.argtypes = [vp, f32p, c_int64]
+    lib.speigs_hip_model_get_params.restype = c_int
+    lib.speigs_hip_model_get_params.argtypes = [vp, f32p, c_int64]
+    lib.speigs_hip_model_infer.restype = c_int
+    lib.speigs_hip_model_infer.argtypes = [vp, c_int64, _I64P, _I64P, _F64P, u64, f32p]
+    lib.speigs_hip_model_grad.restype = c_int
+    lib.speigs_hip_model_grad.argtypes = [vp, c_int64, _I64P, _I64P, _F64P, _F64P, c_int64, u64, _F64P, f32p]
+    lib.speigs_hip_model_train_step.restype = c_int
+    lib.speigs_hip_model_train_step.argtypes = [vp, c_int64, _I64P, _I64P, _F64P, _F64P, c_int64, u64,
+                                                ctypes.c_float, _F64P]
+    lib.speigs_hip_model_destroy.restype = None
+    lib.speigs_hip_model_destroy.argtypes = [vp]
+
+
+class StartModel:
+    """The learned-start network (speigs_hip_model_*): create, load/save
+    parameters, infer the proposed block, loss/gradient and Adam steps."""
+
+    WIDTH = 32
+
+    def __init__(self, layers: int = 3, seed: int = 1, lib=None):
+        self.lib = lib or load()
+        h = ctypes.c_void_p()
+        rc = self.lib.speigs_hip_model_create(layers, seed, ctypes.byref(h))
+        if rc:
+            raise RuntimeError(f"speigs_hip_model_create failed: {rc}")
+        self.handle = h
+        self.nparams = int(self.lib.speigs_hip_model_nparams(h))
+
+    def close(self):
+        if self.handle:
+            self.lib.speigs_hip_model_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.nparams, np.float32)
+        rc = self.lib.speigs_hip_model_get_params(self.handle, _ptr(p, ctypes.c_float), self.nparams)
+        if rc:
+            raise RuntimeError(f"get_params failed: {rc}")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        rc = self.lib.speigs_hip_model_set_params(self.handle, _ptr(p, ctypes.c_float), len(p))
+        if rc:
+            raise RuntimeError(f"set_params failed: {rc}")
+
+    def save(self, path):
+        self.get_params().tofile(str(path))
+
+    def load(self, path):
+        self.set_params(np.fromfile(str(path), np.float32))
+
+    def infer(self, A, seed: int = 88172645463325252) -> np.ndarray:
+        Ap, Ai, Ax = _csc(A)
+        n = len(Ap) - 1
+        X = np.zeros((n, self.WIDTH), np.float32)
+        rc = self.lib.speigs_hip_model_infer(self.handle, n, _ptr(Ap, c_int64), _ptr(Ai, c_int64), _ptr(Ax, c_double),
+                                             seed, _ptr(X, ctypes.c_float))
+        if rc:
+            raise RuntimeError(f"infer failed: {rc}")
+        return X
+
+    def grad(self, A, U, seed: int = 1):
+        Ap, Ai, Ax = _csc(A)
+        n = len(Ap) - 1
+        U = np.asfortranarray(U, np.float64)
+        loss = np.zeros(1)
+        g = np.zeros(self.nparams, np.float32)
+        rc = self.lib.speigs_hip_model_grad(self.handle, n, _ptr(Ap, c_int64), _ptr(Ai, c_int64), _ptr(Ax, c_double),
+                                            U.ctypes.data_as(POINTER(c_double)), U.shape[1], seed,
+                                            _ptr(loss, c_double), _ptr(g, ctypes.c_float))
+        if rc:
+            raise RuntimeError(f"grad failed: {rc}")
+        return float(loss[0]), g
+
+    def train_step(self, A, U, seed: int, lr: float) -> float:
+        Ap, Ai, Ax = _csc(A)
+        n = len(Ap) - 1
+        U = np.asfortranarray(U, np.float64)
+        loss = np.zeros(1)
+        rc = self.lib.speigs_hip_model_train_step(self.handle, n, _ptr(Ap, c_int64), _ptr(Ai, c_int64),
+                                                  _ptr(Ax, c_double), U.ctypes.data_as(POINTER(c_double)),
+                                                  U.shape[1], seed, lr, _ptr(loss, c_double))
+        if rc:
+            raise RuntimeError(f"train_step failed: {rc}")
+        return float(loss[0])
 
 
 def _ptr(a, t):

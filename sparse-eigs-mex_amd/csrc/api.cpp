@@ -22,6 +22,7 @@
 #include "factor_impl.h"
 #include "host_par.h"
 #include "batch.h"
+#include "mpnn.h"
 
 using namespace sehip;
 
@@ -534,10 +535,36 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
 
     double ti = wall();
     auto *fop = dynamic_cast<FactorOp *>(op.get());
-    if (fop && fop->F && hopts->precision == SPEIGS_HIP_FP32_BLOCK && info->path == SPEIGS_PATH_SHIFTINV)
-        rc = lobpcg_dev(op->A, fop->F, sg.s, n, k, opts, hopts, anorm, lambda, V, resid, info, hinfo);
+    const bool block = fop && fop->F && hopts->precision == SPEIGS_HIP_FP32_BLOCK && info->path == SPEIGS_PATH_SHIFTINV;
+    // learned start: the network's block for the block eigensolver, the sum of
+    // its first min(k, 32) columns as the Lanczos start vector
+    float *xl = nullptr;
+    speigs_hip_opts hl = *hopts;
+    std::vector<double> lstart;
+    if (hopts->start == SPEIGS_HIP_START_LEARNED && hopts->model) {
+        if ((rc = mem.alloc(&xl, (size_t)n * MP_WIDTH))) return rc;
+        const uint64_t seed = opts->seed ? opts->seed : 88172645463325252ULL;
+        auto *mdl = const_cast<speigs_hip_model *>(hopts->model);
+        if ((rc = mp_infer_device(mdl, n, Ap, Ai, Ax, seed, xl))) return rc;
+        if (!block) {
+            std::vector<float> X((size_t)n * MP_WIDTH);
+            SEHIP_CHECK(hipMemcpy(X.data(), xl, X.size() * sizeof(float), hipMemcpyDeviceToHost));
+            const int nc = (int)std::min<int64_t>(k, MP_WIDTH);
+            lstart.assign(n, 0.0);
+            for (int64_t i = 0; i < n; i++)
+                for (int c = 0; c < nc; c++) lstart[i] += X[(size_t)i * MP_WIDTH + c];
+            double nr = 0.0;
+            for (double v : lstart) nr += v * v;
+            if (nr > 0) {
+                hl.start = SPEIGS_HIP_START_VECTOR;
+                hl.start_vec = lstart.data();
+            }
+        }
+    }
+    if (block)
+        rc = lobpcg_dev(op->A, fop->F, sg.s, n, k, opts, &hl, anorm, lambda, V, resid, info, hinfo, xl);
     else
-        rc = lanczos_dev(op.get(), n, k, opts, hopts, anorm, lambda, V, resid, info);
+        rc = lanczos_dev(op.get(), n, k, opts, &hl, anorm, lambda, V, resid, info);
     info->t_iter = wall() - ti;
     tp = wall();
     if (auto *fo = dynamic_cast<FactorOp *>(op.get())) {

@@ -310,6 +310,22 @@ void rfrom_cols(const double *Y, int m, int64_t n, double *X, int64_t ldx, hipSt
     k_rtrans<double, double, false><<<dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st>>>(Y, ldx, m, n, X);
 }
 
+__global__ void k_rfrom_f32rows(const float *__restrict__ X, int64_t ldx, int m, int64_t n, double *__restrict__ Y,
+                                int64_t ldy)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= n * m) return;
+    const int64_t r = e / m;
+    const int c = (int)(e - r * m);
+    Y[r * ldy + c] = (double)X[r * ldx + c];
+}
+
+void rfrom_f32rows(const float *X, int64_t ldx, int m, int64_t n, double *Y, int64_t ldy, hipStream_t st)
+{
+    const int64_t tot = n * m;
+    if (tot > 0) k_rfrom_f32rows<<<dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st>>>(X, ldx, m, n, Y, ldy);
+}
+
 // columns [c0, c1) of X: uniform [-0.5, 0.5) from splitmix64 of
 // (seed, column, row) -- a counter-based stream, the same on every run
 __device__ __forceinline__ uint64_t splitmix64(uint64_t z)

@@ -32,6 +32,10 @@ struct InvTask {
 // tile (I, K) of front s, rows/cols clipped to the L11 order c
 __device__ __forceinline__ int tile_w(int c, int K) { return min(TB, c - K * TB); }
 
+// inverse of the lower-triangular diagonal tile, recursively blocked in LDS:
+// the four 16 x 16 diagonal blocks by forward substitution (a thread per
+// column), then two merges  X21 = -X22 (L21 X11)  (16 -> 32 -> 64), each product
+// spread over all threads.  A partial tile (w < 64) is padded with the identity.
 template <class T>
 __global__ void __launch_bounds__(256) k_inv_diag(T *pool, const InvTask *__restrict__ tasks,
                                                   const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
@@ -39,25 +43,50 @@ __global__ void __launch_bounds__(256) k_inv_diag(T *pool, const InvTask *__rest
 {
     __shared__ T a[TB][TB + 1];
     __shared__ T x[TB][TB + 1];
+    __shared__ T tmp[TB / 2][TB / 2 + 1];
     const InvTask t = tasks[blockIdx.x];
     const int32_t F = Fs[t.s], c = ncol[t.s];
     const int k0 = t.J * TB, w = tile_w(c, t.J);
     T *base = pool + foff[t.s] + (int64_t)k0 * F + k0;
-    for (int e = threadIdx.x; e < w * w; e += 256) {
-        const int r = e % w, q = e / w;
-        a[r][q] = r >= q ? base[(int64_t)q * F + r] : T(0);
+    for (int e = threadIdx.x; e < TB * TB; e += 256) {
+        const int r = e % TB, q = e / TB;
+        T v = T(0);
+        if (r < w && q < w) v = r >= q ? base[(int64_t)q * F + r] : T(0);
+        else if (r == q) v = T(1);
+        a[r][q] = v;
+        x[r][q] = T(0);
     }
     __syncthreads();
-    if ((int)threadIdx.x < w) {           // column j of the inverse by forward substitution
-        const int j = threadIdx.x;
-        for (int i = 0; i < j; i++) x[i][j] = T(0);
-        for (int i = j; i < w; i++) {
+    if (threadIdx.x < TB) {                 // 16 x 16 diagonal blocks, one column per thread
+        const int j = threadIdx.x, b1 = (j / 16) * 16 + 16;
+        for (int i = j; i < b1; i++) {
             T s = (i == j) ? T(1) : T(0);
             for (int k = j; k < i; k++) s -= a[i][k] * x[k][j];
             x[i][j] = s / a[i][i];
         }
     }
     __syncthreads();
+    for (int h = 16; h < TB; h *= 2) {      // merge blocks of size h into 2h
+        const int nb = TB / (2 * h);        // pairs
+        // tmp = L21 X11 for every pair p: rows [p2h + h, p2h + 2h), cols [p2h, p2h + h)
+        for (int e = threadIdx.x; e < nb * h * h; e += 256) {
+            const int p = e / (h * h), rr = (e / h) % h, cc = e % h;
+            const int o = p * 2 * h;
+            T s = T(0);
+            for (int k = 0; k < h; k++) s += a[o + h + rr][o + k] * x[o + k][o + cc];
+            tmp[p * h + rr][cc] = s;
+        }
+        __syncthreads();
+        // X21 = -X22 tmp
+        for (int e = threadIdx.x; e < nb * h * h; e += 256) {
+            const int p = e / (h * h), rr = (e / h) % h, cc = e % h;
+            const int o = p * 2 * h;
+            T s = T(0);
+            for (int k = 0; k < h; k++) s += x[o + h + rr][o + h + k] * tmp[p * h + k][cc];
+            x[o + h + rr][o + cc] = -s;
+        }
+        __syncthreads();
+    }
     for (int e = threadIdx.x; e < w * w; e += 256) {
         const int r = e % w, q = e / w;
         if (r >= q) base[(int64_t)q * F + r] = x[r][q];

@@ -130,6 +130,8 @@ void rto_f32cols(const double *X, int64_t ldx, int m, int64_t n, float *Y, hipSt
 void rfrom_f32cols(const float *Y, int m, int64_t n, double *X, int64_t ldx, hipStream_t st);
 void rto_cols(const double *X, int64_t ldx, int m, int64_t n, double *Y, hipStream_t st);
 void rfrom_cols(const double *Y, int m, int64_t n, double *X, int64_t ldx, hipStream_t st);
+// row-major FP32 (ld ldx) -> row-major FP64 (ld ldy), m columns
+void rfrom_f32rows(const float *X, int64_t ldx, int m, int64_t n, double *Y, int64_t ldy, hipStream_t st);
 // columns [c0, c1): uniform [-0.5, 0.5) from a counter-based generator
 void rrandom(double *X, int64_t ldx, int c0, int c1, int64_t n, uint64_t seed, hipStream_t st);
 

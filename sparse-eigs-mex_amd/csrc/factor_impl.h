@@ -115,8 +115,9 @@ int factor_block_solve_d(Factor *F, const double *B, double *X, int nb);
 void factor_block_free(Factor *F);
 
 // block eigensolver preconditioned by the FP32 factor (lobpcg.cpp)
+// Xstart: optional proposed start block (device, n x 32 row-major FP32)
 int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, const speigs_opts *o,
                const speigs_hip_opts *ho, double anorm, double *lambda, double *Vout, double *resid,
-               speigs_info *info, speigs_hip_info *hinfo);
+               speigs_info *info, speigs_hip_info *hinfo, const float *Xstart);
 
 }  // namespace sehip

@@ -125,7 +125,7 @@ void symmetrize(std::vector<double> &H, int d)
 
 int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, const speigs_opts *o,
                const speigs_hip_opts *ho, double anorm, double *lambda, double *Vout, double *resid,
-               speigs_info *info, speigs_hip_info *hinfo)
+               speigs_info *info, speigs_hip_info *hinfo, const float *Xstart)
 {
     int m = (int)std::min<int64_t>(32, std::max<int64_t>(k + 8, k + k / 2));
     if (m > n / 2) m = (int)std::max<int64_t>(k, n / 2);
@@ -156,18 +156,23 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
     if ((rc = W.mem.alloc(&W.C, 96 * 96))) return rc;
     if ((rc = W.mem.alloc(&W.red, 4 * 96))) return rc;
 
-    // start block: column 0 = the reference's xorshift64* start vector (or the
-    // caller's), the other columns from a counter-based generator on the device
+    // start block: the proposed block of the learned start (its first m
+    // columns), else column 0 = the reference's xorshift64* start vector (or
+    // the caller's) and the other columns from a counter-based generator
     {
         uint64_t seed = o->seed ? o->seed : 88172645463325252ULL;
-        std::vector<double> h((size_t)n);
-        if (ho && ho->start == SPEIGS_HIP_START_VECTOR && ho->start_vec)
-            std::memcpy(h.data(), ho->start_vec, (size_t)n * sizeof(double));
-        else
-            uniform_vec(&seed, n, h.data());
-        SEHIP_CHECK(hipMemcpyAsync(v0, h.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
-        rfrom_cols(v0, 1, n, T2, ldT, st);
-        rrandom(T2, ldT, 1, m, n, seed, st);
+        if (Xstart) {
+            rfrom_f32rows(Xstart, 32, std::min(m, 32), n, T2, ldT, st);
+        } else {
+            std::vector<double> h((size_t)n);
+            if (ho && ho->start == SPEIGS_HIP_START_VECTOR && ho->start_vec)
+                std::memcpy(h.data(), ho->start_vec, (size_t)n * sizeof(double));
+            else
+                uniform_vec(&seed, n, h.data());
+            SEHIP_CHECK(hipMemcpyAsync(v0, h.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+            rfrom_cols(v0, 1, n, T2, ldT, st);
+            rrandom(T2, ldT, 1, m, n, seed, st);
+        }
         for (int pass = 0; pass < 2; pass++) {
             std::vector<double> g;
             if ((rc = dgram(W, T2, ldT, m, T2, ldT, m, g))) return rc;

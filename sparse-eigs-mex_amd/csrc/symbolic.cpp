@@ -199,23 +199,35 @@ private:
             rids.push_back(nr);
         }
         std::vector<int32_t>().swap(nodes);
-        // components are independent: dissect them on the free threads, each
-        // into its own slot, and collect the slots in component order
+        // components are independent: the large ones are dissected on free
+        // threads (each into its own slot), the rest by this thread; the slots
+        // are collected in component order
         std::vector<NdList> slot(comps.size());
+        auto one = [&](size_t c) {
+            if ((int64_t)comps[c].size() <= leaf_)
+                slot[c].push_back(leaf_node(comps[c].data(), (int64_t)comps[c].size()));
+            else
+                dissect(comps[c], rids[c], slot[c], -1);
+            std::vector<int32_t>().swap(comps[c]);
+        };
+        // many components with enough work in total (a block-diagonal input):
+        // all of them go to the pool; otherwise only the large ones
+        int64_t total = 0;
+        for (auto &cc : comps) total += (int64_t)cc.size();
+        const bool many = comps.size() >= 16 && total >= 2 * kSpawnMin;
+        std::vector<size_t> large, small;
+        for (size_t c = 0; c < comps.size(); c++)
+            (many || (int64_t)comps[c].size() >= kSpawnMin ? large : small).push_back(c);
         std::atomic<size_t> next{0};
         auto work = [&] {
             for (;;) {
-                const size_t c = next.fetch_add(1);
-                if (c >= comps.size()) break;
-                if ((int64_t)comps[c].size() <= leaf_)
-                    slot[c].push_back(leaf_node(comps[c].data(), (int64_t)comps[c].size()));
-                else
-                    dissect(comps[c], rids[c], slot[c], -1);
-                std::vector<int32_t>().swap(comps[c]);
+                const size_t q = next.fetch_add(1);
+                if (q >= large.size()) break;
+                one(large[q]);
             }
         };
         std::vector<std::thread> th;
-        for (size_t t = 1; t < comps.size(); t++) {
+        for (size_t t = 1; t < large.size(); t++) {
             int f = free_threads_.load();
             while (f > 0 && !free_threads_.compare_exchange_weak(f, f - 1)) {}
             if (f <= 0) break;
@@ -225,6 +237,7 @@ private:
             });
         }
         work();
+        for (size_t c : small) one(c);
         for (auto &t : th) t.join();
         for (auto &sl : slot)
             for (auto &r : sl) roots.push_back(std::move(r));

@@ -171,6 +171,14 @@ def grf_weights(N: int, seed: int, corr: float = 8.0, logstd: float = 1.0) -> np
     return w.reshape(3, n)
 
 
+def grf_lap3d(N: int, seed: int, corr: float = 8.0, logstd: float = 1.0, shift: float = 1e-3, upper: bool = True):
+    """The learned-start family: 3-D graph Laplacian on an N^3 grid whose edge
+    coefficients are exp of a seeded Gaussian random field (correlation length
+    `corr` cells, log-std `logstd`), plus shift*I (constant eigenvector, eigenvalue
+    `shift`, as in the weighted family)."""
+    return weighted_lap3d(N, seed, shift=shift, upper=upper, weights=grf_weights(N, seed, corr, logstd))
+
+
 # ------------------------------------------------------------- batched blocks
 def block_diag(blocks):
     """Assemble CSC blocks (Ap, Ai, Ax) block-diagonally."""
