@@ -89,7 +89,19 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--paths", default="fp64,chosen")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES (reads BYTES, writes BYTES): the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(args.copy_calibration // 4, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": args.copy_calibration}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not supported"}))
         sys.exit(2)
@@ -125,10 +137,17 @@ def main():
     solve_gbs = None
     if hi.get("t_solve", 0) > 0 and hi.get("solves", 0) > 0:
         solve_gbs = hi["bytes_solve"] * hi["solves"] / hi["t_solve"] / 1e9
+    block = hi.get("precision_used") == 2
     kname = ("multi-right-hand-side FP32 supernodal solve (forward + backward, split-K MFMA tiles)"
-             if hi.get("precision_used") == 2 else "supernodal triangular solve pair (forward + backward)")
+             if block else "supernodal triangular solve pair (forward + backward)")
+    # HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs recorded under profiles/
+    traffic = None
+    tfile = ROOT / "profiles" / ("traffic_block_solve_weighted_lap3d_100.json" if block else "")
+    if block and tfile.is_file() and args.N == 100:
+        traffic = json.loads(tfile.read_text()).get("traffic_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(solve_gbs, 1) if solve_gbs else None, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(solve_gbs / HBM_PEAK_GBS, 4) if solve_gbs else None, "traffic": None,
+                "unit": "GB/s", "frac": round(solve_gbs / HBM_PEAK_GBS, 4) if solve_gbs else None,
+                "traffic": round(traffic) if traffic else None,
                 "kernel": kname, "bytes_per_launch": hi.get("bytes_solve"), "launches": hi.get("solves"),
                 "avg_ms": round(1e3 * hi["t_solve"] / hi["solves"], 3) if hi.get("solves") else None}
     cfg = {"workload": f"{args.N}^3 variable-coefficient 3-D Laplacian (weights log-uniform [0.1,10], seed "
