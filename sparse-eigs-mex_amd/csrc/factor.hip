@@ -245,6 +245,118 @@ __global__ void __launch_bounds__(256) k_update(T *pool, const TileTask *__restr
     }
 }
 
+// ---- panel factorization on one wave -----------------------------------------
+// The diagonal tile of panel k lives in registers: lane r holds row r (a[q] =
+// L(r, q), q <= r).  Column p: the pivot comes from lane p by readlane, the
+// column is scaled, and the trailing rows are updated with the column entries
+// broadcast by readlane -- no barrier, no LDS.  Then the inverse X = L_kk^-1 is
+// formed column-wise (lane j holds column j of X) with the rows of L broadcast
+// the same way, and stored to the panel's scratch slot for the TRSM product.
+__device__ __forceinline__ float bcast(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ double bcast(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <class T>
+__global__ void __launch_bounds__(64) k_potrf_reg(T *pool, const TileTask *__restrict__ tasks,
+                                                  const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
+                                                  const int32_t *__restrict__ ncol, const int64_t *__restrict__ col0,
+                                                  const double *__restrict__ diagA, double delta, double piv_rel,
+                                                  int *flag, T *__restrict__ xinv)
+{
+    const TileTask t = tasks[blockIdx.x];
+    const int32_t F = Fs[t.s], c = ncol[t.s];
+    const int k0 = t.k * TB, w = min(TB, c - k0);
+    const int r = threadIdx.x;
+    T *base = pool + foff[t.s] + (int64_t)k0 * F + k0;
+    T a[TB];
+#pragma unroll
+    for (int q = 0; q < TB; q++) a[q] = (r < w && q < w && q <= r) ? base[(int64_t)q * F + r] : T(q == r ? 1 : 0);
+    bool bad = false;
+    const double *dA = diagA + col0[t.s] + k0;
+#pragma unroll
+    for (int p = 0; p < TB; p++) {
+        if (p < w) {
+            T d = bcast(a[p], p);
+            const double ref = fabs(dA[p] + delta);
+            if (!((double)d > piv_rel * ref) || !(d == d)) {
+                bad = true;
+                d = T(1);
+            }
+            const T sd = sqrt(d);
+            a[p] = (r == p) ? sd : (r > p ? a[p] / sd : a[p]);
+#pragma unroll
+            for (int q = p + 1; q < TB; q++) {
+                const T lq = bcast(a[p], q);          // L(q, p)
+                if (r >= q) a[q] -= a[p] * lq;
+            }
+        }
+    }
+    if (bad && r == 0) atomicOr(flag, 1);
+#pragma unroll
+    for (int q = 0; q < TB; q++)
+        if (r < w && q < w && q <= r) base[(int64_t)q * F + r] = a[q];
+    // X = L^-1 (padded with the identity beyond w): lane j holds column j
+    T x[TB];
+#pragma unroll
+    for (int i = 0; i < TB; i++) {
+        T s = (i == r) ? T(1) : T(0);
+#pragma unroll
+        for (int k = 0; k < i; k++) s -= bcast(a[k], i) * x[k];     // L(i, k) X(k, j)
+        x[i] = (i >= r) ? s / bcast(a[i], i) : T(0);
+    }
+    T *X = xinv + (int64_t)blockIdx.x * (TB * TB);
+#pragma unroll
+    for (int i = 0; i < TB; i++) X[(int64_t)r * TB + i] = x[i];      // column-major 64 x 64
+}
+
+// tile row i below the diagonal block: B <- B X^T with X = L_kk^-1 from the
+// panel's scratch slot t.j (MFMA, the product formed as for the Schur update)
+template <class T>
+__global__ void __launch_bounds__(256) k_trsm_mfma(T *pool, const TileTask *__restrict__ tasks,
+                                                   const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
+                                                   const int32_t *__restrict__ ncol, const T *__restrict__ xinv)
+{
+    __shared__ T Bt[TB][TB + 4];   // Bt[k][row] = B(row, k)
+    __shared__ T Xt[TB][TB + 4];   // Xt[k][col] = X(col, k)
+    typedef typename MfmaAcc<T>::V V;
+    const TileTask t = tasks[blockIdx.x];
+    const int32_t F = Fs[t.s], c = ncol[t.s];
+    const int k0 = t.k * TB, w = min(TB, c - k0);
+    const int r0 = max(t.i * TB, k0 + w), r1 = min(F, (t.i + 1) * TB), nr = r1 - r0;
+    if (nr <= 0) return;
+    T *fr = pool + foff[t.s];
+    const T *X = xinv + (int64_t)t.j * (TB * TB);
+    const int w4 = (w + 3) & ~3;
+    for (int e = threadIdx.x; e < TB * TB; e += 256) {
+        const int rr = e % TB, kk = e / TB;
+        Bt[kk][rr] = (rr < nr && kk < w) ? fr[(int64_t)(k0 + kk) * F + r0 + rr] : T(0);
+        Xt[kk][rr] = (kk < w && rr < w) ? X[(int64_t)kk * TB + rr] : T(0);   // X(rr, kk): column kk of X
+    }
+    __syncthreads();
+    V acc[2][2];
+    zero_acc<T>(acc);
+    tile_mma<T, TB + 4>(Bt, Xt, w4, acc);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int rb = (wv & 1) * 32, cb = (wv >> 1) * 32, lr = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int a2 = 0; a2 < 2; a2++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = rb + 16 * a2 + lr, col = cb + 16 * b + MfmaAcc<T>::colof(lk, v);
+                if (row < nr && col < w) fr[(int64_t)(k0 + col) * F + r0 + row] = acc[a2][b][v];
+            }
+}
+
 // Schur update of tile (i, j) on the matrix cores: each wave owns a 32 x 32
 // quarter of the 64 x 64 tile as 2 x 2 blocks of 16 x 16, K = 4 per MFMA.
 // Lanes 0-15 of each result value hold 16 consecutive rows of one column, so
@@ -802,12 +914,30 @@ int run_numeric(Factor *F, double delta)
             k_extend_add_pcb<T><<<dim3((unsigned)P.pcb_cnt), dim3(256), 0, st>>>(
                 pool, F->pcb + P.pcb_off, F->foff, F->Fs, F->ncol, F->relptr, F->rel, F->chptr, F->chl);
         for (size_t k = 0; k < P.k_potrf_cnt.size(); k++) {
-            if (P.k_potrf_cnt[k] > 0)
-                k_potrf<T><<<dim3((unsigned)P.k_potrf_cnt[k]), dim3(256), 0, st>>>(
-                    pool, F->tiles + P.k_potrf_off[k], F->foff, F->Fs, F->ncol, F->col0, F->diagA, delta, piv_rel, F->flag);
-            if (P.k_trsm_cnt[k] > 0)
-                k_trsm<T><<<dim3((unsigned)P.k_trsm_cnt[k]), dim3(256), 0, st>>>(pool, F->tiles + P.k_trsm_off[k],
-                                                                                 F->foff, F->Fs, F->ncol);
+            if (F->use_reg_panel) {
+                // panel in registers (+ its inverse), then TRSM as a product with the inverse
+                // (in groups of kXinvSlots fronts, the inverses' scratch capacity)
+                T *xinv = (T *)F->xinv;
+                const std::vector<int64_t> &cut = P.k_trsm_cut[k];
+                for (size_t g = 0; g + 1 < cut.size(); g++) {
+                    const int64_t p0 = (int64_t)g * kXinvSlots;
+                    const int64_t pc = std::min(kXinvSlots, P.k_potrf_cnt[k] - p0);
+                    k_potrf_reg<T><<<dim3((unsigned)pc), dim3(64), 0, st>>>(
+                        pool, F->tiles + P.k_potrf_off[k] + p0, F->foff, F->Fs, F->ncol, F->col0, F->diagA, delta,
+                        piv_rel, F->flag, xinv);
+                    if (cut[g + 1] > cut[g])
+                        k_trsm_mfma<T><<<dim3((unsigned)(cut[g + 1] - cut[g])), dim3(256), 0, st>>>(
+                            pool, F->tiles + cut[g], F->foff, F->Fs, F->ncol, xinv);
+                }
+            } else {
+                if (P.k_potrf_cnt[k] > 0)
+                    k_potrf<T><<<dim3((unsigned)P.k_potrf_cnt[k]), dim3(256), 0, st>>>(
+                        pool, F->tiles + P.k_potrf_off[k], F->foff, F->Fs, F->ncol, F->col0, F->diagA, delta, piv_rel,
+                        F->flag);
+                if (P.k_trsm_cnt[k] > 0)
+                    k_trsm<T><<<dim3((unsigned)P.k_trsm_cnt[k]), dim3(256), 0, st>>>(pool, F->tiles + P.k_trsm_off[k],
+                                                                                     F->foff, F->Fs, F->ncol);
+            }
             if (P.k_upd_cnt[k] > 0) {
                 if (F->use_mfma)
                     k_update_mfma<T><<<dim3((unsigned)P.k_upd_cnt[k]), dim3(256), 0, st>>>(
@@ -907,6 +1037,7 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     // debug knobs for A/B checks of kernel variants
     if (const char *e = getenv("SPEIGS_HIP_SOLVE_MODE")) F->solve_mode = atoi(e);
     if (const char *e = getenv("SPEIGS_HIP_MFMA")) F->use_mfma = atoi(e) != 0;
+    if (const char *e = getenv("SPEIGS_HIP_REG_PANEL")) F->use_reg_panel = atoi(e) != 0;
     SymbolicOptions so;
     int rc = symbolic_analyze(n, Ap, Ai, so, &F->S);
     if (rc) return rc;
@@ -987,6 +1118,7 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     std::vector<std::vector<int64_t>> bylevel(S.nlevels);
     for (int64_t s = 0; s < ns; s++) bylevel[S.level[s]].push_back(s);
     F->plan.resize(S.nlevels);
+    int64_t xinv_max = 1;
     for (int32_t L = 0; L < S.nlevels; L++) {
         LevelPlan &P = F->plan[L];
         P.fr_off = (int64_t)fronts.size();
@@ -1012,16 +1144,22 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
             for (int64_t s : bylevel[L])
                 if ((S.c1[s] - S.c0[s]) > k * TB) tiles.push_back({(int32_t)s, (int32_t)k, (int32_t)k, (int32_t)k});
             P.k_potrf_cnt.push_back((int64_t)tiles.size() - P.k_potrf_off.back());
+            xinv_max = std::max(xinv_max, std::min(kXinvSlots, P.k_potrf_cnt.back()));
             P.k_trsm_off.push_back((int64_t)tiles.size());
+            P.k_trsm_cut.emplace_back();
+            int64_t ord = 0;     // the front's potrf task ordinal in this step; its inverse's slot = ord % kXinvSlots
             for (int64_t s : bylevel[L]) {
                 const int64_t c = S.c1[s] - S.c0[s], Fv = S.F[s];
                 if (c <= k * TB) continue;
+                if (ord % kXinvSlots == 0) P.k_trsm_cut.back().push_back((int64_t)tiles.size());
                 const int64_t kend = k * TB + std::min<int64_t>(TB, c - k * TB);
                 const int64_t nt = (Fv + TB - 1) / TB;
                 for (int64_t i = k; i < nt; i++)
                     if (std::max(i * TB, kend) < std::min(Fv, (i + 1) * TB))
-                        tiles.push_back({(int32_t)s, (int32_t)k, (int32_t)i, 0});
+                        tiles.push_back({(int32_t)s, (int32_t)k, (int32_t)i, (int32_t)(ord % kXinvSlots)});
+                ord++;
             }
+            P.k_trsm_cut.back().push_back((int64_t)tiles.size());
             P.k_trsm_cnt.push_back((int64_t)tiles.size() - P.k_trsm_off.back());
             P.k_upd_off.push_back((int64_t)tiles.size());
             for (int64_t s : bylevel[L]) {
@@ -1138,6 +1276,11 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
         if ((rc = F->mem.alloc((double **)&F->pool, (size_t)S.pool))) return rc;
     }
     (void)esz;
+    if (F->fp32) {
+        if ((rc = F->mem.alloc((float **)&F->xinv, (size_t)xinv_max * TB * TB))) return rc;
+    } else {
+        if ((rc = F->mem.alloc((double **)&F->xinv, (size_t)xinv_max * TB * TB))) return rc;
+    }
     trace_phase("pool allocation", tp);
     std::vector<double> axv(Ax, Ax + std::max<int64_t>(Ap[n], 1));
     if (Ap[n] == 0) axv.assign(1, 0.0);

@@ -36,10 +36,15 @@ struct UpdTask {
     int32_t s, p, j, pad;
 };
 
+constexpr int64_t kXinvSlots = 2048;   // diagonal-tile inverses held at once (64 x 64 each)
+
 struct LevelPlan {
     std::vector<int64_t> ea_off;      // per child ordinal: offset in the extend-add task array
     std::vector<int64_t> ea_cnt;
     std::vector<int64_t> k_potrf_off, k_potrf_cnt, k_trsm_off, k_trsm_cnt, k_upd_off, k_upd_cnt;
+    // per panel step: start of each group of kXinvSlots fronts in the TRSM tasks
+    // (plus the end); a group's inverses fit the scratch, slot = ordinal % kXinvSlots
+    std::vector<std::vector<int64_t>> k_trsm_cut;
     int64_t fr_off = 0, fr_cnt = 0;   // fronts of the level in the fronts array
     std::vector<int64_t> sea_off, sea_cnt;   // (unused) per-ordinal solve extend-add
     int64_t pcb_off = 0, pcb_cnt = 0;        // factor extend-add: parent column blocks
@@ -73,6 +78,7 @@ struct Factor {
     int64_t solves = 0, ir_steps = 0;
     bool factored = false;
     bool use_mfma = true;
+    bool use_reg_panel = true;       // one-wave register POTRF + TRSM as a product with L_kk^-1
     int solve_mode = 1;               // 0: one workgroup per front, 1: blocked multi-workgroup
     EaTask *sea = nullptr;
     PcbTask *pcb = nullptr, *fin = nullptr;
@@ -93,6 +99,7 @@ struct Factor {
     double t_solve = 0.0, t_invert = 0.0;
     int64_t block_solves = 0, block_rhs = 0;   // multi-right-hand-side solves (FP32 vectors)
     bool inverted = false;            // L11 of every front holds its inverse
+    void *xinv = nullptr;             // per panel launch: inverses of the diagonal tiles (TRSM)
     void *bs_f = nullptr, *bs_d = nullptr;   // block-solve state (FP32 / FP64 vectors)
     ~Factor()
     {
