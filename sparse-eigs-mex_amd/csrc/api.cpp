@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <thread>
 #include <vector>
 
 #include "dev.h"
@@ -452,6 +453,20 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
     StreamGuard sg;
     if (hipStreamCreate(&sg.s) != hipSuccess) return SPEIGS_ERR_INTERNAL;
     DevMem mem;
+    // the SELL matrix of the SpMVs is built on the host and uploaded on a
+    // stream of its own while the device factorizes (joined before use, and
+    // on every early return before `mem` and the streams go away)
+    SellA sell_pre;
+    int sell_rc = 0;
+    StreamGuard sell_sg;
+    std::thread sell_th;
+    struct Joiner {
+        std::thread &t;
+        ~Joiner()
+        {
+            if (t.joinable()) t.join();
+        }
+    } sell_join{sell_th};
     std::unique_ptr<MultOp> op;
     int rc = SPEIGS_OK;
 
@@ -479,6 +494,8 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
             rc = factor_create(n, Ap, Ai, Ax, fopt, sg.s, &fo->F, hinfo);
             if (rc) return rc;
             info->t_analyze = wall() - ta;
+            if (hipStreamCreate(&sell_sg.s) != hipSuccess) return SPEIGS_ERR_INTERNAL;
+            sell_th = std::thread([&]() { sell_rc = sell_build(n, Ap, Ai, Ax, &sell_pre, mem, sell_sg.s); });
             double tf = wall();
             double delta = 0.0;
             int st = factor_numeric(fo->F, 0.0);
@@ -525,7 +542,13 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
     op->n = n;
     op->stream = sg.s;
     double tu = wall();
-    if ((rc = sell_build(n, Ap, Ai, Ax, &op->A, mem, sg.s))) return rc;
+    if (sell_th.joinable()) {
+        sell_th.join();
+        if (sell_rc) return sell_rc;
+        op->A = sell_pre;
+    } else if ((rc = sell_build(n, Ap, Ai, Ax, &op->A, mem, sg.s))) {
+        return rc;
+    }
     hinfo->t_upload += wall() - tu;
     trace_phase("SELL build + upload", tp);
     hinfo->bytes_spmv = spmv_bytes(op->A);
@@ -909,7 +932,6 @@ struct speigs_hip_op {
     SellA A;
     Factor *F = nullptr;
     double *x = nullptr, *y = nullptr;
-    float *xf = nullptr, *yf = nullptr;
     int64_t cap = 0;
     int precision = 0;
     ~speigs_hip_op()
@@ -973,10 +995,6 @@ static int op_buffers(speigs_hip_op *op, int64_t nvec)
     int rc;
     if ((rc = op->mem.alloc(&op->x, (size_t)ld * nvec))) return rc;
     if ((rc = op->mem.alloc(&op->y, (size_t)ld * nvec))) return rc;
-    if (op->precision == SPEIGS_HIP_FP32_BLOCK) {
-        if ((rc = op->mem.alloc(&op->xf, (size_t)ld * nvec))) return rc;
-        if ((rc = op->mem.alloc(&op->yf, (size_t)ld * nvec))) return rc;
-    }
     op->cap = nvec;
     return 0;
 }
@@ -1015,9 +1033,7 @@ int speigs_hip_op_apply(speigs_hip_op *op, const double *x, double *y, speigs_in
             if (generic) {
                 if ((rc = factor_block_solve_d(op->F, xb, yb, nb))) return rc;
             } else {
-                block_to_f32(xb, op->n, op->xf, nb, op->n, op->stream);
-                if ((rc = factor_block_solve(op->F, op->xf, op->yf, nb))) return rc;
-                block_to_f64(op->yf, yb, op->n, nb, op->n, op->stream);
+                if ((rc = factor_block_solve_rows(op->F, xb, 1, op->n, yb, 1, op->n, nb))) return rc;
             }
         }
     } else {

@@ -290,16 +290,6 @@ __global__ void __launch_bounds__(256) k_rtrans(const S *__restrict__ in, int64_
     }
 }
 
-void rto_f32cols(const double *X, int64_t ldx, int m, int64_t n, float *Y, hipStream_t st)
-{
-    k_rtrans<double, float, true><<<dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st>>>(X, ldx, m, n, Y);
-}
-
-void rfrom_f32cols(const float *Y, int m, int64_t n, double *X, int64_t ldx, hipStream_t st)
-{
-    k_rtrans<float, double, false><<<dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st>>>(Y, ldx, m, n, X);
-}
-
 void rto_cols(const double *X, int64_t ldx, int m, int64_t n, double *Y, hipStream_t st)
 {
     k_rtrans<double, double, true><<<dim3((unsigned)((n + 63) / 64)), dim3(256), 0, st>>>(X, ldx, m, n, Y);

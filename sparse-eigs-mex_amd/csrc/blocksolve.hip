@@ -192,8 +192,17 @@ static int invert_impl(Factor *F)
     const Symbolic &S = F->S;
     hipStream_t st = F->st;
     T *pool = (T *)F->pool;
-    std::vector<std::vector<int64_t>> bylevel(S.nlevels);
-    for (int64_t s = 0; s < S.ns; s++) bylevel[S.level[s]].push_back(s);
+    // the fronts' inverses are independent: step st2 handles tile J = nb - 1 - st2
+    // of every front with more than st2 tiles, whatever its tree level, so the
+    // number of steps is the largest front's tile count.  Fronts are taken
+    // largest first so that their long part tasks start early in each launch.
+    std::vector<int64_t> order(S.ns);
+    for (int64_t s = 0; s < S.ns; s++) order[s] = s;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+        return (S.c1[x] - S.c0[x]) > (S.c1[y] - S.c0[y]);
+    });
+    int64_t nbmax = 0;
+    for (int64_t s = 0; s < S.ns; s++) nbmax = std::max<int64_t>(nbmax, (S.c1[s] - S.c0[s] + TB - 1) / TB);
     // all task lists of all steps, uploaded once
     std::vector<InvTask> diag;
     std::vector<InvPart> parts;
@@ -201,17 +210,15 @@ static int invert_impl(Factor *F)
     struct Step { int64_t d_off, d_cnt, p_off, p_cnt, f_off, f_cnt; };
     std::vector<Step> steps;
     int64_t pmax = 1;
-    for (int32_t L = 0; L < S.nlevels; L++) {
-        int64_t nbmax = 0;
-        for (int64_t s : bylevel[L]) nbmax = std::max<int64_t>(nbmax, (S.c1[s] - S.c0[s] + TB - 1) / TB);
+    {
         for (int64_t st2 = 0; st2 < nbmax; st2++) {
             Step sp;
             sp.d_off = (int64_t)diag.size();
             sp.p_off = (int64_t)parts.size();
             sp.f_off = (int64_t)fins.size();
-            for (int64_t s : bylevel[L]) {
+            for (int64_t s : order) {
                 const int64_t nb = (S.c1[s] - S.c0[s] + TB - 1) / TB;
-                if (st2 >= nb) continue;
+                if (st2 >= nb) break;        // sorted by size: no later front has this step
                 const int32_t J = (int32_t)(nb - 1 - st2);
                 diag.push_back({(int32_t)s, J, J, 0});
                 for (int32_t I = J + 1; I < nb; I++) {
@@ -425,150 +432,6 @@ __global__ void __launch_bounds__(256) k_bs_tile(const T *__restrict__ pool, con
     }
 }
 
-// ---- fast path: FP32 vectors, split-K MFMA tile products + fixed-order epilogue
-// Part task (front s, output tile I, K tiles [k0, k1)): P (64 rows x 32
-// right-hand sides) = sum_K op(A)_IK X_K on v_mfma_f32_16x16x4f32, formed as
-// P^T = X^T op(A)^T: wave w owns rows 16w..16w+15 and two 16-column blocks h of
-// right-hand sides; lane l supplies X[k + l/16][16h + l%16] and
-// op(A)[16w + l%16][k + l/16] and receives P[16w + l%16][16h + 4(l/16) + v], so
-// lanes 0..15 of a result register are 16 consecutive rows of one column.
-// A task that is the only part of its output tile (direct = 1) applies the
-// epilogue itself; otherwise P goes to its partial slot (column-major 64 x 32)
-// and a finish task sums the slots in a fixed order.
-struct PartTask {
-    int32_t s, I, k0, k1, direct, pad;
-};
-struct FinTask {
-    int32_t s, I;
-    int64_t p0;      // first part task of this output tile
-    int32_t np, pad;
-};
-
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-// epilogue of output rows orow0 + r of front s for right-hand side v
-template <int MODE>
-__device__ __forceinline__ void bsf_store(float val, int r, int v, int32_t F, int64_t c0, int orow0, int64_t n,
-                                          float *__restrict__ w, float *__restrict__ Yg, float *__restrict__ Xg)
-{
-    if (MODE == 0) Yg[(int64_t)v * n + c0 + orow0 + r] = val;
-    else if (MODE == 1) w[(int64_t)v * F + orow0 + r] -= val;
-    else if (MODE == 2) w[(int64_t)v * F + orow0 + r] = Yg[(int64_t)v * n + c0 + orow0 + r] - val;
-    else Xg[(int64_t)v * n + c0 + orow0 + r] = val;
-}
-
-template <class T, int MODE>
-__global__ void __launch_bounds__(256) k_bsf_part(const T *__restrict__ pool, const PartTask *__restrict__ tasks,
-                                                  BsArgs a, int64_t n, int nb, float *__restrict__ ws,
-                                                  float *__restrict__ Yg, float *__restrict__ Xg,
-                                                  float *__restrict__ part)
-{
-    __shared__ float At[TB][TB + 4];
-    __shared__ float Xs[TB][48];
-    const PartTask t = tasks[blockIdx.x];
-    const int s = t.s, I = t.I;
-    const int32_t F = a.Fs[s], c = a.ncol[s];
-    const T *L = pool + a.foff[s];
-    float *w = ws + a.wsoff[s] * NBMAX;
-    const int64_t c0 = a.col0[s];
-    int orow0, on;
-    if (MODE == 1) { orow0 = c + I * TB; on = min(TB, F - orow0); }
-    else { orow0 = I * TB; on = min(TB, c - orow0); }
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int lr = lane & 15, lk = lane >> 4;
-    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-    for (int K = t.k0; K < t.k1; K++) {
-        int krow0, kn;
-        if (MODE == 2) { krow0 = c + K * TB; kn = min(TB, F - krow0); }
-        else { krow0 = K * TB; kn = min(TB, c - krow0); }
-        __syncthreads();
-        for (int e = threadIdx.x; e < TB * TB; e += 256) {
-            const int p = e & 63, q = e >> 6;
-            if (MODE == 0 || MODE == 1) {   // At[p][q] = L(orow0 + p, krow0 + q)
-                float av = 0.f;
-                if (p < on && q < kn && (MODE == 1 || K < I || p >= q)) av = (float)L[(int64_t)(krow0 + q) * F + orow0 + p];
-                At[p][q] = av;
-            } else {                        // At[q][p] = L(krow0 + p, orow0 + q)
-                float av = 0.f;
-                if (p < kn && q < on && (MODE == 2 || K > I || p >= q)) av = (float)L[(int64_t)(orow0 + q) * F + krow0 + p];
-                At[q][p] = av;
-            }
-        }
-        for (int e = threadIdx.x; e < TB * 32; e += 256) {
-            const int kk = e & 63, v = e >> 6;
-            float xv = 0.f;
-            if (kk < kn && v < nb) {
-                if (MODE == 1) xv = Yg[(int64_t)v * n + c0 + krow0 + kk];
-                else xv = w[(int64_t)v * F + krow0 + kk];
-            }
-            Xs[kk][v] = xv;
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int k = 0; k < TB; k += 4) {
-            const float av = At[16 * wv + lr][k + lk];
-            const float b0 = Xs[k + lk][lr], b1 = Xs[k + lk][16 + lr];
-            acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(b0, av, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(b1, av, acc1, 0, 0, 0);
-        }
-    }
-    const int row = 16 * wv + lr;
-    if (t.direct) {
-        if (row < on) {
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int v0 = 4 * lk + v, v1 = 16 + 4 * lk + v;
-                if (v0 < nb) bsf_store<MODE>(acc0[v], row, v0, F, c0, orow0, n, w, Yg, Xg);
-                if (v1 < nb) bsf_store<MODE>(acc1[v], row, v1, F, c0, orow0, n, w, Yg, Xg);
-            }
-        }
-        return;
-    }
-    float *out = part + (int64_t)blockIdx.x * (TB * 32);
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-        out[(4 * lk + v) * TB + row] = acc0[v];
-        out[(16 + 4 * lk + v) * TB + row] = acc1[v];
-    }
-}
-
-template <int MODE>
-__global__ void __launch_bounds__(256) k_bsf_fin(const FinTask *__restrict__ tasks, BsArgs a, int64_t n, int nb,
-                                                 float *__restrict__ ws, float *__restrict__ Yg,
-                                                 float *__restrict__ Xg, const float *__restrict__ part)
-{
-    const FinTask t = tasks[blockIdx.x];
-    const int s = t.s, I = t.I;
-    const int32_t F = a.Fs[s], c = a.ncol[s];
-    float *w = ws + a.wsoff[s] * NBMAX;
-    const int64_t c0 = a.col0[s];
-    int orow0, on;
-    if (MODE == 1) { orow0 = c + I * TB; on = min(TB, F - orow0); }
-    else { orow0 = I * TB; on = min(TB, c - orow0); }
-    for (int e = threadIdx.x; e < TB * 32; e += 256) {
-        const int r = e & 63, v = e >> 6;
-        if (r >= on || v >= nb) continue;
-        float sum = 0.f;
-        for (int q = 0; q < t.np; q++) sum += part[(t.p0 + q) * (TB * 32) + v * TB + r];
-        bsf_store<MODE>(sum, r, v, F, c0, orow0, n, w, Yg, Xg);
-    }
-}
-
-// backward: ws rows >= c of each front of the level = X at the front's update rows
-__global__ void __launch_bounds__(256) k_bsf_gather(const RowTask *__restrict__ tasks, BsArgs a, int64_t n, int nb,
-                                                    float *__restrict__ ws, const float *__restrict__ Xg)
-{
-    const RowTask t = tasks[blockIdx.x];
-    const int s = t.s;
-    const int32_t F = a.Fs[s], c = a.ncol[s];
-    float *w = ws + a.wsoff[s] * NBMAX;
-    const int32_t *R = a.rows + a.rptr[s];
-    const int i = c + t.I * 256 + threadIdx.x;
-    if (i >= F) return;
-    const int64_t g = R[i - c];
-    for (int v = 0; v < nb; v++) w[(int64_t)v * F + i] = Xg[(int64_t)v * n + g];
-}
-
 template <class V>
 __global__ void k_bs_perm(const V *__restrict__ src, const int64_t *__restrict__ perm, V *__restrict__ dst, int64_t n,
                           int nb, int gather)
@@ -595,131 +458,7 @@ struct BsState {
     RowTask *tasks = nullptr;
     BsPlan plan;
     V *ws = nullptr, *Pg = nullptr, *Yg = nullptr, *Xg = nullptr;
-    // fast path (V = float)
-    PartTask *pt = nullptr;
-    FinTask *ft = nullptr;
-    float *part = nullptr;
-    RowTask *gt = nullptr;
-    std::vector<int64_t> pt_off[4], pt_cnt[4], ft_off[4], ft_cnt[4], gt_off, gt_cnt;
 };
-
-constexpr int KCHUNK = 6;   // K tiles per part task
-
-// part/fin task lists of one mode for the fronts of a level
-void plan_mode(const Symbolic &S, const std::vector<int64_t> &fronts, int mode, std::vector<PartTask> &pt,
-               std::vector<FinTask> &ft, int64_t pbase)
-{
-    for (int64_t s : fronts) {
-        const int64_t c = S.c1[s] - S.c0[s], F = S.F[s];
-        const int64_t nt11 = (c + TB - 1) / TB, nupd = (F - c + TB - 1) / TB;
-        const int64_t nout = mode == 1 ? nupd : nt11;
-        for (int64_t I = 0; I < nout; I++) {
-            int64_t kb, ke;
-            if (mode == 0) { kb = 0; ke = I + 1; }
-            else if (mode == 1) { kb = 0; ke = nt11; }
-            else if (mode == 2) { kb = 0; ke = nupd; }
-            else { kb = I; ke = nt11; }
-            // a front without update rows still needs Z = Y (a finish with no parts)
-            if (ke <= kb && mode != 2) continue;
-            const int64_t np = (ke - kb + KCHUNK - 1) / KCHUNK;
-            FinTask f;
-            f.s = (int32_t)s;
-            f.I = (int32_t)I;
-            f.p0 = (int64_t)pt.size() - pbase;
-            f.pad = 0;
-            for (int64_t k0 = kb; k0 < ke; k0 += KCHUNK)
-                pt.push_back({(int32_t)s, (int32_t)I, (int32_t)k0, (int32_t)std::min<int64_t>(ke, k0 + KCHUNK),
-                              np == 1 ? 1 : 0, 0});
-            f.np = (int32_t)np;
-            if (np != 1) ft.push_back(f);
-        }
-    }
-}
-
-int bsf_prepare(Factor *F, BsState<float> &B)
-{
-    const Symbolic &S = F->S;
-    std::vector<std::vector<int64_t>> bylevel(S.nlevels);
-    for (int64_t s = 0; s < S.ns; s++) bylevel[S.level[s]].push_back(s);
-    std::vector<PartTask> pt;
-    std::vector<FinTask> ft;
-    std::vector<RowTask> gt;
-    int64_t maxpart = 1;
-    for (int32_t L = 0; L < S.nlevels; L++) {
-        for (int mode = 0; mode < 4; mode++) {
-            const int64_t p0 = (int64_t)pt.size(), f0 = (int64_t)ft.size();
-            // part indices of fin tasks are relative to the mode's launch
-            plan_mode(S, bylevel[L], mode, pt, ft, p0);
-            B.pt_off[mode].push_back(p0);
-            B.pt_cnt[mode].push_back((int64_t)pt.size() - p0);
-            B.ft_off[mode].push_back(f0);
-            B.ft_cnt[mode].push_back((int64_t)ft.size() - f0);
-            maxpart = std::max<int64_t>(maxpart, (int64_t)pt.size() - p0);
-        }
-        B.gt_off.push_back((int64_t)gt.size());
-        for (int64_t s : bylevel[L]) {
-            const int64_t c = S.c1[s] - S.c0[s], r = S.F[s] - c;
-            for (int64_t ck = 0; ck * 256 < r; ck++) gt.push_back({(int32_t)s, (int32_t)ck});
-        }
-        B.gt_cnt.push_back((int64_t)gt.size() - B.gt_off.back());
-    }
-    if (pt.empty()) pt.push_back({0, 0, 0, 0, 0, 0});
-    if (ft.empty()) ft.push_back({0, 0, 0, 0, 0});
-    if (gt.empty()) gt.push_back({0, 0});
-    int rc;
-    if ((rc = B.mem.alloc(&B.pt, pt.size()))) return rc;
-    if ((rc = B.mem.alloc(&B.ft, ft.size()))) return rc;
-    if ((rc = B.mem.alloc(&B.gt, gt.size()))) return rc;
-    if ((rc = B.mem.alloc(&B.part, (size_t)maxpart * TB * 32))) return rc;
-    SEHIP_CHECK(hipMemcpyAsync(B.pt, pt.data(), pt.size() * sizeof(PartTask), hipMemcpyHostToDevice, F->st));
-    SEHIP_CHECK(hipMemcpyAsync(B.ft, ft.data(), ft.size() * sizeof(FinTask), hipMemcpyHostToDevice, F->st));
-    SEHIP_CHECK(hipMemcpyAsync(B.gt, gt.data(), gt.size() * sizeof(RowTask), hipMemcpyHostToDevice, F->st));
-    SEHIP_CHECK(hipStreamSynchronize(F->st));
-    return 0;
-}
-
-template <class T>
-int bsf_run(Factor *F, BsState<float> &B, const float *Bin, float *Xout, int nb)
-{
-    const Symbolic &S = F->S;
-    hipStream_t st = F->st;
-    const T *pool = (const T *)F->pool;
-    BsArgs a{F->foff, F->wsoff, F->col0, F->rptr, F->relptr, F->chptr, F->Fs, F->ncol, F->rows, F->rel, F->chl};
-    const unsigned nbk = (unsigned)((F->n + 255) / 256);
-    k_bs_perm<float><<<dim3(nbk), dim3(256), 0, st>>>(Bin, F->perm, B.Pg, F->n, nb, 1);
-    auto mode_launch = [&](int mode, int32_t L) {
-        const int64_t pc = B.pt_cnt[mode][L], fc = B.ft_cnt[mode][L];
-        const PartTask *pt = B.pt + B.pt_off[mode][L];
-        const FinTask *ft = B.ft + B.ft_off[mode][L];
-        const dim3 gp((unsigned)pc), gf((unsigned)fc), blk(256);
-        switch (mode) {
-        case 0: if (pc) k_bsf_part<T, 0><<<gp, blk, 0, st>>>(pool, pt, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part);
-                if (fc) k_bsf_fin<0><<<gf, blk, 0, st>>>(ft, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part); break;
-        case 1: if (pc) k_bsf_part<T, 1><<<gp, blk, 0, st>>>(pool, pt, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part);
-                if (fc) k_bsf_fin<1><<<gf, blk, 0, st>>>(ft, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part); break;
-        case 2: if (pc) k_bsf_part<T, 2><<<gp, blk, 0, st>>>(pool, pt, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part);
-                if (fc) k_bsf_fin<2><<<gf, blk, 0, st>>>(ft, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part); break;
-        default: if (pc) k_bsf_part<T, 3><<<gp, blk, 0, st>>>(pool, pt, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part);
-                 if (fc) k_bsf_fin<3><<<gf, blk, 0, st>>>(ft, a, F->n, nb, B.ws, B.Yg, B.Xg, B.part);
-        }
-    };
-    for (int32_t L = 0; L < S.nlevels; L++) {
-        const BsPlan &P = B.plan;
-        if (P.init_cnt[L])
-            k_bs_init<float><<<dim3((unsigned)P.init_cnt[L]), dim3(256), 0, st>>>(B.tasks + P.init_off[L], a, B.Pg,
-                                                                                  F->n, nb, B.ws);
-        mode_launch(0, L);
-        mode_launch(1, L);
-    }
-    for (int32_t L = S.nlevels - 1; L >= 0; L--) {
-        if (B.gt_cnt[L])
-            k_bsf_gather<<<dim3((unsigned)B.gt_cnt[L]), dim3(256), 0, st>>>(B.gt + B.gt_off[L], a, F->n, nb, B.ws, B.Xg);
-        mode_launch(2, L);
-        mode_launch(3, L);
-    }
-    k_bs_perm<float><<<dim3(nbk), dim3(256), 0, st>>>(B.Xg, F->perm, Xout, F->n, nb, 0);
-    return 0;
-}
 
 template <class V>
 int bs_prepare(Factor *F, BsState<V> &B)
@@ -794,25 +533,6 @@ int bs_run(Factor *F, BsState<V> &B, const V *Bin, V *Xout, int nb)
 
 }  // namespace
 
-int factor_block_solve(Factor *F, const float *Bin, float *Xout, int nb)
-{
-    if (nb < 1 || nb > NBMAX) return SPEIGS_ERR_ARG;
-    if (!F->inverted) return SPEIGS_ERR_FACTOR;
-    if (!F->bs_f) F->bs_f = new BsState<float>();
-    BsState<float> &B = *reinterpret_cast<BsState<float> *>(F->bs_f);
-    if (!B.ready) {
-        int rc = bs_prepare(F, B);
-        if (rc) return rc;
-        if ((rc = bsf_prepare(F, B))) return rc;
-    }
-    const int e = ev_begin(F);
-    const int rc = F->fp32 ? bsf_run<float>(F, B, Bin, Xout, nb) : bsf_run<double>(F, B, Bin, Xout, nb);
-    ev_end(F, e);
-    F->block_solves++;
-    F->block_rhs += nb;
-    return rc;
-}
-
 int factor_block_solve_d(Factor *F, const double *Bin, double *Xout, int nb)
 {
     if (nb < 1 || nb > NBMAX) return SPEIGS_ERR_ARG;
@@ -826,9 +546,9 @@ int factor_block_solve_d(Factor *F, const double *Bin, double *Xout, int nb)
 
 void factor_block_free(Factor *F)
 {
-    delete reinterpret_cast<BsState<float> *>(F->bs_f);
     delete reinterpret_cast<BsState<double> *>(F->bs_d);
-    F->bs_f = F->bs_d = nullptr;
+    F->bs_d = nullptr;
+    rowsolve_free(F);
 }
 
 }  // namespace sehip

@@ -105,10 +105,6 @@ void axpby_residual(const double *b, const double *Ax, const double *x, double d
 // x += d
 void vadd(double *x, const double *d, int64_t n, hipStream_t st);
 
-// ------------------------------------------------------------ block kernels
-void block_to_f32(const double *X, int64_t ld, float *Y, int m, int64_t n, hipStream_t st);
-void block_to_f64(const float *Y, double *X, int64_t ld, int m, int64_t n, hipStream_t st);
-
 // ------------------------------------------------------------ row-major multivectors
 // n x w blocks with element (r, c) at X[r * ld + c] (blockops.hip)
 int64_t rgram_blocks(int64_t n);
@@ -126,8 +122,6 @@ void rresid(const double *AX, int64_t ldax, const double *X, int64_t ldx, const 
             double *R, int64_t ldr, hipStream_t st);
 void rcopy(const double *X, int64_t ldx, int m, int64_t n, double *Y, int64_t ldy, hipStream_t st);
 // layout changes against column-major (ld n) blocks, m <= 96
-void rto_f32cols(const double *X, int64_t ldx, int m, int64_t n, float *Y, hipStream_t st);
-void rfrom_f32cols(const float *Y, int m, int64_t n, double *X, int64_t ldx, hipStream_t st);
 void rto_cols(const double *X, int64_t ldx, int m, int64_t n, double *Y, hipStream_t st);
 void rfrom_cols(const double *Y, int m, int64_t n, double *X, int64_t ldx, hipStream_t st);
 // row-major FP32 (ld ldx) -> row-major FP64 (ld ldy), m columns

@@ -512,27 +512,4 @@ void vadd(double *x, const double *d, int64_t n, hipStream_t st)
     k_vadd<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st>>>(x, d, n);
 }
 
-// ------------------------------------------------------------------ block kernels
-// FP64 (ld) <-> FP32 (ld n) block conversion
-__global__ void k_to_f32(const double *X, int64_t ld, float *Y, int m, int64_t n)
-{
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    for (int t = 0; t < m; t++) Y[(int64_t)t * n + i] = (float)X[(int64_t)t * ld + i];
-}
-__global__ void k_to_f64(const float *Y, double *X, int64_t ld, int m, int64_t n)
-{
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    for (int t = 0; t < m; t++) X[(int64_t)t * ld + i] = (double)Y[(int64_t)t * n + i];
-}
-void block_to_f32(const double *X, int64_t ld, float *Y, int m, int64_t n, hipStream_t st)
-{
-    k_to_f32<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st>>>(X, ld, Y, m, n);
-}
-void block_to_f64(const float *Y, double *X, int64_t ld, int m, int64_t n, hipStream_t st)
-{
-    k_to_f64<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st>>>(Y, X, ld, m, n);
-}
-
 }  // namespace sehip

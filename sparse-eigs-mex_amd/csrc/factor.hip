@@ -1177,6 +1177,17 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
         }
     }
     trace_phase("factor task lists", tp);
+    if (trace_on())
+        for (int32_t L = 0; L < S.nlevels; L++) {
+            int64_t sc = 0, sr = 0, mf = 0;
+            for (int64_t s : bylevel[L]) {
+                sc += S.c1[s] - S.c0[s];
+                sr += S.F[s] - (S.c1[s] - S.c0[s]);
+                mf = std::max<int64_t>(mf, S.F[s]);
+            }
+            fprintf(stderr, "[speigs_hip] level %2d: %6zu fronts, sum c %8lld, sum r %9lld, max F %6lld\n", (int)L,
+                    bylevel[L].size(), (long long)sc, (long long)sr, (long long)mf);
+        }
     // blocked solve plans: forward row blocks in increasing order per front,
     // backward panels in decreasing order per front (ticket = dependency order)
     std::vector<EaTask> sea;

@@ -100,7 +100,7 @@ struct Factor {
     int64_t block_solves = 0, block_rhs = 0;   // multi-right-hand-side solves (FP32 vectors)
     bool inverted = false;            // L11 of every front holds its inverse
     void *xinv = nullptr;             // per panel launch: inverses of the diagonal tiles (TRSM)
-    void *bs_f = nullptr, *bs_d = nullptr;   // block-solve state (FP32 / FP64 vectors)
+    void *rs = nullptr, *bs_d = nullptr;     // block-solve state (row-major FP32 / generic FP64 vectors)
     ~Factor()
     {
         if (hpart) (void)hipHostFree(hpart);
@@ -115,9 +115,12 @@ void ev_end(Factor *F, int i);
 
 // in-place inverse of every front's L11 (chain-free solves); 0 ok
 int factor_invert_l11(Factor *F);
-// multi-right-hand-side solve with the inverted factor: X = (L L^T)^-1 B for
-// nb <= 32 FP32 columns (ld n); requires factor_invert_l11
-int factor_block_solve(Factor *F, const float *B, float *X, int nb);
+// X = (L L^T)^-1 B for nb <= 32 right-hand sides, FP32 arithmetic, after
+// factor_invert_l11; element (i, v) of B at B[i * brs + v * bcs] (likewise X,
+// which may alias B with the same strides)
+int factor_block_solve_rows(Factor *F, const double *B, int64_t brs, int64_t bcs, double *X, int64_t xrs,
+                            int64_t xcs, int nb);
+void rowsolve_free(Factor *F);
 int factor_block_solve_d(Factor *F, const double *B, double *X, int nb);
 void factor_block_free(Factor *F);
 

@@ -13,8 +13,10 @@
 //   X <- smallest m Ritz vectors, P <- their component outside X.
 // Robustness: [W P] is projected out of X and made orthonormal by SVQB
 // (Stathopoulos and Wu 2002: scale, eigendecompose the Gram matrix, drop
-// directions below 1e-12), and the pair of steps is repeated, so S stays
-// orthonormal to rounding and the projected problem is a standard symmetric one.
+// directions below 1e-12), and the pair of steps is repeated when the first can
+// have fallen short (cancellation in the projection, a dropped direction, an
+// ill-conditioned Gram matrix), so S stays orthonormal to rounding and the
+// projected problem is a standard symmetric one.
 // Blocks are row-major in HBM (see LWork).
 // Convergence is the reference's test: ||A x - lambda x||_2 <= tol*max(||A||_1,1)
 // for the k wanted pairs (src/speigs_core.c:243-265).
@@ -59,7 +61,8 @@ int dgemm_tall(LWork &W, const double *V, int64_t ldv, int m, const std::vector<
 }
 
 // Z (columns m .. m+nz of S) <- Z - X (X^T Z); returns Z^T Z after the update
-int project_out(LWork &W, double *S, int64_t ldS, int m, int nz, std::vector<double> &zz)
+// and, in *keep, the smallest ratio of a column's norm after to before
+int project_out(LWork &W, double *S, int64_t ldS, int m, int nz, std::vector<double> &zz, double *keep = nullptr)
 {
     double *Z = S + m;
     std::vector<double> g;
@@ -79,12 +82,21 @@ int project_out(LWork &W, double *S, int64_t ldS, int m, int nz, std::vector<dou
             for (int q = 0; q < m; q++) acc += gx[(size_t)i * m + q] * gx[(size_t)j * m + q];
             zz[(size_t)j * nz + i] -= acc;
         }
+    if (keep) {
+        double kmin = 1.0;
+        for (int j = 0; j < nz; j++) {
+            const double b = g[(size_t)j * ld + m + j], a = zz[(size_t)j * nz + j];
+            if (b > 0) kmin = std::min(kmin, std::sqrt(std::max(a, 0.0) / b));
+        }
+        *keep = kmin;
+    }
     return dgemm_tall(W, S, ldS, m, gx, nz, Z, ldS, 1.0);
 }
 
 // SVQB from the Gram matrix zz of Z: out = Z D U S^-1/2 (kept directions); returns r
+// and, in *cond, the spread of the kept eigenvalues of the scaled Gram matrix
 int svqb(LWork &W, const double *Z, int64_t ldz, int nz, const std::vector<double> &zz, double *out, int64_t ldo,
-         int *r_out)
+         int *r_out, double *cond = nullptr)
 {
     std::vector<double> d(nz);
     for (int i = 0; i < nz; i++) {
@@ -98,6 +110,7 @@ int svqb(LWork &W, const double *Z, int64_t ldz, int nz, const std::vector<doubl
     const double smax = ev[nz - 1];
     int r = 0;
     for (int j = nz - 1; j >= 0 && smax > 0 && ev[j] > 1e-12 * smax; j--) r++;
+    if (cond) *cond = r > 0 ? smax / ev[nz - r] : 0.0;
     std::vector<double> T((size_t)nz * r, 0.0);
     for (int c = 0; c < r; c++) {
         const int j = nz - 1 - c;
@@ -141,7 +154,6 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
     W.st = st;
     const int64_t ldS = (3 * m + 3) / 4 * 4, ldT = (2 * m + 3) / 4 * 4;
     double *S, *AS, *Sn, *ASn, *Pb, *T2, *v0;
-    float *Rf, *Wf;
     if ((rc = W.mem.alloc(&S, (size_t)n * ldS))) return rc;     // [X | W | P]
     if ((rc = W.mem.alloc(&AS, (size_t)n * ldS))) return rc;
     if ((rc = W.mem.alloc(&Sn, (size_t)n * ldS))) return rc;
@@ -149,8 +161,6 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
     if ((rc = W.mem.alloc(&Pb, (size_t)n * m))) return rc;
     if ((rc = W.mem.alloc(&T2, (size_t)n * ldT))) return rc;
     if ((rc = W.mem.alloc(&v0, (size_t)n))) return rc;
-    if ((rc = W.mem.alloc(&Rf, (size_t)n * m))) return rc;
-    if ((rc = W.mem.alloc(&Wf, (size_t)n * m))) return rc;
     if ((rc = W.mem.alloc(&W.partial, (size_t)rgram_blocks(n) * 96 * 96))) return rc;
     if ((rc = W.mem.alloc(&W.G, 96 * 96))) return rc;
     if ((rc = W.mem.alloc(&W.C, 96 * 96))) return rc;
@@ -212,25 +222,31 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
             }
         }
         if (nconv >= k || it >= maxit) break;
-        // W = T R: one FP32 multi-right-hand-side solve
-        rto_f32cols(Z, ldS, m, n, Rf, st);
-        if ((rc = factor_block_solve(F, Rf, Wf, m))) return rc;
-        rfrom_f32cols(Wf, m, n, Z, ldS, st);
+        // W = T R: one FP32 multi-right-hand-side solve, in place on the rows of Z
+        if ((rc = factor_block_solve_rows(F, Z, ldS, 1, Z, ldS, 1, m))) return rc;
         solves++;
         if (np > 0) rcopy(Pb, m, np, n, Z + m, ldS, st);
         int nz = m + np;
         std::vector<double> zz;
-        if ((rc = project_out(W, S, ldS, m, nz, zz))) return rc;
+        double keep = 0.0, cond = 0.0;
+        if ((rc = project_out(W, S, ldS, m, nz, zz, &keep))) return rc;
         int r = 0;
-        if ((rc = svqb(W, Z, ldS, nz, zz, T2, ldT, &r))) return rc;
+        if ((rc = svqb(W, Z, ldS, nz, zz, T2, ldT, &r, &cond))) return rc;
         if (r == 0) break;
         rcopy(T2, ldT, r, n, Z, ldS, st);
+        // the second pass only where the first can have left Z short of
+        // orthonormal: a column that lost more than half its norm to the
+        // projection (Kahan's "twice is enough" test), a dropped direction, or
+        // an ill-conditioned Gram matrix
+        const bool again = keep < 0.5 || r < nz || cond > 1e4;
         nz = r;
-        if ((rc = project_out(W, S, ldS, m, nz, zz))) return rc;
-        if ((rc = svqb(W, Z, ldS, nz, zz, T2, ldT, &r))) return rc;
-        if (r == 0) break;
-        rcopy(T2, ldT, r, n, Z, ldS, st);
-        nz = r;
+        if (again) {
+            if ((rc = project_out(W, S, ldS, m, nz, zz))) return rc;
+            if ((rc = svqb(W, Z, ldS, nz, zz, T2, ldT, &r))) return rc;
+            if (r == 0) break;
+            rcopy(T2, ldT, r, n, Z, ldS, st);
+            nz = r;
+        }
         rspmm(A, Z, ldS, AZ, ldS, nz, st);
         // Rayleigh-Ritz on S = [X Z] (orthonormal): H = S^T A S
         const int d = m + nz;

@@ -1,0 +1,731 @@
+// rowsolve.hip -- the multi-right-hand-side solve of the block eigensolver:
+// X = (L L^T)^-1 B for up to 32 right-hand sides, FP32 vectors, with the
+// diagonal blocks L11 of every front already inverted in place (blocksolve.hip).
+//
+// Vectors are row-major with a row of 32 floats (128 bytes, right-hand sides
+// beyond nb are zero), so gathering a row by index -- the permutation, the
+// children's update rows, X at a front's update rows -- is one contiguous
+// line.  One buffer holds, by rows,
+//     [0, n)              P  = the permuted right-hand sides
+//     [n, n + sum r)      U  = each front's update block (r rows per front)
+//     [n + sum r, ...)    T  = per-column scratch: W1 (forward), Z (backward)
+// plus Y and X (n rows each).  Front s (c columns, r update rows), forward:
+//     W  = [P_c; 0] + sum of the children's U rows at their places in s
+//          (a precomputed pull list per front row, children in order, so the
+//          sums are deterministic and need no atomics; one launch per level
+//          assembles W1 into T and W2 into U for every front with children
+//          or outside the wave-per-front kernels)
+//     Y  = L11^-1 W1,   U_s = W2 - L21 Y
+// backward:
+//     Z  = Y - L21^T X_R,   X_c = L11^-T Z      (X_R = X at the update rows)
+// All four products run on v_mfma_f32_16x16x4f32 with the vector rows as the
+// second operand: lane l of a product holds column l%16 of the right-hand
+// sides, so every vector access is 16 consecutive floats.
+//
+// Fronts with c <= 64 and r <= 64 (most of the tree) are solved whole by one
+// wave each, Y / Z staying in LDS.  The others use 64-row output tiles: a
+// task is one tile and up to RKCHUNK K tiles, each wave owns 16 rows and
+// keeps the next K tile's loads in flight while it multiplies the current
+// one; tasks of a split K range leave partials that a finish kernel sums in
+// a fixed order.  The transposed products (Z, X_c) read L along columns
+// coalesced and transpose each wave's 16 x 64 slice through LDS.
+#include <algorithm>
+#include <cstring>
+
+#include "factor_impl.h"
+#include "host_par.h"
+
+namespace sehip {
+
+namespace {
+
+constexpr int RW = 32;            // floats per vector row
+constexpr int RTILE = 64;         // output rows per tile
+constexpr int RKCHUNK = 4;        // K tiles per part task
+constexpr int TINY = 64;          // wave-per-front limit on c and r
+constexpr int RP = 36;            // LDS row pitch (floats)
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct RsArgs {
+    const int64_t *foff, *col0, *rptr, *chptr, *frow, *uoff, *lptr;
+    const int32_t *Fs, *ncol, *rows, *lst;
+    int64_t n, t0;               // buffer row of U's first row (= n), of T's first row
+};
+
+struct RsPart {                  // output tile I of front s, K tiles [k0, k1)
+    int32_t s, I, k0, k1, direct, pad;
+};
+struct RsFin {
+    int32_t s, I;
+    int64_t p0;                  // first part (relative to the launch)
+    int32_t np, pad;
+};
+struct RsRow {
+    int32_t s, I;                // front, 32-row chunk
+};
+
+__device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// exchange through LDS between the lanes of one wave: the LDS executes one
+// wave's instructions in order, so a read issued after a write sees it (and a
+// later write cannot overtake an earlier read); the wave barrier keeps the
+// compiler from moving the accesses across this point.  No memory fence: it
+// would also wait for the global loads kept in flight across it.
+__device__ __forceinline__ void wave_lds_sync() { __builtin_amdgcn_wave_barrier(); }
+
+// ---- tiny fronts (c <= 64, r <= 64): one wave per front ----------------------
+// Operand layout (MFMA step (kq, t), lane l = 16 g + lc): the product index is
+// k = 16 kq + 4 g + t, the first operand holds A[16 rb + lc][k], the second
+// V[k][16 h + lc]; the result register v of lane l is row 16 rb + 4 g + v,
+// right-hand side 16 h + lc.  Every load of a step is issued before the first
+// product (no stores in between), so a front costs a few memory latencies.
+template <class T>
+__global__ void __launch_bounds__(256) k_rs_tiny_fwd(const T *__restrict__ pool, const int32_t *__restrict__ fronts,
+                                                     int nf, RsArgs a, float *__restrict__ buf,
+                                                     float *__restrict__ Yg)
+{
+    __shared__ float Ys[4][RTILE][RP];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, lc = lane & 15;
+    const int fi = blockIdx.x * 4 + wv;
+    if (fi >= nf) return;       // per wave; no workgroup barrier below
+    const int s = fronts[fi];
+    const int32_t F = a.Fs[s], c = a.ncol[s], r = F - c;
+    const T *L = pool + a.foff[s];
+    const int64_t c0 = a.col0[s], ub = a.n + a.uoff[s];
+    const bool leaf = a.chptr[s + 1] == a.chptr[s];   // not assembled: W1 = P rows, W2 = 0
+    const int64_t tb = leaf ? c0 : a.t0 + c0;
+    const int nq = (c + 15) >> 4, nr = (r + 15) >> 4;
+    float (*Yw)[RP] = Ys[wv];
+
+    float wb[4][4][2], w2[4][4][2];
+#pragma unroll
+    for (int kq = 0; kq < 4; kq++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int k = 16 * kq + 4 * g + t;
+            const bool in = kq < nq && k < c;
+            wb[kq][t][0] = in ? buf[(tb + k) * RW + lc] : 0.f;
+            wb[kq][t][1] = in ? buf[(tb + k) * RW + 16 + lc] : 0.f;
+            const int ur = 16 * kq + 4 * g + t;       // (rb = kq, v = t) of the U rows
+            const bool iu = !leaf && kq < nr && ur < r;
+            w2[kq][t][0] = iu ? buf[(ub + ur) * RW + lc] : 0.f;
+            w2[kq][t][1] = iu ? buf[(ub + ur) * RW + 16 + lc] : 0.f;
+        }
+    // Y = X11 W1 (X11 lower triangular: A[row][k] = L(row, k), k <= row)
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) {
+        if (rb >= nq) break;
+        f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
+        const int row = 16 * rb + lc;
+#pragma unroll
+        for (int kq = 0; kq <= rb; kq++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int k = 16 * kq + 4 * g + t;
+                const float av = (row < c && k <= row) ? (float)L[(int64_t)k * F + row] : 0.f;
+                acc0 = mfma4(av, wb[kq][t][0], acc0);
+                acc1 = mfma4(av, wb[kq][t][1], acc1);
+            }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int yr = 16 * rb + 4 * g + v;
+            Yw[yr][lc] = acc0[v];
+            Yw[yr][16 + lc] = acc1[v];
+            if (yr < c) {
+                Yg[(c0 + yr) * RW + lc] = acc0[v];
+                Yg[(c0 + yr) * RW + 16 + lc] = acc1[v];
+            }
+        }
+    }
+    if (r <= 0) return;
+    wave_lds_sync();
+    float yb[4][4][2];
+#pragma unroll
+    for (int kq = 0; kq < 4; kq++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int k = 16 * kq + 4 * g + t;
+            yb[kq][t][0] = kq < nq ? Yw[k][lc] : 0.f;
+            yb[kq][t][1] = kq < nq ? Yw[k][16 + lc] : 0.f;
+        }
+    // U = W2 - L21 Y
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) {
+        if (rb >= nr) break;
+        f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
+        const int lrow = 16 * rb + lc;
+#pragma unroll
+        for (int kq = 0; kq < 4; kq++) {
+            if (kq >= nq) break;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int k = 16 * kq + 4 * g + t;
+                const float av = (lrow < r && k < c) ? (float)L[(int64_t)k * F + c + lrow] : 0.f;
+                acc0 = mfma4(av, yb[kq][t][0], acc0);
+                acc1 = mfma4(av, yb[kq][t][1], acc1);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int ur = 16 * rb + 4 * g + v;
+            if (ur < r) {
+                buf[(ub + ur) * RW + lc] = w2[rb][v][0] - acc0[v];
+                buf[(ub + ur) * RW + 16 + lc] = w2[rb][v][1] - acc1[v];
+            }
+        }
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_rs_tiny_bwd(const T *__restrict__ pool, const int32_t *__restrict__ fronts,
+                                                     int nf, RsArgs a, const float *__restrict__ Yg,
+                                                     float *__restrict__ Xg)
+{
+    __shared__ float Zs[4][RTILE][RP];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, lc = lane & 15;
+    const int fi = blockIdx.x * 4 + wv;
+    if (fi >= nf) return;
+    const int s = fronts[fi];
+    const int32_t F = a.Fs[s], c = a.ncol[s], r = F - c;
+    const T *L = pool + a.foff[s];
+    const int64_t c0 = a.col0[s];
+    const int32_t *R = a.rows + a.rptr[s];
+    const int nq = (c + 15) >> 4, nr = (r + 15) >> 4;
+    float (*Zw)[RP] = Zs[wv];
+
+    float xb[4][4][2], yv[4][4][2];
+#pragma unroll
+    for (int kq = 0; kq < 4; kq++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int k = 16 * kq + 4 * g + t;
+            const bool in = kq < nr && k < r;
+            const int64_t gr = in ? (int64_t)R[k] : 0;
+            xb[kq][t][0] = in ? Xg[gr * RW + lc] : 0.f;
+            xb[kq][t][1] = in ? Xg[gr * RW + 16 + lc] : 0.f;
+            const bool iy = kq < nq && k < c;          // (rb = kq, v = t) of the Y rows
+            yv[kq][t][0] = iy ? Yg[(c0 + k) * RW + lc] : 0.f;
+            yv[kq][t][1] = iy ? Yg[(c0 + k) * RW + 16 + lc] : 0.f;
+        }
+    // Z = Y - L21^T X_R (A[j][k] = L(c + k, j), k over the r update rows)
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) {
+        if (rb >= nq) break;
+        f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
+        const int j = 16 * rb + lc;
+#pragma unroll
+        for (int kq = 0; kq < 4; kq++) {
+            if (kq >= nr) break;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int k = 16 * kq + 4 * g + t;
+                const float av = (j < c && k < r) ? (float)L[(int64_t)j * F + c + k] : 0.f;
+                acc0 = mfma4(av, xb[kq][t][0], acc0);
+                acc1 = mfma4(av, xb[kq][t][1], acc1);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int zr = 16 * rb + 4 * g + v;
+            Zw[zr][lc] = yv[rb][v][0] - acc0[v];
+            Zw[zr][16 + lc] = yv[rb][v][1] - acc1[v];
+        }
+    }
+    wave_lds_sync();
+    float zb[4][4][2];
+#pragma unroll
+    for (int kq = 0; kq < 4; kq++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int k = 16 * kq + 4 * g + t;
+            zb[kq][t][0] = kq < nq ? Zw[k][lc] : 0.f;
+            zb[kq][t][1] = kq < nq ? Zw[k][16 + lc] : 0.f;
+        }
+    // X_c = X11^T Z (A[j][k] = L(k, j), k >= j)
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) {
+        if (rb >= nq) break;
+        f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
+        const int j = 16 * rb + lc;
+#pragma unroll
+        for (int kq = rb; kq < 4; kq++) {
+            if (kq >= nq) break;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int k = 16 * kq + 4 * g + t;
+                const float av = (j < c && k < c && k >= j) ? (float)L[(int64_t)j * F + k] : 0.f;
+                acc0 = mfma4(av, zb[kq][t][0], acc0);
+                acc1 = mfma4(av, zb[kq][t][1], acc1);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int xr = 16 * rb + 4 * g + v;
+            if (xr < c) {
+                Xg[(c0 + xr) * RW + lc] = acc0[v];
+                Xg[(c0 + xr) * RW + 16 + lc] = acc1[v];
+            }
+        }
+    }
+}
+
+// ---- tile products of the other fronts ---------------------------------------
+// W of the level's fronts: W1 rows to T (row t0 + c0 + i), W2 rows to U
+__global__ void __launch_bounds__(256) k_rs_assemble(const RsRow *__restrict__ tasks, RsArgs a, float *__restrict__ buf)
+{
+    const RsRow t = tasks[blockIdx.x];
+    const int s = t.s;
+    const int32_t F = a.Fs[s], c = a.ncol[s];
+    const int i = t.I * 32 + (threadIdx.x >> 3), q4 = threadIdx.x & 7;
+    if (i >= F) return;
+    const int64_t rid = a.frow[s] + i;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    const int64_t q1 = a.lptr[rid + 1];
+    for (int64_t q = a.lptr[rid]; q < q1; q++) {
+        const f32x4 x = *(const f32x4 *)(buf + (int64_t)a.lst[q] * RW + 4 * q4);
+        acc.x += x.x;
+        acc.y += x.y;
+        acc.z += x.z;
+        acc.w += x.w;
+    }
+    const int64_t dst = i < c ? a.t0 + a.col0[s] + i : a.n + a.uoff[s] + (i - c);
+    *(f32x4 *)(buf + dst * RW + 4 * q4) = acc;
+}
+
+// output row (tile-local) `row`, right-hand side `col` of a tile product
+//   MODE 0  Y      = X11 W1             out: Y rows of the front's columns
+//   MODE 1  U      = W2 - L21 Y         out: U rows (read-modify-write)
+//   MODE 2  Z      = Y - L21^T X_R      out: T rows
+//   MODE 3  X_c    = X11^T Z            out: X rows
+template <int MODE>
+__device__ __forceinline__ void rs_store(float val, int row, int col, const RsArgs &a, int s, int orow0,
+                                         float *__restrict__ buf, float *__restrict__ Yg, float *__restrict__ Xg)
+{
+    const int64_t c0 = a.col0[s];
+    if (MODE == 0) Yg[(c0 + orow0 + row) * RW + col] = val;
+    else if (MODE == 1) buf[(a.n + a.uoff[s] + orow0 + row) * RW + col] -= val;
+    else if (MODE == 2) buf[(a.t0 + c0 + orow0 + row) * RW + col] = Yg[(c0 + orow0 + row) * RW + col] - val;
+    else Xg[(c0 + orow0 + row) * RW + col] = val;
+}
+
+// operands of one 64-deep K tile for one wave (16 output rows): A in a[] (modes
+// 0, 1: directly in operand layout; modes 2, 3: column j = 16 wv + jj of the
+// wave's rows at tile row `lane`, to be transposed through LDS), the vector
+// rows in b0[] / b1[]; index 4 kq + t
+template <class T, int MODE>
+__device__ __forceinline__ void rs_load(const T *__restrict__ L, int F, int c, int I, int K, int kn, int orow0, int on,
+                                        int wv, int g, int lc, int lane, const RsArgs &a, int64_t c0,
+                                        const int32_t *__restrict__ R, const float *__restrict__ buf,
+                                        const float *__restrict__ Yg, const float *__restrict__ Xg, float (&av)[16],
+                                        float (&b0)[16], float (&b1)[16])
+{
+#pragma unroll
+    for (int kq = 0; kq < 4; kq++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int k = 16 * kq + 4 * g + t, kk = K * RTILE + k, x = 4 * kq + t;
+            float v0 = 0.f, v1 = 0.f;
+            if (k < kn) {
+                const float *src;
+                if (MODE == 0 || MODE == 3) src = buf + (a.t0 + c0 + kk) * RW;
+                else if (MODE == 1) src = Yg + (c0 + kk) * RW;
+                else src = Xg + (int64_t)R[kk] * RW;
+                v0 = src[lc];
+                v1 = src[16 + lc];
+            }
+            b0[x] = v0;
+            b1[x] = v1;
+            if (MODE == 0 || MODE == 1) {
+                const int row = 16 * wv + lc;
+                float v = 0.f;
+                if (row < on && k < kn) {
+                    if (MODE == 0) {
+                        if (K < I || k <= row) v = (float)L[(int64_t)kk * F + orow0 + row];
+                    } else {
+                        v = (float)L[(int64_t)kk * F + c + orow0 + row];
+                    }
+                }
+                av[x] = v;
+            }
+        }
+    if (MODE == 2 || MODE == 3) {
+#pragma unroll
+        for (int jj = 0; jj < 16; jj++) {
+            const int j = 16 * wv + jj;      // tile-local output row = front column orow0 + j
+            float v = 0.f;
+            if (j < on && lane < kn) {
+                const int64_t col = (int64_t)(orow0 + j) * F;
+                if (MODE == 2) v = (float)L[col + c + K * RTILE + lane];
+                else if (K > I || lane >= j) v = (float)L[col + K * RTILE + lane];
+            }
+            av[jj] = v;
+        }
+    }
+}
+
+template <class T, int MODE>
+__global__ void __launch_bounds__(256) k_rs_part(const T *__restrict__ pool, const RsPart *__restrict__ tasks, RsArgs a,
+                                                 float *__restrict__ buf, float *__restrict__ Yg,
+                                                 float *__restrict__ Xg, float *__restrict__ part)
+{
+    __shared__ float At[4][RTILE][20];    // modes 2, 3: [k][j] of the wave's A tile
+    const RsPart tk = tasks[blockIdx.x];
+    const int s = tk.s, I = tk.I;
+    const int32_t F = a.Fs[s], c = a.ncol[s], r = F - c;
+    const int orow0 = I * RTILE;
+    const int on = min(RTILE, (MODE == 1 ? r : c) - orow0);
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, lc = lane & 15;
+    if (16 * wv >= on) return;     // this wave's rows are past the tile (no workgroup barrier below)
+    const T *L = pool + a.foff[s];
+    const int64_t c0 = a.col0[s];
+    const int32_t *R = a.rows + a.rptr[s];
+    const int kdim = MODE == 2 ? r : c;
+    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
+    float av[16], b0[16], b1[16];
+    rs_load<T, MODE>(L, F, c, I, tk.k0, min(RTILE, kdim - tk.k0 * RTILE), orow0, on, wv, g, lc, lane, a, c0, R, buf,
+                     Yg, Xg, av, b0, b1);
+    for (int K = tk.k0; K < tk.k1; K++) {
+        float ca[16], c0v[16], c1v[16];
+#pragma unroll
+        for (int x = 0; x < 16; x++) {
+            ca[x] = av[x];
+            c0v[x] = b0[x];
+            c1v[x] = b1[x];
+        }
+        if (K + 1 < tk.k1)     // next tile's loads in flight during this tile's products
+            rs_load<T, MODE>(L, F, c, I, K + 1, min(RTILE, kdim - (K + 1) * RTILE), orow0, on, wv, g, lc, lane, a,
+                             c0, R, buf, Yg, Xg, av, b0, b1);
+        if (MODE == 2 || MODE == 3) {
+#pragma unroll
+            for (int jj = 0; jj < 16; jj++) At[wv][lane][jj] = ca[jj];
+            wave_lds_sync();
+#pragma unroll
+            for (int kq = 0; kq < 4; kq++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) ca[4 * kq + t] = At[wv][16 * kq + 4 * g + t][lc];
+            wave_lds_sync();
+        }
+#pragma unroll
+        for (int x = 0; x < 16; x++) {
+            acc0 = mfma4(ca[x], c0v[x], acc0);
+            acc1 = mfma4(ca[x], c1v[x], acc1);
+        }
+    }
+    if (tk.direct) {
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = 16 * wv + 4 * g + v;
+            if (row < on) {
+                rs_store<MODE>(acc0[v], row, lc, a, s, orow0, buf, Yg, Xg);
+                rs_store<MODE>(acc1[v], row, 16 + lc, a, s, orow0, buf, Yg, Xg);
+            }
+        }
+    } else {
+        float *out = part + (int64_t)blockIdx.x * (RTILE * RW);
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = 16 * wv + 4 * g + v;
+            out[row * RW + lc] = acc0[v];
+            out[row * RW + 16 + lc] = acc1[v];
+        }
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_rs_fin(const RsFin *__restrict__ tasks, RsArgs a, float *__restrict__ buf,
+                                                float *__restrict__ Yg, float *__restrict__ Xg,
+                                                const float *__restrict__ part)
+{
+    const RsFin t = tasks[blockIdx.x];
+    const int s = t.s;
+    const int32_t F = a.Fs[s], c = a.ncol[s];
+    const int orow0 = t.I * RTILE;
+    const int on = min(RTILE, (MODE == 1 ? F - c : c) - orow0);
+    for (int e = threadIdx.x; e < RTILE * RW; e += 256) {
+        const int row = e >> 5, col = e & 31;
+        if (row >= on) continue;
+        // four interleaved sums (fixed order) keep several loads in flight
+        const float *p = part + t.p0 * (RTILE * RW) + e;
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+        int q = 0;
+        for (; q + 4 <= t.np; q += 4) {
+            s0 += p[(int64_t)q * (RTILE * RW)];
+            s1 += p[(int64_t)(q + 1) * (RTILE * RW)];
+            s2 += p[(int64_t)(q + 2) * (RTILE * RW)];
+            s3 += p[(int64_t)(q + 3) * (RTILE * RW)];
+        }
+        for (; q < t.np; q++) s0 += p[(int64_t)q * (RTILE * RW)];
+        rs_store<MODE>((s0 + s1) + (s2 + s3), row, col, a, s, orow0, buf, Yg, Xg);
+    }
+}
+
+// ---- permutation + precision change at the boundary -------------------------
+// buffer row i = B(perm[i], :) (FP64 -> FP32, columns >= nb zero); element
+// (i, v) of B at B[i * rs + v * cs]
+__global__ void __launch_bounds__(256) k_rs_in(const double *__restrict__ B, int64_t rs, int64_t cs,
+                                               const int64_t *__restrict__ perm, int64_t n, int nb,
+                                               float *__restrict__ buf)
+{
+    const int64_t i = (int64_t)blockIdx.x * 8 + (threadIdx.x >> 5);
+    const int v = threadIdx.x & 31;
+    if (i >= n) return;
+    buf[i * RW + v] = v < nb ? (float)B[perm[i] * rs + (int64_t)v * cs] : 0.f;
+}
+
+__global__ void __launch_bounds__(256) k_rs_out(const float *__restrict__ Xg, const int64_t *__restrict__ perm,
+                                                int64_t n, int nb, double *__restrict__ X, int64_t rs, int64_t cs)
+{
+    const int64_t i = (int64_t)blockIdx.x * 8 + (threadIdx.x >> 5);
+    const int v = threadIdx.x & 31;
+    if (i >= n || v >= nb) return;
+    X[perm[i] * rs + (int64_t)v * cs] = (double)Xg[i * RW + v];
+}
+
+// ---- plan -----------------------------------------------------------------
+struct RsLevel {
+    int64_t sm_off = 0, sm_cnt = 0;          // tiny fronts
+    int64_t as_off = 0, as_cnt = 0;          // assembly chunks (every front of the level)
+    int64_t pt_off[4] = {0, 0, 0, 0}, pt_cnt[4] = {0, 0, 0, 0};
+    int64_t ft_off[4] = {0, 0, 0, 0}, ft_cnt[4] = {0, 0, 0, 0};
+};
+
+struct RsState {
+    DevMem mem;
+    std::vector<RsLevel> lev;
+    int32_t *small = nullptr;
+    RsRow *asmt = nullptr;
+    RsPart *pt = nullptr;
+    RsFin *ft = nullptr;
+    float *part = nullptr, *buf = nullptr, *Yg = nullptr, *Xg = nullptr;
+    int64_t *frow = nullptr, *uoff = nullptr, *lptr = nullptr;
+    int32_t *lst = nullptr;
+    int64_t n = 0, t0 = 0;
+};
+
+bool is_tiny(int64_t c, int64_t r) { return c <= TINY && r <= TINY; }
+
+void plan_parts(const Symbolic &S, const std::vector<int64_t> &fronts, int mode, std::vector<RsPart> &pt,
+                std::vector<RsFin> &ft, int64_t pbase)
+{
+    for (int64_t s : fronts) {
+        const int64_t c = S.c1[s] - S.c0[s], r = S.F[s] - c;
+        const int64_t nt11 = (c + RTILE - 1) / RTILE, nupd = (r + RTILE - 1) / RTILE;
+        const int64_t nout = mode == 1 ? nupd : nt11;
+        for (int64_t I = 0; I < nout; I++) {
+            int64_t kb, ke;
+            if (mode == 0) { kb = 0; ke = I + 1; }
+            else if (mode == 1) { kb = 0; ke = nt11; }
+            else if (mode == 2) { kb = 0; ke = nupd; }
+            else { kb = I; ke = nt11; }
+            const int64_t np = (ke - kb + RKCHUNK - 1) / RKCHUNK;
+            if (np == 1) {
+                pt.push_back({(int32_t)s, (int32_t)I, (int32_t)kb, (int32_t)ke, 1, 0});
+                continue;
+            }
+            // np == 0 (mode 2 without update rows): the finish writes Z = Y
+            RsFin f{(int32_t)s, (int32_t)I, (int64_t)pt.size() - pbase, (int32_t)np, 0};
+            for (int64_t k0 = kb; k0 < ke; k0 += RKCHUNK)
+                pt.push_back({(int32_t)s, (int32_t)I, (int32_t)k0, (int32_t)std::min<int64_t>(ke, k0 + RKCHUNK), 0, 0});
+            ft.push_back(f);
+        }
+    }
+}
+
+int rs_prepare(Factor *F, RsState &B)
+{
+    const Symbolic &S = F->S;
+    const int64_t ns = S.ns, n = F->n;
+    std::vector<int64_t> frow(ns + 1, 0), uoff(ns + 1, 0);
+    for (int64_t s = 0; s < ns; s++) {
+        frow[s + 1] = frow[s] + S.F[s];
+        uoff[s + 1] = uoff[s] + (S.F[s] - (S.c1[s] - S.c0[s]));
+    }
+    const int64_t nrid = frow[ns], sumr = uoff[ns];
+    if (2 * n + sumr >= ((int64_t)1 << 31)) return SPEIGS_ERR_MEM;   // buffer rows are int32 in the pull lists
+    B.n = n;
+    B.t0 = n + sumr;
+    // pull lists: row (s, i) <- P row c0 + i (i < c), then each child's U rows
+    // that land on i, children in order
+    const int nt = host_threads();
+    std::vector<int64_t> tot(ns + 1, 0);
+    parallel_for(ns, nt, 256, [&](int64_t s, int) {
+        int64_t t = S.c1[s] - S.c0[s];
+        for (int64_t q = S.chptr[s]; q < S.chptr[s + 1]; q++) {
+            const int64_t ch = S.chl[q];
+            t += S.F[ch] - (S.c1[ch] - S.c0[ch]);
+        }
+        tot[s + 1] = t;
+    });
+    for (int64_t s = 0; s < ns; s++) tot[s + 1] += tot[s];
+    std::vector<int64_t> lptr(nrid + 1);
+    std::vector<int32_t> lst(std::max<int64_t>(tot[ns], 1));
+    lptr[nrid] = tot[ns];
+    parallel_for(ns, nt, 64, [&](int64_t s, int) {
+        const int64_t c = S.c1[s] - S.c0[s], Fv = S.F[s];
+        std::vector<int64_t> cnt(Fv + 1, 0);
+        for (int64_t i = 0; i < c; i++) cnt[i + 1]++;
+        for (int64_t q = S.chptr[s]; q < S.chptr[s + 1]; q++) {
+            const int64_t ch = S.chl[q], rc = S.F[ch] - (S.c1[ch] - S.c0[ch]);
+            const int32_t *rl = S.rel.data() + S.relptr[ch];
+            for (int64_t j = 0; j < rc; j++) cnt[rl[j] + 1]++;
+        }
+        for (int64_t i = 0; i < Fv; i++) cnt[i + 1] += cnt[i];
+        const int64_t base = tot[s];
+        for (int64_t i = 0; i < Fv; i++) lptr[frow[s] + i] = base + cnt[i];
+        for (int64_t i = 0; i < c; i++) lst[base + cnt[i]++] = (int32_t)(S.c0[s] + i);
+        for (int64_t q = S.chptr[s]; q < S.chptr[s + 1]; q++) {
+            const int64_t ch = S.chl[q], rc = S.F[ch] - (S.c1[ch] - S.c0[ch]);
+            const int32_t *rl = S.rel.data() + S.relptr[ch];
+            for (int64_t j = 0; j < rc; j++) lst[base + cnt[rl[j]]++] = (int32_t)(n + uoff[ch] + j);
+        }
+    });
+    // per level: tiny fronts, assembly chunks of every front, product tasks of the others
+    std::vector<std::vector<int64_t>> bylevel(S.nlevels);
+    for (int64_t s = 0; s < ns; s++) bylevel[S.level[s]].push_back(s);
+    std::vector<int32_t> small;
+    std::vector<RsRow> asmt;
+    std::vector<RsPart> pt;
+    std::vector<RsFin> ft;
+    int64_t maxpart = 1;
+    B.lev.assign(S.nlevels, RsLevel());
+    for (int32_t L = 0; L < S.nlevels; L++) {
+        RsLevel &P = B.lev[L];
+        std::vector<int64_t> big;
+        P.sm_off = (int64_t)small.size();
+        for (int64_t s : bylevel[L]) {
+            const int64_t c = S.c1[s] - S.c0[s];
+            if (is_tiny(c, S.F[s] - c)) small.push_back((int32_t)s);
+            else big.push_back(s);
+        }
+        P.sm_cnt = (int64_t)small.size() - P.sm_off;
+        P.as_off = (int64_t)asmt.size();
+        for (int64_t s : bylevel[L]) {
+            const int64_t c = S.c1[s] - S.c0[s];
+            if (is_tiny(c, S.F[s] - c) && S.chptr[s + 1] == S.chptr[s]) continue;   // leaf: read in place
+            for (int64_t I = 0; I * 32 < S.F[s]; I++) asmt.push_back({(int32_t)s, (int32_t)I});
+        }
+        P.as_cnt = (int64_t)asmt.size() - P.as_off;
+        for (int mode = 0; mode < 4; mode++) {
+            const int64_t p0 = (int64_t)pt.size(), f0 = (int64_t)ft.size();
+            plan_parts(S, big, mode, pt, ft, p0);
+            P.pt_off[mode] = p0;
+            P.pt_cnt[mode] = (int64_t)pt.size() - p0;
+            P.ft_off[mode] = f0;
+            P.ft_cnt[mode] = (int64_t)ft.size() - f0;
+            maxpart = std::max<int64_t>(maxpart, P.pt_cnt[mode]);
+        }
+    }
+    if (small.empty()) small.push_back(0);
+    if (asmt.empty()) asmt.push_back({0, 0});
+    if (pt.empty()) pt.push_back({0, 0, 0, 0, 0, 0});
+    if (ft.empty()) ft.push_back({0, 0, 0, 0, 0});
+    int rc;
+    auto up = [&](auto **dst, const auto &v) -> int {
+        int e = B.mem.alloc(dst, v.size());
+        if (e) return e;
+        SEHIP_CHECK(hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, F->st));
+        return 0;
+    };
+    if ((rc = up(&B.small, small)) || (rc = up(&B.asmt, asmt)) || (rc = up(&B.pt, pt)) || (rc = up(&B.ft, ft)) ||
+        (rc = up(&B.frow, frow)) || (rc = up(&B.uoff, uoff)) || (rc = up(&B.lptr, lptr)) || (rc = up(&B.lst, lst)))
+        return rc;
+    if ((rc = B.mem.alloc(&B.part, (size_t)maxpart * RTILE * RW))) return rc;
+    if ((rc = B.mem.alloc(&B.buf, (size_t)(2 * n + sumr) * RW))) return rc;
+    if ((rc = B.mem.alloc(&B.Yg, (size_t)n * RW))) return rc;
+    if ((rc = B.mem.alloc(&B.Xg, (size_t)n * RW))) return rc;
+    SEHIP_CHECK(hipStreamSynchronize(F->st));   // host vectors above go out of scope
+    return 0;
+}
+
+template <class T>
+void rs_run(Factor *F, RsState &B, const double *Bin, int64_t brs, int64_t bcs, double *Xout, int64_t xrs,
+            int64_t xcs, int nb)
+{
+    const Symbolic &S = F->S;
+    hipStream_t st = F->st;
+    const T *pool = (const T *)F->pool;
+    const RsArgs a{F->foff, F->col0, F->rptr, F->chptr, B.frow, B.uoff, B.lptr, F->Fs, F->ncol, F->rows, B.lst, B.n,
+                   B.t0};
+    const unsigned nrb = (unsigned)((F->n + 7) / 8);
+    const dim3 blk(256);
+    k_rs_in<<<dim3(nrb), blk, 0, st>>>(Bin, brs, bcs, F->perm, F->n, nb, B.buf);
+    auto parts = [&](int mode, const RsLevel &P) {
+        const int64_t pc = P.pt_cnt[mode], fc = P.ft_cnt[mode];
+        const RsPart *pt = B.pt + P.pt_off[mode];
+        const RsFin *ft = B.ft + P.ft_off[mode];
+        const dim3 gp((unsigned)pc), gf((unsigned)fc);
+        switch (mode) {
+        case 0:
+            if (pc) k_rs_part<T, 0><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
+            if (fc) k_rs_fin<0><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
+            break;
+        case 1:
+            if (pc) k_rs_part<T, 1><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
+            if (fc) k_rs_fin<1><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
+            break;
+        case 2:
+            if (pc) k_rs_part<T, 2><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
+            if (fc) k_rs_fin<2><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
+            break;
+        default:
+            if (pc) k_rs_part<T, 3><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
+            if (fc) k_rs_fin<3><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
+        }
+    };
+    for (int32_t L = 0; L < S.nlevels; L++) {
+        const RsLevel &P = B.lev[L];
+        if (P.as_cnt) k_rs_assemble<<<dim3((unsigned)P.as_cnt), blk, 0, st>>>(B.asmt + P.as_off, a, B.buf);
+        if (P.sm_cnt)
+            k_rs_tiny_fwd<T><<<dim3((unsigned)((P.sm_cnt + 3) / 4)), blk, 0, st>>>(pool, B.small + P.sm_off,
+                                                                                  (int)P.sm_cnt, a, B.buf, B.Yg);
+        parts(0, P);
+        parts(1, P);
+    }
+    for (int32_t L = S.nlevels - 1; L >= 0; L--) {
+        const RsLevel &P = B.lev[L];
+        parts(2, P);
+        parts(3, P);
+        if (P.sm_cnt)
+            k_rs_tiny_bwd<T><<<dim3((unsigned)((P.sm_cnt + 3) / 4)), blk, 0, st>>>(pool, B.small + P.sm_off,
+                                                                                  (int)P.sm_cnt, a, B.Yg, B.Xg);
+    }
+    k_rs_out<<<dim3(nrb), blk, 0, st>>>(B.Xg, F->perm, F->n, nb, Xout, xrs, xcs);
+}
+
+}  // namespace
+
+int factor_block_solve_rows(Factor *F, const double *Bin, int64_t brs, int64_t bcs, double *Xout, int64_t xrs,
+                            int64_t xcs, int nb)
+{
+    if (nb < 1 || nb > RW) return SPEIGS_ERR_ARG;
+    if (!F->inverted) return SPEIGS_ERR_FACTOR;
+    if (!F->rs) {
+        RsState *B = new RsState();
+        const int rc = rs_prepare(F, *B);
+        if (rc) {
+            delete B;
+            return rc;
+        }
+        F->rs = B;
+    }
+    RsState &B = *reinterpret_cast<RsState *>(F->rs);
+    const int e = ev_begin(F);
+    if (F->fp32) rs_run<float>(F, B, Bin, brs, bcs, Xout, xrs, xcs, nb);
+    else rs_run<double>(F, B, Bin, brs, bcs, Xout, xrs, xcs, nb);
+    ev_end(F, e);
+    F->block_solves++;
+    F->block_rhs += nb;
+    return 0;
+}
+
+void rowsolve_free(Factor *F)
+{
+    delete reinterpret_cast<RsState *>(F->rs);
+    F->rs = nullptr;
+}
+
+}  // namespace sehip
