@@ -81,11 +81,13 @@ __device__ __forceinline__ int lower_bound_i32(const int32_t *a, int n, int v)
     return lo;
 }
 
-// extend-add into parent columns [pcb*64, pcb*64+64): children in order, so the
-// sum is deterministic; rel maps are increasing, so each child contributes a
-// contiguous range of its columns
+// extend-add into the parent block (columns [64 cb, +64), rows [256 rb, +256)):
+// children in order with a barrier between them, so every element's sum is
+// deterministic.  The rel maps are increasing, so a child contributes a
+// contiguous range of its columns and of its rows; thread t takes child row
+// ilo + t, 16 columns at a time with all 16 loads issued before the stores.
 template <class T>
-__global__ void __launch_bounds__(256) k_extend_add_pcb(T *pool, const PcbTask *__restrict__ tasks,
+__global__ void __launch_bounds__(256) k_extend_add_blk(T *pool, const EaBlk *__restrict__ tasks,
                                                         const int64_t *__restrict__ foff,
                                                         const int32_t *__restrict__ Fs, const int32_t *__restrict__ ncol,
                                                         const int64_t *__restrict__ relptr,
@@ -93,19 +95,38 @@ __global__ void __launch_bounds__(256) k_extend_add_pcb(T *pool, const PcbTask *
                                                         const int64_t *__restrict__ chptr,
                                                         const int32_t *__restrict__ chl)
 {
-    const PcbTask t = tasks[blockIdx.x];
+    constexpr int NB = 16;
+    const EaBlk t = tasks[blockIdx.x];
     const int32_t Fp = Fs[t.parent];
     T *P = pool + foff[t.parent];
-    const int c0 = t.pcb * 64, c1 = min(Fp, c0 + 64);
+    const int c0 = t.cb * 64, c1 = min(Fp, c0 + 64);
+    const int R0 = t.rb * 256, R1 = min(Fp, R0 + 256);
     for (int64_t q = chptr[t.parent]; q < chptr[t.parent + 1]; q++) {
         const int ch = chl[q];
         const int32_t Fc = Fs[ch], cc = ncol[ch], rc = Fc - cc;
         const int32_t *rl = rel + relptr[ch];
         const int jlo = lower_bound_i32(rl, rc, c0), jhi = lower_bound_i32(rl, rc, c1);
-        const T *U = pool + foff[ch] + (int64_t)cc * Fc + cc;
-        for (int j = jlo; j < jhi; j++) {
-            const int64_t pc = (int64_t)rl[j] * Fp;
-            for (int i = j + threadIdx.x; i < rc; i += 256) P[pc + rl[i]] += U[(int64_t)j * Fc + i];
+        const int ilo = lower_bound_i32(rl, rc, R0), ihi = lower_bound_i32(rl, rc, R1);
+        const T *U = pool + foff[ch] + (int64_t)cc * Fc + cc;     // r x r, ld Fc
+        const int i = ilo + threadIdx.x;
+        if (jlo < jhi && i < ihi) {
+            const int64_t prow = rl[i];
+            const int jend = min(jhi, i + 1);      // U is lower triangular
+            for (int j = jlo; j < jend; j += NB) {
+                T u[NB], pv[NB];
+                int64_t pos[NB];
+#pragma unroll
+                for (int b = 0; b < NB; b++) {
+                    const int jj = j + b;
+                    const bool ok = jj < jend;
+                    pos[b] = ok ? (int64_t)rl[jj] * Fp + prow : 0;
+                    u[b] = ok ? U[(int64_t)jj * Fc + i] : T(0);
+                    pv[b] = ok ? P[pos[b]] : T(0);
+                }
+#pragma unroll
+                for (int b = 0; b < NB; b++)
+                    if (j + b < jend) P[pos[b]] = pv[b] + u[b];
+            }
         }
         __syncthreads();
     }
@@ -378,16 +399,35 @@ __global__ void __launch_bounds__(256) k_update_mfma(T *pool, const TileTask *__
     if (nr <= 0 || nq <= 0) return;
     T *fr = pool + foff[t.s];
     const int w4 = (w + 3) & ~3;
-    for (int e = threadIdx.x; e < TB * w4; e += 256) {
-        const int r = e % TB, kk = e / TB;
-        const bool in = kk < w;
-        Ai[kk][r] = (in && r < nr) ? fr[(int64_t)(k0 + kk) * F + r0 + r] : T(0);
-        Aj[kk][r] = (in && r < nq) ? fr[(int64_t)(k0 + kk) * F + q0 + r] : T(0);
-    }
-    __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int rb = (wv & 1) * 32, cb = (wv >> 1) * 32;
     const int lr = lane & 15, lk = lane >> 4;
+    // every global load of the task -- both panel tiles and this thread's
+    // elements of C -- is issued before the first store, so the task pays one
+    // memory latency (a store to the front between loads would order them)
+    T va[16], vb[16], cv[2][2][4];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int kk = wv + 4 * q;
+        const bool in = kk < w;
+        va[q] = (in && lane < nr) ? fr[(int64_t)(k0 + kk) * F + r0 + lane] : T(0);
+        vb[q] = (in && lane < nq) ? fr[(int64_t)(k0 + kk) * F + q0 + lane] : T(0);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = rb + 16 * a + lr, col = cb + 16 * b + MfmaAcc<T>::colof(lk, v);
+                cv[a][b][v] = (row < nr && col < nq) ? fr[(int64_t)(q0 + col) * F + r0 + row] : T(0);
+            }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        Ai[wv + 4 * q][lane] = va[q];
+        Aj[wv + 4 * q][lane] = vb[q];
+    }
+    __syncthreads();
     V acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; a++)
@@ -413,7 +453,7 @@ __global__ void __launch_bounds__(256) k_update_mfma(T *pool, const TileTask *__
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int col = cb + 16 * b + MfmaAcc<T>::colof(lk, v);
-                if (row < nr && col < nq) fr[(int64_t)(q0 + col) * F + r0 + row] -= acc[a][b][v];
+                if (row < nr && col < nq) fr[(int64_t)(q0 + col) * F + r0 + row] = cv[a][b][v] - acc[a][b][v];
             }
         }
 }
@@ -911,8 +951,8 @@ int run_numeric(Factor *F, double delta)
     for (int32_t L = 0; L < S.nlevels; L++) {
         const LevelPlan &P = F->plan[L];
         if (P.pcb_cnt > 0)
-            k_extend_add_pcb<T><<<dim3((unsigned)P.pcb_cnt), dim3(256), 0, st>>>(
-                pool, F->pcb + P.pcb_off, F->foff, F->Fs, F->ncol, F->relptr, F->rel, F->chptr, F->chl);
+            k_extend_add_blk<T><<<dim3((unsigned)P.pcb_cnt), dim3(256), 0, st>>>(
+                pool, F->eab + P.pcb_off, F->foff, F->Fs, F->ncol, F->relptr, F->rel, F->chptr, F->chl);
         for (size_t k = 0; k < P.k_potrf_cnt.size(); k++) {
             if (F->use_reg_panel) {
                 // panel in registers (+ its inverse), then TRSM as a product with the inverse
@@ -1214,14 +1254,17 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
             for (int64_t pp = (S.c1[s2] - S.c0[s2] + 63) / 64 - 1; pp >= 0; pp--) bblk.push_back({(int32_t)s2, (int32_t)pp});
         P.bb_cnt = (int64_t)bblk.size() - P.bb_off;
     }
-    std::vector<PcbTask> pcb, fin;
+    std::vector<PcbTask> fin;
+    std::vector<EaBlk> eab;
     for (int32_t L = 0; L < S.nlevels; L++) {
         LevelPlan &P = F->plan[L];
-        P.pcb_off = (int64_t)pcb.size();
+        P.pcb_off = (int64_t)eab.size();
         for (int64_t s2 : bylevel[L])
             if (S.chptr[s2 + 1] > S.chptr[s2])
-                for (int64_t cb = 0; cb * 64 < S.F[s2]; cb++) pcb.push_back({(int32_t)s2, (int32_t)cb});
-        P.pcb_cnt = (int64_t)pcb.size() - P.pcb_off;
+                for (int64_t cb = 0; cb * 64 < S.F[s2]; cb++)
+                    for (int64_t rb = cb / 4; rb * 256 < S.F[s2]; rb++)   // the lower part: rows >= 64 cb
+                        eab.push_back({(int32_t)s2, (int32_t)cb, (int32_t)rb, 0});
+        P.pcb_cnt = (int64_t)eab.size() - P.pcb_off;
         P.fin_off = (int64_t)fin.size();
         for (int64_t s2 : bylevel[L])
             for (int64_t cb = 0; cb * 256 < S.F[s2]; cb++) fin.push_back({(int32_t)s2, (int32_t)cb});
@@ -1255,9 +1298,9 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     if ((rc = F->mem.alloc(&F->tick_b, (size_t)ns + 1))) return rc;
     SEHIP_CHECK(hipMemsetAsync(F->tick_f, 0, ((size_t)ns + 1) * sizeof(unsigned), st));
     SEHIP_CHECK(hipMemsetAsync(F->tick_b, 0, ((size_t)ns + 1) * sizeof(unsigned), st));
-    if (pcb.empty()) pcb.push_back({0, 0});
+    if (eab.empty()) eab.push_back({0, 0, 0, 0});
     if (fin.empty()) fin.push_back({0, 0});
-    if ((rc = upload(F.get(), &F->pcb, pcb))) return rc;
+    if ((rc = upload(F.get(), &F->eab, eab))) return rc;
     if ((rc = upload(F.get(), &F->fin, fin))) return rc;
     if (sea.empty()) sea.push_back({0, 0, 0, 0});
     if (ea.empty()) ea.push_back({0, 0, 0, 0});

@@ -14,8 +14,11 @@ struct EaTask {   // extend-add: child front columns [cb*64, cb*64+64) into the 
     int32_t child, parent, cb, pad;
 };
 
-struct PcbTask {   // a 64-column block of a parent front that receives extend-adds
+struct PcbTask {   // a block of a front: (front, block index)
     int32_t parent, pcb;
+};
+struct EaBlk {     // extend-add target: parent front, 64-column block, 256-row block
+    int32_t parent, cb, rb, pad;
 };
 
 struct TileTask {   // panel k of front s; tile row i, tile column j
@@ -47,7 +50,7 @@ struct LevelPlan {
     std::vector<std::vector<int64_t>> k_trsm_cut;
     int64_t fr_off = 0, fr_cnt = 0;   // fronts of the level in the fronts array
     std::vector<int64_t> sea_off, sea_cnt;   // (unused) per-ordinal solve extend-add
-    int64_t pcb_off = 0, pcb_cnt = 0;        // factor extend-add: parent column blocks
+    int64_t pcb_off = 0, pcb_cnt = 0;        // factor extend-add: parent (column, row) blocks
     int64_t fin_off = 0, fin_cnt = 0;        // solve init: (front, 256-row chunk)
     int64_t fb_off = 0, fb_cnt = 0, bb_off = 0, bb_cnt = 0;   // blocked solve work items
     int64_t ub_off = 0, ub_cnt = 0;                            // backward update-row chunks
@@ -81,7 +84,8 @@ struct Factor {
     bool use_reg_panel = true;       // one-wave register POTRF + TRSM as a product with L_kk^-1
     int solve_mode = 1;               // 0: one workgroup per front, 1: blocked multi-workgroup
     EaTask *sea = nullptr;
-    PcbTask *pcb = nullptr, *fin = nullptr;
+    PcbTask *fin = nullptr;
+    EaBlk *eab = nullptr;             // factor extend-add blocks
     UpdTask *upd = nullptr;
     double *bpart = nullptr;
     int64_t *partoff = nullptr;

@@ -296,21 +296,11 @@ __global__ void __launch_bounds__(256) k_rs_assemble(const RsRow *__restrict__ t
     *(f32x4 *)(buf + dst * RW + 4 * q4) = acc;
 }
 
-// output row (tile-local) `row`, right-hand side `col` of a tile product
+// the four tile products (output rows of one 64-row tile, all right-hand sides):
 //   MODE 0  Y      = X11 W1             out: Y rows of the front's columns
 //   MODE 1  U      = W2 - L21 Y         out: U rows (read-modify-write)
 //   MODE 2  Z      = Y - L21^T X_R      out: T rows
 //   MODE 3  X_c    = X11^T Z            out: X rows
-template <int MODE>
-__device__ __forceinline__ void rs_store(float val, int row, int col, const RsArgs &a, int s, int orow0,
-                                         float *__restrict__ buf, float *__restrict__ Yg, float *__restrict__ Xg)
-{
-    const int64_t c0 = a.col0[s];
-    if (MODE == 0) Yg[(c0 + orow0 + row) * RW + col] = val;
-    else if (MODE == 1) buf[(a.n + a.uoff[s] + orow0 + row) * RW + col] -= val;
-    else if (MODE == 2) buf[(a.t0 + c0 + orow0 + row) * RW + col] = Yg[(c0 + orow0 + row) * RW + col] - val;
-    else Xg[(c0 + orow0 + row) * RW + col] = val;
-}
 
 // operands of one 64-deep K tile for one wave (16 output rows): A in a[] (modes
 // 0, 1: directly in operand layout; modes 2, 3: column j = 16 wv + jj of the
@@ -416,12 +406,29 @@ __global__ void __launch_bounds__(256) k_rs_part(const T *__restrict__ pool, con
         }
     }
     if (tk.direct) {
+        // modes 1, 2 read before they write: all reads first (a store between
+        // them would serialize the loads)
+        float o0[4], o1[4];
+        const float *src = MODE == 1 ? buf + (a.n + a.uoff[s] + orow0) * RW : Yg + (c0 + orow0) * RW;
+        float *dst = MODE == 0 ? Yg + (c0 + orow0) * RW
+                   : MODE == 1 ? buf + (a.n + a.uoff[s] + orow0) * RW
+                   : MODE == 2 ? buf + (a.t0 + c0 + orow0) * RW : Xg + (c0 + orow0) * RW;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = 16 * wv + 4 * g + v;
+            o0[v] = 0.f;
+            o1[v] = 0.f;
+            if ((MODE == 1 || MODE == 2) && row < on) {
+                o0[v] = src[row * RW + lc];
+                o1[v] = src[row * RW + 16 + lc];
+            }
+        }
 #pragma unroll
         for (int v = 0; v < 4; v++) {
             const int row = 16 * wv + 4 * g + v;
             if (row < on) {
-                rs_store<MODE>(acc0[v], row, lc, a, s, orow0, buf, Yg, Xg);
-                rs_store<MODE>(acc1[v], row, 16 + lc, a, s, orow0, buf, Yg, Xg);
+                dst[row * RW + lc] = (MODE == 1 || MODE == 2) ? o0[v] - acc0[v] : acc0[v];
+                dst[row * RW + 16 + lc] = (MODE == 1 || MODE == 2) ? o1[v] - acc1[v] : acc1[v];
             }
         }
     } else {
@@ -440,26 +447,35 @@ __global__ void __launch_bounds__(256) k_rs_fin(const RsFin *__restrict__ tasks,
                                                 float *__restrict__ Yg, float *__restrict__ Xg,
                                                 const float *__restrict__ part)
 {
+    constexpr int PER = RTILE * RW / 256;      // elements per thread
     const RsFin t = tasks[blockIdx.x];
     const int s = t.s;
     const int32_t F = a.Fs[s], c = a.ncol[s];
     const int orow0 = t.I * RTILE;
     const int on = min(RTILE, (MODE == 1 ? F - c : c) - orow0);
-    for (int e = threadIdx.x; e < RTILE * RW; e += 256) {
-        const int row = e >> 5, col = e & 31;
-        if (row >= on) continue;
-        // four interleaved sums (fixed order) keep several loads in flight
-        const float *p = part + t.p0 * (RTILE * RW) + e;
-        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-        int q = 0;
-        for (; q + 4 <= t.np; q += 4) {
-            s0 += p[(int64_t)q * (RTILE * RW)];
-            s1 += p[(int64_t)(q + 1) * (RTILE * RW)];
-            s2 += p[(int64_t)(q + 2) * (RTILE * RW)];
-            s3 += p[(int64_t)(q + 3) * (RTILE * RW)];
-        }
-        for (; q < t.np; q++) s0 += p[(int64_t)q * (RTILE * RW)];
-        rs_store<MODE>((s0 + s1) + (s2 + s3), row, col, a, s, orow0, buf, Yg, Xg);
+    const int64_t c0 = a.col0[s];
+    // partial sums in a fixed order, each step's PER loads independent
+    float sum[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) sum[u] = 0.f;
+    const float *p = part + t.p0 * (RTILE * RW) + threadIdx.x;
+    for (int q = 0; q < t.np; q++)
+#pragma unroll
+        for (int u = 0; u < PER; u++) sum[u] += p[(int64_t)q * (RTILE * RW) + 256 * u];
+    const float *src = MODE == 1 ? buf + (a.n + a.uoff[s] + orow0) * RW : Yg + (c0 + orow0) * RW;
+    float *dst = MODE == 0 ? Yg + (c0 + orow0) * RW
+               : MODE == 1 ? buf + (a.n + a.uoff[s] + orow0) * RW
+               : MODE == 2 ? buf + (a.t0 + c0 + orow0) * RW : Xg + (c0 + orow0) * RW;
+    float o[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int e = threadIdx.x + 256 * u;
+        o[u] = ((MODE == 1 || MODE == 2) && (e >> 5) < on) ? src[e] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int e = threadIdx.x + 256 * u;
+        if ((e >> 5) < on) dst[e] = (MODE == 1 || MODE == 2) ? o[u] - sum[u] : sum[u];
     }
 }
 
