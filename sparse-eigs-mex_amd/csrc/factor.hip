@@ -218,15 +218,42 @@ __global__ void __launch_bounds__(256) k_trsm(T *pool, const TileTask *__restric
     }
 }
 
+// Update tile gi of a panel step, decoded instead of listed: ufid[gi] is the
+// front's ordinal f in the step's POTRF list pt, upref[f] its first tile; a
+// front's tiles run over column tiles j >= jstart (jstart = k when the panel
+// leaves rows of tile k below it) and, within a column, rows i >= j.
+__device__ __forceinline__ TileTask upd_decode(const TileTask *__restrict__ pt, const int32_t *__restrict__ ufid,
+                                               const int64_t *__restrict__ upref, const int32_t *__restrict__ Fs,
+                                               const int32_t *__restrict__ ncol, int64_t gi)
+{
+    const int f = ufid[gi];
+    TileTask t = pt[f];
+    const int32_t F = Fs[t.s], c = ncol[t.s];
+    const int kend = t.k * TB + min(TB, c - t.k * TB);
+    const int nt = (F + TB - 1) / TB;
+    const int jstart = kend < min(F, (t.k + 1) * TB) ? t.k : t.k + 1;
+    const int m = nt - jstart;
+    int64_t rem = gi - upref[f];
+    int tt = 0;
+    while (rem >= m - tt) {
+        rem -= m - tt;
+        tt++;
+    }
+    t.j = jstart + tt;
+    t.i = t.j + (int)rem;
+    return t;
+}
+
 // Schur update of tile (i, j): C -= L_i L_j^T over the panel (vector FMA, 4x4 per thread)
 template <class T>
-__global__ void __launch_bounds__(256) k_update(T *pool, const TileTask *__restrict__ tasks,
+__global__ void __launch_bounds__(256) k_update(T *pool, const TileTask *__restrict__ pt,
+                                                const int32_t *__restrict__ ufid, const int64_t *__restrict__ upref,
                                                 const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
                                                 const int32_t *__restrict__ ncol)
 {
     __shared__ T Ai[TB][TB + 4];   // [kk][row]
     __shared__ T Aj[TB][TB + 4];
-    const TileTask t = tasks[blockIdx.x];
+    const TileTask t = upd_decode(pt, ufid, upref, Fs, ncol, blockIdx.x);
     const int32_t F = Fs[t.s], c = ncol[t.s];
     const int k0 = t.k * TB, w = min(TB, c - k0), kend = k0 + w;
     const int r0 = max(t.i * TB, kend), r1 = min(F, (t.i + 1) * TB);
@@ -268,11 +295,13 @@ __global__ void __launch_bounds__(256) k_update(T *pool, const TileTask *__restr
 
 // ---- panel factorization on one wave -----------------------------------------
 // The diagonal tile of panel k lives in registers: lane r holds row r (a[q] =
-// L(r, q), q <= r).  Column p: the pivot comes from lane p by readlane, the
-// column is scaled, and the trailing rows are updated with the column entries
-// broadcast by readlane -- no barrier, no LDS.  Then the inverse X = L_kk^-1 is
-// formed column-wise (lane j holds column j of X) with the rows of L broadcast
-// the same way, and stored to the panel's scratch slot for the TRSM product.
+// L(r, q) for q <= r; entries above the diagonal are scratch that never feeds
+// the lower part).  Column p: the pivot comes from lane p by readlane, the
+// column is scaled, and each later column q is updated with L(q, p) broadcast
+// from lane q.  The same broadcasts drive the inverse: lane r carries column r
+// of X = L_kk^-1 as a right-looking forward substitution (t <- e_r;
+// x_p = t_p / L(p, p); t_q -= L(q, p) x_p), so the inverse costs FMAs only.
+// X is stored column-major to the panel's scratch slot for the TRSM product.
 __device__ __forceinline__ float bcast(float v, int lane)
 {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
@@ -297,9 +326,13 @@ __global__ void __launch_bounds__(64) k_potrf_reg(T *pool, const TileTask *__res
     const int k0 = t.k * TB, w = min(TB, c - k0);
     const int r = threadIdx.x;
     T *base = pool + foff[t.s] + (int64_t)k0 * F + k0;
-    T a[TB];
+    // rows beyond w are the identity (their columns stay e_r, their X rows too)
+    T a[TB], x[TB];
 #pragma unroll
-    for (int q = 0; q < TB; q++) a[q] = (r < w && q < w && q <= r) ? base[(int64_t)q * F + r] : T(q == r ? 1 : 0);
+    for (int q = 0; q < TB; q++) {
+        a[q] = (r < w && q < w && q <= r) ? base[(int64_t)q * F + r] : T(q == r ? 1 : 0);
+        x[q] = T(q == r ? 1 : 0);
+    }
     bool bad = false;
     const double *dA = diagA + col0[t.s] + k0;
 #pragma unroll
@@ -311,12 +344,17 @@ __global__ void __launch_bounds__(64) k_potrf_reg(T *pool, const TileTask *__res
                 bad = true;
                 d = T(1);
             }
+            // true divisions (not a reciprocal): the same rounding as a
+            // left-looking substitution, which the multiplicity tests rely on
             const T sd = sqrt(d);
-            a[p] = (r == p) ? sd : (r > p ? a[p] / sd : a[p]);
+            a[p] = (r == p) ? sd : a[p] / sd;
+            const T xp = x[p] / sd;
+            x[p] = xp;
 #pragma unroll
             for (int q = p + 1; q < TB; q++) {
                 const T lq = bcast(a[p], q);          // L(q, p)
-                if (r >= q) a[q] -= a[p] * lq;
+                a[q] -= a[p] * lq;
+                x[q] -= lq * xp;
             }
         }
     }
@@ -324,18 +362,9 @@ __global__ void __launch_bounds__(64) k_potrf_reg(T *pool, const TileTask *__res
 #pragma unroll
     for (int q = 0; q < TB; q++)
         if (r < w && q < w && q <= r) base[(int64_t)q * F + r] = a[q];
-    // X = L^-1 (padded with the identity beyond w): lane j holds column j
-    T x[TB];
-#pragma unroll
-    for (int i = 0; i < TB; i++) {
-        T s = (i == r) ? T(1) : T(0);
-#pragma unroll
-        for (int k = 0; k < i; k++) s -= bcast(a[k], i) * x[k];     // L(i, k) X(k, j)
-        x[i] = (i >= r) ? s / bcast(a[i], i) : T(0);
-    }
     T *X = xinv + (int64_t)blockIdx.x * (TB * TB);
 #pragma unroll
-    for (int i = 0; i < TB; i++) X[(int64_t)r * TB + i] = x[i];      // column-major 64 x 64
+    for (int i = 0; i < TB; i++) X[(int64_t)r * TB + i] = x[i];      // column r of X, column-major 64 x 64
 }
 
 // tile row i below the diagonal block: B <- B X^T with X = L_kk^-1 from the
@@ -383,14 +412,16 @@ __global__ void __launch_bounds__(256) k_trsm_mfma(T *pool, const TileTask *__re
 // Lanes 0-15 of each result value hold 16 consecutive rows of one column, so
 // the read-modify-write of the column-major front is coalesced.
 template <class T>
-__global__ void __launch_bounds__(256) k_update_mfma(T *pool, const TileTask *__restrict__ tasks,
+__global__ void __launch_bounds__(256) k_update_mfma(T *pool, const TileTask *__restrict__ pt,
+                                                     const int32_t *__restrict__ ufid,
+                                                     const int64_t *__restrict__ upref,
                                                      const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
                                                      const int32_t *__restrict__ ncol)
 {
     __shared__ T Ai[TB][TB + 4];   // [kk][row]
     __shared__ T Aj[TB][TB + 4];
     typedef typename MfmaAcc<T>::V V;
-    const TileTask t = tasks[blockIdx.x];
+    const TileTask t = upd_decode(pt, ufid, upref, Fs, ncol, blockIdx.x);
     const int32_t F = Fs[t.s], c = ncol[t.s];
     const int k0 = t.k * TB, w = min(TB, c - k0), kend = k0 + w;
     const int r0 = max(t.i * TB, kend), r1 = min(F, (t.i + 1) * TB);
@@ -979,12 +1010,15 @@ int run_numeric(Factor *F, double delta)
                                                                                      F->foff, F->Fs, F->ncol);
             }
             if (P.k_upd_cnt[k] > 0) {
+                const TileTask *pt = F->tiles + P.k_potrf_off[k];
+                const int32_t *uf = F->ufid + P.k_upd_off[k];
+                const int64_t *up = F->upref + P.k_upref_off[k];
                 if (F->use_mfma)
-                    k_update_mfma<T><<<dim3((unsigned)P.k_upd_cnt[k]), dim3(256), 0, st>>>(
-                        pool, F->tiles + P.k_upd_off[k], F->foff, F->Fs, F->ncol);
+                    k_update_mfma<T><<<dim3((unsigned)P.k_upd_cnt[k]), dim3(256), 0, st>>>(pool, pt, uf, up, F->foff,
+                                                                                           F->Fs, F->ncol);
                 else
-                    k_update<T><<<dim3((unsigned)P.k_upd_cnt[k]), dim3(256), 0, st>>>(pool, F->tiles + P.k_upd_off[k],
-                                                                                      F->foff, F->Fs, F->ncol);
+                    k_update<T><<<dim3((unsigned)P.k_upd_cnt[k]), dim3(256), 0, st>>>(pool, pt, uf, up, F->foff, F->Fs,
+                                                                                      F->ncol);
             }
         }
     }
@@ -1154,6 +1188,8 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     // plans: extend-add tasks, tile tasks, fronts per level
     std::vector<EaTask> ea;
     std::vector<TileTask> tiles;
+    std::vector<int32_t> ufid;
+    std::vector<int64_t> upref;
     std::vector<int32_t> fronts;
     std::vector<std::vector<int64_t>> bylevel(S.nlevels);
     for (int64_t s = 0; s < ns; s++) bylevel[S.level[s]].push_back(s);
@@ -1201,19 +1237,25 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
             }
             P.k_trsm_cut.back().push_back((int64_t)tiles.size());
             P.k_trsm_cnt.push_back((int64_t)tiles.size() - P.k_trsm_off.back());
-            P.k_upd_off.push_back((int64_t)tiles.size());
+            // update tiles: per front (in POTRF order) its tile count and ordinal
+            P.k_upd_off.push_back((int64_t)ufid.size());
+            P.k_upref_off.push_back((int64_t)upref.size());
+            int64_t acc = 0;
+            int32_t f = 0;
             for (int64_t s : bylevel[L]) {
                 const int64_t c = S.c1[s] - S.c0[s], Fv = S.F[s];
                 if (c <= k * TB) continue;
                 const int64_t kend = k * TB + std::min<int64_t>(TB, c - k * TB);
                 const int64_t nt = (Fv + TB - 1) / TB;
-                for (int64_t j = k; j < nt; j++) {
-                    if (std::max(j * TB, kend) >= std::min(Fv, (j + 1) * TB)) continue;
-                    for (int64_t i = j; i < nt; i++)
-                        tiles.push_back({(int32_t)s, (int32_t)k, (int32_t)i, (int32_t)j});
-                }
+                const int64_t jstart = kend < std::min(Fv, (k + 1) * TB) ? k : k + 1;
+                const int64_t m = nt - jstart, cnt = m * (m + 1) / 2;
+                upref.push_back(acc);
+                ufid.insert(ufid.end(), (size_t)cnt, f);
+                acc += cnt;
+                f++;
             }
-            P.k_upd_cnt.push_back((int64_t)tiles.size() - P.k_upd_off.back());
+            upref.push_back(acc);
+            P.k_upd_cnt.push_back(acc);
         }
     }
     trace_phase("factor task lists", tp);
@@ -1357,6 +1399,9 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     if ((rc = upload(F.get(), &F->perm, S.perm))) return rc;
     if ((rc = upload(F.get(), &F->ea, ea))) return rc;
     if ((rc = upload(F.get(), &F->tiles, tiles))) return rc;
+    if (ufid.empty()) ufid.push_back(0);
+    if ((rc = upload(F.get(), &F->ufid, ufid))) return rc;
+    if ((rc = upload(F.get(), &F->upref, upref))) return rc;
     if ((rc = F->mem.alloc(&F->flag, 1))) return rc;
     F->wssize = wsoff[ns];
     if ((rc = F->mem.alloc(&F->ws, (size_t)std::max<int64_t>(F->wssize, 1)))) return rc;

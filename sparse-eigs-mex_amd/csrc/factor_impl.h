@@ -44,7 +44,7 @@ constexpr int64_t kXinvSlots = 2048;   // diagonal-tile inverses held at once (6
 struct LevelPlan {
     std::vector<int64_t> ea_off;      // per child ordinal: offset in the extend-add task array
     std::vector<int64_t> ea_cnt;
-    std::vector<int64_t> k_potrf_off, k_potrf_cnt, k_trsm_off, k_trsm_cnt, k_upd_off, k_upd_cnt;
+    std::vector<int64_t> k_potrf_off, k_potrf_cnt, k_trsm_off, k_trsm_cnt, k_upd_off, k_upd_cnt, k_upref_off;
     // per panel step: start of each group of kXinvSlots fronts in the TRSM tasks
     // (plus the end); a group's inverses fit the scratch, slot = ordinal % kXinvSlots
     std::vector<std::vector<int64_t>> k_trsm_cut;
@@ -71,6 +71,8 @@ struct Factor {
     int64_t *perm = nullptr;
     EaTask *ea = nullptr;
     TileTask *tiles = nullptr;
+    int32_t *ufid = nullptr;          // update tiles: front ordinal in the step's POTRF list
+    int64_t *upref = nullptr;         // per step and front: first update tile
     std::vector<LevelPlan> plan;
     int *flag = nullptr;
     double *ws = nullptr, *xp = nullptr, *yp = nullptr, *r = nullptr, *d = nullptr, *ax = nullptr, *partial = nullptr;
