@@ -280,7 +280,11 @@ private:
             roots.push_back(leaf_node(q.data(), cnt));
             return;
         }
-        // choose the separator level: smallest level with both sides >= 30%
+        // choose the separator level among those leaving both sides >= 30%:
+        // the smallest size x (1 + |A - B| / 2n), so that a slightly larger
+        // level that halves the region wins over a smaller lopsided one (on
+        // 3-D grids a corner-rooted structure has near-equal middle levels;
+        // balanced splits there cut factor flops by ~10%)
         std::vector<int64_t> lsz(h + 1, 0);
         for (int64_t i = 0; i < cnt; i++) lsz[lev_[q[i]]]++;
         int32_t sep = -1, med = -1;
@@ -289,8 +293,12 @@ private:
             const int64_t before = cum;
             cum += lsz[l];
             if (med < 0 && 2 * cum >= cnt) med = l;
-            if (l > 0 && l < h && 10 * before >= 3 * cnt && 10 * (cnt - cum) >= 3 * cnt)
-                if (sep < 0 || lsz[l] < best) { sep = l; best = lsz[l]; }
+            if (l > 0 && l < h && 10 * before >= 3 * cnt && 10 * (cnt - cum) >= 3 * cnt) {
+                // compare lsz (1 + |before - after| / 2 cnt) in integers
+                const int64_t imb = before > cnt - cum ? before - (cnt - cum) : (cnt - cum) - before;
+                const int64_t sc = lsz[l] * (2 * cnt + imb);
+                if (sep < 0 || sc < best) { sep = l; best = sc; }
+            }
         }
         if (sep < 0) sep = med;
         sep = std::min(std::max(sep, 1), h - 1);
