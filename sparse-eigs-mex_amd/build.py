@@ -30,11 +30,18 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall", "-Wno
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", str(PKG.parent / "include")]
 
 
+# host-only sources (ordering, symbolic analysis, dense projected problems): AVX2 +
+# FMA, present on every x86-64 host an MI355X is paired with
+HOST_ISA = ["-mavx2", "-mfma"]
+
+
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.name + ".o")
     cmd = [hipcc(), *COMMON, "-c", str(src), "-o", str(obj)]
     if src.suffix == ".hip":
         cmd[1:1] = ["-x", "hip"]
+    else:
+        cmd[1:1] = HOST_ISA
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

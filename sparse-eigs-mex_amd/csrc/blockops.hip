@@ -13,13 +13,16 @@ namespace sehip {
 
 // ------------------------------------------------------------------ Gram
 // G = A^T B (p x q, p, q <= 96, column-major ld p) on the FP64 matrix cores.
-// Each wave owns a contiguous range of rows and all (<= 6 x 6) output tiles of
-// 16 x 16; v_mfma_f64_16x16x4 takes its operands straight from memory, 4 rows
-// per step: lane l loads A[r + l/16][16 ti + l%16] and B[r + l/16][16 tj + l%16]
-// (four 128-byte row pieces per load) and holds G[16 ti + l/16 + 4v][16 tj +
-// l%16].  The next step's operands are loaded before the current step's
-// products.  One partial per wave, then a fixed-order sum.
-constexpr int RG_WAVES = 1024;     // waves over the rows (4 per block)
+// A workgroup owns a contiguous range of rows; its four waves split the (<= 6
+// x 6) output tiles of 16 x 16 into 2 x 2 groups of <= 3 x 3 (each wave loads
+// only its column blocks, the shared rows come through L1), which keeps the
+// accumulators at 72 registers and several waves per SIMD in flight.
+// v_mfma_f64_16x16x4 takes its operands straight from memory, 4 rows per step:
+// lane l loads A[r + l/16][16 ti + l%16] and B[r + l/16][16 tj + l%16] (128-byte
+// row pieces) and holds G[16 ti + l/16 + 4v][16 tj + l%16].  The next step's
+// operands are loaded before the current step's products.  One partial per
+// workgroup, then a fixed-order sum.
+constexpr int RG_WAVES = 1024;     // workgroups over the rows
 
 typedef double rd4 __attribute__((ext_vector_type(4)));
 
@@ -27,49 +30,51 @@ __global__ void __launch_bounds__(256) k_rgram_partial(const double *__restrict_
                                                        const double *__restrict__ B, int64_t ldb, int q, int64_t n,
                                                        int64_t rows_per_wave, double *__restrict__ partial)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t wid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int lr = lane & 15, lk = lane >> 4;
     const int pt = (p + 15) >> 4, qt = (q + 15) >> 4;
-    rd4 acc[6][6];
+    const int a0 = 3 * (wv & 1), b0 = 3 * (wv >> 1);     // this wave's tile group
+    if (a0 >= pt || b0 >= qt) return;                     // no workgroup barrier below
+    rd4 acc[3][3];
 #pragma unroll
-    for (int a = 0; a < 6; a++)
+    for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int b = 0; b < 6; b++) acc[a][b] = rd4{0.0, 0.0, 0.0, 0.0};
-    const int64_t r0 = wid * rows_per_wave, r1 = min(n, r0 + rows_per_wave);
-    double xa[6], xb[6];
+        for (int b = 0; b < 3; b++) acc[a][b] = rd4{0.0, 0.0, 0.0, 0.0};
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_wave, r1 = min(n, r0 + rows_per_wave);
+    double xa[3], xb[3];
     auto load = [&](int64_t r) {
         const int64_t row = r + lk;
         const bool in = row < r1;
 #pragma unroll
-        for (int a = 0; a < 6; a++) {
-            const int c = 16 * a + lr;
-            xa[a] = (a < pt && in && c < p) ? A[row * lda + c] : 0.0;
-            xb[a] = (a < qt && in && c < q) ? B[row * ldb + c] : 0.0;
+        for (int a = 0; a < 3; a++) {
+            const int ca = 16 * (a0 + a) + lr, cb = 16 * (b0 + a) + lr;
+            xa[a] = (a0 + a < pt && in && ca < p) ? A[row * lda + ca] : 0.0;
+            xb[a] = (b0 + a < qt && in && cb < q) ? B[row * ldb + cb] : 0.0;
         }
     };
     if (r0 < r1) load(r0);
     for (int64_t r = r0; r < r1; r += 4) {
-        double ca[6], cb[6];
+        double ca[3], cb[3];
 #pragma unroll
-        for (int a = 0; a < 6; a++) { ca[a] = xa[a]; cb[a] = xb[a]; }
+        for (int a = 0; a < 3; a++) { ca[a] = xa[a]; cb[a] = xb[a]; }
         if (r + 4 < r1) load(r + 4);
 #pragma unroll
-        for (int a = 0; a < 6; a++)
+        for (int a = 0; a < 3; a++)
 #pragma unroll
-            for (int b = 0; b < 6; b++)
-                if (a < pt && b < qt) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[a], cb[b], acc[a][b], 0, 0, 0);
+            for (int b = 0; b < 3; b++)
+                if (a0 + a < pt && b0 + b < qt)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[a], cb[b], acc[a][b], 0, 0, 0);
     }
-    double *out = partial + wid * p * q;
+    double *out = partial + (int64_t)blockIdx.x * p * q;
 #pragma unroll
-    for (int a = 0; a < 6; a++)
+    for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int b = 0; b < 6; b++) {
-            if (a >= pt || b >= qt) continue;
-            const int j = 16 * b + lr;
+        for (int b = 0; b < 3; b++) {
+            if (a0 + a >= pt || b0 + b >= qt) continue;
+            const int j = 16 * (b0 + b) + lr;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int i = 16 * a + lk + 4 * v;
+                const int i = 16 * (a0 + a) + lk + 4 * v;
                 if (i < p && j < q) out[(int64_t)j * p + i] = acc[a][b][v];
             }
         }
@@ -101,7 +106,7 @@ __global__ void __launch_bounds__(256) k_rgram_reduce(const double *__restrict__
 
 static int64_t rgram_rpw(int64_t n) { return ((n + RG_WAVES - 1) / RG_WAVES + 3) / 4 * 4; }
 
-// partial slots (waves) of one Gram: a multiple of 4 waves covering n rows
+// partial slots (workgroups) of one Gram: a multiple of 4 covering n rows
 int64_t rgram_blocks(int64_t n)
 {
     const int64_t w = (n + rgram_rpw(n) - 1) / rgram_rpw(n);
@@ -112,7 +117,7 @@ void rgram(const double *A, int64_t lda, int p, const double *B, int64_t ldb, in
            double *G, hipStream_t st)
 {
     const int64_t rpw = rgram_rpw(n), nw = rgram_blocks(n);
-    k_rgram_partial<<<dim3((unsigned)(nw / 4)), dim3(256), 0, st>>>(A, lda, p, B, ldb, q, n, rpw, partial);
+    k_rgram_partial<<<dim3((unsigned)nw), dim3(256), 0, st>>>(A, lda, p, B, ldb, q, n, rpw, partial);
     k_rgram_reduce<<<dim3((unsigned)((p * q + 63) / 64)), dim3(256), 0, st>>>(partial, nw, p * q, G);
 }
 
@@ -127,8 +132,10 @@ constexpr int RM_CP = 100;   // LDS pitch of C rows
 
 constexpr int RM_GROUPS = 4;   // 16-row groups per wave
 
-__global__ void __launch_bounds__(256) k_rgemm(const double *__restrict__ V, int64_t ldv, int m,
-                                               const double *__restrict__ C, int p, double *__restrict__ X,
+// X may be V itself (same rows, same stride): a wave reads all of its 16 rows'
+// inputs before it writes any output row, and no other wave touches them
+__global__ void __launch_bounds__(256) k_rgemm(const double *V, int64_t ldv, int m,
+                                               const double *__restrict__ C, int p, double *X,
                                                int64_t ldx, int64_t n, double beta)
 {
     __shared__ double Cs[96][RM_CP];   // Cs[k][c] = C[k + c m]
@@ -194,9 +201,10 @@ void rgemm(const double *V, int64_t ldv, int m, const double *C, int p, double *
 
 // ------------------------------------------------------------------ SpMM
 // Y = A X for row-major X, Y (nv <= 96 columns): SELL-64 slices; a 16-lane
-// group per row, lane j of the group on column t0 + j, so each nonzero gathers
-// one contiguous 128-byte piece of row col of X.  A block = 16 rows, grid.y
-// over 16-column chunks.
+// group per row, lane j of the group on columns j, 16 + j, ..., so each
+// nonzero's value and column are read once for all columns and each gather of
+// row col of X is a contiguous 128-byte piece.  A block = 16 rows.
+template <int NC>
 __global__ void __launch_bounds__(256) k_sell_rspmm(int64_t nslices, const int64_t *__restrict__ sptr,
                                                     const int32_t *__restrict__ swid, const int32_t *__restrict__ col,
                                                     const double *__restrict__ val, const double *__restrict__ X,
@@ -206,26 +214,37 @@ __global__ void __launch_bounds__(256) k_sell_rspmm(int64_t nslices, const int64
     const int64_t row = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
     const int64_t sl = row >> 6;
     if (sl >= nslices || row >= n) return;
-    const int t = blockIdx.y * 16 + (threadIdx.x & 15);
-    const bool on = t < nv;
+    const int t = threadIdx.x & 15;
     int64_t pp = sptr[sl] + (row & 63);
     const int w = swid[sl];
-    double s = 0.0;
+    double s[NC];
+#pragma unroll
+    for (int j = 0; j < NC; j++) s[j] = 0.0;
     for (int k = 0; k < w; k++, pp += 64) {
         const double a = val[pp];
-        const int64_t c = col[pp];
-        if (on) s = fma(a, X[c * ldx + t], s);
+        const double *xr = X + (int64_t)col[pp] * ldx + t;
+#pragma unroll
+        for (int j = 0; j < NC; j++)
+            if (16 * j + t < nv) s[j] = fma(a, xr[16 * j], s[j]);
     }
-    if (on) Y[row * ldy + t] = s;
+#pragma unroll
+    for (int j = 0; j < NC; j++)
+        if (16 * j + t < nv) Y[row * ldy + 16 * j + t] = s[j];
 }
 
 void rspmm(const SellA &S, const double *X, int64_t ldx, double *Y, int64_t ldy, int nv, hipStream_t st)
 {
-    k_sell_rspmm<<<dim3((unsigned)((S.n + 15) / 16), (unsigned)((nv + 15) / 16)), dim3(256), 0, st>>>(
-        S.nslices, S.sptr, S.swid, S.col, S.val, X, ldx, Y, ldy, S.n, nv);
+    const dim3 g((unsigned)((S.n + 15) / 16)), b(256);
+    switch ((nv + 15) / 16) {
+    case 1: k_sell_rspmm<1><<<g, b, 0, st>>>(S.nslices, S.sptr, S.swid, S.col, S.val, X, ldx, Y, ldy, S.n, nv); break;
+    case 2: k_sell_rspmm<2><<<g, b, 0, st>>>(S.nslices, S.sptr, S.swid, S.col, S.val, X, ldx, Y, ldy, S.n, nv); break;
+    case 3: k_sell_rspmm<3><<<g, b, 0, st>>>(S.nslices, S.sptr, S.swid, S.col, S.val, X, ldx, Y, ldy, S.n, nv); break;
+    case 4: k_sell_rspmm<4><<<g, b, 0, st>>>(S.nslices, S.sptr, S.swid, S.col, S.val, X, ldx, Y, ldy, S.n, nv); break;
+    case 5: k_sell_rspmm<5><<<g, b, 0, st>>>(S.nslices, S.sptr, S.swid, S.col, S.val, X, ldx, Y, ldy, S.n, nv); break;
+    default: k_sell_rspmm<6><<<g, b, 0, st>>>(S.nslices, S.sptr, S.swid, S.col, S.val, X, ldx, Y, ldy, S.n, nv);
+    }
 }
 
-// ------------------------------------------------------------------ elementwise
 __global__ void k_rresid(const double *__restrict__ AX, int64_t ldax, const double *__restrict__ X, int64_t ldx,
                          const double *__restrict__ lam, int m, int64_t n, double *__restrict__ R, int64_t ldr)
 {
@@ -252,6 +271,52 @@ __global__ void k_rcopy(const double *__restrict__ X, int64_t ldx, int m, int64_
     const int c = (int)(e - r * m);
     Y[r * ldy + c] = X[r * ldx + c];
 }
+
+// R = AX - X diag(lam) (m <= 32 columns) and, for the first k columns, the sums
+// of squares: per block in a fixed grid-stride order, then one fixed-order sum
+constexpr int RN_BLOCKS = 1024;
+__global__ void __launch_bounds__(256) k_rresid_norm(const double *__restrict__ AX, int64_t ldax,
+                                                     const double *__restrict__ X, int64_t ldx,
+                                                     const double *__restrict__ lam, int m, int k, int64_t n,
+                                                     double *__restrict__ R, int64_t ldr, double *__restrict__ part)
+{
+    __shared__ double red[8][33];
+    const int c = threadIdx.x & 31, rl = threadIdx.x >> 5;
+    const double lc = c < m ? lam[c] : 0.0;
+    double acc = 0.0;
+    for (int64_t r = (int64_t)blockIdx.x * 8 + rl; r < n; r += (int64_t)RN_BLOCKS * 8) {
+        if (c < m) {
+            const double v = AX[r * ldax + c] - lc * X[r * ldx + c];
+            R[r * ldr + c] = v;
+            acc = fma(v, v, acc);
+        }
+    }
+    red[rl][c] = c < k ? acc : 0.0;
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        double t = 0.0;
+        for (int i = 0; i < 8; i++) t += red[i][threadIdx.x];
+        part[(int64_t)blockIdx.x * 32 + threadIdx.x] = t;
+    }
+}
+
+__global__ void k_norm_reduce(const double *__restrict__ part, int k, double *__restrict__ out)
+{
+    const int c = threadIdx.x;
+    if (c >= k) return;
+    double t = 0.0;
+    for (int b = 0; b < RN_BLOCKS; b++) t += part[(int64_t)b * 32 + c];
+    out[c] = t;
+}
+
+void rresid_norms(const double *AX, int64_t ldax, const double *X, int64_t ldx, const double *lam, int m, int k,
+                  int64_t n, double *R, int64_t ldr, double *part, double *out, hipStream_t st)
+{
+    k_rresid_norm<<<dim3(RN_BLOCKS), dim3(256), 0, st>>>(AX, ldax, X, ldx, lam, m, k, n, R, ldr, part);
+    k_norm_reduce<<<dim3(1), dim3(32), 0, st>>>(part, k, out);
+}
+
+int64_t rresid_norms_part() { return (int64_t)RN_BLOCKS * 32; }
 
 void rcopy(const double *X, int64_t ldx, int m, int64_t n, double *Y, int64_t ldy, hipStream_t st)
 {

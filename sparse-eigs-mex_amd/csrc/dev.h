@@ -112,7 +112,8 @@ int64_t rgram_blocks(int64_t n);
 // rgram_blocks(n) * p * q doubles
 void rgram(const double *A, int64_t lda, int p, const double *B, int64_t ldb, int q, int64_t n, double *partial,
            double *G, hipStream_t st);
-// X = V C + beta X, V n x m, C m x p column-major on the device, m, p <= 96; X must not alias V
+// X = V C + beta X, V n x m, C m x p column-major on the device, m, p <= 96; X may be V
+// itself (same stride, p <= m) but must not overlap it otherwise
 void rgemm(const double *V, int64_t ldv, int m, const double *C, int p, double *X, int64_t ldx, int64_t n,
            double beta, hipStream_t st);
 // Y = A X (nv <= 96)
@@ -121,6 +122,11 @@ void rspmm(const SellA &S, const double *X, int64_t ldx, double *Y, int64_t ldy,
 void rresid(const double *AX, int64_t ldax, const double *X, int64_t ldx, const double *lam, int m, int64_t n,
             double *R, int64_t ldr, hipStream_t st);
 void rcopy(const double *X, int64_t ldx, int m, int64_t n, double *Y, int64_t ldy, hipStream_t st);
+// R = AX - X diag(lam), m <= 32, and out[c] = ||R[:, c]||^2 for c < k (deterministic);
+// part must hold rresid_norms_part() doubles
+void rresid_norms(const double *AX, int64_t ldax, const double *X, int64_t ldx, const double *lam, int m, int k,
+                  int64_t n, double *R, int64_t ldr, double *part, double *out, hipStream_t st);
+int64_t rresid_norms_part();
 // layout changes against column-major (ld n) blocks, m <= 96
 void rto_cols(const double *X, int64_t ldx, int m, int64_t n, double *Y, hipStream_t st);
 void rfrom_cols(const double *Y, int m, int64_t n, double *X, int64_t ldx, hipStream_t st);

@@ -119,11 +119,20 @@ int dense_syev(int64_t n, double *A, double *d)
             for (int64_t i = k + 1; i < n; i++) aj[i] -= v[i] * pj + p[i] * vj;
         }
         e[k] = alpha;
-        for (int64_t r = 0; r < n; r++) {
-            double s = 0.0;
-            for (int64_t i = k + 1; i < n; i++) s += Q[r + i * n] * v[i];
-            s *= 2.0;
-            for (int64_t i = k + 1; i < n; i++) Q[r + i * n] -= s * v[i];
+        // Q <- Q (I - 2 v v^T): t = Q v by column sweeps, then each column of Q
+        // minus 2 v_i t (contiguous, vectorisable)
+        for (int64_t r = 0; r < n; r++) p[r] = 0.0;
+        for (int64_t i = k + 1; i < n; i++) {
+            const double vi = v[i];
+            const double *__restrict__ qi = Q.data() + i * n;
+            double *__restrict__ t = p.data();
+            for (int64_t r = 0; r < n; r++) t[r] += qi[r] * vi;
+        }
+        for (int64_t i = k + 1; i < n; i++) {
+            const double vi2 = 2.0 * v[i];
+            double *__restrict__ qi = Q.data() + i * n;
+            const double *__restrict__ t = p.data();
+            for (int64_t r = 0; r < n; r++) qi[r] -= vi2 * t[r];
         }
     }
     for (int64_t i = 0; i < n; i++) d[i] = A[i + i * n];
@@ -158,7 +167,8 @@ int dense_syev(int64_t n, double *A, double *d)
                 pp = s * r;
                 d[i + 1] = g + pp;
                 g = c * r - b;
-                double *zi = Q.data() + i * n, *zi1 = Q.data() + (i + 1) * n;
+                double *__restrict__ zi = Q.data() + i * n;
+                double *__restrict__ zi1 = Q.data() + (i + 1) * n;
                 for (int64_t q = 0; q < n; q++) {
                     const double zf = zi1[q];
                     zi1[q] = s * zi[q] + c * zf;

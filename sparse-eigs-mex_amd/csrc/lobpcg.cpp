@@ -38,8 +38,19 @@ struct LWork {
     int64_t n = 0;
     hipStream_t st = nullptr;
     DevMem mem;
-    double *partial = nullptr, *G = nullptr, *C = nullptr, *red = nullptr;
+    double *partial = nullptr, *G = nullptr, *C = nullptr, *red = nullptr, *npart = nullptr, *nrm = nullptr;
 };
+
+// R = AX - X theta into R (m columns) and the squared norms of its first k columns
+int resid_norms(LWork &W, const double *AX, const double *X, int64_t ld, const double *lam_dev, int m, int k,
+                double *R, std::vector<double> &out)
+{
+    rresid_norms(AX, ld, X, ld, lam_dev, m, k, W.n, R, ld, W.npart, W.nrm, W.st);
+    out.resize(k);
+    SEHIP_CHECK(hipMemcpyAsync(out.data(), W.nrm, (size_t)k * sizeof(double), hipMemcpyDeviceToHost, W.st));
+    SEHIP_CHECK(hipStreamSynchronize(W.st));
+    return 0;
+}
 
 int dgram(LWork &W, const double *A, int64_t lda, int p, const double *B, int64_t ldb, int q,
           std::vector<double> &out)
@@ -51,7 +62,7 @@ int dgram(LWork &W, const double *A, int64_t lda, int p, const double *B, int64_
     return 0;
 }
 
-// Y = V Cm + beta Y  (V n x m, Cm m x p on the host, column-major; Y must not alias V)
+// Y = V Cm + beta Y  (V n x m, Cm m x p on the host, column-major; Y may be V itself)
 int dgemm_tall(LWork &W, const double *V, int64_t ldv, int m, const std::vector<double> &Cm, int p, double *Y,
                int64_t ldy, double beta)
 {
@@ -153,18 +164,19 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
     W.n = n;
     W.st = st;
     const int64_t ldS = (3 * m + 3) / 4 * 4, ldT = (2 * m + 3) / 4 * 4;
-    double *S, *AS, *Sn, *ASn, *Pb, *T2, *v0;
+    double *S, *AS, *Sn, *ASn, *T2, *v0;
     if ((rc = W.mem.alloc(&S, (size_t)n * ldS))) return rc;     // [X | W | P]
     if ((rc = W.mem.alloc(&AS, (size_t)n * ldS))) return rc;
     if ((rc = W.mem.alloc(&Sn, (size_t)n * ldS))) return rc;
     if ((rc = W.mem.alloc(&ASn, (size_t)n * ldS))) return rc;
-    if ((rc = W.mem.alloc(&Pb, (size_t)n * m))) return rc;
     if ((rc = W.mem.alloc(&T2, (size_t)n * ldT))) return rc;
     if ((rc = W.mem.alloc(&v0, (size_t)n))) return rc;
     if ((rc = W.mem.alloc(&W.partial, (size_t)rgram_blocks(n) * 96 * 96))) return rc;
     if ((rc = W.mem.alloc(&W.G, 96 * 96))) return rc;
     if ((rc = W.mem.alloc(&W.C, 96 * 96))) return rc;
     if ((rc = W.mem.alloc(&W.red, 4 * 96))) return rc;
+    if ((rc = W.mem.alloc(&W.npart, (size_t)rresid_norms_part()))) return rc;
+    if ((rc = W.mem.alloc(&W.nrm, 32))) return rc;
 
     // start block: the proposed block of the learned start (its first m
     // columns), else column 0 = the reference's xorshift64* start vector (or
@@ -209,31 +221,29 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
     std::vector<double> res(m);
     double *lam_dev = W.red;
     for (;; it++) {
-        double *Z = S + m, *AZ = AS + m;
+        // S = [X | P | W] (no P in the first iteration): Z = [P W] is contiguous
+        double *Z = S + m, *AZ = AS + m, *Wb = S + m + np;
         SEHIP_CHECK(hipMemcpyAsync(lam_dev, theta.data(), m * sizeof(double), hipMemcpyHostToDevice, st));
-        rresid(AS, ldS, S, ldS, lam_dev, m, n, Z, ldS, st);
         {
             std::vector<double> g;
-            if ((rc = dgram(W, Z, ldS, k, Z, ldS, k, g))) return rc;
+            if ((rc = resid_norms(W, AS, S, ldS, lam_dev, m, (int)k, Wb, g))) return rc;
             nconv = 0;
             for (int i = 0; i < k; i++) {
-                res[i] = std::sqrt(std::max(g[(size_t)i * k + i], 0.0));
+                res[i] = std::sqrt(std::max(g[i], 0.0));
                 if (res[i] <= rtol) nconv++;
             }
         }
         if (nconv >= k || it >= maxit) break;
-        // W = T R: one FP32 multi-right-hand-side solve, in place on the rows of Z
-        if ((rc = factor_block_solve_rows(F, Z, ldS, 1, Z, ldS, 1, m))) return rc;
+        // W = T R: one FP32 multi-right-hand-side solve, in place on the rows of S
+        if ((rc = factor_block_solve_rows(F, Wb, ldS, 1, Wb, ldS, 1, m))) return rc;
         solves++;
-        if (np > 0) rcopy(Pb, m, np, n, Z + m, ldS, st);
         int nz = m + np;
         std::vector<double> zz;
         double keep = 0.0, cond = 0.0;
         if ((rc = project_out(W, S, ldS, m, nz, zz, &keep))) return rc;
         int r = 0;
-        if ((rc = svqb(W, Z, ldS, nz, zz, T2, ldT, &r, &cond))) return rc;
+        if ((rc = svqb(W, Z, ldS, nz, zz, Z, ldS, &r, &cond))) return rc;   // in place: row-local
         if (r == 0) break;
-        rcopy(T2, ldT, r, n, Z, ldS, st);
         // the second pass only where the first can have left Z short of
         // orthonormal: a column that lost more than half its norm to the
         // projection (Kahan's "twice is enough" test), a dropped direction, or
@@ -242,28 +252,42 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
         nz = r;
         if (again) {
             if ((rc = project_out(W, S, ldS, m, nz, zz))) return rc;
-            if ((rc = svqb(W, Z, ldS, nz, zz, T2, ldT, &r))) return rc;
+            if ((rc = svqb(W, Z, ldS, nz, zz, Z, ldS, &r))) return rc;
             if (r == 0) break;
-            rcopy(T2, ldT, r, n, Z, ldS, st);
             nz = r;
         }
         rspmm(A, Z, ldS, AZ, ldS, nz, st);
-        // Rayleigh-Ritz on S = [X Z] (orthonormal): H = S^T A S
+        // Rayleigh-Ritz on S = [X Z] (orthonormal): H = S^T A S.  X holds the
+        // previous Ritz vectors, so X^T A X = diag(theta) (as in Knyazev's and
+        // SciPy's LOBPCG) and only S^T A Z is formed: (m + nz) x nz
         const int d = m + nz;
-        std::vector<double> H;
-        if ((rc = dgram(W, S, ldS, d, AS, ldS, d, H))) return rc;
+        std::vector<double> G2, H((size_t)d * d, 0.0);
+        if ((rc = dgram(W, S, ldS, d, AZ, ldS, nz, G2))) return rc;
+        for (int t = 0; t < m; t++) H[(size_t)t * d + t] = theta[t];
+        for (int j = 0; j < nz; j++)
+            for (int t = 0; t < d; t++) {
+                H[(size_t)(m + j) * d + t] = G2[(size_t)j * d + t];
+                if (t < m) H[(size_t)t * d + m + j] = G2[(size_t)j * d + t];
+            }
         symmetrize(H, d);
-        std::vector<double> ev(d);
-        if (dense_syev(d, H.data(), ev.data())) return SPEIGS_ERR_LAPACK;
-        for (int t = 0; t < m; t++) theta[t] = ev[t];
-        std::vector<double> C((size_t)d * m), Cz((size_t)nz * m);
+        // only the m smallest Ritz pairs are kept: all values by QL, those m
+        // vectors by inverse iteration on the tridiagonal (O(d^2 m), not the
+        // O(d^3) of accumulating every rotation)
+        DenseTri tri;
+        if (dense_tri_values(d, H.data(), tri)) return SPEIGS_ERR_LAPACK;
+        std::vector<int64_t> sel(m);
+        for (int t = 0; t < m; t++) sel[t] = t;
+        // C2 = [C | C_P]: the Ritz coefficients and those of the new P (the Z
+        // rows of C), so [X_new P_new] = S C2 is one pass over S
+        std::vector<double> C2((size_t)d * 2 * m, 0.0);
+        if (dense_tri_vectors(tri, m, sel.data(), C2.data(), d)) return SPEIGS_ERR_LAPACK;
         for (int t = 0; t < m; t++) {
-            for (int i = 0; i < d; i++) C[(size_t)t * d + i] = H[(size_t)t * d + i];
-            for (int i = 0; i < nz; i++) Cz[(size_t)t * nz + i] = H[(size_t)t * d + m + i];
+            theta[t] = tri.w[t];
+            for (int i = m; i < d; i++) C2[(size_t)(m + t) * d + i] = C2[(size_t)t * d + i];
         }
-        if ((rc = dgemm_tall(W, S, ldS, d, C, m, Sn, ldS, 0.0))) return rc;      // new X
-        if ((rc = dgemm_tall(W, AS, ldS, d, C, m, ASn, ldS, 0.0))) return rc;    // new A X
-        if ((rc = dgemm_tall(W, Z, ldS, nz, Cz, m, Pb, m, 0.0))) return rc;      // new P
+        std::vector<double> C(C2.begin(), C2.begin() + (size_t)d * m);
+        if ((rc = dgemm_tall(W, S, ldS, d, C2, 2 * m, Sn, ldS, 0.0))) return rc;   // new X, new P
+        if ((rc = dgemm_tall(W, AS, ldS, d, C, m, ASn, ldS, 0.0))) return rc;       // new A X
         std::swap(S, Sn);
         std::swap(AS, ASn);
         np = m;
@@ -277,11 +301,10 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
     std::vector<double> lam(m);
     for (int t = 0; t < m; t++) lam[t] = xax[(size_t)t * m + t] / xx[(size_t)t * m + t];
     SEHIP_CHECK(hipMemcpyAsync(lam_dev, lam.data(), m * sizeof(double), hipMemcpyHostToDevice, st));
-    rresid(AS, ldS, S, ldS, lam_dev, m, n, S + m, ldS, st);
-    if ((rc = dgram(W, S + m, ldS, k, S + m, ldS, k, rr))) return rc;
+    if ((rc = resid_norms(W, AS, S, ldS, lam_dev, m, (int)k, S + m, rr))) return rc;
     nconv = 0;
     for (int t = 0; t < k; t++) {
-        const double r = std::sqrt(std::max(rr[(size_t)t * k + t], 0.0)) / std::sqrt(xx[(size_t)t * m + t]);
+        const double r = std::sqrt(std::max(rr[t], 0.0)) / std::sqrt(xx[(size_t)t * m + t]);
         lambda[t] = lam[t];
         if (resid) resid[t] = r;
         if (r <= rtol) nconv++;
