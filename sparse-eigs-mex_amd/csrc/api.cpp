@@ -568,7 +568,7 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
         if ((rc = mem.alloc(&xl, (size_t)n * MP_WIDTH))) return rc;
         const uint64_t seed = opts->seed ? opts->seed : 88172645463325252ULL;
         auto *mdl = const_cast<speigs_hip_model *>(hopts->model);
-        if ((rc = mp_infer_device(mdl, n, Ap, Ai, Ax, seed, xl))) return rc;
+        if ((rc = mp_infer_sell(mdl, op->A, seed, xl))) return rc;
         if (!block) {
             std::vector<float> X((size_t)n * MP_WIDTH);
             SEHIP_CHECK(hipMemcpy(X.data(), xl, X.size() * sizeof(float), hipMemcpyDeviceToHost));

@@ -13,5 +13,8 @@ int64_t mp_nparams(int layers);
 // output block X (n x 32, row-major FP32, device memory) for the graph of A
 int mp_infer_device(speigs_hip_model *m, int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax,
                     uint64_t seed, float *Xdev);
+// the same from the solver's device SELL matrix of A (no host graph build)
+struct SellA;
+int mp_infer_sell(speigs_hip_model *m, const SellA &A, uint64_t seed, float *Xdev);
 
 }  // namespace sehip

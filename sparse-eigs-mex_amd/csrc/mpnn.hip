@@ -133,16 +133,18 @@ __global__ void k_mp_place(const float *__restrict__ Q, const double *__restrict
     H[i * CH + c0 + c] = (float)(Q[i * CH + c] * scale[c]);
 }
 
-// Out_i = so_i sum_j |a_ij| si_j H_j (32 channels): a 32-lane group per row
+// Out_i = so_i sum_j |a_ij| si_j H_j for the first NCH of the 32 channels (32:
+// a layer's messages, 4: the diffused input channels): an NCH-lane group per row
+template <int NCH>
 __global__ void __launch_bounds__(256) k_mp_spmm(int64_t nslices, const int64_t *__restrict__ sptr,
                                                  const int32_t *__restrict__ swid, const int32_t *__restrict__ col,
                                                  const double *__restrict__ val, const float *__restrict__ H,
                                                  const double *__restrict__ si, const double *__restrict__ so,
                                                  float *__restrict__ Out, int64_t n)
 {
-    const int64_t row = (int64_t)blockIdx.x * 8 + (threadIdx.x >> 5);
+    const int64_t row = (int64_t)blockIdx.x * (256 / NCH) + (threadIdx.x / NCH);
     if (row >= n) return;
-    const int c = threadIdx.x & 31;
+    const int c = threadIdx.x % NCH;
     const int64_t sl = row >> 6;
     int64_t pp = sptr[sl] + (row & 63);
     const int w = swid[sl];
@@ -384,10 +386,92 @@ int graph_build(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *A
     return 0;
 }
 
+// The graph from the solver's SELL matrix of A (already on the device): the
+// same slices, columns and padding, values |a_ij| off the diagonal and 0 on it;
+// per row the diagonal d (first entry in the row's own column: padding entries
+// follow the stored ones), s = sum |a_ij| in the row's column order, 1/d, and
+// per block the partial sums of log d, log s and their counts.
+__global__ void __launch_bounds__(256) k_mp_graph(int64_t n, const int64_t *__restrict__ sptr,
+                                                  const int32_t *__restrict__ swid, const int32_t *__restrict__ col,
+                                                  const double *__restrict__ val, double *__restrict__ val2,
+                                                  double *__restrict__ d, double *__restrict__ s,
+                                                  double *__restrict__ dinv, double *__restrict__ part)
+{
+    __shared__ double red[4][256];
+    const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double ld = 0.0, cd = 0.0, ls = 0.0, cs = 0.0;
+    if (row < n) {
+        const int64_t sl = row >> 6;
+        int64_t pp = sptr[sl] + (row & 63);
+        const int w = swid[sl];
+        double dv = 0.0, sv = 0.0;
+        bool seen = false;
+        for (int k = 0; k < w; k++, pp += 64) {
+            const double v = val[pp];
+            if (col[pp] == row) {
+                if (!seen) dv = v;
+                seen = true;
+                val2[pp] = 0.0;
+            } else {
+                val2[pp] = fabs(v);
+                sv += fabs(v);
+            }
+        }
+        d[row] = dv;
+        s[row] = sv;
+        dinv[row] = dv > 0 ? 1.0 / dv : 0.0;
+        if (dv > 0) { ld = log(dv); cd = 1.0; }
+        if (sv > 0) { ls = log(sv); cs = 1.0; }
+    }
+    red[0][threadIdx.x] = ld;
+    red[1][threadIdx.x] = cd;
+    red[2][threadIdx.x] = ls;
+    red[3][threadIdx.x] = cs;
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double t = 0.0;
+        for (int i = 0; i < 256; i++) t += red[threadIdx.x][i];
+        part[(int64_t)blockIdx.x * 4 + threadIdx.x] = t;
+    }
+}
+
+int graph_from_sell(const SellA &A, Graph &G, DevMem &mem, hipStream_t st)
+{
+    const int64_t n = A.n, nb = (n + 255) / 256;
+    G.n = n;
+    G.S = A;
+    double *val2 = nullptr, *part = nullptr;
+    int rc;
+    if ((rc = mem.alloc(&val2, (size_t)std::max<int64_t>(A.stored, 1)))) return rc;
+    if ((rc = mem.alloc(&G.d, (size_t)n))) return rc;
+    if ((rc = mem.alloc(&G.s, (size_t)n))) return rc;
+    if ((rc = mem.alloc(&G.dinv, (size_t)n))) return rc;
+    if ((rc = mem.alloc(&part, (size_t)nb * 4))) return rc;
+    k_mp_graph<<<dim3((unsigned)nb), dim3(256), 0, st>>>(n, A.sptr, A.swid, A.col, A.val, val2, G.d, G.s, G.dinv,
+                                                         part);
+    G.S.val = val2;
+    std::vector<double> hp((size_t)nb * 4);
+    SEHIP_CHECK(hipMemcpyAsync(hp.data(), part, hp.size() * 8, hipMemcpyDeviceToHost, st));
+    SEHIP_CHECK(hipStreamSynchronize(st));
+    double t[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t b = 0; b < nb; b++)
+        for (int c = 0; c < 4; c++) t[c] += hp[b * 4 + c];
+    G.mlogd = t[1] > 0 ? t[0] / t[1] : 0.0;
+    G.mlogs = t[3] > 0 ? t[2] / t[3] : 0.0;
+    return 0;
+}
+
 void spmm(const Graph &G, const float *H, const double *si, const double *so, float *Out, hipStream_t st)
 {
-    k_mp_spmm<<<dim3((unsigned)((G.n + 7) / 8)), dim3(256), 0, st>>>(G.S.nslices, G.S.sptr, G.S.swid, G.S.col,
-                                                                     G.S.val, H, si, so, Out, G.n);
+    k_mp_spmm<32><<<dim3((unsigned)((G.n + 7) / 8)), dim3(256), 0, st>>>(G.S.nslices, G.S.sptr, G.S.swid, G.S.col,
+                                                                         G.S.val, H, si, so, Out, G.n);
+}
+
+// the diffusion of the 4 input noise channels (channels 0..3 of H and Out)
+void spmm4(const Graph &G, const float *H, const double *so, float *Out, hipStream_t st)
+{
+    k_mp_spmm<4><<<dim3((unsigned)((G.n + 63) / 64)), dim3(256), 0, st>>>(G.S.nslices, G.S.sptr, G.S.swid, G.S.col,
+                                                                          G.S.val, H, nullptr, so, Out, G.n);
 }
 
 constexpr int DIFF1 = 8, DIFF2 = 64;   // diffusion steps of the multi-scale inputs
@@ -409,7 +493,7 @@ int features(speigs_hip_model *m, const Graph &G, uint64_t seed, float *H, float
     for (int stage = 0; stage < 2; stage++) {
         const int target = stage == 0 ? DIFF1 : DIFF2;
         for (; done < target; done++) {
-            spmm(G, Q, nullptr, G.dinv, Q2, st);
+            spmm4(G, Q, G.dinv, Q2, st);
             std::swap(Q, Q2);
         }
         k_mp_sumsq<<<dim3((unsigned)nb), dim3(256), 0, st>>>(Q, n, part);
@@ -617,6 +701,24 @@ int mp_infer_device(speigs_hip_model *m, int64_t n, const int64_t *Ap, const int
     if ((rc = forward(m, G, seed, B.Hs, B.Ms))) return rc;
     SEHIP_CHECK(hipMemcpyAsync(Xdev, B.Hs[m->layers], (size_t)n * CH * 4, hipMemcpyDeviceToDevice, m->st));
     SEHIP_CHECK(hipStreamSynchronize(m->st));
+    return 0;
+}
+
+int mp_infer_sell(speigs_hip_model *m, const SellA &A, uint64_t seed, float *Xdev)
+{
+    double tp = wall();
+    DevMem gm;
+    Graph G;
+    int rc;
+    if ((rc = graph_from_sell(A, G, gm, m->st))) return rc;
+    trace_phase("start network: graph", tp);
+    Buffers B;
+    if ((rc = buffers(m, A.n, 0, B, false))) return rc;
+    trace_phase("start network: buffers", tp);
+    if ((rc = forward(m, G, seed, B.Hs, B.Ms))) return rc;
+    SEHIP_CHECK(hipMemcpyAsync(Xdev, B.Hs[m->layers], (size_t)A.n * CH * 4, hipMemcpyDeviceToDevice, m->st));
+    SEHIP_CHECK(hipStreamSynchronize(m->st));
+    trace_phase("start network: forward", tp);
     return 0;
 }
 
