@@ -296,10 +296,25 @@ __global__ void __launch_bounds__(256) k_rs_assemble(const RsRow *__restrict__ t
     *(f32x4 *)(buf + dst * RW + 4 * q4) = acc;
 }
 
+// backward: X at the update rows of the level's larger fronts, gathered once
+// into their U rows (free after the forward sweep), so the transposed product
+// streams them like the forward one streams Y
+__global__ void __launch_bounds__(256) k_rs_gather(const RsRow *__restrict__ tasks, RsArgs a,
+                                                   const float *__restrict__ Xg, float *__restrict__ buf)
+{
+    const RsRow t = tasks[blockIdx.x];
+    const int s = t.s;
+    const int32_t r = a.Fs[s] - a.ncol[s];
+    const int i = t.I * 32 + (threadIdx.x >> 3), q4 = threadIdx.x & 7;
+    if (i >= r) return;
+    const int64_t g = a.rows[a.rptr[s] + i];
+    *(f32x4 *)(buf + (a.n + a.uoff[s] + i) * RW + 4 * q4) = *(const f32x4 *)(Xg + g * RW + 4 * q4);
+}
+
 // the four tile products (output rows of one 64-row tile, all right-hand sides):
 //   MODE 0  Y      = X11 W1             out: Y rows of the front's columns
 //   MODE 1  U      = W2 - L21 Y         out: U rows (read-modify-write)
-//   MODE 2  Z      = Y - L21^T X_R      out: T rows
+//   MODE 2  Z      = Y - L21^T X_R      out: T rows (X_R from the U rows)
 //   MODE 3  X_c    = X11^T Z            out: X rows
 
 // operands of one 64-deep K tile for one wave (16 output rows): A in a[] (modes
@@ -309,7 +324,7 @@ __global__ void __launch_bounds__(256) k_rs_assemble(const RsRow *__restrict__ t
 template <class T, int MODE>
 __device__ __forceinline__ void rs_load(const T *__restrict__ L, int F, int c, int I, int K, int kn, int orow0, int on,
                                         int wv, int g, int lc, int lane, const RsArgs &a, int64_t c0,
-                                        const int32_t *__restrict__ R, const float *__restrict__ buf,
+                                        int64_t ub, const float *__restrict__ buf,
                                         const float *__restrict__ Yg, const float *__restrict__ Xg, float (&av)[16],
                                         float (&b0)[16], float (&b1)[16])
 {
@@ -323,7 +338,7 @@ __device__ __forceinline__ void rs_load(const T *__restrict__ L, int F, int c, i
                 const float *src;
                 if (MODE == 0 || MODE == 3) src = buf + (a.t0 + c0 + kk) * RW;
                 else if (MODE == 1) src = Yg + (c0 + kk) * RW;
-                else src = Xg + (int64_t)R[kk] * RW;
+                else src = buf + (ub + kk) * RW;     // X_R, gathered into the U rows
                 v0 = src[lc];
                 v1 = src[16 + lc];
             }
@@ -372,11 +387,11 @@ __global__ void __launch_bounds__(256) k_rs_part(const T *__restrict__ pool, con
     if (16 * wv >= on) return;     // this wave's rows are past the tile (no workgroup barrier below)
     const T *L = pool + a.foff[s];
     const int64_t c0 = a.col0[s];
-    const int32_t *R = a.rows + a.rptr[s];
+    const int64_t ub = a.n + a.uoff[s];      // mode 2: X at the update rows
     const int kdim = MODE == 2 ? r : c;
     f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
     float av[16], b0[16], b1[16];
-    rs_load<T, MODE>(L, F, c, I, tk.k0, min(RTILE, kdim - tk.k0 * RTILE), orow0, on, wv, g, lc, lane, a, c0, R, buf,
+    rs_load<T, MODE>(L, F, c, I, tk.k0, min(RTILE, kdim - tk.k0 * RTILE), orow0, on, wv, g, lc, lane, a, c0, ub, buf,
                      Yg, Xg, av, b0, b1);
     for (int K = tk.k0; K < tk.k1; K++) {
         float ca[16], c0v[16], c1v[16];
@@ -388,7 +403,7 @@ __global__ void __launch_bounds__(256) k_rs_part(const T *__restrict__ pool, con
         }
         if (K + 1 < tk.k1)     // next tile's loads in flight during this tile's products
             rs_load<T, MODE>(L, F, c, I, K + 1, min(RTILE, kdim - (K + 1) * RTILE), orow0, on, wv, g, lc, lane, a,
-                             c0, R, buf, Yg, Xg, av, b0, b1);
+                             c0, ub, buf, Yg, Xg, av, b0, b1);
         if (MODE == 2 || MODE == 3) {
 #pragma unroll
             for (int jj = 0; jj < 16; jj++) At[wv][lane][jj] = ca[jj];
@@ -505,6 +520,7 @@ __global__ void __launch_bounds__(256) k_rs_out(const float *__restrict__ Xg, co
 struct RsLevel {
     int64_t sm_off = 0, sm_cnt = 0;          // tiny fronts
     int64_t as_off = 0, as_cnt = 0;          // assembly chunks (every front of the level)
+    int64_t gx_off = 0, gx_cnt = 0;          // backward gathers of X_R (the larger fronts)
     int64_t pt_off[4] = {0, 0, 0, 0}, pt_cnt[4] = {0, 0, 0, 0};
     int64_t ft_off[4] = {0, 0, 0, 0}, ft_cnt[4] = {0, 0, 0, 0};
 };
@@ -513,7 +529,7 @@ struct RsState {
     DevMem mem;
     std::vector<RsLevel> lev;
     int32_t *small = nullptr;
-    RsRow *asmt = nullptr;
+    RsRow *asmt = nullptr, *gxt = nullptr;
     RsPart *pt = nullptr;
     RsFin *ft = nullptr;
     float *part = nullptr, *buf = nullptr, *Yg = nullptr, *Xg = nullptr;
@@ -603,7 +619,7 @@ int rs_prepare(Factor *F, RsState &B)
     std::vector<std::vector<int64_t>> bylevel(S.nlevels);
     for (int64_t s = 0; s < ns; s++) bylevel[S.level[s]].push_back(s);
     std::vector<int32_t> small;
-    std::vector<RsRow> asmt;
+    std::vector<RsRow> asmt, gxt;
     std::vector<RsPart> pt;
     std::vector<RsFin> ft;
     int64_t maxpart = 1;
@@ -625,6 +641,12 @@ int rs_prepare(Factor *F, RsState &B)
             for (int64_t I = 0; I * 32 < S.F[s]; I++) asmt.push_back({(int32_t)s, (int32_t)I});
         }
         P.as_cnt = (int64_t)asmt.size() - P.as_off;
+        P.gx_off = (int64_t)gxt.size();
+        for (int64_t s : big) {
+            const int64_t r = S.F[s] - (S.c1[s] - S.c0[s]);
+            for (int64_t I = 0; I * 32 < r; I++) gxt.push_back({(int32_t)s, (int32_t)I});
+        }
+        P.gx_cnt = (int64_t)gxt.size() - P.gx_off;
         for (int mode = 0; mode < 4; mode++) {
             const int64_t p0 = (int64_t)pt.size(), f0 = (int64_t)ft.size();
             plan_parts(S, big, mode, pt, ft, p0);
@@ -637,6 +659,7 @@ int rs_prepare(Factor *F, RsState &B)
     }
     if (small.empty()) small.push_back(0);
     if (asmt.empty()) asmt.push_back({0, 0});
+    if (gxt.empty()) gxt.push_back({0, 0});
     if (pt.empty()) pt.push_back({0, 0, 0, 0, 0, 0});
     if (ft.empty()) ft.push_back({0, 0, 0, 0, 0});
     int rc;
@@ -646,7 +669,7 @@ int rs_prepare(Factor *F, RsState &B)
         SEHIP_CHECK(hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, F->st));
         return 0;
     };
-    if ((rc = up(&B.small, small)) || (rc = up(&B.asmt, asmt)) || (rc = up(&B.pt, pt)) || (rc = up(&B.ft, ft)) ||
+    if ((rc = up(&B.small, small)) || (rc = up(&B.asmt, asmt)) || (rc = up(&B.gxt, gxt)) || (rc = up(&B.pt, pt)) || (rc = up(&B.ft, ft)) ||
         (rc = up(&B.frow, frow)) || (rc = up(&B.uoff, uoff)) || (rc = up(&B.lptr, lptr)) || (rc = up(&B.lst, lst)))
         return rc;
     if ((rc = B.mem.alloc(&B.part, (size_t)maxpart * RTILE * RW))) return rc;
@@ -703,6 +726,7 @@ void rs_run(Factor *F, RsState &B, const double *Bin, int64_t brs, int64_t bcs, 
     }
     for (int32_t L = S.nlevels - 1; L >= 0; L--) {
         const RsLevel &P = B.lev[L];
+        if (P.gx_cnt) k_rs_gather<<<dim3((unsigned)P.gx_cnt), blk, 0, st>>>(B.gxt + P.gx_off, a, B.Xg, B.buf);
         parts(2, P);
         parts(3, P);
         if (P.sm_cnt)
