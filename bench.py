@@ -81,7 +81,7 @@ def cpu_baseline(seconds_budget: float, k: int, tol: float, seed: int):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=7)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--N", type=int, default=100)
     ap.add_argument("--k", type=int, default=20)
@@ -160,6 +160,7 @@ def main():
            "t_numeric": hi.get("t_numeric"), "t_invert": hi.get("t_invert"), "t_solve": hi.get("t_solve")}
     for p, (ts, rr) in results.items():
         cfg[f"{p}_median_s"] = round(statistics.median(ts), 4)
+        cfg[f"{p}_mad_s"] = round(statistics.median([abs(t - statistics.median(ts)) for t in ts]), 4)
         cfg[f"{p}_nops"] = rr.info["nops"]
     if "fp64" in results and "chosen" in results:
         f64 = statistics.median(results["fp64"][0])
