@@ -93,84 +93,65 @@ __global__ void __launch_bounds__(256) k_inv_diag(T *pool, const InvTask *__rest
     }
 }
 
-// Split-K products of one inversion step J (MFMA): part task (s, J, I, k0, k1)
-// writes P = sum_{k0<=K<k1} X_IK L_KJ (64 x 64, column-major) to its scratch
-// slot, with J < k0 < k1 <= I + 1 and X_II lower triangular.
-struct InvPart {
-    int32_t s, J, I, k0, k1, pad;
-};
-// finish task: L_IJ <- -(sum of its np partials, fixed order) X_JJ
-struct InvFin {
-    int32_t s, J, I, np;
-    int64_t p0;
+// Recursive doubling over the 64-column tiles of L11: with the inverses X11,
+// X22 of two adjacent diagonal blocks of b tiles known,
+//     inv([L11 0; L21 L22]) = [X11 0; -X22 L21 X11  X22],
+// formed as T = L21 X11 (scratch) then X21 = -X22 T (over L21).  Level b
+// merges every pair of b-tile blocks of every front at once (two launches), so
+// the whole inverse takes 2 log2(tiles) launches after the diagonal tiles.
+// Task: output tile (i, j) of the pair whose second block starts at tile row
+// R1 and whose first block starts at tile C1, with b tiles per block; T of the
+// pair at scratch tiles tslot + i b + j (column-major 64 x 64).
+struct InvG {
+    int32_t s, b, R1, C1, i, j;
+    int64_t tslot;
 };
 
 constexpr int IP = TB + 4;   // LDS row pitch of the inverse kernels
 
-template <class T>
-__global__ void __launch_bounds__(256) k_inv_part(const T *__restrict__ pool, const InvPart *__restrict__ tasks,
+// MODE 0: T(i, j) = sum_{k=j}^{b-1} L(R1+i, C1+k) X(C1+k, C1+j)
+// MODE 1: L(R1+i, C1+j) = -sum_{k=0}^{i} X(R1+i, R1+k) T(k, j)
+// (X diagonal tiles lower triangular; tiles clipped to the order c of L11)
+template <class T, int MODE>
+__global__ void __launch_bounds__(256) k_inv_gemm(T *pool, const InvG *__restrict__ tasks,
                                                   const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
-                                                  const int32_t *__restrict__ ncol, T *__restrict__ part)
+                                                  const int32_t *__restrict__ ncol, T *__restrict__ scratch)
 {
-    __shared__ T Xt[TB][IP];   // Xt[kk][row] = X_IK(row, kk)
-    __shared__ T Lt[TB][IP];   // Lt[kk][col] = L_KJ(kk, col)
+    __shared__ T At[TB][IP];   // At[kk][row] = A(row, kk)
+    __shared__ T Bt[TB][IP];   // Bt[kk][col] = B(kk, col)
     typedef typename MfmaAcc<T>::V V;
-    const InvPart t = tasks[blockIdx.x];
+    const InvG t = tasks[blockIdx.x];
     const int32_t F = Fs[t.s], c = ncol[t.s];
-    const int J = t.J, I = t.I;
-    const int wJ = tile_w(c, J), wI = tile_w(c, I);
-    const T *fr = pool + foff[t.s];
+    T *fr = pool + foff[t.s];
+    const int ti = t.R1 + t.i;                   // output tile row (absolute)
+    const int tj = t.C1 + t.j;                   // output tile column (absolute)
+    const int wi = tile_w(c, ti), wj = tile_w(c, tj);
+    const int kbeg = MODE == 0 ? t.j : 0, kend = MODE == 0 ? t.b : t.i + 1;
     V acc[2][2];
     zero_acc<T>(acc);
-    for (int K = t.k0; K < t.k1; K++) {
-        const int wK = tile_w(c, K);
+    for (int k = kbeg; k < kend; k++) {
+        const int tk = (MODE == 0 ? t.C1 : t.R1) + k;   // the inner tile (absolute)
+        const int wk = tile_w(c, tk);
+        if (wk <= 0) break;
         __syncthreads();
         for (int e = threadIdx.x; e < TB * TB; e += 256) {
             const int r = e & 63, q = e >> 6;
-            T xv = T(0);
-            if (r < wI && q < wK && (K < I || r >= q)) xv = fr[(int64_t)(K * TB + q) * F + I * TB + r];
-            Xt[q][r] = xv;
-            Lt[r][q] = (r < wK && q < wJ) ? fr[(int64_t)(J * TB + q) * F + K * TB + r] : T(0);
+            T av = T(0), bv = T(0);
+            if (MODE == 0) {
+                // A = L(ti, tk)(r, q); B = X(tk, tj)(r, q), lower on the diagonal tile
+                if (r < wi && q < wk) av = fr[(int64_t)(tk * TB + q) * F + ti * TB + r];
+                if (r < wk && q < wj && (k > t.j || r >= q)) bv = fr[(int64_t)(tj * TB + q) * F + tk * TB + r];
+            } else {
+                // A = X(ti, tk)(r, q), lower on the diagonal tile; B = T(k, j)(r, q)
+                if (r < wi && q < wk && (k < t.i || r >= q)) av = fr[(int64_t)(tk * TB + q) * F + ti * TB + r];
+                bv = scratch[(t.tslot + (int64_t)k * t.b + t.j) * (TB * TB) + q * TB + r];
+            }
+            At[q][r] = av;
+            Bt[r][q] = bv;
         }
         __syncthreads();
-        tile_mma<T, IP>(Xt, Lt, (wK + 3) & ~3, acc);
+        tile_mma<T, IP>(At, Bt, (wk + 3) & ~3, acc);
     }
-    T *out = part + (int64_t)blockIdx.x * (TB * TB);
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int rb = (wv & 1) * 32, cb = (wv >> 1) * 32, lr = lane & 15, lk = lane >> 4;
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 2; b++)
-#pragma unroll
-            for (int v = 0; v < 4; v++)
-                out[(cb + 16 * b + MfmaAcc<T>::colof(lk, v)) * TB + rb + 16 * a + lr] = acc[a][b][v];
-}
-
-template <class T>
-__global__ void __launch_bounds__(256) k_inv_fin(T *pool, const InvFin *__restrict__ tasks,
-                                                 const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
-                                                 const int32_t *__restrict__ ncol, const T *__restrict__ part)
-{
-    __shared__ T Tt[TB][IP];   // Tt[kk][row] = P(row, kk)
-    __shared__ T Xd[TB][IP];   // Xd[kk][col] = X_JJ(kk, col)
-    typedef typename MfmaAcc<T>::V V;
-    const InvFin t = tasks[blockIdx.x];
-    const int32_t F = Fs[t.s], c = ncol[t.s];
-    const int J = t.J, I = t.I;
-    const int wJ = tile_w(c, J), wI = tile_w(c, I);
-    T *fr = pool + foff[t.s];
-    for (int e = threadIdx.x; e < TB * TB; e += 256) {
-        const int r = e & 63, q = e >> 6;
-        T sum = T(0);
-        for (int p = 0; p < t.np; p++) sum += part[(t.p0 + p) * (TB * TB) + e];
-        Tt[q][r] = sum;
-        Xd[r][q] = (r < wJ && q < wJ && r >= q) ? fr[(int64_t)(J * TB + q) * F + J * TB + r] : T(0);
-    }
-    __syncthreads();
-    V acc[2][2];
-    zero_acc<T>(acc);
-    tile_mma<T, IP>(Tt, Xd, (wJ + 3) & ~3, acc);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int rb = (wv & 1) * 32, cb = (wv >> 1) * 32, lr = lane & 15, lk = lane >> 4;
 #pragma unroll
@@ -180,11 +161,13 @@ __global__ void __launch_bounds__(256) k_inv_fin(T *pool, const InvFin *__restri
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int row = rb + 16 * a + lr, col = cb + 16 * b + MfmaAcc<T>::colof(lk, v);
-                if (row < wI && col < wJ) fr[(int64_t)(J * TB + col) * F + I * TB + row] = -acc[a][b][v];
+                if (MODE == 0)
+                    scratch[(t.tslot + (int64_t)t.i * t.b + t.j) * (TB * TB) + col * TB + row] =
+                        (row < wi && col < wj) ? acc[a][b][v] : T(0);
+                else if (row < wi && col < wj)
+                    fr[(int64_t)(tj * TB + col) * F + ti * TB + row] = -acc[a][b][v];
             }
 }
-
-constexpr int INV_KCHUNK = 8;   // K tiles per part task
 
 template <class T>
 static int invert_impl(Factor *F)
@@ -192,78 +175,53 @@ static int invert_impl(Factor *F)
     const Symbolic &S = F->S;
     hipStream_t st = F->st;
     T *pool = (T *)F->pool;
-    // the fronts' inverses are independent: step st2 handles tile J = nb - 1 - st2
-    // of every front with more than st2 tiles, whatever its tree level, so the
-    // number of steps is the largest front's tile count.  Fronts are taken
-    // largest first so that their long part tasks start early in each launch.
-    std::vector<int64_t> order(S.ns);
-    for (int64_t s = 0; s < S.ns; s++) order[s] = s;
-    std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
-        return (S.c1[x] - S.c0[x]) > (S.c1[y] - S.c0[y]);
-    });
-    int64_t nbmax = 0;
-    for (int64_t s = 0; s < S.ns; s++) nbmax = std::max<int64_t>(nbmax, (S.c1[s] - S.c0[s] + TB - 1) / TB);
-    // all task lists of all steps, uploaded once
+    // every diagonal tile of every front, then the merge levels
     std::vector<InvTask> diag;
-    std::vector<InvPart> parts;
-    std::vector<InvFin> fins;
-    struct Step { int64_t d_off, d_cnt, p_off, p_cnt, f_off, f_cnt; };
-    std::vector<Step> steps;
-    int64_t pmax = 1;
-    {
-        for (int64_t st2 = 0; st2 < nbmax; st2++) {
-            Step sp;
-            sp.d_off = (int64_t)diag.size();
-            sp.p_off = (int64_t)parts.size();
-            sp.f_off = (int64_t)fins.size();
-            for (int64_t s : order) {
-                const int64_t nb = (S.c1[s] - S.c0[s] + TB - 1) / TB;
-                if (st2 >= nb) break;        // sorted by size: no later front has this step
-                const int32_t J = (int32_t)(nb - 1 - st2);
-                diag.push_back({(int32_t)s, J, J, 0});
-                for (int32_t I = J + 1; I < nb; I++) {
-                    InvFin f;
-                    f.s = (int32_t)s;
-                    f.J = J;
-                    f.I = I;
-                    f.p0 = (int64_t)parts.size() - sp.p_off;
-                    for (int32_t k0 = J + 1; k0 <= I; k0 += INV_KCHUNK)
-                        parts.push_back({(int32_t)s, J, I, k0, std::min<int32_t>(I + 1, k0 + INV_KCHUNK), 0});
-                    f.np = (int32_t)((int64_t)parts.size() - sp.p_off - f.p0);
-                    fins.push_back(f);
-                }
-            }
-            sp.d_cnt = (int64_t)diag.size() - sp.d_off;
-            sp.p_cnt = (int64_t)parts.size() - sp.p_off;
-            sp.f_cnt = (int64_t)fins.size() - sp.f_off;
-            pmax = std::max(pmax, sp.p_cnt);
-            steps.push_back(sp);
-        }
+    int64_t nbmax = 0;
+    for (int64_t s = 0; s < S.ns; s++) {
+        const int64_t nb = (S.c1[s] - S.c0[s] + TB - 1) / TB;
+        nbmax = std::max(nbmax, nb);
+        for (int32_t J = 0; J < nb; J++) diag.push_back({(int32_t)s, J, J, 0});
     }
     if (diag.empty()) return 0;
-    if (parts.empty()) parts.push_back({0, 0, 0, 0, 0, 0});
-    if (fins.empty()) fins.push_back({0, 0, 0, 0, 0});
+    std::vector<InvG> g;
+    struct Level { int64_t off, cnt; };
+    std::vector<Level> lev;
+    int64_t tmax = 1;
+    for (int64_t b = 1; b < nbmax; b *= 2) {
+        Level L{(int64_t)g.size(), 0};
+        int64_t tslot = 0;
+        for (int64_t s = 0; s < S.ns; s++) {
+            const int64_t nb = (S.c1[s] - S.c0[s] + TB - 1) / TB;
+            for (int64_t c1 = 0; c1 + b < nb; c1 += 2 * b) {
+                const int64_t r1 = c1 + b, b2 = std::min(b, nb - r1);
+                for (int64_t i = 0; i < b2; i++)
+                    for (int64_t j = 0; j < b; j++)
+                        g.push_back({(int32_t)s, (int32_t)b, (int32_t)r1, (int32_t)c1, (int32_t)i, (int32_t)j, tslot});
+                tslot += b2 * b;
+            }
+        }
+        L.cnt = (int64_t)g.size() - L.off;
+        if (L.cnt) lev.push_back(L);
+        tmax = std::max(tmax, tslot);
+    }
+    if (g.empty()) g.push_back({0, 1, 0, 0, 0, 0, 0});
     DevMem tmp;
     InvTask *d_diag = nullptr;
-    InvPart *d_parts = nullptr;
-    InvFin *d_fins = nullptr;
+    InvG *d_g = nullptr;
     T *scratch = nullptr;
     int rc;
     if ((rc = tmp.alloc(&d_diag, diag.size()))) return rc;
-    if ((rc = tmp.alloc(&d_parts, parts.size()))) return rc;
-    if ((rc = tmp.alloc(&d_fins, fins.size()))) return rc;
-    if ((rc = tmp.alloc(&scratch, (size_t)pmax * TB * TB))) return rc;
+    if ((rc = tmp.alloc(&d_g, g.size()))) return rc;
+    if ((rc = tmp.alloc(&scratch, (size_t)tmax * TB * TB))) return rc;
     SEHIP_CHECK(hipMemcpyAsync(d_diag, diag.data(), diag.size() * sizeof(InvTask), hipMemcpyHostToDevice, st));
-    SEHIP_CHECK(hipMemcpyAsync(d_parts, parts.data(), parts.size() * sizeof(InvPart), hipMemcpyHostToDevice, st));
-    SEHIP_CHECK(hipMemcpyAsync(d_fins, fins.data(), fins.size() * sizeof(InvFin), hipMemcpyHostToDevice, st));
-    for (const Step &sp : steps) {
-        k_inv_diag<T><<<dim3((unsigned)sp.d_cnt), dim3(256), 0, st>>>(pool, d_diag + sp.d_off, F->foff, F->Fs, F->ncol);
-        if (sp.f_cnt) {
-            k_inv_part<T><<<dim3((unsigned)sp.p_cnt), dim3(256), 0, st>>>(pool, d_parts + sp.p_off, F->foff, F->Fs,
-                                                                         F->ncol, scratch);
-            k_inv_fin<T><<<dim3((unsigned)sp.f_cnt), dim3(256), 0, st>>>(pool, d_fins + sp.f_off, F->foff, F->Fs,
-                                                                        F->ncol, scratch);
-        }
+    SEHIP_CHECK(hipMemcpyAsync(d_g, g.data(), g.size() * sizeof(InvG), hipMemcpyHostToDevice, st));
+    k_inv_diag<T><<<dim3((unsigned)diag.size()), dim3(256), 0, st>>>(pool, d_diag, F->foff, F->Fs, F->ncol);
+    for (const Level &L : lev) {
+        k_inv_gemm<T, 0><<<dim3((unsigned)L.cnt), dim3(256), 0, st>>>(pool, d_g + L.off, F->foff, F->Fs, F->ncol,
+                                                                     scratch);
+        k_inv_gemm<T, 1><<<dim3((unsigned)L.cnt), dim3(256), 0, st>>>(pool, d_g + L.off, F->foff, F->Fs, F->ncol,
+                                                                     scratch);
     }
     SEHIP_CHECK(hipStreamSynchronize(st));
     return 0;

@@ -218,69 +218,103 @@ __global__ void __launch_bounds__(256) k_trsm(T *pool, const TileTask *__restric
     }
 }
 
+// Schur updates are outer-blocked: blocks of KB panels (256 columns).  After
+// panel k the columns left in its block, [kend, bend), are updated at once with
+// panel k alone (the "inner" update, so the next panels of the block see their
+// final values); everything right of the block, [bend, F), is updated once per
+// block with all of its panels, K = bend - 256 kb <= 256 (the "outer" update,
+// after the block's last panel).  The trailing matrix's C tiles are then read
+// and written once per 256 columns instead of once per panel.
+//
 // Update tile gi of a panel step, decoded instead of listed: ufid[gi] is the
 // front's ordinal f in the step's POTRF list pt, upref[f] its first tile; a
-// front's tiles run over column tiles j >= jstart (jstart = k when the panel
-// leaves rows of tile k below it) and, within a column, rows i >= j.
-__device__ __forceinline__ TileTask upd_decode(const TileTask *__restrict__ pt, const int32_t *__restrict__ ufid,
-                                               const int64_t *__restrict__ upref, const int32_t *__restrict__ Fs,
-                                               const int32_t *__restrict__ ncol, int64_t gi)
+// front's tiles run over column tiles j >= j0 and, within a column, rows
+// i >= j.  The result is the tile's K range and its clipped rows / columns.
+struct UpdTile {
+    int32_t s;
+    int k0, kw, r0, r1, q0, q1;
+};
+
+__device__ __forceinline__ UpdTile upd_decode(const TileTask *__restrict__ pt, const int32_t *__restrict__ ufid,
+                                              const int64_t *__restrict__ upref, const int32_t *__restrict__ Fs,
+                                              const int32_t *__restrict__ ncol, int64_t gi, int outer)
 {
     const int f = ufid[gi];
-    TileTask t = pt[f];
+    const TileTask t = pt[f];
     const int32_t F = Fs[t.s], c = ncol[t.s];
     const int kend = t.k * TB + min(TB, c - t.k * TB);
+    const int kb = t.k / KB, bend = min((kb + 1) * KB * TB, c);
     const int nt = (F + TB - 1) / TB;
-    const int jstart = kend < min(F, (t.k + 1) * TB) ? t.k : t.k + 1;
-    const int m = nt - jstart;
+    int lo, hi, j0;
+    UpdTile u;
+    u.s = t.s;
+    if (outer) {
+        lo = bend;
+        hi = F;
+        j0 = bend / TB;
+        u.k0 = kb * KB * TB;
+        u.kw = bend - u.k0;
+    } else {
+        lo = kend;
+        hi = bend;
+        j0 = kend < min(F, (t.k + 1) * TB) ? t.k : t.k + 1;
+        u.k0 = t.k * TB;
+        u.kw = kend - u.k0;
+    }
+    const int m = nt - j0;
     int64_t rem = gi - upref[f];
     int tt = 0;
     while (rem >= m - tt) {
         rem -= m - tt;
         tt++;
     }
-    t.j = jstart + tt;
-    t.i = t.j + (int)rem;
-    return t;
+    const int j = j0 + tt, i = j + (int)rem;
+    u.r0 = max(i * TB, lo);
+    u.r1 = min(F, (i + 1) * TB);
+    u.q0 = max(j * TB, lo);
+    u.q1 = min(hi, (j + 1) * TB);
+    return u;
 }
 
-// Schur update of tile (i, j): C -= L_i L_j^T over the panel (vector FMA, 4x4 per thread)
+// Schur update of a C tile: C -= L_i L_j^T over K = [k0, k0 + kw) (vector FMA,
+// 4 x 4 per thread, 64-column slabs through LDS)
 template <class T>
 __global__ void __launch_bounds__(256) k_update(T *pool, const TileTask *__restrict__ pt,
                                                 const int32_t *__restrict__ ufid, const int64_t *__restrict__ upref,
-                                                const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
-                                                const int32_t *__restrict__ ncol)
+                                                int outer, const int64_t *__restrict__ foff,
+                                                const int32_t *__restrict__ Fs, const int32_t *__restrict__ ncol)
 {
     __shared__ T Ai[TB][TB + 4];   // [kk][row]
     __shared__ T Aj[TB][TB + 4];
-    const TileTask t = upd_decode(pt, ufid, upref, Fs, ncol, blockIdx.x);
-    const int32_t F = Fs[t.s], c = ncol[t.s];
-    const int k0 = t.k * TB, w = min(TB, c - k0), kend = k0 + w;
-    const int r0 = max(t.i * TB, kend), r1 = min(F, (t.i + 1) * TB);
-    const int q0 = max(t.j * TB, kend), q1 = min(F, (t.j + 1) * TB);
-    const int nr = r1 - r0, nq = q1 - q0;
+    const UpdTile u = upd_decode(pt, ufid, upref, Fs, ncol, blockIdx.x, outer);
+    const int32_t F = Fs[u.s];
+    const int r0 = u.r0, q0 = u.q0, nr = u.r1 - u.r0, nq = u.q1 - u.q0;
     if (nr <= 0 || nq <= 0) return;
-    T *fr = pool + foff[t.s];
-    for (int e = threadIdx.x; e < TB * w; e += 256) {
-        const int r = e % TB, kk = e / TB;
-        Ai[kk][r] = r < nr ? fr[(int64_t)(k0 + kk) * F + r0 + r] : T(0);
-        Aj[kk][r] = r < nq ? fr[(int64_t)(k0 + kk) * F + q0 + r] : T(0);
-    }
-    __syncthreads();
+    T *fr = pool + foff[u.s];
     const int ri = 4 * (threadIdx.x & 15), qj = 4 * (threadIdx.x >> 4);
     T acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int b = 0; b < 4; b++) acc[a][b] = T(0);
-    for (int kk = 0; kk < w; kk++) {
-        T x[4], y[4];
+    for (int sb = 0; sb < u.kw; sb += TB) {
+        const int w = min(TB, u.kw - sb), kc = u.k0 + sb;
+        __syncthreads();
+        for (int e = threadIdx.x; e < TB * w; e += 256) {
+            const int r = e % TB, kk = e / TB;
+            Ai[kk][r] = r < nr ? fr[(int64_t)(kc + kk) * F + r0 + r] : T(0);
+            Aj[kk][r] = r < nq ? fr[(int64_t)(kc + kk) * F + q0 + r] : T(0);
+        }
+        __syncthreads();
+        for (int kk = 0; kk < w; kk++) {
+            T x[4], y[4];
 #pragma unroll
-        for (int a = 0; a < 4; a++) { x[a] = Ai[kk][ri + a]; y[a] = Aj[kk][qj + a]; }
+            for (int a = 0; a < 4; a++) { x[a] = Ai[kk][ri + a]; y[a] = Aj[kk][qj + a]; }
 #pragma unroll
-        for (int a = 0; a < 4; a++)
+            for (int a = 0; a < 4; a++)
 #pragma unroll
-            for (int b = 0; b < 4; b++) acc[a][b] = fma(x[a], y[b], acc[a][b]);
+                for (int b = 0; b < 4; b++) acc[a][b] = fma(x[a], y[b], acc[a][b]);
+        }
     }
 #pragma unroll
     for (int b = 0; b < 4; b++) {
@@ -407,43 +441,42 @@ __global__ void __launch_bounds__(256) k_trsm_mfma(T *pool, const TileTask *__re
             }
 }
 
-// Schur update of tile (i, j) on the matrix cores: each wave owns a 32 x 32
-// quarter of the 64 x 64 tile as 2 x 2 blocks of 16 x 16, K = 4 per MFMA.
-// Lanes 0-15 of each result value hold 16 consecutive rows of one column, so
-// the read-modify-write of the column-major front is coalesced.
+// Schur update of a C tile on the matrix cores: each wave owns a 32 x 32
+// quarter of the 64 x 64 tile as 2 x 2 blocks of 16 x 16, K = 4 per MFMA, the K
+// range in 64-column slabs staged through LDS while the next slab's loads are
+// in flight.  Lanes 0-15 of each result value hold 16 consecutive rows of one
+// column, so the read-modify-write of the column-major front is coalesced.
+// Every global load is issued before the first store (a store into the front
+// between loads would order them).
 template <class T>
 __global__ void __launch_bounds__(256) k_update_mfma(T *pool, const TileTask *__restrict__ pt,
                                                      const int32_t *__restrict__ ufid,
-                                                     const int64_t *__restrict__ upref,
+                                                     const int64_t *__restrict__ upref, int outer,
                                                      const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
                                                      const int32_t *__restrict__ ncol)
 {
     __shared__ T Ai[TB][TB + 4];   // [kk][row]
     __shared__ T Aj[TB][TB + 4];
     typedef typename MfmaAcc<T>::V V;
-    const TileTask t = upd_decode(pt, ufid, upref, Fs, ncol, blockIdx.x);
-    const int32_t F = Fs[t.s], c = ncol[t.s];
-    const int k0 = t.k * TB, w = min(TB, c - k0), kend = k0 + w;
-    const int r0 = max(t.i * TB, kend), r1 = min(F, (t.i + 1) * TB);
-    const int q0 = max(t.j * TB, kend), q1 = min(F, (t.j + 1) * TB);
-    const int nr = r1 - r0, nq = q1 - q0;
+    const UpdTile u = upd_decode(pt, ufid, upref, Fs, ncol, blockIdx.x, outer);
+    const int32_t F = Fs[u.s];
+    const int r0 = u.r0, q0 = u.q0, nr = u.r1 - u.r0, nq = u.q1 - u.q0;
     if (nr <= 0 || nq <= 0) return;
-    T *fr = pool + foff[t.s];
-    const int w4 = (w + 3) & ~3;
+    T *fr = pool + foff[u.s];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int rb = (wv & 1) * 32, cb = (wv >> 1) * 32;
     const int lr = lane & 15, lk = lane >> 4;
-    // every global load of the task -- both panel tiles and this thread's
-    // elements of C -- is issued before the first store, so the task pays one
-    // memory latency (a store to the front between loads would order them)
     T va[16], vb[16], cv[2][2][4];
+    auto load = [&](int kc, int w) {
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const int kk = wv + 4 * q;
-        const bool in = kk < w;
-        va[q] = (in && lane < nr) ? fr[(int64_t)(k0 + kk) * F + r0 + lane] : T(0);
-        vb[q] = (in && lane < nq) ? fr[(int64_t)(k0 + kk) * F + q0 + lane] : T(0);
-    }
+        for (int q = 0; q < 16; q++) {
+            const int kk = wv + 4 * q;
+            const bool in = kk < w;
+            va[q] = (in && lane < nr) ? fr[(int64_t)(kc + kk) * F + r0 + lane] : T(0);
+            vb[q] = (in && lane < nq) ? fr[(int64_t)(kc + kk) * F + q0 + lane] : T(0);
+        }
+    };
+    load(u.k0, min(TB, u.kw));
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
@@ -453,28 +486,33 @@ __global__ void __launch_bounds__(256) k_update_mfma(T *pool, const TileTask *__
                 const int row = rb + 16 * a + lr, col = cb + 16 * b + MfmaAcc<T>::colof(lk, v);
                 cv[a][b][v] = (row < nr && col < nq) ? fr[(int64_t)(q0 + col) * F + r0 + row] : T(0);
             }
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-        Ai[wv + 4 * q][lane] = va[q];
-        Aj[wv + 4 * q][lane] = vb[q];
-    }
-    __syncthreads();
     V acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) acc[a][b] = V{T(0), T(0), T(0), T(0)};
-    for (int kk = 0; kk < w4; kk += 4) {
-        T xi[2], xj[2];
+    for (int sb = 0; sb < u.kw; sb += TB) {
+        const int w = min(TB, u.kw - sb), w4 = (w + 3) & ~3;
+        if (sb > 0) __syncthreads();          // the previous slab's products are done
 #pragma unroll
-        for (int a = 0; a < 2; a++) {
-            xi[a] = Ai[kk + lk][rb + 16 * a + lr];
-            xj[a] = Aj[kk + lk][cb + 16 * a + lr];
+        for (int q = 0; q < 16; q++) {
+            Ai[wv + 4 * q][lane] = va[q];
+            Aj[wv + 4 * q][lane] = vb[q];
         }
+        __syncthreads();
+        if (sb + TB < u.kw) load(u.k0 + sb + TB, min(TB, u.kw - sb - TB));
+        for (int kk = 0; kk < w4; kk += 4) {
+            T xi[2], xj[2];
 #pragma unroll
-        for (int a = 0; a < 2; a++)
+            for (int a = 0; a < 2; a++) {
+                xi[a] = Ai[kk + lk][rb + 16 * a + lr];
+                xj[a] = Aj[kk + lk][cb + 16 * a + lr];
+            }
 #pragma unroll
-            for (int b = 0; b < 2; b++) acc[a][b] = MfmaAcc<T>::blk(xi[a], xj[b], acc[a][b]);
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) acc[a][b] = MfmaAcc<T>::blk(xi[a], xj[b], acc[a][b]);
+        }
     }
 #pragma unroll
     for (int a = 0; a < 2; a++)
@@ -1009,16 +1047,19 @@ int run_numeric(Factor *F, double delta)
                     k_trsm<T><<<dim3((unsigned)P.k_trsm_cnt[k]), dim3(256), 0, st>>>(pool, F->tiles + P.k_trsm_off[k],
                                                                                      F->foff, F->Fs, F->ncol);
             }
-            if (P.k_upd_cnt[k] > 0) {
-                const TileTask *pt = F->tiles + P.k_potrf_off[k];
-                const int32_t *uf = F->ufid + P.k_upd_off[k];
-                const int64_t *up = F->upref + P.k_upref_off[k];
+            // inner then outer updates: disjoint columns of the trailing matrix
+            const TileTask *pt = F->tiles + P.k_potrf_off[k];
+            for (int outer = 0; outer < 2; outer++) {
+                const int64_t cnt = outer ? P.k_out_cnt[k] : P.k_upd_cnt[k];
+                if (cnt <= 0) continue;
+                const int32_t *uf = F->ufid + (outer ? P.k_out_off[k] : P.k_upd_off[k]);
+                const int64_t *up = F->upref + (outer ? P.k_outref_off[k] : P.k_upref_off[k]);
                 if (F->use_mfma)
-                    k_update_mfma<T><<<dim3((unsigned)P.k_upd_cnt[k]), dim3(256), 0, st>>>(pool, pt, uf, up, F->foff,
-                                                                                           F->Fs, F->ncol);
+                    k_update_mfma<T><<<dim3((unsigned)cnt), dim3(256), 0, st>>>(pool, pt, uf, up, outer, F->foff,
+                                                                                F->Fs, F->ncol);
                 else
-                    k_update<T><<<dim3((unsigned)P.k_upd_cnt[k]), dim3(256), 0, st>>>(pool, pt, uf, up, F->foff, F->Fs,
-                                                                                      F->ncol);
+                    k_update<T><<<dim3((unsigned)cnt), dim3(256), 0, st>>>(pool, pt, uf, up, outer, F->foff, F->Fs,
+                                                                           F->ncol);
             }
         }
     }
@@ -1237,25 +1278,48 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
             }
             P.k_trsm_cut.back().push_back((int64_t)tiles.size());
             P.k_trsm_cnt.push_back((int64_t)tiles.size() - P.k_trsm_off.back());
-            // update tiles: per front (in POTRF order) its tile count and ordinal
+            // update tiles, per front (in POTRF order) its tile count and ordinal:
+            // inner = columns [kend, bend) of the panel's block, outer = columns
+            // [bend, F) after the block's last panel (see upd_decode)
             P.k_upd_off.push_back((int64_t)ufid.size());
             P.k_upref_off.push_back((int64_t)upref.size());
-            int64_t acc = 0;
+            P.k_out_off.push_back((int64_t)ufid.size());
+            int64_t acc = 0, acco = 0;
             int32_t f = 0;
+            std::vector<int64_t> ocnt;
             for (int64_t s : bylevel[L]) {
                 const int64_t c = S.c1[s] - S.c0[s], Fv = S.F[s];
                 if (c <= k * TB) continue;
                 const int64_t kend = k * TB + std::min<int64_t>(TB, c - k * TB);
                 const int64_t nt = (Fv + TB - 1) / TB;
-                const int64_t jstart = kend < std::min(Fv, (k + 1) * TB) ? k : k + 1;
-                const int64_t m = nt - jstart, cnt = m * (m + 1) / 2;
+                const int64_t bend = std::min<int64_t>((k / KB + 1) * KB * TB, c);
+                const int64_t j0 = kend < std::min(Fv, (k + 1) * TB) ? k : k + 1;
+                const int64_t j1 = (bend + TB - 1) / TB;
+                int64_t cnt = 0;
+                for (int64_t j = j0; j < j1; j++) cnt += nt - j;
                 upref.push_back(acc);
                 ufid.insert(ufid.end(), (size_t)cnt, f);
                 acc += cnt;
+                int64_t oc = 0;
+                if (k == j1 - 1 && bend < Fv) {
+                    const int64_t m = nt - bend / TB;
+                    oc = m * (m + 1) / 2;
+                }
+                ocnt.push_back(oc);
                 f++;
             }
             upref.push_back(acc);
             P.k_upd_cnt.push_back(acc);
+            // the outer tiles of this step follow in ufid, their prefix in upref
+            P.k_out_off.back() = (int64_t)ufid.size();
+            P.k_outref_off.push_back((int64_t)upref.size());
+            for (size_t q = 0; q < ocnt.size(); q++) {
+                upref.push_back(acco);
+                ufid.insert(ufid.end(), (size_t)ocnt[q], (int32_t)q);
+                acco += ocnt[q];
+            }
+            upref.push_back(acco);
+            P.k_out_cnt.push_back(acco);
         }
     }
     trace_phase("factor task lists", tp);

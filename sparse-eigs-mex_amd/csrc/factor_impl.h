@@ -9,6 +9,7 @@
 namespace sehip {
 
 constexpr int TB = 64;   // tile edge
+constexpr int KB = 4;    // panels per outer block of the Schur updates (K = 256)
 
 struct EaTask {   // extend-add: child front columns [cb*64, cb*64+64) into the parent
     int32_t child, parent, cb, pad;
@@ -45,6 +46,7 @@ struct LevelPlan {
     std::vector<int64_t> ea_off;      // per child ordinal: offset in the extend-add task array
     std::vector<int64_t> ea_cnt;
     std::vector<int64_t> k_potrf_off, k_potrf_cnt, k_trsm_off, k_trsm_cnt, k_upd_off, k_upd_cnt, k_upref_off;
+    std::vector<int64_t> k_out_off, k_out_cnt, k_outref_off;   // the outer (block-end) updates
     // per panel step: start of each group of kXinvSlots fronts in the TRSM tasks
     // (plus the end); a group's inverses fit the scratch, slot = ordinal % kXinvSlots
     std::vector<std::vector<int64_t>> k_trsm_cut;
