@@ -303,7 +303,8 @@ def speigs(A, k: int, which: str | int = "sm", opts: Opts | dict | None = None, 
         fn = getattr(lib, prefix + "speigs")
         rc = fn(n, _ptr(Ap, c_int64), _ptr(Ai, c_int64), _ptr(Ax, c_double), int(k), w, ctypes.byref(o),
                 _ptr(lam, c_double), _ptr(V, c_double), _ptr(res, c_double), ctypes.byref(info))
-    return Result(rc, lam, None if V is None else V[:, :n].T.copy(), res, info.as_dict(), hinfo)
+    # V[:, :n].T is the n x k column-major result as a Fortran-ordered view (no copy)
+    return Result(rc, lam, None if V is None else V[:, :n].T, res, info.as_dict(), hinfo)
 
 
 def speigs_z(A_p, A_i, Axr, Axi, k: int, which="sm", opts: dict | None = None, lib=None, prefix: str = ""):

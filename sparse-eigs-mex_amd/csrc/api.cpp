@@ -297,11 +297,46 @@ struct StreamGuard {
     }
 };
 
+// A lower bound on the bandwidth of every ordering of the stored pattern: a
+// stored entry (r, c) puts r within kd positions of c, so the nodes reachable
+// from v in d steps along the column lists lie within d kd positions of v and
+// kd >= (|ball(v, d)| - 1) / (2 d).  The BFS from v stops after about `budget`
+// nodes.
+int64_t band_lower_bound(int64_t n, const int64_t *Ap, const int64_t *Ai, int64_t v, int64_t budget)
+{
+    std::vector<uint8_t> seen(n, 0);
+    std::vector<int64_t> q;
+    q.reserve(budget + 64);
+    q.push_back(v);
+    seen[v] = 1;
+    int64_t lb = 0, head = 0;
+    for (int64_t d = 1; head < (int64_t)q.size() && (int64_t)q.size() < budget; d++) {
+        const int64_t end = (int64_t)q.size();
+        for (; head < end; head++) {
+            const int64_t c = q[head];
+            for (int64_t p = Ap[c]; p < Ap[c + 1]; p++) {
+                const int64_t r = Ai[p];
+                if (r >= 0 && r < n && !seen[r]) {
+                    seen[r] = 1;
+                    q.push_back(r);
+                }
+            }
+        }
+        lb = std::max(lb, ((int64_t)q.size() - 1 + 2 * d - 1) / (2 * d));
+    }
+    return lb;
+}
+
 int try_banded(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax, const speigs_opts *o, double anorm,
                HostSolveOp *op, speigs_info *info)
 {
     int bmax = o->band_max ? o->band_max : 512;
     if (bmax <= 0) return 0;
+    // the gate below needs kd <= bmax and (kd + 1) n <= 2e7; a lower bound on
+    // kd above either limit settles it without the RCM ordering (the BFS starts
+    // at the last column, which on upper-triangle storage reaches the most)
+    const int64_t kcap = std::min<int64_t>(bmax, (int64_t)(2.0e7 / (double)n) - 1);
+    if (n > 1 && band_lower_bound(n, Ap, Ai, n - 1, 1 << 16) > kcap) return 0;
     double tb = wall();
     std::vector<int64_t> pm, ip(n);
     rcm(n, Ap, Ai, pm);

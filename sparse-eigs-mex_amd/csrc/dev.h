@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
+#include <thread>
 #include <vector>
 
 #include "../../include/speigs.h"
@@ -59,6 +60,33 @@ struct DevMem {
         for (void *q : ptrs) (void)hipFree(q);
         ptrs.clear();
     }
+};
+
+// ------------------------------------------------------------ host <-> device copies
+// Copies between pageable host memory and the device through pinned staging
+// buffers, several host threads in parallel (hostcopy.cpp).  copy_to_host
+// returns once dst holds the data (the copy is ordered after the work queued
+// on st); copy_to_device returns once src may be reused, the data reaching the
+// device in stream order on st.
+int copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t st);
+int copy_to_host_2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t rows,
+                    hipStream_t st);
+int copy_to_device(void *dst, const void *src, size_t bytes, hipStream_t st);
+
+// Writes every page of a host buffer once (each byte written is its own value)
+// on helper threads, so that a later copy into memory the caller has just
+// allocated does not stop for page faults; join() (or the destructor) waits.
+class HostTouch {
+public:
+    HostTouch() = default;
+    HostTouch(const HostTouch &) = delete;
+    HostTouch &operator=(const HostTouch &) = delete;
+    ~HostTouch() { join(); }
+    void start(void *p, size_t bytes);
+    void join();
+
+private:
+    std::vector<std::thread> th_;
 };
 
 // ------------------------------------------------------------ SELL matrix

@@ -206,8 +206,8 @@ int sell_build(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax
     if ((rc = mem.alloc(&S->val, tot))) return rc;
     SEHIP_CHECK(hipMemcpyAsync(S->sptr, sptr.data(), (nsl + 1) * 8, hipMemcpyHostToDevice, st));
     SEHIP_CHECK(hipMemcpyAsync(S->swid, swid.data(), nsl * 4, hipMemcpyHostToDevice, st));
-    SEHIP_CHECK(hipMemcpyAsync(S->col, col.data(), tot * 4, hipMemcpyHostToDevice, st));
-    SEHIP_CHECK(hipMemcpyAsync(S->val, val.data(), tot * 8, hipMemcpyHostToDevice, st));
+    if ((rc = copy_to_device(S->col, col.data(), tot * 4, st))) return rc;
+    if ((rc = copy_to_device(S->val, val.data(), tot * 8, st))) return rc;
     SEHIP_CHECK(hipStreamSynchronize(st));
     return 0;
 }

@@ -1134,8 +1134,7 @@ int upload(Factor *F, D **dst, const std::vector<D> &src)
 {
     int rc = F->mem.alloc(dst, src.size());
     if (rc) return rc;
-    if (!src.empty())
-        SEHIP_CHECK(hipMemcpyAsync(*dst, src.data(), src.size() * sizeof(D), hipMemcpyHostToDevice, F->st));
+    if (!src.empty()) return copy_to_device(*dst, src.data(), src.size() * sizeof(D), F->st);
     return 0;
 }
 

@@ -115,6 +115,10 @@ int lanczos_dev(DevOperator *op, int64_t n, int64_t k, const speigs_opts *o, con
     const double tol = o->tol > 0 ? o->tol : 1e-14;
     const double scale = anorm > 1.0 ? anorm : 1.0;
     const double rtol = tol * scale, small = 1e-13 * scale;
+    // the caller's eigenvector array is faulted in on helper threads while the
+    // device iterates
+    HostTouch touch;
+    if (Vout) touch.start(Vout, (size_t)n * std::min<int64_t>(k, ncv) * sizeof(double));
 
     Work W;
     W.n = n;
@@ -287,8 +291,10 @@ int lanczos_dev(DevOperator *op, int64_t n, int64_t k, const speigs_opts *o, con
     }
     if (Vout) {
         const int64_t nw = std::min<int64_t>(k, ncv);
-        SEHIP_CHECK(hipMemcpy2DAsync(Vout, (size_t)n * sizeof(double), W.X, (size_t)W.ld * sizeof(double),
-                                     (size_t)n * sizeof(double), (size_t)nw, hipMemcpyDeviceToHost, W.st));
+        touch.join();
+        int rc2 = copy_to_host_2d(Vout, (size_t)n * sizeof(double), W.X, (size_t)W.ld * sizeof(double),
+                                  (size_t)n * sizeof(double), (size_t)nw, W.st);
+        if (rc2) return rc2;
     }
     SEHIP_CHECK(hipStreamSynchronize(W.st));
     info->nconv = nconv;

@@ -157,6 +157,10 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
     const int64_t maxit = o->maxit ? std::max<int64_t>(o->maxit, 50) : 500;
     const double tol = o->tol > 0 ? o->tol : 1e-14;
     const double rtol = tol * (anorm > 1.0 ? anorm : 1.0);
+    // the caller's eigenvector array is faulted in on helper threads while the
+    // device iterates
+    HostTouch touch;
+    if (Vout) touch.start(Vout, (size_t)n * k * sizeof(double));
     int rc = factor_invert_l11(F);
     if (rc) return rc;
 
@@ -311,7 +315,8 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
     }
     if (Vout) {
         rto_cols(S, ldS, (int)k, n, T2, st);   // T2 holds n x 2m >= n x k doubles
-        SEHIP_CHECK(hipMemcpyAsync(Vout, T2, (size_t)n * k * sizeof(double), hipMemcpyDeviceToHost, st));
+        touch.join();
+        if ((rc = copy_to_host(Vout, T2, (size_t)n * k * sizeof(double), st))) return rc;
     }
     SEHIP_CHECK(hipStreamSynchronize(st));
     info->nconv = nconv;
