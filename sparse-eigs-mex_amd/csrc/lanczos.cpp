@@ -12,8 +12,10 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "dev.h"
+#include "host_par.h"
 
 namespace sehip {
 
@@ -27,9 +29,70 @@ uint64_t xorshift_next(uint64_t *s)
     return x * 0x2545F4914F6CDD1DULL;
 }
 
+namespace {
+
+// The xorshift step is linear over GF(2)^64: a map is stored as the images of
+// the 64 basis vectors, so that the state J steps ahead is a product with the
+// J-th power of the step's map (Brent's jump-ahead for xorshift generators).
+struct Gf2Map {
+    uint64_t col[64];
+};
+
+uint64_t gf2_apply(const Gf2Map &A, uint64_t x)
+{
+    uint64_t y = 0;
+    for (int b = 0; b < 64; b++)
+        if ((x >> b) & 1) y ^= A.col[b];
+    return y;
+}
+
+Gf2Map gf2_compose(const Gf2Map &A, const Gf2Map &B)   // A after B
+{
+    Gf2Map C;
+    for (int b = 0; b < 64; b++) C.col[b] = gf2_apply(A, B.col[b]);
+    return C;
+}
+
+Gf2Map xorshift_power(int64_t J)
+{
+    Gf2Map step, P;
+    for (int b = 0; b < 64; b++) {
+        uint64_t x = (uint64_t)1 << b;
+        (void)xorshift_next(&x);
+        step.col[b] = x;
+        P.col[b] = (uint64_t)1 << b;
+    }
+    for (; J > 0; J >>= 1) {
+        if (J & 1) P = gf2_compose(step, P);
+        step = gf2_compose(step, step);
+    }
+    return P;
+}
+
+}  // namespace
+
+// v[i] from the i-th state of the sequence; long vectors in chunks on host
+// threads, each chunk started at its state by a jump-ahead (the same values)
 void uniform_vec(uint64_t *s, int64_t n, double *v)
 {
-    for (int64_t i = 0; i < n; i++) v[i] = (double)(xorshift_next(s) >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    auto run = [v](uint64_t st, int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; i++) v[i] = (double)(xorshift_next(&st) >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+        return st;
+    };
+    const int T = (int)std::min<int64_t>(host_threads(), n / 65536);
+    if (T <= 1) {
+        *s = run(*s, 0, n);
+        return;
+    }
+    const int64_t len = (n + T - 1) / T;
+    const Gf2Map jump = xorshift_power(len);
+    std::vector<uint64_t> st0(T), st1(T);
+    st0[0] = *s;
+    for (int t = 1; t < T; t++) st0[t] = gf2_apply(jump, st0[t - 1]);
+    parallel_for(T, T, 1, [&](int64_t t, int) {
+        st1[t] = run(st0[t], t * len, std::min(n, (t + 1) * len));
+    });
+    *s = st1[T - 1];
 }
 
 static double host_sumsq(int64_t n, const double *x)
@@ -157,7 +220,7 @@ int lanczos_dev(DevOperator *op, int64_t n, int64_t k, const speigs_opts *o, con
         if (nr == 0.0) return SPEIGS_ERR_INTERNAL;
         double s = 1.0 / nr;
         for (int64_t i = 0; i < n; i++) hv[i] *= s;
-        SEHIP_CHECK(hipMemcpyAsync(W.V, hv.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, W.st));
+        if ((rc = copy_to_device(W.V, hv.data(), (size_t)n * sizeof(double), W.st))) return rc;
     }
 
     int64_t j = 0, restarts = 0, nconv = 0;

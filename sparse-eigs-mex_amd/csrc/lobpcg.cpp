@@ -195,7 +195,7 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
                 std::memcpy(h.data(), ho->start_vec, (size_t)n * sizeof(double));
             else
                 uniform_vec(&seed, n, h.data());
-            SEHIP_CHECK(hipMemcpyAsync(v0, h.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice, st));
+            if ((rc = copy_to_device(v0, h.data(), (size_t)n * sizeof(double), st))) return rc;
             rfrom_cols(v0, 1, n, T2, ldT, st);
             rrandom(T2, ldT, 1, m, n, seed, st);
         }

@@ -38,27 +38,44 @@ struct Graph {
     std::vector<int32_t> adj;
 };
 
-void build_graph(int64_t n, const int64_t *Ap, const int64_t *Ai, Graph &g)
+// Adjacency of the stored strict upper triangle, both directions.  Row x
+// lists its lower neighbours (column x's entries above the diagonal, in stored
+// order), then its higher neighbours j > x in ascending j -- the order a serial
+// pass over the columns produces; built in parallel, the second half of each
+// row filled through atomic cursors and then sorted.
+void build_graph(int64_t n, const int64_t *Ap, const int64_t *Ai, Graph &g, int nt)
 {
-    std::vector<int64_t> deg(n + 1, 0);
-    for (int64_t j = 0; j < n; j++)
-        for (int64_t q = Ap[j]; q < Ap[j + 1]; q++) {
-            const int64_t i = Ai[q];
-            if (i < j) { deg[i]++; deg[j]++; }
-        }
-    g.n = n;
-    g.xadj.assign(n + 1, 0);
-    for (int64_t i = 0; i < n; i++) g.xadj[i + 1] = g.xadj[i] + deg[i];
-    g.adj.resize(std::max<int64_t>(g.xadj[n], 1));
-    for (int64_t i = 0; i < n; i++) deg[i] = g.xadj[i];
-    for (int64_t j = 0; j < n; j++)
+    std::vector<int64_t> lo(n, 0), up(n, 0);
+    parallel_for(n, nt, 8192, [&](int64_t j, int) {
+        int64_t c = 0;
         for (int64_t q = Ap[j]; q < Ap[j + 1]; q++) {
             const int64_t i = Ai[q];
             if (i < j) {
-                g.adj[deg[i]++] = (int32_t)j;
-                g.adj[deg[j]++] = (int32_t)i;
+                c++;
+                __atomic_fetch_add(&up[i], (int64_t)1, __ATOMIC_RELAXED);
             }
         }
+        lo[j] = c;
+    });
+    g.n = n;
+    g.xadj.assign(n + 1, 0);
+    for (int64_t i = 0; i < n; i++) g.xadj[i + 1] = g.xadj[i] + lo[i] + up[i];
+    g.adj.resize(std::max<int64_t>(g.xadj[n], 1));
+    // cursors into the upper halves
+    parallel_for(n, nt, 65536, [&](int64_t i, int) { up[i] = g.xadj[i] + lo[i]; });
+    parallel_for(n, nt, 8192, [&](int64_t j, int) {
+        int64_t p = g.xadj[j];
+        for (int64_t q = Ap[j]; q < Ap[j + 1]; q++) {
+            const int64_t i = Ai[q];
+            if (i < j) {
+                g.adj[p++] = (int32_t)i;
+                g.adj[__atomic_fetch_add(&up[i], (int64_t)1, __ATOMIC_RELAXED)] = (int32_t)j;
+            }
+        }
+    });
+    parallel_for(n, nt, 8192, [&](int64_t i, int) {
+        std::sort(g.adj.begin() + g.xadj[i] + lo[i], g.adj.begin() + g.xadj[i + 1]);
+    });
 }
 
 // Region ids live in one array shared by the threads that dissect disjoint
@@ -243,6 +260,21 @@ private:
             for (auto &r : sl) roots.push_back(std::move(r));
     }
 
+    // connected region r whose nodes are listed in the order of a BFS from
+    // nodes[0], lev_ holding the levels (height h): no component search, no
+    // first BFS
+    void split_ordered(std::vector<int32_t> &nodes, int32_t r, NdList &roots, int32_t h)
+    {
+        if ((int64_t)nodes.size() <= leaf_) {
+            roots.push_back(leaf_node(nodes.data(), (int64_t)nodes.size()));
+            return;
+        }
+        std::vector<int32_t> &q = scratch().q;
+        if (q.size() < nodes.size()) q.resize(nodes.size());
+        std::copy(nodes.begin(), nodes.end(), q.begin());
+        dissect(nodes, r, roots, h);
+    }
+
     // connected region r with nodes.size() > leaf; h0 >= 0: the thread's queue
     // and lev_ already hold the BFS from nodes[0], of height h0
     void dissect(std::vector<int32_t> &nodes, int32_t r, NdList &roots, int32_t h0)
@@ -337,6 +369,17 @@ private:
         }
         for (int32_t v : A) rid_st(rid_.data(), v, ra);
         for (int32_t v : B) rid_st(rid_.data(), v, rb);
+        // A (levels below sep, and the separator nodes moved to it) lists its
+        // nodes in the order of a BFS from s inside A, at the same levels --
+        // a shortest path from s to a node below sep stays below sep, and a
+        // moved node is reached from its BFS parent at sep - 1 -- once the
+        // moved nodes are back at level sep; A is connected
+        int32_t ha = sep - 1;
+        for (int32_t v : S)
+            if (lev_[v] == sep - 1) {
+                lev_[v] = sep;
+                ha = sep;
+            }
         std::vector<int32_t>().swap(nodes);
         NdList ka, kb;
         bool spawned = false;
@@ -345,14 +388,14 @@ private:
             int f = free_threads_.load();
             while (f > 0 && !free_threads_.compare_exchange_weak(f, f - 1)) {}
             if (f > 0) {
-                th = std::thread([this, &A, &ka, ra] {
-                    split_components(A, ra, ka);
+                th = std::thread([this, &A, &ka, ra, ha] {
+                    split_ordered(A, ra, ka, ha);
                     free_threads_.fetch_add(1);
                 });
                 spawned = true;
             }
         }
-        if (!spawned) split_components(A, ra, ka);
+        if (!spawned) split_ordered(A, ra, ka, ha);
         split_components(B, rb, kb);
         if (spawned) th.join();
         std::vector<int32_t>().swap(A);
@@ -371,11 +414,11 @@ int symbolic_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, const Symb
 {
     if (n >= (int64_t)1 << 31) return SPEIGS_ERR_ARG;
     double t0 = wall();
+    const int nt = host_threads();
     Graph g;
-    build_graph(n, Ap, Ai, g);
+    build_graph(n, Ap, Ai, g, nt);
     S->n = n;
     S->perm.assign(n, 0);
-    const int nt = host_threads();
     Dissection nd(g, std::max<int64_t>(o.leaf, 1), S->perm, nt);
     nd.run();
     S->iperm.assign(n, 0);
