@@ -531,6 +531,8 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
             info->t_analyze = wall() - ta;
             if (hipStreamCreate(&sell_sg.s) != hipSuccess) return SPEIGS_ERR_INTERNAL;
             sell_th = std::thread([&]() { sell_rc = sell_build(n, Ap, Ai, Ax, &sell_pre, mem, sell_sg.s); });
+            // the block eigensolver's solve plan is built while the device factorizes
+            if (hopts->precision == SPEIGS_HIP_FP32_BLOCK) rowsolve_prepare_async(fo->F);
             double tf = wall();
             double delta = 0.0;
             int st = factor_numeric(fo->F, 0.0);

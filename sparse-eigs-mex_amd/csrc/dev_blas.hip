@@ -129,8 +129,10 @@ int sell_build(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax
     // Host construction in parallel over the stored columns.  Each row of the
     // full symmetric matrix lists its entries in increasing column order (ties
     // by value), whatever the order of the input, so the SpMV's summation
-    // order is a function of the matrix alone.
-    const int nt = host_threads();
+    // order is a function of the matrix alone.  It runs beside the numeric
+    // factorization, whose launching thread must keep a core: half the
+    // analysis threads.
+    const int nt = std::max(1, host_threads() / 2);
     std::unique_ptr<std::atomic<int32_t>[]> rl(new std::atomic<int32_t>[n]);
     parallel_for(n, nt, 65536, [&](int64_t i, int) { rl[i].store(0, std::memory_order_relaxed); });
     parallel_for(n, nt, 4096, [&](int64_t j, int) {

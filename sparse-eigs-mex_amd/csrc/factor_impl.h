@@ -128,6 +128,9 @@ int factor_invert_l11(Factor *F);
 // which may alias B with the same strides)
 int factor_block_solve_rows(Factor *F, const double *B, int64_t brs, int64_t bcs, double *X, int64_t xrs,
                             int64_t xcs, int nb);
+// starts building the row solve's plan on a helper thread (it reads only the
+// symbolic structure); the first factor_block_solve_rows joins it
+void rowsolve_prepare_async(Factor *F);
 void rowsolve_free(Factor *F);
 int factor_block_solve_d(Factor *F, const double *B, double *X, int nb);
 void factor_block_free(Factor *F);

@@ -31,6 +31,8 @@
 // coalesced and transpose each wave's 16 x 64 slice through LDS.
 #include <algorithm>
 #include <cstring>
+#include <thread>
+#include <vector>
 
 #include "factor_impl.h"
 #include "host_par.h"
@@ -525,7 +527,23 @@ struct RsLevel {
     int64_t ft_off[4] = {0, 0, 0, 0}, ft_cnt[4] = {0, 0, 0, 0};
 };
 
+// host-side plan of the solve (task lists, pull lists), uploaded at first use
+struct RsHost {
+    std::vector<int32_t> small, lst;
+    std::vector<RsRow> asmt, gxt;
+    std::vector<RsPart> pt;
+    std::vector<RsFin> ft;
+    std::vector<int64_t> frow, uoff, lptr;
+    int64_t maxpart = 1, sumr = 0;
+};
+
 struct RsState {
+    // the plan is built on a helper thread (started while the device
+    // factorizes); the first solve joins it and uploads
+    std::thread th;
+    int plan_rc = 0;
+    bool ready = false;
+    RsHost host;
     DevMem mem;
     std::vector<RsLevel> lev;
     int32_t *small = nullptr;
@@ -536,6 +554,10 @@ struct RsState {
     int64_t *frow = nullptr, *uoff = nullptr, *lptr = nullptr;
     int32_t *lst = nullptr;
     int64_t n = 0, t0 = 0;
+    ~RsState()
+    {
+        if (th.joinable()) th.join();
+    }
 };
 
 bool is_tiny(int64_t c, int64_t r) { return c <= TINY && r <= TINY; }
@@ -567,22 +589,29 @@ void plan_parts(const Symbolic &S, const std::vector<int64_t> &fronts, int mode,
     }
 }
 
-int rs_prepare(Factor *F, RsState &B)
+// the plan: reads only the symbolic structure, so it can run beside the
+// numeric factorization
+int rs_plan(const Factor *F, RsState &B)
 {
     const Symbolic &S = F->S;
     const int64_t ns = S.ns, n = F->n;
-    std::vector<int64_t> frow(ns + 1, 0), uoff(ns + 1, 0);
+    RsHost &H = B.host;
+    std::vector<int64_t> &frow = H.frow, &uoff = H.uoff, &lptr = H.lptr;
+    frow.assign(ns + 1, 0);
+    uoff.assign(ns + 1, 0);
     for (int64_t s = 0; s < ns; s++) {
         frow[s + 1] = frow[s] + S.F[s];
         uoff[s + 1] = uoff[s] + (S.F[s] - (S.c1[s] - S.c0[s]));
     }
     const int64_t nrid = frow[ns], sumr = uoff[ns];
+    H.sumr = sumr;
     if (2 * n + sumr >= ((int64_t)1 << 31)) return SPEIGS_ERR_MEM;   // buffer rows are int32 in the pull lists
     B.n = n;
     B.t0 = n + sumr;
     // pull lists: row (s, i) <- P row c0 + i (i < c), then each child's U rows
-    // that land on i, children in order
-    const int nt = host_threads();
+    // that land on i, children in order (a quarter of the analysis threads:
+    // this runs beside the SELL build and the factorization's launches)
+    const int nt = std::max(1, host_threads() / 4);
     std::vector<int64_t> tot(ns + 1, 0);
     parallel_for(ns, nt, 256, [&](int64_t s, int) {
         int64_t t = S.c1[s] - S.c0[s];
@@ -593,8 +622,9 @@ int rs_prepare(Factor *F, RsState &B)
         tot[s + 1] = t;
     });
     for (int64_t s = 0; s < ns; s++) tot[s + 1] += tot[s];
-    std::vector<int64_t> lptr(nrid + 1);
-    std::vector<int32_t> lst(std::max<int64_t>(tot[ns], 1));
+    lptr.assign(nrid + 1, 0);
+    std::vector<int32_t> &lst = H.lst;
+    lst.assign(std::max<int64_t>(tot[ns], 1), 0);
     lptr[nrid] = tot[ns];
     parallel_for(ns, nt, 64, [&](int64_t s, int) {
         const int64_t c = S.c1[s] - S.c0[s], Fv = S.F[s];
@@ -618,11 +648,11 @@ int rs_prepare(Factor *F, RsState &B)
     // per level: tiny fronts, assembly chunks of every front, product tasks of the others
     std::vector<std::vector<int64_t>> bylevel(S.nlevels);
     for (int64_t s = 0; s < ns; s++) bylevel[S.level[s]].push_back(s);
-    std::vector<int32_t> small;
-    std::vector<RsRow> asmt, gxt;
-    std::vector<RsPart> pt;
-    std::vector<RsFin> ft;
-    int64_t maxpart = 1;
+    std::vector<int32_t> &small = H.small;
+    std::vector<RsRow> &asmt = H.asmt, &gxt = H.gxt;
+    std::vector<RsPart> &pt = H.pt;
+    std::vector<RsFin> &ft = H.ft;
+    int64_t &maxpart = H.maxpart;
     B.lev.assign(S.nlevels, RsLevel());
     for (int32_t L = 0; L < S.nlevels; L++) {
         RsLevel &P = B.lev[L];
@@ -662,21 +692,29 @@ int rs_prepare(Factor *F, RsState &B)
     if (gxt.empty()) gxt.push_back({0, 0});
     if (pt.empty()) pt.push_back({0, 0, 0, 0, 0, 0});
     if (ft.empty()) ft.push_back({0, 0, 0, 0, 0});
+    return 0;
+}
+
+// device buffers and the plan's upload; the host plan is released after
+int rs_upload(Factor *F, RsState &B)
+{
+    RsHost &H = B.host;
+    const int64_t n = F->n;
     int rc;
     auto up = [&](auto **dst, const auto &v) -> int {
         int e = B.mem.alloc(dst, v.size());
         if (e) return e;
-        SEHIP_CHECK(hipMemcpyAsync(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice, F->st));
-        return 0;
+        return copy_to_device(*dst, v.data(), v.size() * sizeof(v[0]), F->st);
     };
-    if ((rc = up(&B.small, small)) || (rc = up(&B.asmt, asmt)) || (rc = up(&B.gxt, gxt)) || (rc = up(&B.pt, pt)) || (rc = up(&B.ft, ft)) ||
-        (rc = up(&B.frow, frow)) || (rc = up(&B.uoff, uoff)) || (rc = up(&B.lptr, lptr)) || (rc = up(&B.lst, lst)))
+    if ((rc = up(&B.small, H.small)) || (rc = up(&B.asmt, H.asmt)) || (rc = up(&B.gxt, H.gxt)) ||
+        (rc = up(&B.pt, H.pt)) || (rc = up(&B.ft, H.ft)) || (rc = up(&B.frow, H.frow)) || (rc = up(&B.uoff, H.uoff)) ||
+        (rc = up(&B.lptr, H.lptr)) || (rc = up(&B.lst, H.lst)))
         return rc;
-    if ((rc = B.mem.alloc(&B.part, (size_t)maxpart * RTILE * RW))) return rc;
-    if ((rc = B.mem.alloc(&B.buf, (size_t)(2 * n + sumr) * RW))) return rc;
+    if ((rc = B.mem.alloc(&B.part, (size_t)H.maxpart * RTILE * RW))) return rc;
+    if ((rc = B.mem.alloc(&B.buf, (size_t)(2 * n + H.sumr) * RW))) return rc;
     if ((rc = B.mem.alloc(&B.Yg, (size_t)n * RW))) return rc;
     if ((rc = B.mem.alloc(&B.Xg, (size_t)n * RW))) return rc;
-    SEHIP_CHECK(hipStreamSynchronize(F->st));   // host vectors above go out of scope
+    B.host = RsHost();
     return 0;
 }
 
@@ -743,16 +781,15 @@ int factor_block_solve_rows(Factor *F, const double *Bin, int64_t brs, int64_t b
 {
     if (nb < 1 || nb > RW) return SPEIGS_ERR_ARG;
     if (!F->inverted) return SPEIGS_ERR_FACTOR;
-    if (!F->rs) {
-        RsState *B = new RsState();
-        const int rc = rs_prepare(F, *B);
-        if (rc) {
-            delete B;
-            return rc;
-        }
-        F->rs = B;
-    }
+    rowsolve_prepare_async(F);
     RsState &B = *reinterpret_cast<RsState *>(F->rs);
+    if (!B.ready) {
+        if (B.th.joinable()) B.th.join();
+        if (B.plan_rc) return B.plan_rc;
+        const int rc = rs_upload(F, B);
+        if (rc) return rc;
+        B.ready = true;
+    }
     const int e = ev_begin(F);
     if (F->fp32) rs_run<float>(F, B, Bin, brs, bcs, Xout, xrs, xcs, nb);
     else rs_run<double>(F, B, Bin, brs, bcs, Xout, xrs, xcs, nb);
@@ -760,6 +797,14 @@ int factor_block_solve_rows(Factor *F, const double *Bin, int64_t brs, int64_t b
     F->block_solves++;
     F->block_rhs += nb;
     return 0;
+}
+
+void rowsolve_prepare_async(Factor *F)
+{
+    if (F->rs) return;
+    RsState *B = new RsState();
+    F->rs = B;
+    B->th = std::thread([F, B] { B->plan_rc = rs_plan(F, *B); });
 }
 
 void rowsolve_free(Factor *F)
