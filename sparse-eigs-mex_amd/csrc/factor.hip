@@ -368,12 +368,14 @@ __global__ void __launch_bounds__(64) k_potrf_reg(T *pool, const TileTask *__res
         x[q] = T(q == r ? 1 : 0);
     }
     bool bad = false;
-    const double *dA = diagA + col0[t.s] + k0;
+    // lane r holds A(k0 + r, k0 + r); the pivot test of step p broadcasts it
+    // (a load inside the step chain would put a memory latency on every step)
+    const double dAr = r < w ? diagA[col0[t.s] + k0 + r] : 0.0;
 #pragma unroll
     for (int p = 0; p < TB; p++) {
         if (p < w) {
             T d = bcast(a[p], p);
-            const double ref = fabs(dA[p] + delta);
+            const double ref = fabs(bcast(dAr, p) + delta);
             if (!((double)d > piv_rel * ref) || !(d == d)) {
                 bad = true;
                 d = T(1);
@@ -396,9 +398,15 @@ __global__ void __launch_bounds__(64) k_potrf_reg(T *pool, const TileTask *__res
 #pragma unroll
     for (int q = 0; q < TB; q++)
         if (r < w && q < w && q <= r) base[(int64_t)q * F + r] = a[q];
+    // column r of X (column-major 64 x 64) is lane r's x; transposed through
+    // LDS so that each store instruction writes one contiguous column
+    __shared__ T xs[TB][TB + 1];
+#pragma unroll
+    for (int i = 0; i < TB; i++) xs[i][r] = x[i];
+    __syncthreads();
     T *X = xinv + (int64_t)blockIdx.x * (TB * TB);
 #pragma unroll
-    for (int i = 0; i < TB; i++) X[(int64_t)r * TB + i] = x[i];      // column r of X, column-major 64 x 64
+    for (int j = 0; j < TB; j++) X[(int64_t)j * TB + r] = xs[r][j];
 }
 
 // tile row i below the diagonal block: B <- B X^T with X = L_kk^-1 from the
