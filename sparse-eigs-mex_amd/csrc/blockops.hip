@@ -14,9 +14,10 @@ namespace sehip {
 // ------------------------------------------------------------------ Gram
 // G = A^T B (p x q, p, q <= 96, column-major ld p) on the FP64 matrix cores.
 // A workgroup owns a contiguous range of rows; its four waves split the (<= 6
-// x 6) output tiles of 16 x 16 into 2 x 2 groups of <= 3 x 3 (each wave loads
-// only its column blocks, the shared rows come through L1), which keeps the
-// accumulators at 72 registers and several waves per SIMD in flight.
+// x 6) output tiles of 16 x 16 into 2 x 2 groups of <= 3 x 3, halving the tile
+// rows and the tile columns (each wave loads only its column blocks, the
+// shared rows come through L1), which keeps the accumulators at 72 registers
+// and several waves per SIMD in flight.
 // v_mfma_f64_16x16x4 takes its operands straight from memory, 4 rows per step:
 // lane l loads A[r + l/16][16 ti + l%16] and B[r + l/16][16 tj + l%16] (128-byte
 // row pieces) and holds G[16 ti + l/16 + 4v][16 tj + l%16].  The next step's
@@ -33,7 +34,10 @@ __global__ void __launch_bounds__(256) k_rgram_partial(const double *__restrict_
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int lr = lane & 15, lk = lane >> 4;
     const int pt = (p + 15) >> 4, qt = (q + 15) >> 4;
-    const int a0 = 3 * (wv & 1), b0 = 3 * (wv >> 1);     // this wave's tile group
+    // halves of the tile rows and of the tile columns (<= 3 each): the waves'
+    // shares of the products differ by at most one row or column of tiles
+    const int ga = (pt + 1) >> 1, gb = (qt + 1) >> 1;
+    const int a0 = ga * (wv & 1), b0 = gb * (wv >> 1);   // this wave's tile group
     if (a0 >= pt || b0 >= qt) return;                     // no workgroup barrier below
     rd4 acc[3][3];
 #pragma unroll
@@ -48,8 +52,8 @@ __global__ void __launch_bounds__(256) k_rgram_partial(const double *__restrict_
 #pragma unroll
         for (int a = 0; a < 3; a++) {
             const int ca = 16 * (a0 + a) + lr, cb = 16 * (b0 + a) + lr;
-            xa[a] = (a0 + a < pt && in && ca < p) ? A[row * lda + ca] : 0.0;
-            xb[a] = (b0 + a < qt && in && cb < q) ? B[row * ldb + cb] : 0.0;
+            xa[a] = (a < ga && a0 + a < pt && in && ca < p) ? A[row * lda + ca] : 0.0;
+            xb[a] = (a < gb && b0 + a < qt && in && cb < q) ? B[row * ldb + cb] : 0.0;
         }
     };
     if (r0 < r1) load(r0);
@@ -62,7 +66,7 @@ __global__ void __launch_bounds__(256) k_rgram_partial(const double *__restrict_
         for (int a = 0; a < 3; a++)
 #pragma unroll
             for (int b = 0; b < 3; b++)
-                if (a0 + a < pt && b0 + b < qt)
+                if (a < ga && b < gb && a0 + a < pt && b0 + b < qt)
                     acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[a], cb[b], acc[a][b], 0, 0, 0);
     }
     double *out = partial + (int64_t)blockIdx.x * p * q;
@@ -70,7 +74,7 @@ __global__ void __launch_bounds__(256) k_rgram_partial(const double *__restrict_
     for (int a = 0; a < 3; a++)
 #pragma unroll
         for (int b = 0; b < 3; b++) {
-            if (a0 + a >= pt || b0 + b >= qt) continue;
+            if (a >= ga || b >= gb || a0 + a >= pt || b0 + b >= qt) continue;
             const int j = 16 * (b0 + b) + lr;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
@@ -124,8 +128,9 @@ void rgram(const double *A, int64_t lda, int p, const double *B, int64_t ldb, in
 // ------------------------------------------------------------------ tall GEMM
 // X = V C + beta X (V n x m, C m x p column-major, m, p <= 96; X must not alias
 // V) on the FP64 matrix cores.  C is staged in LDS once per block; each wave
-// owns 16 rows and all (<= 6) 16-column output tiles, loads its whole 16 x m
-// slice of V up front (all loads in flight), then runs the steps of 4 inner
+// owns 16-row groups and all (<= 6) 16-column output tiles, loads a group's
+// whole 16 x m slice of V at once (all loads in flight, the next group's during
+// the current group's products), then runs the steps of 4 inner
 // indices: lane l supplies V[r + l%16][k + l/16] and C[k + l/16][16 tj + l%16]
 // and holds X[r + l/16 + 4v][16 tj + l%16] (stores of four 128-byte row pieces).
 constexpr int RM_CP = 100;   // LDS pitch of C rows
@@ -154,17 +159,24 @@ __global__ void __launch_bounds__(256) k_rgemm(const double *V, int64_t ldv, int
     const int lane = threadIdx.x & 63;
     const int lr = lane & 15, lk = lane >> 4;
     const int pt = (p + 15) >> 4;
-    for (int g = 0; g < RM_GROUPS; g++) {
-        const int64_t r0 = (((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * RM_GROUPS + g) * 16;
-        if (r0 >= n) break;
+    const int64_t rbase = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * RM_GROUPS * 16;
+    // the next group's rows are loaded while the current group's products run
+    double va[24], vn[24];
+    auto load = [&](double *dst, int64_t r0) {
         const int64_t arow = r0 + lr;
         const bool rin = arow < n;
-        double va[24];
 #pragma unroll
         for (int s2 = 0; s2 < 24; s2++) {
             const int kk = 4 * s2 + lk;
-            va[s2] = (rin && kk < m) ? V[arow * ldv + kk] : 0.0;
+            dst[s2] = (rin && kk < m) ? V[arow * ldv + kk] : 0.0;
         }
+    };
+    if (rbase < n) load(va, rbase);
+#pragma unroll
+    for (int g = 0; g < RM_GROUPS; g++) {
+        const int64_t r0 = rbase + g * 16;
+        if (r0 >= n) break;
+        if (g + 1 < RM_GROUPS && r0 + 16 < n) load(vn, r0 + 16);
         rd4 acc[6];
 #pragma unroll
         for (int b = 0; b < 6; b++) acc[b] = rd4{0.0, 0.0, 0.0, 0.0};
@@ -189,6 +201,8 @@ __global__ void __launch_bounds__(256) k_rgemm(const double *V, int64_t ldv, int
                 }
             }
         }
+#pragma unroll
+        for (int s2 = 0; s2 < 24; s2++) va[s2] = vn[s2];
     }
 }
 

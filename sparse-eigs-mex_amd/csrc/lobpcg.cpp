@@ -13,7 +13,8 @@
 //   X <- smallest m Ritz vectors, P <- their component outside X.
 // Robustness: [W P] is projected out of X and made orthonormal by SVQB
 // (Stathopoulos and Wu 2002: scale, eigendecompose the Gram matrix, drop
-// directions below 1e-12), and the pair of steps is repeated when the first can
+// directions below 1e-12) -- both from one Gram matrix [X Z]^T Z and applied as
+// one product over [X Z] -- and the pair of steps is repeated when the first can
 // have fallen short (cancellation in the projection, a dropped direction, an
 // ill-conditioned Gram matrix), so S stays orthonormal to rounding and the
 // projected problem is a standard symmetric one.
@@ -71,43 +72,10 @@ int dgemm_tall(LWork &W, const double *V, int64_t ldv, int m, const std::vector<
     return 0;
 }
 
-// Z (columns m .. m+nz of S) <- Z - X (X^T Z); returns Z^T Z after the update
-// and, in *keep, the smallest ratio of a column's norm after to before
-int project_out(LWork &W, double *S, int64_t ldS, int m, int nz, std::vector<double> &zz, double *keep = nullptr)
-{
-    double *Z = S + m;
-    std::vector<double> g;
-    int rc = dgram(W, S, ldS, m + nz, Z, ldS, nz, g);   // rows 0..m-1: X^T Z, rows m..: Z^T Z
-    if (rc) return rc;
-    const int ld = m + nz;
-    std::vector<double> gx((size_t)m * nz);
-    zz.assign((size_t)nz * nz, 0.0);
-    for (int j = 0; j < nz; j++) {
-        for (int i = 0; i < m; i++) gx[(size_t)j * m + i] = -g[(size_t)j * ld + i];
-        for (int i = 0; i < nz; i++) zz[(size_t)j * nz + i] = g[(size_t)j * ld + m + i];
-    }
-    // (Z - X Gx)^T (Z - X Gx) = Z^T Z - Gx^T Gx for orthonormal X
-    for (int j = 0; j < nz; j++)
-        for (int i = 0; i < nz; i++) {
-            double acc = 0.0;
-            for (int q = 0; q < m; q++) acc += gx[(size_t)i * m + q] * gx[(size_t)j * m + q];
-            zz[(size_t)j * nz + i] -= acc;
-        }
-    if (keep) {
-        double kmin = 1.0;
-        for (int j = 0; j < nz; j++) {
-            const double b = g[(size_t)j * ld + m + j], a = zz[(size_t)j * nz + j];
-            if (b > 0) kmin = std::min(kmin, std::sqrt(std::max(a, 0.0) / b));
-        }
-        *keep = kmin;
-    }
-    return dgemm_tall(W, S, ldS, m, gx, nz, Z, ldS, 1.0);
-}
-
-// SVQB from the Gram matrix zz of Z: out = Z D U S^-1/2 (kept directions); returns r
-// and, in *cond, the spread of the kept eigenvalues of the scaled Gram matrix
-int svqb(LWork &W, const double *Z, int64_t ldz, int nz, const std::vector<double> &zz, double *out, int64_t ldo,
-         int *r_out, double *cond = nullptr)
+// SVQB transform from the Gram matrix zz of Z: T = D U S^-1/2 over the kept
+// directions (nz x r, column-major), so that Z T is orthonormal; returns r and,
+// in *cond, the spread of the kept eigenvalues of the scaled Gram matrix
+int svqb_transform(int nz, const std::vector<double> &zz, std::vector<double> &T, int *r_out, double *cond)
 {
     std::vector<double> d(nz);
     for (int i = 0; i < nz; i++) {
@@ -122,18 +90,67 @@ int svqb(LWork &W, const double *Z, int64_t ldz, int nz, const std::vector<doubl
     int r = 0;
     for (int j = nz - 1; j >= 0 && smax > 0 && ev[j] > 1e-12 * smax; j--) r++;
     if (cond) *cond = r > 0 ? smax / ev[nz - r] : 0.0;
-    std::vector<double> T((size_t)nz * r, 0.0);
+    T.assign((size_t)nz * r, 0.0);
     for (int c = 0; c < r; c++) {
         const int j = nz - 1 - c;
         const double sc = 1.0 / std::sqrt(ev[j]);
         for (int i = 0; i < nz; i++) T[(size_t)c * nz + i] = d[i] * gs[(size_t)j * nz + i] * sc;
     }
-    if (r > 0) {
-        int rc = dgemm_tall(W, Z, ldz, nz, T, r, out, ldo, 0.0);
-        if (rc) return rc;
-    }
     *r_out = r;
     return 0;
+}
+
+// SVQB of Z: out = Z T (out may be Z itself: the product is row-local)
+int svqb(LWork &W, const double *Z, int64_t ldz, int nz, const std::vector<double> &zz, double *out, int64_t ldo,
+         int *r_out, double *cond = nullptr)
+{
+    std::vector<double> T;
+    int rc = svqb_transform(nz, zz, T, r_out, cond);
+    if (rc) return rc;
+    if (*r_out > 0) return dgemm_tall(W, Z, ldz, nz, T, *r_out, out, ldo, 0.0);
+    return 0;
+}
+
+// Z (columns m .. m+nz of S) <- (Z - X X^T Z) T with T the SVQB transform of
+// the projected block, as one product over [X Z]: Z <- [X Z] [-X^T Z T ; T].
+// The projected Gram matrix is formed from [X Z]^T Z (X orthonormal); *keep is
+// the smallest ratio of a column's norm after the projection to before, *cond
+// the SVQB spread; returns the kept count in *r
+int project_svqb(LWork &W, double *S, int64_t ldS, int m, int nz, int *r, double *keep, double *cond)
+{
+    double *Z = S + m;
+    std::vector<double> g;
+    int rc = dgram(W, S, ldS, m + nz, Z, ldS, nz, g);   // rows 0..m-1: X^T Z, rows m..: Z^T Z
+    if (rc) return rc;
+    const int ld = m + nz;
+    std::vector<double> zz((size_t)nz * nz);
+    for (int j = 0; j < nz; j++)
+        for (int i = 0; i < nz; i++) {
+            double acc = g[(size_t)j * ld + m + i];
+            for (int q = 0; q < m; q++) acc -= g[(size_t)i * ld + q] * g[(size_t)j * ld + q];
+            zz[(size_t)j * nz + i] = acc;
+        }
+    double kmin = 1.0;
+    for (int j = 0; j < nz; j++) {
+        const double b = g[(size_t)j * ld + m + j], a = zz[(size_t)j * nz + j];
+        if (b > 0) kmin = std::min(kmin, std::sqrt(std::max(a, 0.0) / b));
+    }
+    *keep = kmin;
+    std::vector<double> T;
+    if ((rc = svqb_transform(nz, zz, T, r, cond))) return rc;
+    const int rr = *r;
+    if (rr == 0) return 0;
+    // C = [-(X^T Z) T ; T], (m + nz) x rr column-major
+    std::vector<double> C((size_t)ld * rr);
+    for (int c = 0; c < rr; c++) {
+        for (int q = 0; q < m; q++) {
+            double acc = 0.0;
+            for (int j = 0; j < nz; j++) acc += g[(size_t)j * ld + q] * T[(size_t)c * nz + j];
+            C[(size_t)c * ld + q] = -acc;
+        }
+        for (int j = 0; j < nz; j++) C[(size_t)c * ld + m + j] = T[(size_t)c * nz + j];
+    }
+    return dgemm_tall(W, S, ldS, ld, C, rr, Z, ldS, 0.0);
 }
 
 void symmetrize(std::vector<double> &H, int d)
@@ -242,11 +259,9 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
         if ((rc = factor_block_solve_rows(F, Wb, ldS, 1, Wb, ldS, 1, m))) return rc;
         solves++;
         int nz = m + np;
-        std::vector<double> zz;
         double keep = 0.0, cond = 0.0;
-        if ((rc = project_out(W, S, ldS, m, nz, zz, &keep))) return rc;
         int r = 0;
-        if ((rc = svqb(W, Z, ldS, nz, zz, Z, ldS, &r, &cond))) return rc;   // in place: row-local
+        if ((rc = project_svqb(W, S, ldS, m, nz, &r, &keep, &cond))) return rc;
         if (r == 0) break;
         // the second pass only where the first can have left Z short of
         // orthonormal: a column that lost more than half its norm to the
@@ -255,8 +270,8 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
         const bool again = keep < 0.5 || r < nz || cond > 1e4;
         nz = r;
         if (again) {
-            if ((rc = project_out(W, S, ldS, m, nz, zz))) return rc;
-            if ((rc = svqb(W, Z, ldS, nz, zz, Z, ldS, &r))) return rc;
+            double keep2 = 0.0, cond2 = 0.0;
+            if ((rc = project_svqb(W, S, ldS, m, nz, &r, &keep2, &cond2))) return rc;
             if (r == 0) break;
             nz = r;
         }
