@@ -4,6 +4,10 @@
 // solve2 (src/speigs_core.c:933-970, :86-100).
 #pragma once
 #include <cstdint>
+#include <memory>
+#include <new>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "dev.h"
@@ -15,6 +19,29 @@ namespace sehip {
 // every leaf region) is one supernode whose columns are contiguous in the new
 // order and whose row structure below the diagonal block is the boundary of
 // its region -- the set of later (ancestor-separator) nodes adjacent to it.
+// Allocator that leaves elements default-initialised (no zero fill): for large
+// index arrays that are written in parallel right after they are sized.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <class U>
+    NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+    template <class U>
+    void construct(U *p) noexcept(std::is_nothrow_default_constructible<U>::value)
+    {
+        ::new ((void *)p) U;
+    }
+    template <class U, class... Args>
+    void construct(U *p, Args &&...args)
+    {
+        ::new ((void *)p) U(std::forward<Args>(args)...);
+    }
+};
+
 struct Symbolic {
     int64_t n = 0;
     std::vector<int64_t> perm, iperm;          // perm[new] = old
@@ -22,14 +49,15 @@ struct Symbolic {
     std::vector<int64_t> c0, c1;               // column range (new indices)
     std::vector<int64_t> parent;               // -1 for roots
     std::vector<int64_t> chptr, chl;           // children lists (CSR)
-    std::vector<int64_t> rptr, rows;           // boundary rows, sorted, CSR
+    std::vector<int64_t> rptr;                 // boundary rows, sorted, CSR
+    std::vector<int64_t, NoInitAlloc<int64_t>> rows;
     std::vector<int32_t> level;                // 0 = leaves
     int32_t nlevels = 0;
     std::vector<int64_t> F;                    // front order c + r
     std::vector<int64_t> foff;                 // front offset (elements) in the pool
     int64_t pool = 0;
     std::vector<int64_t> relptr;               // per supernode: offset of its rel map
-    std::vector<int32_t> rel;                  // positions of its rows in the parent front
+    std::vector<int32_t, NoInitAlloc<int32_t>> rel;   // positions of its rows in the parent front
     double flops = 0.0;
     int64_t nnzL = 0;
     double t_order = 0.0, t_symbolic = 0.0;

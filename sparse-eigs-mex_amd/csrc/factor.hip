@@ -1137,8 +1137,8 @@ double device_norm(Factor *F, const double *v)
     return std::sqrt(s);
 }
 
-template <class D>
-int upload(Factor *F, D **dst, const std::vector<D> &src)
+template <class D, class Al>
+int upload(Factor *F, D **dst, const std::vector<D, Al> &src)
 {
     int rc = F->mem.alloc(dst, src.size());
     if (rc) return rc;
@@ -1175,15 +1175,16 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
         nc[s] = (int32_t)(S.c1[s] - S.c0[s]);
         wsoff[s + 1] = wsoff[s] + S.F[s];
     }
-    for (size_t i = 0; i < S.rows.size(); i++) rows32[i] = (int32_t)S.rows[i];
+    const int nt = host_threads();
+    parallel_for((int64_t)S.rows.size(), nt, 65536, [&](int64_t i, int) { rows32[i] = (int32_t)S.rows[i]; });
     for (size_t i = 0; i < S.chl.size(); i++) chl32[i] = (int32_t)S.chl[i];
     // assembly of A's entries: upper triangle as stored (i <= j)
     std::vector<int64_t> snode(n);
-    for (int64_t s = 0; s < ns; s++)
+    parallel_for(ns, nt, 256, [&](int64_t s, int) {
         for (int64_t j = S.c0[s]; j < S.c1[s]; j++) snode[j] = s;
+    });
     // columns in parallel, each into its own range of the map (map order =
     // column order, as a serial scan would give)
-    const int nt = host_threads();
     std::vector<int64_t> cnt(n + 1, 0);
     parallel_for(n, nt, 4096, [&](int64_t j, int) {
         int64_t c = 0;

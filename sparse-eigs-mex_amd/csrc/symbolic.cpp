@@ -169,10 +169,7 @@ private:
             ch.push_back((int64_t)c0.size() - 1);
         }
         c0.push_back(next_pos_);
-        for (int32_t v : nd.cols) {
-            perm_[next_pos_++] = v;
-            rid_[v] = 0;
-        }
+        for (int32_t v : nd.cols) perm_[next_pos_++] = v;
         c1.push_back(next_pos_);
         kids.push_back(std::move(ch));
         NdList().swap(nd.kids);
@@ -421,8 +418,8 @@ int symbolic_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, const Symb
     S->perm.assign(n, 0);
     Dissection nd(g, std::max<int64_t>(o.leaf, 1), S->perm, nt);
     nd.run();
-    S->iperm.assign(n, 0);
-    for (int64_t i = 0; i < n; i++) S->iperm[S->perm[i]] = i;
+    S->iperm.resize(n);
+    parallel_for(n, nt, 65536, [&](int64_t i, int) { S->iperm[S->perm[i]] = i; });
     double t1 = wall();
     S->t_order = t1 - t0;
 
@@ -499,12 +496,14 @@ int symbolic_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, const Symb
     // relative positions of each supernode's rows in its parent's front
     S->relptr.assign(ns + 1, 0);
     for (int64_t s = 0; s < ns; s++) S->relptr[s + 1] = S->relptr[s] + (S->rptr[s + 1] - S->rptr[s]);
-    S->rel.assign(std::max<int64_t>(S->relptr[ns], 1), 0);
-    for (int64_t s = 0; s < ns; s++) {
+    S->rel.resize(std::max<int64_t>(S->relptr[ns], 1));
+    S->rel[0] = 0;
+    std::atomic<int> bad{0};
+    parallel_for(ns, nt, 256, [&](int64_t s, int) {
         const int64_t p = S->parent[s];
         if (p < 0) {
-            if (S->rptr[s + 1] != S->rptr[s]) return SPEIGS_ERR_INTERNAL;
-            continue;
+            if (S->rptr[s + 1] != S->rptr[s]) bad.store(1);
+            return;
         }
         const int64_t cp = S->c1[p] - S->c0[p];
         const int64_t *Rp = S->rows.data() + S->rptr[p], nRp = S->rptr[p + 1] - S->rptr[p];
@@ -513,16 +512,23 @@ int symbolic_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, const Symb
             const int64_t v = S->rows[q];
             int64_t pos;
             if (v < S->c1[p]) {
-                if (v < S->c0[p]) return SPEIGS_ERR_INTERNAL;
+                if (v < S->c0[p]) {
+                    bad.store(1);
+                    return;
+                }
                 pos = v - S->c0[p];
             } else {
                 while (k < nRp && Rp[k] < v) k++;
-                if (k >= nRp || Rp[k] != v) return SPEIGS_ERR_INTERNAL;
+                if (k >= nRp || Rp[k] != v) {
+                    bad.store(1);
+                    return;
+                }
                 pos = cp + k;
             }
             S->rel[S->relptr[s] + (q - S->rptr[s])] = (int32_t)pos;
         }
-    }
+    });
+    if (bad.load()) return SPEIGS_ERR_INTERNAL;
     S->t_symbolic = wall() - t1;
     return SPEIGS_OK;
 }
