@@ -409,6 +409,89 @@ __global__ void __launch_bounds__(64) k_potrf_reg(T *pool, const TileTask *__res
     for (int j = 0; j < TB; j++) X[(int64_t)j * TB + r] = xs[r][j];
 }
 
+// The same factorization and inverse on four waves: wave w holds columns
+// 16w .. 16w+15 of every row (lane r = row r), so the rank-1 updates of a step
+// are spread over the four SIMDs instead of one wave issuing all of them.
+// Step p: the wave owning column p forms the pivot and column p of L and of X
+// and leaves them in LDS (two buffers, alternating by step, so one barrier per
+// step suffices); every wave then reads the step's column with one wait and
+// updates its own columns q > p.  Each element sees the same operations in the
+// same order as in k_potrf_reg, so the factor and the inverse are the same to
+// the bit.
+template <class T>
+__global__ void __launch_bounds__(256) k_potrf_reg4(T *pool, const TileTask *__restrict__ tasks,
+                                                    const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
+                                                    const int32_t *__restrict__ ncol, const int64_t *__restrict__ col0,
+                                                    const double *__restrict__ diagA, double delta, double piv_rel,
+                                                    int *flag, T *__restrict__ xinv)
+{
+    __shared__ __align__(16) T Lb[2][TB];
+    __shared__ T Xb[2][TB];
+    __shared__ T xs[TB][TB + 1];
+    const TileTask t = tasks[blockIdx.x];
+    const int32_t F = Fs[t.s], c = ncol[t.s];
+    const int k0 = t.k * TB, w = min(TB, c - k0);
+    const int r = threadIdx.x & 63, wv = threadIdx.x >> 6, q0 = 16 * wv;
+    T *base = pool + foff[t.s] + (int64_t)k0 * F + k0;
+    // rows beyond w are the identity (their columns stay e_r, their X rows too)
+    T a[16], x[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int q = q0 + j;
+        a[j] = (r < w && q < w && q <= r) ? base[(int64_t)q * F + r] : T(q == r ? 1 : 0);
+        x[j] = T(q == r ? 1 : 0);
+    }
+    bool bad = false;
+    const double dAr = r < w ? diagA[col0[t.s] + k0 + r] : 0.0;
+#pragma unroll
+    for (int p = 0; p < TB; p++) {
+        if (p >= w) continue;                     // w is the same for the whole block
+        const int jp = p & 15, b = p & 1;
+        if (wv == (p >> 4)) {
+            T d = bcast(a[jp], p);
+            const double ref = fabs(bcast(dAr, p) + delta);
+            if (!((double)d > piv_rel * ref) || !(d == d)) {
+                bad = true;
+                d = T(1);
+            }
+            // true divisions (not a reciprocal), as in k_potrf_reg
+            const T sd = sqrt(d);
+            a[jp] = (r == p) ? sd : a[jp] / sd;
+            const T xp = x[jp] / sd;
+            x[jp] = xp;
+            Lb[b][r] = a[jp];
+            Xb[b][r] = xp;
+        }
+        __syncthreads();
+        if (q0 + 15 > p) {
+            const T lr = Lb[b][r], xr = Xb[b][r];
+            T lq[16];                                   // L(q, p), the same for every lane
+#pragma unroll
+            for (int j = 0; j < 16; j++) lq[j] = Lb[b][q0 + j];
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                const T an = a[j] - lr * lq[j], xn = x[j] - lq[j] * xr;
+                const bool upd = q0 + j > p;
+                a[j] = upd ? an : a[j];
+                x[j] = upd ? xn : x[j];
+            }
+        }
+    }
+    if (bad && r == 0) atomicOr(flag, 1);
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int q = q0 + j;
+        if (r < w && q < w && q <= r) base[(int64_t)q * F + r] = a[j];
+    }
+    // column r of X (column-major 64 x 64) is row r's x, transposed through LDS
+#pragma unroll
+    for (int j = 0; j < 16; j++) xs[q0 + j][r] = x[j];
+    __syncthreads();
+    T *X = xinv + (int64_t)blockIdx.x * (TB * TB);
+#pragma unroll
+    for (int j = 0; j < 16; j++) X[(int64_t)(q0 + j) * TB + r] = xs[r][q0 + j];
+}
+
 // tile row i below the diagonal block: B <- B X^T with X = L_kk^-1 from the
 // panel's scratch slot t.j (MFMA, the product formed as for the Schur update)
 template <class T>
@@ -1039,9 +1122,14 @@ int run_numeric(Factor *F, double delta)
                 for (size_t g = 0; g + 1 < cut.size(); g++) {
                     const int64_t p0 = (int64_t)g * kXinvSlots;
                     const int64_t pc = std::min(kXinvSlots, P.k_potrf_cnt[k] - p0);
-                    k_potrf_reg<T><<<dim3((unsigned)pc), dim3(64), 0, st>>>(
-                        pool, F->tiles + P.k_potrf_off[k] + p0, F->foff, F->Fs, F->ncol, F->col0, F->diagA, delta,
-                        piv_rel, F->flag, xinv);
+                    if (F->potrf_waves == 4)
+                        k_potrf_reg4<T><<<dim3((unsigned)pc), dim3(256), 0, st>>>(
+                            pool, F->tiles + P.k_potrf_off[k] + p0, F->foff, F->Fs, F->ncol, F->col0, F->diagA,
+                            delta, piv_rel, F->flag, xinv);
+                    else
+                        k_potrf_reg<T><<<dim3((unsigned)pc), dim3(64), 0, st>>>(
+                            pool, F->tiles + P.k_potrf_off[k] + p0, F->foff, F->Fs, F->ncol, F->col0, F->diagA,
+                            delta, piv_rel, F->flag, xinv);
                     if (cut[g + 1] > cut[g])
                         k_trsm_mfma<T><<<dim3((unsigned)(cut[g + 1] - cut[g])), dim3(256), 0, st>>>(
                             pool, F->tiles + cut[g], F->foff, F->Fs, F->ncol, xinv);
@@ -1160,6 +1248,7 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     if (const char *e = getenv("SPEIGS_HIP_SOLVE_MODE")) F->solve_mode = atoi(e);
     if (const char *e = getenv("SPEIGS_HIP_MFMA")) F->use_mfma = atoi(e) != 0;
     if (const char *e = getenv("SPEIGS_HIP_REG_PANEL")) F->use_reg_panel = atoi(e) != 0;
+    if (const char *e = getenv("SPEIGS_HIP_POTRF_WAVES")) F->potrf_waves = atoi(e) == 1 ? 1 : 4;
     SymbolicOptions so;
     int rc = symbolic_analyze(n, Ap, Ai, so, &F->S);
     if (rc) return rc;

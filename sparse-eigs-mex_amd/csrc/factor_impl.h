@@ -85,7 +85,8 @@ struct Factor {
     int64_t solves = 0, ir_steps = 0;
     bool factored = false;
     bool use_mfma = true;
-    bool use_reg_panel = true;       // one-wave register POTRF + TRSM as a product with L_kk^-1
+    bool use_reg_panel = true;       // register POTRF + TRSM as a product with L_kk^-1
+    int potrf_waves = 4;             // register POTRF on 4 waves (columns split) or 1
     int solve_mode = 1;               // 0: one workgroup per front, 1: blocked multi-workgroup
     EaTask *sea = nullptr;
     PcbTask *fin = nullptr;
