@@ -44,6 +44,7 @@ namespace {
 constexpr int RW = 32;            // floats per vector row
 constexpr int RTILE = 64;         // output rows per tile
 constexpr int RKCHUNK = 4;        // K tiles per part task
+constexpr int RBP = 36;           // LDS pitch of a staged vector row (rows k and k + 4 on other banks)
 constexpr int TINY = 64;          // wave-per-front limit on c and r
 constexpr int RP = 36;            // LDS row pitch (floats)
 
@@ -323,30 +324,19 @@ __global__ void __launch_bounds__(256) k_rs_gather(const RsRow *__restrict__ tas
 // 0, 1: directly in operand layout; modes 2, 3: column j = 16 wv + jj of the
 // wave's rows at tile row `lane`, to be transposed through LDS), the vector
 // rows in b0[] / b1[]; index 4 kq + t
+// the wave's A operands of K tile K: modes 0, 1 lane (g, lc) holds
+// A(row 16 wv + lc, k = 16 kq + 4 g + t) at x = 4 kq + t; modes 2, 3 the
+// wave's 16 output columns of L along k (lane = k), transposed through LDS later
 template <class T, int MODE>
-__device__ __forceinline__ void rs_load(const T *__restrict__ L, int F, int c, int I, int K, int kn, int orow0, int on,
-                                        int wv, int g, int lc, int lane, const RsArgs &a, int64_t c0,
-                                        int64_t ub, const float *__restrict__ buf,
-                                        const float *__restrict__ Yg, const float *__restrict__ Xg, float (&av)[16],
-                                        float (&b0)[16], float (&b1)[16])
+__device__ __forceinline__ void rs_load_a(const T *__restrict__ L, int F, int c, int I, int K, int kn, int orow0,
+                                          int on, int wv, int g, int lc, int lane, float (&av)[16])
 {
+    if (MODE == 0 || MODE == 1) {
 #pragma unroll
-    for (int kq = 0; kq < 4; kq++)
+        for (int kq = 0; kq < 4; kq++)
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const int k = 16 * kq + 4 * g + t, kk = K * RTILE + k, x = 4 * kq + t;
-            float v0 = 0.f, v1 = 0.f;
-            if (k < kn) {
-                const float *src;
-                if (MODE == 0 || MODE == 3) src = buf + (a.t0 + c0 + kk) * RW;
-                else if (MODE == 1) src = Yg + (c0 + kk) * RW;
-                else src = buf + (ub + kk) * RW;     // X_R, gathered into the U rows
-                v0 = src[lc];
-                v1 = src[16 + lc];
-            }
-            b0[x] = v0;
-            b1[x] = v1;
-            if (MODE == 0 || MODE == 1) {
+            for (int t = 0; t < 4; t++) {
+                const int k = 16 * kq + 4 * g + t, kk = K * RTILE + k, x = 4 * kq + t;
                 const int row = 16 * wv + lc;
                 float v = 0.f;
                 if (row < on && k < kn) {
@@ -358,8 +348,7 @@ __device__ __forceinline__ void rs_load(const T *__restrict__ L, int F, int c, i
                 }
                 av[x] = v;
             }
-        }
-    if (MODE == 2 || MODE == 3) {
+    } else {
 #pragma unroll
         for (int jj = 0; jj < 16; jj++) {
             const int j = 16 * wv + jj;      // tile-local output row = front column orow0 + j
@@ -374,54 +363,97 @@ __device__ __forceinline__ void rs_load(const T *__restrict__ L, int F, int c, i
     }
 }
 
+// the K tile's vector rows (64 x 32 floats), loaded by the whole workgroup:
+// thread e takes row e / 4, floats 8 (e % 4) .. +7
+template <int MODE>
+__device__ __forceinline__ void rs_load_b(const RsArgs &a, int64_t c0, int64_t ub, int K, int kn,
+                                          const float *__restrict__ buf, const float *__restrict__ Yg, float4 (&bv)[2])
+{
+    const int row = threadIdx.x >> 2, col = 8 * (threadIdx.x & 3);
+    const int64_t kk = (int64_t)K * RTILE + row;
+    bv[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+    bv[1] = bv[0];
+    if (row < kn) {
+        const float *src = (MODE == 0 || MODE == 3) ? buf + (a.t0 + c0 + kk) * RW
+                         : MODE == 1                ? Yg + (c0 + kk) * RW
+                                                    : buf + (ub + kk) * RW;     // X_R, gathered into the U rows
+        bv[0] = *reinterpret_cast<const float4 *>(src + col);
+        bv[1] = *reinterpret_cast<const float4 *>(src + col + 4);
+    }
+}
+
 template <class T, int MODE>
 __global__ void __launch_bounds__(256) k_rs_part(const T *__restrict__ pool, const RsPart *__restrict__ tasks, RsArgs a,
                                                  float *__restrict__ buf, float *__restrict__ Yg,
                                                  float *__restrict__ Xg, float *__restrict__ part)
 {
-    __shared__ float At[4][RTILE][20];    // modes 2, 3: [k][j] of the wave's A tile
+    __shared__ float At[4][RTILE][20];             // modes 2, 3: [k][j] of the wave's A tile
+    __shared__ __align__(16) float Bs[2][RTILE][RBP];   // the K tile's vector rows, double-buffered
     const RsPart tk = tasks[blockIdx.x];
     const int s = tk.s, I = tk.I;
     const int32_t F = a.Fs[s], c = a.ncol[s], r = F - c;
     const int orow0 = I * RTILE;
     const int on = min(RTILE, (MODE == 1 ? r : c) - orow0);
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, lc = lane & 15;
-    if (16 * wv >= on) return;     // this wave's rows are past the tile (no workgroup barrier below)
+    // a wave whose rows are past the tile still loads its share of the vector
+    // tiles and takes part in the barriers
+    const bool active = 16 * wv < on;
     const T *L = pool + a.foff[s];
     const int64_t c0 = a.col0[s];
     const int64_t ub = a.n + a.uoff[s];      // mode 2: X at the update rows
     const int kdim = MODE == 2 ? r : c;
     f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
-    float av[16], b0[16], b1[16];
-    rs_load<T, MODE>(L, F, c, I, tk.k0, min(RTILE, kdim - tk.k0 * RTILE), orow0, on, wv, g, lc, lane, a, c0, ub, buf,
-                     Yg, Xg, av, b0, b1);
+    float av[16];
+    float4 bv[2];
+    auto kn_of = [&](int K) { return min(RTILE, kdim - K * RTILE); };
+    if (active) rs_load_a<T, MODE>(L, F, c, I, tk.k0, kn_of(tk.k0), orow0, on, wv, g, lc, lane, av);
+    rs_load_b<MODE>(a, c0, ub, tk.k0, kn_of(tk.k0), buf, Yg, bv);
+    {
+        float *dst = &Bs[0][threadIdx.x >> 2][8 * (threadIdx.x & 3)];
+        *reinterpret_cast<float4 *>(dst) = bv[0];
+        *reinterpret_cast<float4 *>(dst + 4) = bv[1];
+    }
+    __syncthreads();
     for (int K = tk.k0; K < tk.k1; K++) {
-        float ca[16], c0v[16], c1v[16];
+        const int bb = (K - tk.k0) & 1;
+        float ca[16];
 #pragma unroll
-        for (int x = 0; x < 16; x++) {
-            ca[x] = av[x];
-            c0v[x] = b0[x];
-            c1v[x] = b1[x];
+        for (int x = 0; x < 16; x++) ca[x] = av[x];
+        const bool more = K + 1 < tk.k1;
+        if (more) {     // next tile's loads in flight during this tile's products
+            if (active) rs_load_a<T, MODE>(L, F, c, I, K + 1, kn_of(K + 1), orow0, on, wv, g, lc, lane, av);
+            rs_load_b<MODE>(a, c0, ub, K + 1, kn_of(K + 1), buf, Yg, bv);
         }
-        if (K + 1 < tk.k1)     // next tile's loads in flight during this tile's products
-            rs_load<T, MODE>(L, F, c, I, K + 1, min(RTILE, kdim - (K + 1) * RTILE), orow0, on, wv, g, lc, lane, a,
-                             c0, ub, buf, Yg, Xg, av, b0, b1);
-        if (MODE == 2 || MODE == 3) {
+        if (active) {
+            if (MODE == 2 || MODE == 3) {
 #pragma unroll
-            for (int jj = 0; jj < 16; jj++) At[wv][lane][jj] = ca[jj];
-            wave_lds_sync();
+                for (int jj = 0; jj < 16; jj++) At[wv][lane][jj] = ca[jj];
+                wave_lds_sync();
+#pragma unroll
+                for (int kq = 0; kq < 4; kq++)
+#pragma unroll
+                    for (int t = 0; t < 4; t++) ca[4 * kq + t] = At[wv][16 * kq + 4 * g + t][lc];
+                wave_lds_sync();
+            }
 #pragma unroll
             for (int kq = 0; kq < 4; kq++)
 #pragma unroll
-                for (int t = 0; t < 4; t++) ca[4 * kq + t] = At[wv][16 * kq + 4 * g + t][lc];
-            wave_lds_sync();
+                for (int t = 0; t < 4; t++) {
+                    const int x = 4 * kq + t, k = 16 * kq + 4 * g + t;
+                    acc0 = mfma4(ca[x], Bs[bb][k][lc], acc0);
+                    acc1 = mfma4(ca[x], Bs[bb][k][16 + lc], acc1);
+                }
         }
-#pragma unroll
-        for (int x = 0; x < 16; x++) {
-            acc0 = mfma4(ca[x], c0v[x], acc0);
-            acc1 = mfma4(ca[x], c1v[x], acc1);
+        if (more) {
+            // the other buffer was last read in the previous tile, before the
+            // barrier that ended it
+            float *dst = &Bs[bb ^ 1][threadIdx.x >> 2][8 * (threadIdx.x & 3)];
+            *reinterpret_cast<float4 *>(dst) = bv[0];
+            *reinterpret_cast<float4 *>(dst + 4) = bv[1];
         }
+        __syncthreads();
     }
+    if (!active) return;
     if (tk.direct) {
         // modes 1, 2 read before they write: all reads first (a store between
         // them would serialize the loads)
