@@ -5,7 +5,7 @@ on one MI355X: the chosen path against the plain FP64 GPU path.
 
 Chosen path (precision SPEIGS_HIP_FP32_BLOCK): FP32 multifrontal factor on MFMA
 with the diagonal blocks inverted in place, used as the preconditioner of a
-block eigensolver (LOBPCG, block 30) -- one FP32 multi-right-hand-side solve per
+block eigensolver (LOBPCG, block k + 8 = 28) -- one FP32 multi-right-hand-side solve per
 iteration, Ritz pairs and residuals against A in FP64, so the answer meets the
 same FP64 residual test.  FP32 factor + per-apply iterative refinement
 (SPEIGS_HIP_FP32_IR) is available as --paths fp32ir.
