@@ -137,27 +137,45 @@ private:
     // BFS inside region r from s; fills q[0..cnt), lev_, returns cnt and height
     int64_t bfs(int32_t r, int32_t s, int32_t &h, std::vector<int32_t> &q)
     {
-        int32_t *rid = rid_.data();
-        int64_t head = 0, tail = 0;
+        int32_t *rid = rid_.data(), *lev = lev_.data(), *qq = q.data();
+        const int64_t *xadj = g_.xadj.data();
+        const int32_t *adj = g_.adj.data();
         const int32_t vis = -r;       // temporary mark
-        q[tail++] = s;
-        lev_[s] = 0;
+        int64_t head = 0, tail = 0;
+        qq[tail++] = s;
+        lev[s] = 0;
         rid_st(rid, s, vis);
-        h = 0;
+        // the queue is in level order: the nodes being expanded are at level
+        // cur until head reaches lend.  The memory a later expansion touches
+        // is prefetched in three stages (its xadj entry, adjacency list and
+        // neighbours' ids), which does not change the visiting order.
+        int32_t cur = 0;
+        int64_t lend = 1;
+        constexpr int64_t kA = 16;
         while (head < tail) {
-            const int32_t v = q[head++];
-            const int32_t lv = lev_[v] + 1;
-            for (int64_t p = g_.xadj[v]; p < g_.xadj[v + 1]; p++) {
-                const int32_t u = g_.adj[p];
+            if (head == lend) {
+                cur++;
+                lend = tail;
+            }
+            if (head + 3 * kA < tail) __builtin_prefetch(xadj + qq[head + 3 * kA]);
+            if (head + 2 * kA < tail) __builtin_prefetch(adj + xadj[qq[head + 2 * kA]]);
+            if (head + kA < tail) {
+                const int32_t w = qq[head + kA];
+                for (int64_t p = xadj[w]; p < xadj[w + 1]; p++) __builtin_prefetch(rid + adj[p], 1);
+            }
+            const int32_t v = qq[head++];
+            const int32_t lv = cur + 1;
+            for (int64_t p = xadj[v]; p < xadj[v + 1]; p++) {
+                const int32_t u = adj[p];
                 if (rid_ld(rid, u) == r) {
                     rid_st(rid, u, vis);
-                    lev_[u] = lv;
-                    if (lv > h) h = lv;
-                    q[tail++] = u;
+                    lev[u] = lv;
+                    qq[tail++] = u;
                 }
             }
         }
-        for (int64_t i = 0; i < tail; i++) rid_st(rid, q[i], r);
+        h = lev[qq[tail - 1]];        // the last node queued is on the last level
+        for (int64_t i = 0; i < tail; i++) rid_st(rid, qq[i], r);
         return tail;
     }
 
