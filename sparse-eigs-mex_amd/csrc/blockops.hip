@@ -234,12 +234,29 @@ __global__ void __launch_bounds__(256) k_sell_rspmm(int64_t nslices, const int64
     double s[NC];
 #pragma unroll
     for (int j = 0; j < NC; j++) s[j] = 0.0;
-    for (int k = 0; k < w; k++, pp += 64) {
-        const double a = val[pp];
-        const double *xr = X + (int64_t)col[pp] * ldx + t;
+    // eight nonzeros at a time: their columns and values, then all of their X
+    // loads, are in flight together; the sums run in the nonzeros' order
+    for (int k0 = 0; k0 < w; k0 += 8, pp += 8 * 64) {
+        const int kc = min(8, w - k0);
+        double a[8];
+        int64_t cc[8];
 #pragma unroll
-        for (int j = 0; j < NC; j++)
-            if (16 * j + t < nv) s[j] = fma(a, xr[16 * j], s[j]);
+        for (int u = 0; u < 8; u++) {
+            a[u] = u < kc ? val[pp + 64 * u] : 0.0;
+            cc[u] = u < kc ? (int64_t)col[pp + 64 * u] : row;
+        }
+        double xv[8][NC];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+#pragma unroll
+            for (int j = 0; j < NC; j++)
+                xv[u][j] = (u < kc && 16 * j + t < nv) ? X[cc[u] * ldx + 16 * j + t] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (u < kc)
+#pragma unroll
+                for (int j = 0; j < NC; j++)
+                    if (16 * j + t < nv) s[j] = fma(a[u], xv[u][j], s[j]);
     }
 #pragma unroll
     for (int j = 0; j < NC; j++)
