@@ -101,6 +101,53 @@ int svqb_transform(int nz, const std::vector<double> &zz, std::vector<double> &T
     return 0;
 }
 
+// Cholesky form of the same orthonormalisation: with the Gram matrix scaled to
+// unit diagonal, D zz D = R^T R and T = D R^-1 (upper triangular).  Returns 1
+// (use SVQB) when a pivot falls below 1e-8 -- a Gram matrix too close to
+// singular, whose near-null directions SVQB drops; *cond = (max R_ii / min
+// R_ii)^2 estimates the scaled Gram matrix's condition number.
+int chol_transform(int nz, const std::vector<double> &zz, std::vector<double> &T, double *cond)
+{
+    std::vector<double> d(nz), R((size_t)nz * nz, 0.0);
+    for (int i = 0; i < nz; i++) {
+        const double g = zz[(size_t)i * nz + i];
+        if (!(g > 0)) return 1;
+        d[i] = 1.0 / std::sqrt(g);
+    }
+    // upper R, column-major: R[j * nz + i], i <= j
+    for (int j = 0; j < nz; j++) {
+        for (int i = 0; i <= j; i++) {
+            double acc = d[i] * 0.5 * (zz[(size_t)j * nz + i] + zz[(size_t)i * nz + j]) * d[j];
+            for (int q = 0; q < i; q++) acc -= R[(size_t)i * nz + q] * R[(size_t)j * nz + q];
+            if (i < j) {
+                R[(size_t)j * nz + i] = acc / R[(size_t)i * nz + i];
+            } else {
+                if (!(acc > 1e-8)) return 1;
+                R[(size_t)j * nz + j] = std::sqrt(acc);
+            }
+        }
+    }
+    double rmax = 0.0, rmin = 1e300;
+    for (int i = 0; i < nz; i++) {
+        rmax = std::max(rmax, R[(size_t)i * nz + i]);
+        rmin = std::min(rmin, R[(size_t)i * nz + i]);
+    }
+    *cond = (rmax / rmin) * (rmax / rmin);
+    // T = D R^-1: back substitution column by column
+    T.assign((size_t)nz * nz, 0.0);
+    for (int j = 0; j < nz; j++) {
+        T[(size_t)j * nz + j] = 1.0 / R[(size_t)j * nz + j];
+        for (int i = j - 1; i >= 0; i--) {
+            double acc = 0.0;
+            for (int q = i + 1; q <= j; q++) acc += R[(size_t)q * nz + i] * T[(size_t)j * nz + q];
+            T[(size_t)j * nz + i] = -acc / R[(size_t)i * nz + i];
+        }
+    }
+    for (int j = 0; j < nz; j++)
+        for (int i = 0; i <= j; i++) T[(size_t)j * nz + i] *= d[i];
+    return 0;
+}
+
 // SVQB of Z: out = Z T (out may be Z itself: the product is row-local)
 int svqb(LWork &W, const double *Z, int64_t ldz, int nz, const std::vector<double> &zz, double *out, int64_t ldo,
          int *r_out, double *cond = nullptr)
@@ -137,8 +184,13 @@ int project_svqb(LWork &W, double *S, int64_t ldS, int m, int nz, int *r, double
         if (b > 0) kmin = std::min(kmin, std::sqrt(std::max(a, 0.0) / b));
     }
     *keep = kmin;
+    // the Cholesky form when the projected block is well conditioned, SVQB
+    // (which can drop directions) otherwise
     std::vector<double> T;
-    if ((rc = svqb_transform(nz, zz, T, r, cond))) return rc;
+    if (chol_transform(nz, zz, T, cond) == 0)
+        *r = nz;
+    else if ((rc = svqb_transform(nz, zz, T, r, cond)))
+        return rc;
     const int rr = *r;
     if (rr == 0) return 0;
     // C = [-(X^T Z) T ; T], (m + nz) x rr column-major
