@@ -336,7 +336,7 @@ int try_banded(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax
     // kd above either limit settles it without the RCM ordering (the BFS starts
     // at the last column, which on upper-triangle storage reaches the most)
     const int64_t kcap = std::min<int64_t>(bmax, (int64_t)(2.0e7 / (double)n) - 1);
-    if (n > 1 && band_lower_bound(n, Ap, Ai, n - 1, 1 << 16) > kcap) return 0;
+    if (n > 1 && band_lower_bound(n, Ap, Ai, n - 1, 1 << 14) > kcap) return 0;
     double tb = wall();
     std::vector<int64_t> pm, ip(n);
     rcm(n, Ap, Ai, pm);

@@ -48,12 +48,19 @@ __global__ void __launch_bounds__(256) k_inv_diag(T *pool, const InvTask *__rest
     const int32_t F = Fs[t.s], c = ncol[t.s];
     const int k0 = t.J * TB, w = tile_w(c, t.J);
     T *base = pool + foff[t.s] + (int64_t)k0 * F + k0;
-    for (int e = threadIdx.x; e < TB * TB; e += 256) {
-        const int r = e % TB, q = e / TB;
+    T va[TB * TB / 256];      // all of a thread's loads in flight before its LDS stores
+#pragma unroll
+    for (int u = 0; u < TB * TB / 256; u++) {
+        const int e = threadIdx.x + 256 * u, r = e % TB, q = e / TB;
         T v = T(0);
         if (r < w && q < w) v = r >= q ? base[(int64_t)q * F + r] : T(0);
         else if (r == q) v = T(1);
-        a[r][q] = v;
+        va[u] = v;
+    }
+#pragma unroll
+    for (int u = 0; u < TB * TB / 256; u++) {
+        const int e = threadIdx.x + 256 * u, r = e % TB, q = e / TB;
+        a[r][q] = va[u];
         x[r][q] = T(0);
     }
     __syncthreads();
@@ -133,9 +140,11 @@ __global__ void __launch_bounds__(256) k_inv_gemm(T *pool, const InvG *__restric
         const int tk = (MODE == 0 ? t.C1 : t.R1) + k;   // the inner tile (absolute)
         const int wk = tile_w(c, tk);
         if (wk <= 0) break;
-        __syncthreads();
-        for (int e = threadIdx.x; e < TB * TB; e += 256) {
-            const int r = e & 63, q = e >> 6;
+        // all 32 loads of a thread in flight before its LDS stores
+        T va[TB * TB / 256], vb[TB * TB / 256];
+#pragma unroll
+        for (int u = 0; u < TB * TB / 256; u++) {
+            const int e = threadIdx.x + 256 * u, r = e & 63, q = e >> 6;
             T av = T(0), bv = T(0);
             if (MODE == 0) {
                 // A = L(ti, tk)(r, q); B = X(tk, tj)(r, q), lower on the diagonal tile
@@ -146,8 +155,15 @@ __global__ void __launch_bounds__(256) k_inv_gemm(T *pool, const InvG *__restric
                 if (r < wi && q < wk && (k < t.i || r >= q)) av = fr[(int64_t)(tk * TB + q) * F + ti * TB + r];
                 bv = scratch[(t.tslot + (int64_t)k * t.b + t.j) * (TB * TB) + q * TB + r];
             }
-            At[q][r] = av;
-            Bt[r][q] = bv;
+            va[u] = av;
+            vb[u] = bv;
+        }
+        __syncthreads();          // the previous tile's products are done
+#pragma unroll
+        for (int u = 0; u < TB * TB / 256; u++) {
+            const int e = threadIdx.x + 256 * u, r = e & 63, q = e >> 6;
+            At[q][r] = va[u];
+            Bt[r][q] = vb[u];
         }
         __syncthreads();
         tile_mma<T, IP>(At, Bt, (wk + 3) & ~3, acc);

@@ -510,10 +510,19 @@ __global__ void __launch_bounds__(256) k_trsm_mfma(T *pool, const TileTask *__re
     T *fr = pool + foff[t.s];
     const T *X = xinv + (int64_t)t.j * (TB * TB);
     const int w4 = (w + 3) & ~3;
-    for (int e = threadIdx.x; e < TB * TB; e += 256) {
-        const int rr = e % TB, kk = e / TB;
-        Bt[kk][rr] = (rr < nr && kk < w) ? fr[(int64_t)(k0 + kk) * F + r0 + rr] : T(0);
-        Xt[kk][rr] = (kk < w && rr < w) ? X[(int64_t)kk * TB + rr] : T(0);   // X(rr, kk): column kk of X
+    // all 32 loads of a thread in flight before its LDS stores
+    T vb[TB * TB / 256], vx[TB * TB / 256];
+#pragma unroll
+    for (int u = 0; u < TB * TB / 256; u++) {
+        const int e = threadIdx.x + 256 * u, rr = e % TB, kk = e / TB;
+        vb[u] = (rr < nr && kk < w) ? fr[(int64_t)(k0 + kk) * F + r0 + rr] : T(0);
+        vx[u] = (kk < w && rr < w) ? X[(int64_t)kk * TB + rr] : T(0);   // X(rr, kk): column kk of X
+    }
+#pragma unroll
+    for (int u = 0; u < TB * TB / 256; u++) {
+        const int e = threadIdx.x + 256 * u, rr = e % TB, kk = e / TB;
+        Bt[kk][rr] = vb[u];
+        Xt[kk][rr] = vx[u];
     }
     __syncthreads();
     V acc[2][2];

@@ -288,12 +288,24 @@ __global__ void __launch_bounds__(256) k_rs_assemble(const RsRow *__restrict__ t
     const int64_t rid = a.frow[s] + i;
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     const int64_t q1 = a.lptr[rid + 1];
-    for (int64_t q = a.lptr[rid]; q < q1; q++) {
-        const f32x4 x = *(const f32x4 *)(buf + (int64_t)a.lst[q] * RW + 4 * q4);
-        acc.x += x.x;
-        acc.y += x.y;
-        acc.z += x.z;
-        acc.w += x.w;
+    // four list entries at a time: their indices, then their rows, in flight
+    // together; the sums stay in list order
+    for (int64_t q = a.lptr[rid]; q < q1; q += 4) {
+        int32_t idx[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) idx[u] = q + u < q1 ? a.lst[q + u] : 0;
+        f32x4 x[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            x[u] = q + u < q1 ? *(const f32x4 *)(buf + (int64_t)idx[u] * RW + 4 * q4) : f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (q + u < q1) {
+                acc.x += x[u].x;
+                acc.y += x[u].y;
+                acc.z += x[u].z;
+                acc.w += x[u].w;
+            }
     }
     const int64_t dst = i < c ? a.t0 + a.col0[s] + i : a.n + a.uoff[s] + (i - c);
     *(f32x4 *)(buf + dst * RW + 4 * q4) = acc;
