@@ -164,3 +164,28 @@ def test_analyze_deterministic_across_threads(hiplib, monkeypatch, case):
     assert np.array_equal(p1, p8) and np.array_equal(s1, s8) and ns1 == ns8
     assert np.array_equal(np.sort(p1), np.arange(n))
     assert np.array_equal(np.unique(s1), np.arange(ns1))
+
+
+@pytest.mark.parametrize("N,seed,perm_sha,snode_sha,nnz_l,nsuper", [
+    (30, 3, "680a8cd8cf9ea15d", "e480163c2b78dc3e", 4718718, 1247),
+    (24, 7, "a8c2225333b1c43b", "c80161f20d4db7e4", 1975935, 601),
+])
+def test_analyze_ordering_unchanged(hiplib, N, seed, perm_sha, snode_sha, nnz_l, nsuper):
+    """The nested-dissection ordering and supernode partition are integer-only
+    and deterministic: pinned here so that a change meant to make the analysis
+    faster (prefetching, parallel passes, reused BFS structures) cannot change
+    the permutation, the fill or the factor's shape unnoticed."""
+    import ctypes
+    import hashlib
+    A = pkg.gen.weighted_lap3d(N, seed=seed)
+    Ap, Ai = (np.ascontiguousarray(a) for a in A[:2])
+    n = len(Ap) - 1
+    hiplib.speigs_hip_analyze.argtypes = [ctypes.c_int64] + [ctypes.c_void_p] * 4 + [ctypes.POINTER(pkg.HipInfo)]
+    perm = np.zeros(n, np.int64)
+    snode = np.zeros(n, np.int64)
+    hi = pkg.HipInfo()
+    assert hiplib.speigs_hip_analyze(n, Ap.ctypes.data, Ai.ctypes.data, perm.ctypes.data, snode.ctypes.data,
+                                     ctypes.byref(hi)) == 0
+    assert hashlib.sha256(perm.tobytes()).hexdigest()[:16] == perm_sha
+    assert hashlib.sha256(snode.tobytes()).hexdigest()[:16] == snode_sha
+    assert hi.nnz_L == nnz_l and hi.nsuper == nsuper
