@@ -469,6 +469,10 @@ int symbolic_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, const Symb
     // their parent's level starts), nodes of a level in parallel
     std::vector<std::vector<int64_t>> rowset(ns);
     std::vector<std::vector<int32_t>> mark(nt);
+    // the inverse permutation in 32 bits (half the cache footprint of the
+    // lookups below)
+    std::vector<int32_t> ip32(n);
+    parallel_for(n, nt, 65536, [&](int64_t i, int) { ip32[i] = (int32_t)S->iperm[i]; });
     for (int32_t L = 0; L < S->nlevels; L++) {
         const std::vector<int64_t> &nodes = bylevel[L];
         parallel_for((int64_t)nodes.size(), nt, 16, [&](int64_t i, int tid) {
@@ -483,7 +487,7 @@ int symbolic_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, const Symb
             for (int64_t col = S->c0[s]; col < hi; col++) {
                 const int32_t old = (int32_t)S->perm[col];
                 for (int64_t p = g.xadj[old]; p < g.xadj[old + 1]; p++) {
-                    const int64_t v = S->iperm[g.adj[p]];
+                    const int64_t v = ip32[g.adj[p]];
                     if (v >= hi && mk[v] != (int32_t)s) { mk[v] = (int32_t)s; cand.push_back(v); }
                 }
             }
@@ -495,6 +499,7 @@ int symbolic_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, const Symb
     S->rows.resize(S->rptr[ns]);
     parallel_for(ns, nt, 256, [&](int64_t s, int) {
         std::copy(rowset[s].begin(), rowset[s].end(), S->rows.begin() + S->rptr[s]);
+        std::vector<int64_t>().swap(rowset[s]);      // freed in parallel too
     });
     std::vector<std::vector<int64_t>>().swap(rowset);
     // front sizes, offsets, counts
