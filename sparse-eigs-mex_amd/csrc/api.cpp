@@ -43,12 +43,19 @@ int key_desc(const void *a, const void *b) { return -key_asc(a, b); }
 // 1 full symmetric, 2 upper only, 3 lower only, 0 not symmetric (src/speigs_core.c:363-401)
 int symmetry_kind(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax)
 {
-    bool up = false, lo = false;
-    for (int64_t c = 0; c < n && !(up && lo); c++)
+    // which triangles hold entries: a parallel scan (chunks stop once both
+    // have been seen)
+    std::atomic<int> seen{0};     // bit 0: above the diagonal, bit 1: below
+    parallel_for(n, host_threads(), 65536, [&](int64_t c, int) {
+        if (seen.load(std::memory_order_relaxed) == 3) return;
+        int f = 0;
         for (int64_t q = Ap[c]; q < Ap[c + 1]; q++) {
-            if (Ai[q] < c) up = true;
-            else if (Ai[q] > c) lo = true;
+            if (Ai[q] < c) f |= 1;
+            else if (Ai[q] > c) f |= 2;
         }
+        if (f & ~seen.load(std::memory_order_relaxed)) seen.fetch_or(f, std::memory_order_relaxed);
+    });
+    const bool up = seen.load() & 1, lo = seen.load() & 2;
     if (up && !lo) return 2;
     if (lo && !up) return 3;
     if (!up && !lo) return 1;
@@ -230,14 +237,31 @@ void rcm(int64_t n, const int64_t *Ap, const int64_t *Ai, std::vector<int64_t> &
 
 double norm1_upper(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax)
 {
+    // |A| column sums from the upper triangle: column j adds its entries to
+    // cs[j] and, mirrored, to cs[i] of each row i < j.  Threads take column
+    // ranges; a mirrored entry whose row lies in an earlier range is queued and
+    // added after all ranges are done, in range order, so each cs[i] sums its
+    // terms in the order of the serial loop and the result is bitwise the same.
     std::vector<double> cs(n, 0.0);
-    for (int64_t j = 0; j < n; j++)
-        for (int64_t q = Ap[j]; q < Ap[j + 1]; q++) {
-            int64_t i = Ai[q];
-            if (i > j) continue;
-            cs[j] += std::fabs(Ax[q]);
-            if (i != j) cs[i] += std::fabs(Ax[q]);
-        }
+    const int T = (int)std::max<int64_t>(1, std::min<int64_t>(host_threads(), n / 65536));
+    const int64_t per = (n + T - 1) / T;
+    std::vector<std::vector<std::pair<int64_t, double>>> late(T);
+    parallel_for(T, T, 1, [&](int64_t t, int) {
+        const int64_t c0 = t * per, c1 = std::min(n, c0 + per);
+        for (int64_t j = c0; j < c1; j++)
+            for (int64_t q = Ap[j]; q < Ap[j + 1]; q++) {
+                const int64_t i = Ai[q];
+                if (i > j) continue;
+                cs[j] += std::fabs(Ax[q]);
+                if (i == j) continue;
+                if (i >= c0)
+                    cs[i] += std::fabs(Ax[q]);
+                else
+                    late[t].push_back({i, std::fabs(Ax[q])});
+            }
+    });
+    for (int t = 0; t < T; t++)
+        for (const auto &e : late[t]) cs[e.first] += e.second;
     double m = 0.0;
     for (double v : cs) m = std::max(m, v);
     return m;
@@ -432,6 +456,15 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
               speigs_which which, const speigs_opts *opts, const speigs_hip_opts *hopts, double *lambda, double *V,
               double *resid, speigs_info *info, speigs_hip_info *hinfo)
 {
+    // declared first, destroyed last: with SPEIGS_HIP_TRACE=1 it reports the
+    // time the other locals' destructors take (device memory, streams)
+    struct TraceRelease {
+        double t = 0.0;
+        ~TraceRelease()
+        {
+            if (t > 0.0) trace_phase("release (memory, streams)", t);
+        }
+    } trace_release;
     speigs_opts dflt;
     if (!opts) {
         speigs_default_opts(&dflt);
@@ -649,6 +682,13 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
         }
     }
     trace_phase("final order", tp);
+    if (trace_on()) {
+        op.reset();
+        trace_phase("release operator", tp);
+        mem.release();
+        trace_phase("release workspace", tp);
+        trace_release.t = wall();
+    }
     return rc;
 }
 

@@ -114,6 +114,42 @@ def test_gpu_paths_fail_loudly_without_device(hiplib):
     assert _solve(A, 4, "lm").rc == -5          # SPEIGS_ERR_INTERNAL, no CPU fallback
 
 
+def test_norm_matches_serial_sum_order(hiplib):
+    """info.anorm (max |A| column sum, set before the device is needed) equals
+    the column sums accumulated in the order of one serial pass over the
+    stored upper triangle, bitwise, with rows coupled across the whole range
+    so that the threaded sum must defer entries of earlier column ranges."""
+    if hiplib.speigs_hip_device_count() > 0:
+        pytest.skip("a HIP device is present: the call would run the eigensolver")
+    rng = np.random.default_rng(11)
+    n = 300_000
+    far = rng.integers(0, np.arange(1, n), size=n - 1)          # a row < j for every column j >= 1
+    cols = [np.arange(n), np.arange(1, n), np.arange(1, n)]
+    rows = [np.arange(n), np.arange(n - 1), far]
+    col = np.concatenate(cols)
+    row = np.concatenate(rows)
+    o = np.lexsort((row, col))
+    col, row = col[o], row[o]
+    keep = np.ones(col.size, bool)
+    keep[1:] = (col[1:] != col[:-1]) | (row[1:] != row[:-1])
+    col, row = col[keep], row[keep]
+    val = rng.uniform(-3.0, 3.0, col.size) * 10.0 ** rng.integers(-8, 8, col.size)
+    val[row == col] = np.abs(val[row == col]) + 1.0
+    Ap = np.zeros(n + 1, np.int64)
+    np.add.at(Ap, col + 1, 1)
+    Ap = np.cumsum(Ap)
+    # serial order: entry q adds |a| to cs[col], then (off the diagonal) to cs[row]
+    tgt = np.stack([col, row], 1).ravel()
+    add = np.stack([np.abs(val), np.where(row == col, 0.0, np.abs(val))], 1).ravel()
+    live = np.ones(tgt.size, bool)
+    live[1::2] = row != col
+    cs = np.zeros(n)
+    np.add.at(cs, tgt[live], add[live])
+    r = _solve((Ap, row.astype(np.int64), val), 4, "sm", vectors=False)
+    assert r.rc == -5
+    assert r.info["anorm"] == cs.max()
+
+
 @pytest.mark.parametrize("case", ["full", "upper", "weighted_upper", "blocks", "chain"])
 def test_fast_rcm_equals_reference_scan(hiplib, oracle, case):
     """The O(n+nnz) RCM of the banded gate reproduces the reference's O(n^2)
