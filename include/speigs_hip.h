@@ -83,6 +83,13 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
 /* number of visible HIP devices (0 when none); never fails */
 int speigs_hip_device_count(void);
 
+/* Device memory freed at the end of a call is kept per device and reused by
+ * later calls (up to SPEIGS_HIP_CACHE_MB megabytes per device, default a
+ * quarter of its memory; 0 disables keeping).  This returns every kept block
+ * to the HIP runtime.  No counterpart in the reference (its solver runs on the
+ * host); for callers that share the device with other allocators. */
+void speigs_hip_release_cache(void);
+
 /* Ordering + supernodal symbolic analysis only: host code, no device needed.
  * Replaces cholmod_l_analyze (src/speigs_core.c:933-936).  perm[n] (may be
  * NULL) receives the fill-reducing order, perm[new] = old; snode[n] (may be

@@ -122,6 +122,8 @@ def _bind_ext(lib: ctypes.CDLL) -> None:
                               POINTER(HipOpts), _F64P, _F64P, _F64P, POINTER(Info), POINTER(HipInfo)]
     lib.speigs_hip_device_count.restype = c_int
     lib.speigs_hip_device_count.argtypes = []
+    lib.speigs_hip_release_cache.restype = None
+    lib.speigs_hip_release_cache.argtypes = []
     vp, u64, f32p = ctypes.c_void_p, ctypes.c_uint64, POINTER(ctypes.c_float)
     lib.speigs_hip_model_create.restype = c_int
     lib.speigs_hip_model_create.argtypes = [c_int, u64, POINTER(vp)]
@@ -327,6 +329,12 @@ def speigs_z(A_p, A_i, Axr, Axi, k: int, which="sm", opts: dict | None = None, l
             SM if which in ("sm", SM) else LM, ctypes.byref(o), _ptr(lam, c_double), _ptr(Vr, c_double),
             _ptr(Vi, c_double), _ptr(res, c_double), ctypes.byref(info))
     return Result(rc, lam, (Vr.T + 1j * Vi.T), res, info.as_dict())
+
+
+def release_cache(lib=None) -> None:
+    """Return the device memory the library keeps between calls to the HIP
+    runtime (speigs_hip_release_cache)."""
+    (lib or load()).speigs_hip_release_cache()
 
 
 def errmsg(code: int, lib=None) -> str:

@@ -683,6 +683,14 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
     }
     trace_phase("final order", tp);
     if (trace_on()) {
+        if (auto *fo = dynamic_cast<FactorOp *>(op.get())) {
+            if (fo->F) {
+                factor_block_free(fo->F);
+                trace_phase("release solve state", tp);
+                fo->F->mem.release();
+                trace_phase("release factor memory", tp);
+            }
+        }
         op.reset();
         trace_phase("release operator", tp);
         mem.release();

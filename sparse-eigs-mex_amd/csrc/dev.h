@@ -44,22 +44,36 @@ inline int64_t pad_ld(int64_t n) { return (n + 31) / 32 * 32; }
 inline int64_t nblocks_rows(int64_t n) { return (n + kRowsPerBlock - 1) / kRowsPerBlock; }
 
 // ------------------------------------------------------------ device memory
+// Device buffers come from a per-device cache of freed blocks (devcache.cpp):
+// dev_alloc reuses a kept block of the request's size class or allocates one;
+// dev_release waits for the blocks' devices to finish their queued work and
+// keeps the blocks for later requests (up to a cap per device).
+struct DevBlock {
+    void *p;
+    size_t bytes;
+    int dev;
+};
+int dev_alloc(void **p, size_t bytes, int *dev, size_t *bytes_out);
+void dev_release(std::vector<DevBlock> &blocks);
+
 struct DevMem {
-    std::vector<void *> ptrs;
+    std::vector<DevBlock> blocks;
+    DevMem() = default;
+    DevMem(const DevMem &) = delete;
+    DevMem &operator=(const DevMem &) = delete;
     ~DevMem() { release(); }
     template <class T> int alloc(T **p, size_t count) {
         void *q = nullptr;
+        int dev = 0;
+        size_t got = 0;
         if (count == 0) count = 1;
-        hipError_t e = hipMalloc(&q, count * sizeof(T));
-        if (e != hipSuccess) { (void)hipGetLastError(); *p = nullptr; return SPEIGS_ERR_MEM; }
-        ptrs.push_back(q);
+        const int rc = dev_alloc(&q, count * sizeof(T), &dev, &got);
+        if (rc) { *p = nullptr; return rc; }
+        blocks.push_back({q, got, dev});
         *p = static_cast<T *>(q);
         return 0;
     }
-    void release() {
-        for (void *q : ptrs) (void)hipFree(q);
-        ptrs.clear();
-    }
+    void release() { dev_release(blocks); }
 };
 
 // ------------------------------------------------------------ host <-> device copies

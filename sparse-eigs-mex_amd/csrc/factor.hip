@@ -29,10 +29,17 @@ namespace sehip {
 
 
 // ------------------------------------------------------------------ kernels
+// p[0 .. n) = 0 with 16-byte stores over a grid stride (p 16-byte aligned);
+// one element per thread ran at ~0.7 TB/s on the 2.3 GB FP32 pool
 template <class T>
-__global__ void k_zero(T *p, int64_t n)
+__global__ void __launch_bounds__(256) k_zero(T *p, int64_t n)
 {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t n16 = n * (int64_t)sizeof(T) / 16;
+    float4 *q = reinterpret_cast<float4 *>(p);
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n16; i += stride)
+        q[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const int64_t i = n16 * 16 / (int64_t)sizeof(T) + (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = T(0);
 }
 
@@ -1114,7 +1121,9 @@ int run_numeric(Factor *F, double delta)
     T *pool = (T *)F->pool;
     const double piv_rel = F->fp32 ? 1e-5 : 1e-11;
     SEHIP_CHECK(hipMemsetAsync(F->flag, 0, sizeof(int), st));
-    k_zero<T><<<dim3((unsigned)((S.pool + 255) / 256)), dim3(256), 0, st>>>(pool, S.pool);
+    if ((uintptr_t)pool % 16) return SPEIGS_ERR_INTERNAL;
+    k_zero<T><<<dim3((unsigned)std::min<int64_t>(4096, (S.pool * (int64_t)sizeof(T) / 16 + 255) / 256 + 1)), dim3(256), 0,
+                st>>>(pool, S.pool);
     k_assemble<T><<<dim3((unsigned)((F->nasm + 255) / 256)), dim3(256), 0, st>>>(pool, F->asm_dst, F->asm_src, F->Ax,
                                                                                 F->nasm, delta);
     for (int32_t L = 0; L < S.nlevels; L++) {

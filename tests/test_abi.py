@@ -107,6 +107,10 @@ def test_host_nonsymmetric_rejected(hiplib):
     assert _solve((Ap, Ai, Ax), 3, "sm", vectors=False).rc == -6
 
 
+def test_release_cache_without_device(hiplib):
+    pkg.release_cache()        # nothing kept: no device call, no error
+
+
 def test_gpu_paths_fail_loudly_without_device(hiplib):
     if hiplib.speigs_hip_device_count() > 0:
         pytest.skip("a HIP device is present")

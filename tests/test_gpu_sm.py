@@ -213,3 +213,29 @@ def test_block_solve_fp32(hiplib, monkeypatch, N, nrhs):
     assert e_g64 <= 1e-3
     assert e_f64 <= 1e-3
     assert e_fg <= 1e-4
+
+
+def test_device_memory_reuse(hiplib, monkeypatch):
+    """Device blocks kept between calls (and reused by calls of other sizes)
+    leave every result bitwise unchanged: a solve repeated after a smaller one,
+    after the kept blocks were released, and with keeping turned off."""
+    A = gen.weighted_lap3d(24, seed=5)
+    B = gen.weighted_lap3d(20, seed=6)
+    ho = {"precision": 2}
+
+    def solve(M):
+        r = _solve(M, 10, "sm", opts={"tol": 1e-11}, hip_opts=ho)
+        assert r.rc == 0 and r.info["flag"] == 0
+        return r
+
+    def same(r, s):
+        assert np.array_equal(r.lam, s.lam) and np.array_equal(r.V, s.V) and np.array_equal(r.resid, s.resid)
+
+    ra, rb = solve(A), solve(B)
+    same(solve(A), ra)
+    same(solve(B), rb)
+    pkg.release_cache()
+    same(solve(A), ra)
+    monkeypatch.setenv("SPEIGS_HIP_CACHE_MB", "0")
+    same(solve(B), rb)
+    same(solve(A), ra)
