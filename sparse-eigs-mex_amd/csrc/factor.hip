@@ -1286,7 +1286,7 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     parallel_for((int64_t)S.rows.size(), nt, 65536, [&](int64_t i, int) { rows32[i] = (int32_t)S.rows[i]; });
     for (size_t i = 0; i < S.chl.size(); i++) chl32[i] = (int32_t)S.chl[i];
     // assembly of A's entries: upper triangle as stored (i <= j)
-    std::vector<int64_t> snode(n);
+    std::vector<int64_t, NoInitAlloc<int64_t>> snode(n);
     parallel_for(ns, nt, 256, [&](int64_t s, int) {
         for (int64_t j = S.c0[s]; j < S.c1[s]; j++) snode[j] = s;
     });
@@ -1299,7 +1299,8 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
         cnt[j + 1] = c;
     });
     for (int64_t j = 0; j < n; j++) cnt[j + 1] += cnt[j];
-    std::vector<int64_t> dst(cnt[n]), src(cnt[n]);
+    // filled by the column loop below (first touch on its threads)
+    std::vector<int64_t, NoInitAlloc<int64_t>> dst(cnt[n]), src(cnt[n]);
     std::vector<char> hasdiag(n, 0);
     std::vector<double> diagA(n, 0.0);
     auto local_row = [&](int64_t s, int64_t row) -> int64_t {
@@ -1341,6 +1342,40 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     F->nasm = (int64_t)dst.size();
     F->nnzA = Ap[n];
     trace_phase("assembly map", tp);
+    // The arrays that are final by now go to the device (A's values straight
+    // from the caller's array).  Copying them on a helper thread while the task
+    // lists below are built was measured: the serial list building ran twice as
+    // slowly beside the staging copies, for no net gain.
+    struct Early {
+        void *d;
+        const void *h;
+        size_t bytes;
+    };
+    std::vector<Early> early;
+    auto stage = [&](auto **d, const auto *h, size_t count) -> int {
+        static_assert(sizeof(**d) == sizeof(*h), "device and host element types differ");
+        const int r = F->mem.alloc(d, std::max<size_t>(count, 1));
+        if (r == 0 && count > 0) early.push_back({(void *)*d, (const void *)h, count * sizeof(**d)});
+        return r;
+    };
+    if ((rc = stage(&F->Ax, Ax, (size_t)Ap[n]))) return rc;
+    if ((rc = stage(&F->diagA, diagA.data(), diagA.size()))) return rc;
+    if ((rc = stage(&F->asm_dst, dst.data(), dst.size()))) return rc;
+    if ((rc = stage(&F->asm_src, src.data(), src.size()))) return rc;
+    if ((rc = stage(&F->foff, S.foff.data(), S.foff.size()))) return rc;
+    if ((rc = stage(&F->wsoff, wsoff.data(), wsoff.size()))) return rc;
+    if ((rc = stage(&F->col0, S.c0.data(), S.c0.size()))) return rc;
+    if ((rc = stage(&F->rptr, S.rptr.data(), S.rptr.size()))) return rc;
+    if ((rc = stage(&F->relptr, S.relptr.data(), S.relptr.size()))) return rc;
+    if ((rc = stage(&F->chptr, S.chptr.data(), S.chptr.size()))) return rc;
+    if ((rc = stage(&F->Fs, Fs.data(), Fs.size()))) return rc;
+    if ((rc = stage(&F->ncol, nc.data(), nc.size()))) return rc;
+    if ((rc = stage(&F->rows, rows32.data(), rows32.size()))) return rc;
+    if ((rc = stage(&F->rel, S.rel.data(), S.rel.size()))) return rc;
+    if ((rc = stage(&F->chl, chl32.data(), chl32.size()))) return rc;
+    if ((rc = stage(&F->perm, S.perm.data(), S.perm.size()))) return rc;
+    for (const Early &e : early)
+        if ((rc = copy_to_device(e.d, e.h, e.bytes, st))) return rc;
     // plans: extend-add tasks, tile tasks, fronts per level
     std::vector<EaTask> ea;
     std::vector<TileTask> tiles;
@@ -1557,25 +1592,7 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
         if ((rc = F->mem.alloc((double **)&F->xinv, (size_t)xinv_max * TB * TB))) return rc;
     }
     trace_phase("pool allocation", tp);
-    std::vector<double> axv(Ax, Ax + std::max<int64_t>(Ap[n], 1));
-    if (Ap[n] == 0) axv.assign(1, 0.0);
-    if ((rc = upload(F.get(), &F->Ax, axv))) return rc;
-    if ((rc = upload(F.get(), &F->diagA, diagA))) return rc;
-    if ((rc = upload(F.get(), &F->asm_dst, dst))) return rc;
-    if ((rc = upload(F.get(), &F->asm_src, src))) return rc;
-    if ((rc = upload(F.get(), &F->foff, S.foff))) return rc;
-    if ((rc = upload(F.get(), &F->wsoff, wsoff))) return rc;
-    if ((rc = upload(F.get(), &F->col0, S.c0))) return rc;
-    if ((rc = upload(F.get(), &F->rptr, S.rptr))) return rc;
-    if ((rc = upload(F.get(), &F->relptr, S.relptr))) return rc;
-    if ((rc = upload(F.get(), &F->chptr, S.chptr))) return rc;
-    if ((rc = upload(F.get(), &F->Fs, Fs))) return rc;
-    if ((rc = upload(F.get(), &F->ncol, nc))) return rc;
-    if ((rc = upload(F.get(), &F->rows, rows32))) return rc;
-    if ((rc = upload(F.get(), &F->rel, S.rel))) return rc;
-    if ((rc = upload(F.get(), &F->chl, chl32))) return rc;
     if ((rc = upload(F.get(), &F->fronts, fronts))) return rc;
-    if ((rc = upload(F.get(), &F->perm, S.perm))) return rc;
     if ((rc = upload(F.get(), &F->ea, ea))) return rc;
     if ((rc = upload(F.get(), &F->tiles, tiles))) return rc;
     if (ufid.empty()) ufid.push_back(0);
