@@ -43,16 +43,90 @@ __global__ void __launch_bounds__(256) k_zero(T *p, int64_t n)
     if (i < n) p[i] = T(0);
 }
 
-// pool[dst[e]] = A value (+ delta on diagonals); src < 0 marks a diagonal entry ~src
-template <class T>
-__global__ void k_assemble(T *pool, const int64_t *__restrict__ dst, const int64_t *__restrict__ src,
-                           const double *__restrict__ Ax, int64_t ne, double delta)
+// ---- assembly of A into the fronts, from A's own CSC arrays on the device.
+// An entry (i, j), i <= j, of the stored pattern lands in front s = snode[col]
+// of its permuted column col = min(iperm i, iperm j), at the front row of
+// row = max(...): row - c0[s] inside the supernode, else c + the position of
+// row in the supernode's sorted row list (binary search).
+
+// snode[j] = the supernode of permuted column j
+__global__ void __launch_bounds__(256) k_snode_fill(const int64_t *__restrict__ col0,
+                                                    const int32_t *__restrict__ ncol, int64_t ns,
+                                                    int32_t *__restrict__ snode)
 {
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= ne) return;
-    const int64_t s = src[e];
-    const double v = s >= 0 ? Ax[s] : (s == INT64_MIN ? 0.0 : Ax[~s]) + delta;
-    pool[dst[e]] = (T)v;
+    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s >= ns) return;
+    const int64_t c0 = col0[s];
+    for (int j = 0; j < ncol[s]; j++) snode[c0 + j] = (int32_t)s;
+}
+
+// diagA[iperm j] = A(j, j) for the stored diagonal entries (diagA zeroed before)
+__global__ void __launch_bounds__(256) k_diag_values(const int64_t *__restrict__ Ap, const int64_t *__restrict__ Ai,
+                                                     const double *__restrict__ Ax,
+                                                     const int64_t *__restrict__ iperm, int64_t n,
+                                                     double *__restrict__ diagA)
+{
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    for (int64_t q = Ap[j]; q < Ap[j + 1]; q++)
+        if (Ai[q] == j) diagA[iperm[j]] = Ax[q];
+}
+
+// every diagonal of the pool = delta: the value of a diagonal A does not store
+template <class T>
+__global__ void __launch_bounds__(256) k_diag_delta(T *pool, const int32_t *__restrict__ snode,
+                                                    const int64_t *__restrict__ col0, const int32_t *__restrict__ Fs,
+                                                    const int64_t *__restrict__ foff, int64_t n, double delta)
+{
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const int32_t s = snode[j];
+    const int64_t l = j - col0[s];
+    pool[foff[s] + l + l * Fs[s]] = (T)(0.0 + delta);
+}
+
+// pool entry of (i, j) = A value (+ delta on diagonals), one thread per stored
+// column; an entry outside the symbolic structure sets bit 2 of *flag
+template <class T>
+__global__ void __launch_bounds__(256) k_assemble(T *pool, const int64_t *__restrict__ Ap,
+                                                  const int64_t *__restrict__ Ai, const double *__restrict__ Ax,
+                                                  const int64_t *__restrict__ iperm, const int32_t *__restrict__ snode,
+                                                  const int64_t *__restrict__ col0, const int32_t *__restrict__ ncol,
+                                                  const int32_t *__restrict__ Fs, const int64_t *__restrict__ rptr,
+                                                  const int32_t *__restrict__ rows, const int64_t *__restrict__ foff,
+                                                  int64_t n, double delta, int *__restrict__ flag)
+{
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const int64_t nj = iperm[j];
+    for (int64_t q = Ap[j]; q < Ap[j + 1]; q++) {
+        const int64_t i = Ai[q];
+        if (i > j) continue;
+        const int64_t ni = iperm[i];
+        const int64_t col = min(ni, nj), row = max(ni, nj);
+        const int32_t s = snode[col];
+        const int64_t c0 = col0[s];
+        const int c = ncol[s];
+        int64_t lr;
+        if (row < c0 + c) {
+            lr = row - c0;
+        } else {
+            int64_t lo = rptr[s], hi = rptr[s + 1];
+            const int64_t end = hi;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if (rows[mid] < row) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo == end || rows[lo] != row) {
+                atomicOr(flag, 2);
+                continue;
+            }
+            lr = c + (lo - rptr[s]);
+        }
+        const double v = i == j ? Ax[q] + delta : Ax[q];
+        pool[foff[s] + lr + (col - c0) * Fs[s]] = (T)v;
+    }
 }
 
 
@@ -1124,8 +1198,10 @@ int run_numeric(Factor *F, double delta)
     if ((uintptr_t)pool % 16) return SPEIGS_ERR_INTERNAL;
     k_zero<T><<<dim3((unsigned)std::min<int64_t>(4096, (S.pool * (int64_t)sizeof(T) / 16 + 255) / 256 + 1)), dim3(256), 0,
                 st>>>(pool, S.pool);
-    k_assemble<T><<<dim3((unsigned)((F->nasm + 255) / 256)), dim3(256), 0, st>>>(pool, F->asm_dst, F->asm_src, F->Ax,
-                                                                                F->nasm, delta);
+    const dim3 gn((unsigned)((F->n + 255) / 256));
+    k_diag_delta<T><<<gn, dim3(256), 0, st>>>(pool, F->snode, F->col0, F->Fs, F->foff, F->n, delta);
+    k_assemble<T><<<gn, dim3(256), 0, st>>>(pool, F->Apd, F->Aid, F->Ax, F->iperm, F->snode, F->col0, F->ncol, F->Fs,
+                                            F->rptr, F->rows, F->foff, F->n, delta, F->flag);
     for (int32_t L = 0; L < S.nlevels; L++) {
         const LevelPlan &P = F->plan[L];
         if (P.pcb_cnt > 0)
@@ -1180,6 +1256,7 @@ int run_numeric(Factor *F, double delta)
     int hflag = 0;
     SEHIP_CHECK(hipMemcpyAsync(&hflag, F->flag, sizeof(int), hipMemcpyDeviceToHost, st));
     SEHIP_CHECK(hipStreamSynchronize(st));
+    if (hflag & 2) return SPEIGS_ERR_INTERNAL;   // A outside the symbolic structure
     return hflag ? 1 : 0;
 }
 
@@ -1285,65 +1362,9 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     const int nt = host_threads();
     parallel_for((int64_t)S.rows.size(), nt, 65536, [&](int64_t i, int) { rows32[i] = (int32_t)S.rows[i]; });
     for (size_t i = 0; i < S.chl.size(); i++) chl32[i] = (int32_t)S.chl[i];
-    // assembly of A's entries: upper triangle as stored (i <= j)
-    std::vector<int64_t, NoInitAlloc<int64_t>> snode(n);
-    parallel_for(ns, nt, 256, [&](int64_t s, int) {
-        for (int64_t j = S.c0[s]; j < S.c1[s]; j++) snode[j] = s;
-    });
-    // columns in parallel, each into its own range of the map (map order =
-    // column order, as a serial scan would give)
-    std::vector<int64_t> cnt(n + 1, 0);
-    parallel_for(n, nt, 4096, [&](int64_t j, int) {
-        int64_t c = 0;
-        for (int64_t q = Ap[j]; q < Ap[j + 1]; q++) c += Ai[q] <= j;
-        cnt[j + 1] = c;
-    });
-    for (int64_t j = 0; j < n; j++) cnt[j + 1] += cnt[j];
-    // filled by the column loop below (first touch on its threads)
-    std::vector<int64_t, NoInitAlloc<int64_t>> dst(cnt[n]), src(cnt[n]);
-    std::vector<char> hasdiag(n, 0);
-    std::vector<double> diagA(n, 0.0);
-    auto local_row = [&](int64_t s, int64_t row) -> int64_t {
-        if (row < S.c1[s]) return row - S.c0[s];
-        const int64_t *R = S.rows.data() + S.rptr[s], nr = S.rptr[s + 1] - S.rptr[s];
-        const int64_t *pos = std::lower_bound(R, R + nr, row);
-        if (pos == R + nr || *pos != row) return -1;
-        return (S.c1[s] - S.c0[s]) + (pos - R);
-    };
-    std::atomic<int> bad{0};
-    parallel_for(n, nt, 4096, [&](int64_t j, int) {
-        int64_t o = cnt[j];
-        for (int64_t q = Ap[j]; q < Ap[j + 1]; q++) {
-            const int64_t i = Ai[q];
-            if (i > j) continue;
-            const int64_t ni = S.iperm[i], nj = S.iperm[j];
-            const int64_t col = std::min(ni, nj), row = std::max(ni, nj);
-            const int64_t s = snode[col];
-            const int64_t lr = local_row(s, row);
-            if (lr < 0) { bad.store(1); return; }
-            dst[o] = S.foff[s] + lr + (col - S.c0[s]) * S.F[s];
-            if (i == j) {
-                src[o] = ~q;
-                hasdiag[col] = 1;
-                diagA[col] = Ax[q];
-            } else {
-                src[o] = q;
-            }
-            o++;
-        }
-    });
-    if (bad.load()) return SPEIGS_ERR_INTERNAL;
-    for (int64_t j = 0; j < n; j++)
-        if (!hasdiag[j]) {
-            const int64_t s = snode[j], l = j - S.c0[s];
-            dst.push_back(S.foff[s] + l + l * S.F[s]);
-            src.push_back(INT64_MIN);
-        }
-    F->nasm = (int64_t)dst.size();
     F->nnzA = Ap[n];
-    trace_phase("assembly map", tp);
-    // The arrays that are final by now go to the device (A's values straight
-    // from the caller's array).  Copying them on a helper thread while the task
+    // The arrays that are final by now go to the device (A's CSC arrays
+    // straight from the caller's).  Copying them on a helper thread while the task
     // lists below are built was measured: the serial list building ran twice as
     // slowly beside the staging copies, for no net gain.
     struct Early {
@@ -1359,9 +1380,9 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
         return r;
     };
     if ((rc = stage(&F->Ax, Ax, (size_t)Ap[n]))) return rc;
-    if ((rc = stage(&F->diagA, diagA.data(), diagA.size()))) return rc;
-    if ((rc = stage(&F->asm_dst, dst.data(), dst.size()))) return rc;
-    if ((rc = stage(&F->asm_src, src.data(), src.size()))) return rc;
+    if ((rc = stage(&F->Apd, Ap, (size_t)n + 1))) return rc;
+    if ((rc = stage(&F->Aid, Ai, (size_t)Ap[n]))) return rc;
+    if ((rc = stage(&F->iperm, S.iperm.data(), S.iperm.size()))) return rc;
     if ((rc = stage(&F->foff, S.foff.data(), S.foff.size()))) return rc;
     if ((rc = stage(&F->wsoff, wsoff.data(), wsoff.size()))) return rc;
     if ((rc = stage(&F->col0, S.c0.data(), S.c0.size()))) return rc;
@@ -1376,6 +1397,14 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     if ((rc = stage(&F->perm, S.perm.data(), S.perm.size()))) return rc;
     for (const Early &e : early)
         if ((rc = copy_to_device(e.d, e.h, e.bytes, st))) return rc;
+    // the supernode of each column and A's diagonal, once
+    if ((rc = F->mem.alloc(&F->snode, (size_t)n))) return rc;
+    if ((rc = F->mem.alloc(&F->diagA, (size_t)n))) return rc;
+    k_snode_fill<<<dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st>>>(F->col0, F->ncol, ns, F->snode);
+    SEHIP_CHECK(hipMemsetAsync(F->diagA, 0, (size_t)n * sizeof(double), st));
+    k_diag_values<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st>>>(F->Apd, F->Aid, F->Ax, F->iperm, n,
+                                                                          F->diagA);
+    trace_phase("assembly data", tp);
     // plans: extend-add tasks, tile tasks, fronts per level
     std::vector<EaTask> ea;
     std::vector<TileTask> tiles;

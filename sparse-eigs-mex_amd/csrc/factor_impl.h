@@ -67,7 +67,8 @@ struct Factor {
     bool fp32 = false;
     void *pool = nullptr;
     double *Ax = nullptr, *diagA = nullptr;
-    int64_t *asm_dst = nullptr, *asm_src = nullptr, nasm = 0;
+    int64_t *Apd = nullptr, *Aid = nullptr, *iperm = nullptr;   // A's CSC arrays, iperm[old] = new
+    int32_t *snode = nullptr;                                     // supernode of each permuted column
     int64_t *foff = nullptr, *wsoff = nullptr, *col0 = nullptr, *rptr = nullptr, *relptr = nullptr, *chptr = nullptr;
     int32_t *Fs = nullptr, *ncol = nullptr, *rows = nullptr, *rel = nullptr, *chl = nullptr, *fronts = nullptr;
     int64_t *perm = nullptr;
