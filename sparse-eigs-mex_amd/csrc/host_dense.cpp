@@ -284,12 +284,20 @@ int ql_values(int64_t n, double *d, double *e)
     return 0;
 }
 
+// workspace of tri_shift_solve, allocated once per dense_tri_vectors call
+struct TriWork {
+    std::vector<double> u0, u1, u2, l;
+    std::vector<char> sw;
+    explicit TriWork(int64_t n) : u0(n), u1(n), u2(n), l(n), sw(n) {}
+};
+
 // solve (T - sigma I) x = b in place, T = tridiag(e, d, e), Gaussian
 // elimination with partial pivoting; tiny pivots replaced by eps3
-void tri_shift_solve(int64_t n, const double *d, const double *e, double sigma, double eps3, double *x)
+void tri_shift_solve(int64_t n, const double *d, const double *e, double sigma, double eps3, double *x, TriWork &w)
 {
-    std::vector<double> u0(n), u1(n), u2(n), l(n);
-    std::vector<char> sw(n, 0);
+    double *u0 = w.u0.data(), *u1 = w.u1.data(), *u2 = w.u2.data(), *l = w.l.data();
+    char *sw = w.sw.data();
+    std::fill(sw, sw + n, 0);
     // row i of the working matrix: (a_i,i, a_i,i+1, a_i,i+2)
     double a0 = d[0] - sigma, a1 = n > 1 ? e[0] : 0.0, a2 = 0.0;
     for (int64_t i = 0; i + 1 < n; i++) {
@@ -375,6 +383,7 @@ int dense_tri_vectors(const DenseTri &T, int64_t k, const int64_t *sel, double *
     std::iota(pos.begin(), pos.end(), 0);
     std::sort(pos.begin(), pos.end(), [&](int64_t a, int64_t b) { return sel[a] < sel[b]; });
     std::vector<std::vector<double>> X(k, std::vector<double>(n));
+    TriWork tw(n);
     int64_t cstart = 0;
     double prev_sigma = 0.0;
     for (int64_t q = 0; q < k; q++) {
@@ -391,7 +400,7 @@ int dense_tri_vectors(const DenseTri &T, int64_t k, const int64_t *sel, double *
             x[i] = (double)((z * 0x2545F4914F6CDD1DULL) >> 11) * (1.0 / 9007199254740992.0) - 0.5;
         }
         for (int it = 0; it < 4; it++) {
-            tri_shift_solve(n, d0.data(), e0.data(), sigma, eps3, x.data());
+            tri_shift_solve(n, d0.data(), e0.data(), sigma, eps3, x.data(), tw);
             for (int64_t p2 = cstart; p2 < q; p2++) {     // reorthogonalise within the cluster
                 const std::vector<double> &y = X[pos[p2]];
                 double dot = 0.0;
@@ -405,19 +414,30 @@ int dense_tri_vectors(const DenseTri &T, int64_t k, const int64_t *sel, double *
             for (int64_t i = 0; i < n; i++) x[i] /= nr;
         }
     }
-    // back-transformation: x <- H_0 H_1 ... H_{n-3} x
-    for (int64_t t = 0; t < k; t++) {
-        std::vector<double> &x = X[t];
-        for (int64_t kk = n - 3; kk >= 0; kk--) {
-            const double *v = T.A.data() + kk * n;
-            double s = 0.0;
-            for (int64_t i = kk + 1; i < n; i++) s += v[i] * x[i];
-            s *= 2.0;
-            if (s != 0.0)
-                for (int64_t i = kk + 1; i < n; i++) x[i] -= s * v[i];
+    // back-transformation: x <- H_0 H_1 ... H_{n-3} x for all k vectors at
+    // once, the vectors interleaved (Xr[i k + t]) so that the loops over t
+    // vectorize; each vector's dot product still sums over i in order
+    std::vector<double> Xr((size_t)n * k), sv(k);
+    for (int64_t t = 0; t < k; t++)
+        for (int64_t i = 0; i < n; i++) Xr[(size_t)i * k + t] = X[t][i];
+    for (int64_t kk = n - 3; kk >= 0; kk--) {
+        const double *v = T.A.data() + kk * n;
+        std::fill(sv.begin(), sv.end(), 0.0);
+        for (int64_t i = kk + 1; i < n; i++) {
+            const double vi = v[i];
+            const double *xr = Xr.data() + (size_t)i * k;
+            for (int64_t t = 0; t < k; t++) sv[t] += vi * xr[t];
         }
-        std::memcpy(Vsel + (size_t)t * ldv, x.data(), (size_t)n * sizeof(double));
+        for (int64_t t = 0; t < k; t++) sv[t] *= 2.0;
+        for (int64_t i = kk + 1; i < n; i++) {
+            const double vi = v[i];
+            double *xr = Xr.data() + (size_t)i * k;
+            for (int64_t t = 0; t < k; t++)
+                if (sv[t] != 0.0) xr[t] -= sv[t] * vi;
+        }
     }
+    for (int64_t t = 0; t < k; t++)
+        for (int64_t i = 0; i < n; i++) Vsel[(size_t)t * ldv + i] = Xr[(size_t)i * k + t];
     return 0;
 }
 
