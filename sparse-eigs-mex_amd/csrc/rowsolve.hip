@@ -86,12 +86,12 @@ __device__ __forceinline__ void wave_lds_sync() { __builtin_amdgcn_wave_barrier(
 // V[k][16 h + lc]; the result register v of lane l is row 16 rb + 4 g + v,
 // right-hand side 16 h + lc.  Every load of a step is issued before the first
 // product (no stores in between), so a front costs a few memory latencies.
-template <class T>
+template <class T, int NQ, int NR>
 __global__ void __launch_bounds__(256) k_rs_tiny_fwd(const T *__restrict__ pool, const int32_t *__restrict__ fronts,
                                                      int nf, RsArgs a, float *__restrict__ buf,
                                                      float *__restrict__ Yg)
 {
-    __shared__ float Ys[4][RTILE][RP];
+    __shared__ float Ys[4][16 * NQ][RP];   // the wave's Y rows (LDS sized to the class)
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, lc = lane & 15;
     const int fi = blockIdx.x * 4 + wv;
     if (fi >= nf) return;       // per wave; no workgroup barrier below
@@ -101,27 +101,32 @@ __global__ void __launch_bounds__(256) k_rs_tiny_fwd(const T *__restrict__ pool,
     const int64_t c0 = a.col0[s], ub = a.n + a.uoff[s];
     const bool leaf = a.chptr[s + 1] == a.chptr[s];   // not assembled: W1 = P rows, W2 = 0
     const int64_t tb = leaf ? c0 : a.t0 + c0;
-    const int nq = (c + 15) >> 4, nr = (r + 15) >> 4;
     float (*Yw)[RP] = Ys[wv];
 
-    float wb[4][4][2], w2[4][4][2];
+    // NQ, NR >= the front's 16-row blocks of c and r (the launch's class);
+    // blocks past c or r hold zeros, so their products add exact zeros
+    float wb[NQ][4][2], w2[NR][4][2];
 #pragma unroll
-    for (int kq = 0; kq < 4; kq++)
+    for (int kq = 0; kq < NQ; kq++)
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             const int k = 16 * kq + 4 * g + t;
-            const bool in = kq < nq && k < c;
+            const bool in = k < c;
             wb[kq][t][0] = in ? buf[(tb + k) * RW + lc] : 0.f;
             wb[kq][t][1] = in ? buf[(tb + k) * RW + 16 + lc] : 0.f;
+        }
+#pragma unroll
+    for (int kq = 0; kq < NR; kq++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
             const int ur = 16 * kq + 4 * g + t;       // (rb = kq, v = t) of the U rows
-            const bool iu = !leaf && kq < nr && ur < r;
+            const bool iu = !leaf && ur < r;
             w2[kq][t][0] = iu ? buf[(ub + ur) * RW + lc] : 0.f;
             w2[kq][t][1] = iu ? buf[(ub + ur) * RW + 16 + lc] : 0.f;
         }
     // Y = X11 W1 (X11 lower triangular: A[row][k] = L(row, k), k <= row)
 #pragma unroll
-    for (int rb = 0; rb < 4; rb++) {
-        if (rb >= nq) break;
+    for (int rb = 0; rb < NQ; rb++) {
         f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
         const int row = 16 * rb + lc;
 #pragma unroll
@@ -146,24 +151,22 @@ __global__ void __launch_bounds__(256) k_rs_tiny_fwd(const T *__restrict__ pool,
     }
     if (r <= 0) return;
     wave_lds_sync();
-    float yb[4][4][2];
+    float yb[NQ][4][2];
 #pragma unroll
-    for (int kq = 0; kq < 4; kq++)
+    for (int kq = 0; kq < NQ; kq++)
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             const int k = 16 * kq + 4 * g + t;
-            yb[kq][t][0] = kq < nq ? Yw[k][lc] : 0.f;
-            yb[kq][t][1] = kq < nq ? Yw[k][16 + lc] : 0.f;
+            yb[kq][t][0] = Yw[k][lc];
+            yb[kq][t][1] = Yw[k][16 + lc];
         }
     // U = W2 - L21 Y
 #pragma unroll
-    for (int rb = 0; rb < 4; rb++) {
-        if (rb >= nr) break;
+    for (int rb = 0; rb < NR; rb++) {
         f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
         const int lrow = 16 * rb + lc;
 #pragma unroll
-        for (int kq = 0; kq < 4; kq++) {
-            if (kq >= nq) break;
+        for (int kq = 0; kq < NQ; kq++) {
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 const int k = 16 * kq + 4 * g + t;
@@ -183,12 +186,12 @@ __global__ void __launch_bounds__(256) k_rs_tiny_fwd(const T *__restrict__ pool,
     }
 }
 
-template <class T>
+template <class T, int NQ, int NR>
 __global__ void __launch_bounds__(256) k_rs_tiny_bwd(const T *__restrict__ pool, const int32_t *__restrict__ fronts,
                                                      int nf, RsArgs a, const float *__restrict__ Yg,
                                                      float *__restrict__ Xg)
 {
-    __shared__ float Zs[4][RTILE][RP];
+    __shared__ float Zs[4][16 * NQ][RP];   // the wave's Z rows (LDS sized to the class)
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, lc = lane & 15;
     const int fi = blockIdx.x * 4 + wv;
     if (fi >= nf) return;
@@ -197,32 +200,37 @@ __global__ void __launch_bounds__(256) k_rs_tiny_bwd(const T *__restrict__ pool,
     const T *L = pool + a.foff[s];
     const int64_t c0 = a.col0[s];
     const int32_t *R = a.rows + a.rptr[s];
-    const int nq = (c + 15) >> 4, nr = (r + 15) >> 4;
     float (*Zw)[RP] = Zs[wv];
 
-    float xb[4][4][2], yv[4][4][2];
+    // NQ, NR >= the front's 16-row blocks of c and r (the launch's class);
+    // blocks past c or r hold zeros, so their products add exact zeros
+    float xb[NR][4][2], yv[NQ][4][2];
 #pragma unroll
-    for (int kq = 0; kq < 4; kq++)
+    for (int kq = 0; kq < NR; kq++)
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             const int k = 16 * kq + 4 * g + t;
-            const bool in = kq < nr && k < r;
+            const bool in = k < r;
             const int64_t gr = in ? (int64_t)R[k] : 0;
             xb[kq][t][0] = in ? Xg[gr * RW + lc] : 0.f;
             xb[kq][t][1] = in ? Xg[gr * RW + 16 + lc] : 0.f;
-            const bool iy = kq < nq && k < c;          // (rb = kq, v = t) of the Y rows
+        }
+#pragma unroll
+    for (int kq = 0; kq < NQ; kq++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int k = 16 * kq + 4 * g + t;
+            const bool iy = k < c;          // (rb = kq, v = t) of the Y rows
             yv[kq][t][0] = iy ? Yg[(c0 + k) * RW + lc] : 0.f;
             yv[kq][t][1] = iy ? Yg[(c0 + k) * RW + 16 + lc] : 0.f;
         }
     // Z = Y - L21^T X_R (A[j][k] = L(c + k, j), k over the r update rows)
 #pragma unroll
-    for (int rb = 0; rb < 4; rb++) {
-        if (rb >= nq) break;
+    for (int rb = 0; rb < NQ; rb++) {
         f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
         const int j = 16 * rb + lc;
 #pragma unroll
-        for (int kq = 0; kq < 4; kq++) {
-            if (kq >= nr) break;
+        for (int kq = 0; kq < NR; kq++) {
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 const int k = 16 * kq + 4 * g + t;
@@ -239,24 +247,22 @@ __global__ void __launch_bounds__(256) k_rs_tiny_bwd(const T *__restrict__ pool,
         }
     }
     wave_lds_sync();
-    float zb[4][4][2];
+    float zb[NQ][4][2];
 #pragma unroll
-    for (int kq = 0; kq < 4; kq++)
+    for (int kq = 0; kq < NQ; kq++)
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             const int k = 16 * kq + 4 * g + t;
-            zb[kq][t][0] = kq < nq ? Zw[k][lc] : 0.f;
-            zb[kq][t][1] = kq < nq ? Zw[k][16 + lc] : 0.f;
+            zb[kq][t][0] = Zw[k][lc];
+            zb[kq][t][1] = Zw[k][16 + lc];
         }
     // X_c = X11^T Z (A[j][k] = L(k, j), k >= j)
 #pragma unroll
-    for (int rb = 0; rb < 4; rb++) {
-        if (rb >= nq) break;
+    for (int rb = 0; rb < NQ; rb++) {
         f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
         const int j = 16 * rb + lc;
 #pragma unroll
-        for (int kq = rb; kq < 4; kq++) {
-            if (kq >= nq) break;
+        for (int kq = rb; kq < NQ; kq++) {
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 const int k = 16 * kq + 4 * g + t;
@@ -564,7 +570,8 @@ __global__ void __launch_bounds__(256) k_rs_out(const float *__restrict__ Xg, co
 
 // ---- plan -----------------------------------------------------------------
 struct RsLevel {
-    int64_t sm_off = 0, sm_cnt = 0;          // tiny fronts
+    int64_t sm_off = 0, sm_cnt = 0;          // tiny fronts, by class (tiny_class)
+    int64_t tc_off[9] = {}, tc_cnt[9] = {};
     int64_t as_off = 0, as_cnt = 0;          // assembly chunks (every front of the level)
     int64_t gx_off = 0, gx_cnt = 0;          // backward gathers of X_R (the larger fronts)
     int64_t pt_off[4] = {0, 0, 0, 0}, pt_cnt[4] = {0, 0, 0, 0};
@@ -605,6 +612,13 @@ struct RsState {
 };
 
 bool is_tiny(int64_t c, int64_t r) { return c <= TINY && r <= TINY; }
+
+// launch class of a tiny front: 3 (16-row blocks of c: 1, 2 or 4) + (of r)
+int tiny_class(int64_t c, int64_t r)
+{
+    auto b = [](int64_t x) { return x <= 16 ? 0 : x <= 32 ? 1 : 2; };
+    return 3 * b(c) + b(r);
+}
 
 void plan_parts(const Symbolic &S, const std::vector<int64_t> &fronts, int mode, std::vector<RsPart> &pt,
                 std::vector<RsFin> &ft, int64_t pbase)
@@ -702,10 +716,16 @@ int rs_plan(const Factor *F, RsState &B)
         RsLevel &P = B.lev[L];
         std::vector<int64_t> big;
         P.sm_off = (int64_t)small.size();
+        std::vector<int32_t> cls[9];
         for (int64_t s : bylevel[L]) {
             const int64_t c = S.c1[s] - S.c0[s];
-            if (is_tiny(c, S.F[s] - c)) small.push_back((int32_t)s);
+            if (is_tiny(c, S.F[s] - c)) cls[tiny_class(c, S.F[s] - c)].push_back((int32_t)s);
             else big.push_back(s);
+        }
+        for (int q = 0; q < 9; q++) {
+            P.tc_off[q] = (int64_t)small.size();
+            P.tc_cnt[q] = (int64_t)cls[q].size();
+            small.insert(small.end(), cls[q].begin(), cls[q].end());
         }
         P.sm_cnt = (int64_t)small.size() - P.sm_off;
         P.as_off = (int64_t)asmt.size();
@@ -762,6 +782,33 @@ int rs_upload(Factor *F, RsState &B)
     return 0;
 }
 
+// the tiny fronts of a level, one launch per class present
+template <class T, bool FWD, int NQ, int NR>
+void tiny_one(const RsLevel &P, int q, RsState &B, const T *pool, const RsArgs &a, hipStream_t st)
+{
+    const int64_t cnt = P.tc_cnt[q];
+    if (cnt <= 0) return;
+    const dim3 g((unsigned)((cnt + 3) / 4)), blk(256);
+    if (FWD)
+        k_rs_tiny_fwd<T, NQ, NR><<<g, blk, 0, st>>>(pool, B.small + P.tc_off[q], (int)cnt, a, B.buf, B.Yg);
+    else
+        k_rs_tiny_bwd<T, NQ, NR><<<g, blk, 0, st>>>(pool, B.small + P.tc_off[q], (int)cnt, a, B.Yg, B.Xg);
+}
+
+template <class T, bool FWD>
+void tiny_launch(const RsLevel &P, RsState &B, const T *pool, const RsArgs &a, hipStream_t st)
+{
+    tiny_one<T, FWD, 1, 1>(P, 0, B, pool, a, st);
+    tiny_one<T, FWD, 1, 2>(P, 1, B, pool, a, st);
+    tiny_one<T, FWD, 1, 4>(P, 2, B, pool, a, st);
+    tiny_one<T, FWD, 2, 1>(P, 3, B, pool, a, st);
+    tiny_one<T, FWD, 2, 2>(P, 4, B, pool, a, st);
+    tiny_one<T, FWD, 2, 4>(P, 5, B, pool, a, st);
+    tiny_one<T, FWD, 4, 1>(P, 6, B, pool, a, st);
+    tiny_one<T, FWD, 4, 2>(P, 7, B, pool, a, st);
+    tiny_one<T, FWD, 4, 4>(P, 8, B, pool, a, st);
+}
+
 template <class T>
 void rs_run(Factor *F, RsState &B, const double *Bin, int64_t brs, int64_t bcs, double *Xout, int64_t xrs,
             int64_t xcs, int nb)
@@ -800,9 +847,7 @@ void rs_run(Factor *F, RsState &B, const double *Bin, int64_t brs, int64_t bcs, 
     for (int32_t L = 0; L < S.nlevels; L++) {
         const RsLevel &P = B.lev[L];
         if (P.as_cnt) k_rs_assemble<<<dim3((unsigned)P.as_cnt), blk, 0, st>>>(B.asmt + P.as_off, a, B.buf);
-        if (P.sm_cnt)
-            k_rs_tiny_fwd<T><<<dim3((unsigned)((P.sm_cnt + 3) / 4)), blk, 0, st>>>(pool, B.small + P.sm_off,
-                                                                                  (int)P.sm_cnt, a, B.buf, B.Yg);
+        if (P.sm_cnt) tiny_launch<T, true>(P, B, pool, a, st);
         parts(0, P);
         parts(1, P);
     }
@@ -811,9 +856,7 @@ void rs_run(Factor *F, RsState &B, const double *Bin, int64_t brs, int64_t bcs, 
         if (P.gx_cnt) k_rs_gather<<<dim3((unsigned)P.gx_cnt), blk, 0, st>>>(B.gxt + P.gx_off, a, B.Xg, B.buf);
         parts(2, P);
         parts(3, P);
-        if (P.sm_cnt)
-            k_rs_tiny_bwd<T><<<dim3((unsigned)((P.sm_cnt + 3) / 4)), blk, 0, st>>>(pool, B.small + P.sm_off,
-                                                                                  (int)P.sm_cnt, a, B.Yg, B.Xg);
+        if (P.sm_cnt) tiny_launch<T, false>(P, B, pool, a, st);
     }
     k_rs_out<<<dim3(nrb), blk, 0, st>>>(B.Xg, F->perm, F->n, nb, Xout, xrs, xcs);
 }
