@@ -49,15 +49,16 @@ __global__ void __launch_bounds__(256) k_zero(T *p, int64_t n)
 // row = max(...): row - c0[s] inside the supernode, else c + the position of
 // row in the supernode's sorted row list (binary search).
 
-// snode[j] = the supernode of permuted column j
+// snode[j] = the supernode of permuted column j (a wave per supernode: the
+// root's thousands of columns are not left to one thread)
 __global__ void __launch_bounds__(256) k_snode_fill(const int64_t *__restrict__ col0,
                                                     const int32_t *__restrict__ ncol, int64_t ns,
                                                     int32_t *__restrict__ snode)
 {
-    const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t s = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (s >= ns) return;
     const int64_t c0 = col0[s];
-    for (int j = 0; j < ncol[s]; j++) snode[c0 + j] = (int32_t)s;
+    for (int j = threadIdx.x & 63; j < ncol[s]; j += 64) snode[c0 + j] = (int32_t)s;
 }
 
 // diagA[iperm j] = A(j, j) for the stored diagonal entries (diagA zeroed before)
@@ -1400,7 +1401,7 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     // the supernode of each column and A's diagonal, once
     if ((rc = F->mem.alloc(&F->snode, (size_t)n))) return rc;
     if ((rc = F->mem.alloc(&F->diagA, (size_t)n))) return rc;
-    k_snode_fill<<<dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st>>>(F->col0, F->ncol, ns, F->snode);
+    k_snode_fill<<<dim3((unsigned)((ns + 3) / 4)), dim3(256), 0, st>>>(F->col0, F->ncol, ns, F->snode);
     SEHIP_CHECK(hipMemsetAsync(F->diagA, 0, (size_t)n * sizeof(double), st));
     k_diag_values<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st>>>(F->Apd, F->Aid, F->Ax, F->iperm, n,
                                                                           F->diagA);

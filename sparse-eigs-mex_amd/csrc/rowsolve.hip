@@ -589,11 +589,13 @@ struct RsHost {
 };
 
 struct RsState {
-    // the plan is built on a helper thread (started while the device
-    // factorizes); the first solve joins it and uploads
+    // the plan is built and uploaded (on a stream of its own) by a helper
+    // thread started while the device factorizes; the first solve joins it and
+    // waits for that stream
     std::thread th;
     int plan_rc = 0;
     bool ready = false;
+    hipStream_t up_st = nullptr;
     RsHost host;
     DevMem mem;
     std::vector<RsLevel> lev;
@@ -608,6 +610,7 @@ struct RsState {
     ~RsState()
     {
         if (th.joinable()) th.join();
+        if (up_st) (void)hipStreamDestroy(up_st);
     }
 };
 
@@ -760,7 +763,7 @@ int rs_plan(const Factor *F, RsState &B)
 }
 
 // device buffers and the plan's upload; the host plan is released after
-int rs_upload(Factor *F, RsState &B)
+int rs_upload(Factor *F, RsState &B, hipStream_t st)
 {
     RsHost &H = B.host;
     const int64_t n = F->n;
@@ -768,7 +771,7 @@ int rs_upload(Factor *F, RsState &B)
     auto up = [&](auto **dst, const auto &v) -> int {
         int e = B.mem.alloc(dst, v.size());
         if (e) return e;
-        return copy_to_device(*dst, v.data(), v.size() * sizeof(v[0]), F->st);
+        return copy_to_device(*dst, v.data(), v.size() * sizeof(v[0]), st);
     };
     if ((rc = up(&B.small, H.small)) || (rc = up(&B.asmt, H.asmt)) || (rc = up(&B.gxt, H.gxt)) ||
         (rc = up(&B.pt, H.pt)) || (rc = up(&B.ft, H.ft)) || (rc = up(&B.frow, H.frow)) || (rc = up(&B.uoff, H.uoff)) ||
@@ -873,8 +876,7 @@ int factor_block_solve_rows(Factor *F, const double *Bin, int64_t brs, int64_t b
     if (!B.ready) {
         if (B.th.joinable()) B.th.join();
         if (B.plan_rc) return B.plan_rc;
-        const int rc = rs_upload(F, B);
-        if (rc) return rc;
+        SEHIP_CHECK(hipStreamSynchronize(B.up_st));
         B.ready = true;
     }
     const int e = ev_begin(F);
@@ -891,7 +893,17 @@ void rowsolve_prepare_async(Factor *F)
     if (F->rs) return;
     RsState *B = new RsState();
     F->rs = B;
-    B->th = std::thread([F, B] { B->plan_rc = rs_plan(F, *B); });
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) (void)hipGetLastError();
+    B->th = std::thread([F, B, dev] {
+        if ((B->plan_rc = rs_plan(F, *B))) return;
+        if (hipSetDevice(dev) != hipSuccess || hipStreamCreate(&B->up_st) != hipSuccess) {
+            (void)hipGetLastError();
+            B->plan_rc = SPEIGS_ERR_INTERNAL;
+            return;
+        }
+        B->plan_rc = rs_upload(F, *B, B->up_st);
+    });
 }
 
 void rowsolve_free(Factor *F)
