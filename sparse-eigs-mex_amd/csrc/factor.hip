@@ -312,6 +312,8 @@ __global__ void __launch_bounds__(256) k_trsm(T *pool, const TileTask *__restric
 // front's ordinal f in the step's POTRF list pt, upref[f] its first tile; a
 // front's tiles run over column tiles j >= j0 and, within a column, rows
 // i >= j.  The result is the tile's K range and its clipped rows / columns.
+// outer: 0 inner update; 1 the outer update's first min(KB, m) tile columns
+// (the next block's panels); 2 the rest of the outer update.
 struct UpdTile {
     int32_t s;
     int k0, kw, r0, r1, q0, q1;
@@ -345,6 +347,11 @@ __device__ __forceinline__ UpdTile upd_decode(const TileTask *__restrict__ pt, c
     }
     const int m = nt - j0;
     int64_t rem = gi - upref[f];
+    if (outer == 2) {
+        // the far part: after the first min(KB, m) tile columns of the outer range
+        const int64_t kk = min(KB, m);
+        rem += kk * m - kk * (kk - 1) / 2;
+    }
     int tt = 0;
     while (rem >= m - tt) {
         rem -= m - tt;
@@ -1203,6 +1210,7 @@ int run_numeric(Factor *F, double delta)
     k_diag_delta<T><<<gn, dim3(256), 0, st>>>(pool, F->snode, F->col0, F->Fs, F->foff, F->n, delta);
     k_assemble<T><<<gn, dim3(256), 0, st>>>(pool, F->Apd, F->Aid, F->Ax, F->iperm, F->snode, F->col0, F->ncol, F->Fs,
                                             F->rptr, F->rows, F->foff, F->n, delta, F->flag);
+    bool far_pending = false;     // a far part of a block-end update is queued on st2
     for (int32_t L = 0; L < S.nlevels; L++) {
         const LevelPlan &P = F->plan[L];
         if (P.pcb_cnt > 0)
@@ -1240,18 +1248,46 @@ int run_numeric(Factor *F, double delta)
             }
             // inner then outer updates: disjoint columns of the trailing matrix
             const TileTask *pt = F->tiles + P.k_potrf_off[k];
-            for (int outer = 0; outer < 2; outer++) {
-                const int64_t cnt = outer ? P.k_out_cnt[k] : P.k_upd_cnt[k];
-                if (cnt <= 0) continue;
-                const int32_t *uf = F->ufid + (outer ? P.k_out_off[k] : P.k_upd_off[k]);
-                const int64_t *up = F->upref + (outer ? P.k_outref_off[k] : P.k_upref_off[k]);
+            auto launch = [&](int outer, int64_t cnt, int64_t off, int64_t refoff, hipStream_t s2) {
+                if (cnt <= 0) return;
+                const int32_t *uf = F->ufid + off;
+                const int64_t *up = F->upref + refoff;
                 if (F->use_mfma)
-                    k_update_mfma<T><<<dim3((unsigned)cnt), dim3(256), 0, st>>>(pool, pt, uf, up, outer, F->foff,
+                    k_update_mfma<T><<<dim3((unsigned)cnt), dim3(256), 0, s2>>>(pool, pt, uf, up, outer, F->foff,
                                                                                 F->Fs, F->ncol);
                 else
-                    k_update<T><<<dim3((unsigned)cnt), dim3(256), 0, st>>>(pool, pt, uf, up, outer, F->foff, F->Fs,
+                    k_update<T><<<dim3((unsigned)cnt), dim3(256), 0, s2>>>(pool, pt, uf, up, outer, F->foff, F->Fs,
                                                                            F->ncol);
+            };
+            launch(0, P.k_upd_cnt[k], P.k_upd_off[k], P.k_upref_off[k], st);
+            // Look-ahead: the block-end update's near part (the next block's
+            // columns, which the next panels need) stays on this stream, its far
+            // part runs on st2 beside the next block's POTRF / TRSM chain.  A
+            // far part and the next block-end update write the same columns, so
+            // that one waits for it; tiles see their updates in the same order,
+            // and the factor is bitwise the same.
+            const int64_t nnear = P.k_out_cnt[k], nfar = P.k_out2_cnt[k];
+            if (nnear > 0 || nfar > 0) {
+                if (far_pending) {
+                    SEHIP_CHECK(hipStreamWaitEvent(st, F->ev_far, 0));
+                    far_pending = false;
+                }
+                if (nfar > 0) {
+                    SEHIP_CHECK(hipEventRecord(F->ev_ready, st));
+                    SEHIP_CHECK(hipStreamWaitEvent(F->st2, F->ev_ready, 0));
+                }
+                launch(1, nnear, P.k_out_off[k], P.k_outref_off[k], st);
+                if (nfar > 0) {
+                    launch(2, nfar, P.k_out2_off[k], P.k_out2ref_off[k], F->st2);
+                    SEHIP_CHECK(hipEventRecord(F->ev_far, F->st2));
+                    far_pending = true;
+                }
             }
+        }
+        // the next level's extend-add reads every update block of this one
+        if (far_pending) {
+            SEHIP_CHECK(hipStreamWaitEvent(st, F->ev_far, 0));
+            far_pending = false;
         }
     }
     int hflag = 0;
@@ -1338,6 +1374,11 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     std::unique_ptr<Factor> F(new Factor());
     F->opt = o;
     F->st = st;
+    if (hipStreamCreate(&F->st2) != hipSuccess || hipEventCreateWithFlags(&F->ev_ready, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&F->ev_far, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        return SPEIGS_ERR_INTERNAL;
+    }
     F->n = n;
     F->fp32 = o.precision == SPEIGS_HIP_FP32_IR || o.precision == SPEIGS_HIP_FP32_BLOCK;
     // debug knobs for A/B checks of kernel variants
@@ -1466,7 +1507,7 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
             P.k_out_off.push_back((int64_t)ufid.size());
             int64_t acc = 0, acco = 0;
             int32_t f = 0;
-            std::vector<int64_t> ocnt;
+            std::vector<int64_t> ocnt, omv;
             for (int64_t s : bylevel[L]) {
                 const int64_t c = S.c1[s] - S.c0[s], Fv = S.F[s];
                 if (c <= k * TB) continue;
@@ -1480,19 +1521,29 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
                 upref.push_back(acc);
                 ufid.insert(ufid.end(), (size_t)cnt, f);
                 acc += cnt;
-                int64_t oc = 0;
+                int64_t oc = 0, om = 0;
                 if (k == j1 - 1 && bend < Fv) {
-                    const int64_t m = nt - bend / TB;
-                    oc = m * (m + 1) / 2;
+                    om = nt - bend / TB;          // tile columns of the outer range
+                    oc = om * (om + 1) / 2;
                 }
                 ocnt.push_back(oc);
+                omv.push_back(om);
                 f++;
             }
             upref.push_back(acc);
             P.k_upd_cnt.push_back(acc);
-            // the outer tiles of this step follow in ufid, their prefix in upref
+            // the outer tiles of this step follow in ufid, their prefix in upref:
+            // first the next block's tile columns, then (a list of their own)
+            // the columns right of it
             P.k_out_off.back() = (int64_t)ufid.size();
             P.k_outref_off.push_back((int64_t)upref.size());
+            std::vector<int64_t> ofar(ocnt.size(), 0);
+            for (size_t q = 0; q < ocnt.size(); q++) {
+                if (ocnt[q] == 0) continue;
+                const int64_t m = omv[q], kk = std::min<int64_t>(KB, m);
+                ofar[q] = ocnt[q] - (kk * m - kk * (kk - 1) / 2);
+                ocnt[q] -= ofar[q];
+            }
             for (size_t q = 0; q < ocnt.size(); q++) {
                 upref.push_back(acco);
                 ufid.insert(ufid.end(), (size_t)ocnt[q], (int32_t)q);
@@ -1500,6 +1551,16 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
             }
             upref.push_back(acco);
             P.k_out_cnt.push_back(acco);
+            P.k_out2_off.push_back((int64_t)ufid.size());
+            P.k_out2ref_off.push_back((int64_t)upref.size());
+            int64_t accf = 0;
+            for (size_t q = 0; q < ofar.size(); q++) {
+                upref.push_back(accf);
+                ufid.insert(ufid.end(), (size_t)ofar[q], (int32_t)q);
+                accf += ofar[q];
+            }
+            upref.push_back(accf);
+            P.k_out2_cnt.push_back(accf);
         }
     }
     trace_phase("factor task lists", tp);

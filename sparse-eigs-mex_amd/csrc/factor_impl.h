@@ -46,7 +46,10 @@ struct LevelPlan {
     std::vector<int64_t> ea_off;      // per child ordinal: offset in the extend-add task array
     std::vector<int64_t> ea_cnt;
     std::vector<int64_t> k_potrf_off, k_potrf_cnt, k_trsm_off, k_trsm_cnt, k_upd_off, k_upd_cnt, k_upref_off;
-    std::vector<int64_t> k_out_off, k_out_cnt, k_outref_off;   // the outer (block-end) updates
+    // the outer (block-end) updates: the next block's columns (out) on the
+    // factor stream, the columns right of it (out2) on the look-ahead stream
+    std::vector<int64_t> k_out_off, k_out_cnt, k_outref_off;
+    std::vector<int64_t> k_out2_off, k_out2_cnt, k_out2ref_off;
     // per panel step: start of each group of kXinvSlots fronts in the TRSM tasks
     // (plus the end); a group's inverses fit the scratch, slot = ordinal % kXinvSlots
     std::vector<std::vector<int64_t>> k_trsm_cut;
@@ -62,6 +65,8 @@ struct Factor {
     Symbolic S;
     FactorOptions opt;
     hipStream_t st = nullptr;
+    hipStream_t st2 = nullptr;              // look-ahead: the far part of block-end updates
+    hipEvent_t ev_ready = nullptr, ev_far = nullptr;
     DevMem mem;
     int64_t n = 0, nnzA = 0;
     bool fp32 = false;
@@ -113,6 +118,10 @@ struct Factor {
     void *rs = nullptr, *bs_d = nullptr;     // block-solve state (row-major FP32 / generic FP64 vectors)
     ~Factor()
     {
+        if (st2) (void)hipStreamSynchronize(st2);
+        if (ev_ready) (void)hipEventDestroy(ev_ready);
+        if (ev_far) (void)hipEventDestroy(ev_far);
+        if (st2) (void)hipStreamDestroy(st2);
         if (hpart) (void)hipHostFree(hpart);
         if (herr) (void)hipHostFree(herr);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
