@@ -134,6 +134,28 @@ def test_weighted_vs_oracle(hiplib, oracle, precision):
     print("precision", precision, "nops", r.info["nops"], o.info["nops"], "hip", r.hip_info)
 
 
+@pytest.mark.parametrize("prec", [0, 2])
+def test_potrf_waves_bitwise(hiplib, monkeypatch, prec):
+    """The panel factorization on one wave and blocked over four waves applies
+    the same operations to every element in the same order: the factors, and
+    so the solves, agree to the bit (FP64 factor, and FP32 with block solves)."""
+    A = gen.weighted_lap3d(28, seed=12, upper=True)
+    n = len(A[0]) - 1
+    B = np.asfortranarray(np.random.default_rng(8).standard_normal((n, 3)))
+    out = []
+    for waves in ("1", "4"):
+        monkeypatch.setenv("SPEIGS_HIP_POTRF_WAVES", waves)
+        rc_, op, hi = _op(hiplib, A, 1, prec, 0.0)
+        assert rc_ == 0
+        X = np.zeros((n, 3), order="F")
+        assert hiplib.speigs_hip_op_apply(op, B.ctypes.data, X.ctypes.data, 3) == 0
+        hiplib.speigs_hip_op_destroy(op)
+        out.append(X)
+    R = gen.spmv_upper(*A, out[1]) - B
+    assert np.linalg.norm(R) / np.linalg.norm(B) <= (1e-12 if prec == 0 else 1e-3)
+    assert np.array_equal(out[0], out[1])
+
+
 @pytest.mark.parametrize("mode,mfma,prec", [("0", "0", 0), ("1", "0", 0), ("1", "1", 0), ("1", "0", 1), ("1", "1", 1),
                                             ("2", "1", 0), ("2", "1", 1)])
 def test_kernel_variants_agree(hiplib, monkeypatch, mode, mfma, prec):
