@@ -138,8 +138,7 @@ void rgram(const double *A, int64_t lda, int p, const double *B, int64_t ldb, in
 // X = V C + beta X (V n x m, C m x p column-major, m, p <= 96; X must not alias
 // V) on the FP64 matrix cores.  C is staged in LDS once per block; each wave
 // owns 16-row groups and all (<= 6) 16-column output tiles, loads a group's
-// whole 16 x m slice of V at once (all loads in flight, the next group's during
-// the current group's products), then runs the steps of 4 inner
+// whole 16 x m slice of V at once (all loads in flight), then runs the steps of 4 inner
 // indices: lane l supplies V[r + l%16][k + l/16] and C[k + l/16][16 tj + l%16]
 // and holds X[r + l/16 + 4v][16 tj + l%16] (stores of four 128-byte row pieces).
 constexpr int RM_CP = 100;   // LDS pitch of C rows
@@ -169,23 +168,20 @@ __global__ void __launch_bounds__(256) k_rgemm(const double *V, int64_t ldv, int
     const int lr = lane & 15, lk = lane >> 4;
     const int pt = (p + 15) >> 4;
     const int64_t rbase = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * RM_GROUPS * 16;
-    // the next group's rows are loaded while the current group's products run
-    double va[24], vn[24];
-    auto load = [&](double *dst, int64_t r0) {
+    // one group's 16 x m slice of V in registers at a time: loading the next
+    // group's during the current products (double the registers, occupancy 2
+    // instead of 3) was slower -- 375 vs 342 us for 84 -> 56 columns at 10^6 rows
+    for (int g = 0; g < RM_GROUPS; g++) {
+        const int64_t r0 = rbase + g * 16;
+        if (r0 >= n) break;
+        double va[24];
         const int64_t arow = r0 + lr;
         const bool rin = arow < n;
 #pragma unroll
         for (int s2 = 0; s2 < 24; s2++) {
             const int kk = 4 * s2 + lk;
-            dst[s2] = (rin && kk < m) ? V[arow * ldv + kk] : 0.0;
+            va[s2] = (rin && kk < m) ? V[arow * ldv + kk] : 0.0;
         }
-    };
-    if (rbase < n) load(va, rbase);
-#pragma unroll
-    for (int g = 0; g < RM_GROUPS; g++) {
-        const int64_t r0 = rbase + g * 16;
-        if (r0 >= n) break;
-        if (g + 1 < RM_GROUPS && r0 + 16 < n) load(vn, r0 + 16);
         rd4 acc[6];
 #pragma unroll
         for (int b = 0; b < 6; b++) acc[b] = rd4{0.0, 0.0, 0.0, 0.0};
@@ -210,8 +206,6 @@ __global__ void __launch_bounds__(256) k_rgemm(const double *V, int64_t ldv, int
                 }
             }
         }
-#pragma unroll
-        for (int s2 = 0; s2 < 24; s2++) va[s2] = vn[s2];
     }
 }
 

@@ -258,6 +258,9 @@ def test_device_memory_reuse(hiplib, monkeypatch):
     same(solve(B), rb)
     pkg.release_cache()
     same(solve(A), ra)
+    monkeypatch.setenv("SPEIGS_HIP_CACHE_MB", "1")      # small blocks kept, the large ones returned
+    same(solve(A), ra)
+    same(solve(B), rb)
     monkeypatch.setenv("SPEIGS_HIP_CACHE_MB", "0")
     same(solve(B), rb)
     same(solve(A), ra)
