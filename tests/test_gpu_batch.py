@@ -98,8 +98,8 @@ def test_ref8_blocks_batched(hiplib):
     rc.check_blocks(lambda A, k, which: pkg.speigs(A, k, which), N1=12, N2=10, k=6)
 
 
-def test_c4_batched_small(hiplib, monkeypatch):
-    """the C4 generator at 64 components (sizes 500..5,000): batched and serial
+def test_many_components_batched_small(hiplib, monkeypatch):
+    """the many-component generator at 64 components (sizes 500..5,000): batched and serial
     device solves agree; every returned vector lives in one component."""
     Ap, Ai, Ax, sizes = gen.batched_components(64, seed=4, nmin=500, nmax=5000)
     A = (Ap, Ai, Ax)

@@ -178,7 +178,7 @@ def test_kernel_variants_agree(hiplib, monkeypatch, mode, mfma, prec):
 
 
 def test_inverse_solve_weighted_100(hiplib, monkeypatch):
-    """inverse-based (chain-free) FP64 solve on a 40^3 C3-family matrix: residual at rounding level."""
+    """inverse-based (chain-free) FP64 solve on a 40^3 weighted 3-D Laplacian: residual at rounding level."""
     monkeypatch.setenv("SPEIGS_HIP_SOLVE_MODE", "2")
     A = gen.weighted_lap3d(40, seed=3, upper=True)
     n = len(A[0]) - 1
