@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "factor_impl.h"
@@ -233,8 +234,7 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
     // device iterates
     HostTouch touch;
     if (Vout) touch.start(Vout, (size_t)n * k * sizeof(double));
-    int rc = factor_invert_l11(F);
-    if (rc) return rc;
+    int rc = 0;
 
     LWork W;
     W.n = n;
@@ -254,20 +254,49 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
     if ((rc = W.mem.alloc(&W.npart, (size_t)rresid_norms_part()))) return rc;
     if ((rc = W.mem.alloc(&W.nrm, 32))) return rc;
 
-    // start block: the proposed block of the learned start (its first m
-    // columns), else column 0 = the reference's xorshift64* start vector (or
-    // the caller's) and the other columns from a counter-based generator
-    {
-        uint64_t seed = o->seed ? o->seed : 88172645463325252ULL;
-        if (Xstart) {
-            rfrom_f32rows(Xstart, 32, std::min(m, 32), n, T2, ldT, st);
-        } else {
-            std::vector<double> h((size_t)n);
+    // The start vector is generated and uploaded (on the factor's second
+    // stream) by a helper thread while the device inverts the diagonal blocks.
+    uint64_t seed = o->seed ? o->seed : 88172645463325252ULL;
+    std::vector<double> h;
+    int start_rc = 0;
+    std::thread start_th;
+    struct JoinStart {
+        std::thread &t;
+        ~JoinStart()
+        {
+            if (t.joinable()) t.join();
+        }
+    } join_start{start_th};
+    if (!Xstart) {
+        int dev = 0;
+        SEHIP_CHECK(hipGetDevice(&dev));
+        start_th = std::thread([&, dev] {
+            if (hipSetDevice(dev) != hipSuccess) {
+                start_rc = SPEIGS_ERR_INTERNAL;
+                return;
+            }
+            h.resize((size_t)n);
             if (ho && ho->start == SPEIGS_HIP_START_VECTOR && ho->start_vec)
                 std::memcpy(h.data(), ho->start_vec, (size_t)n * sizeof(double));
             else
                 uniform_vec(&seed, n, h.data());
-            if ((rc = copy_to_device(v0, h.data(), (size_t)n * sizeof(double), st))) return rc;
+            if ((start_rc = copy_to_device(v0, h.data(), (size_t)n * sizeof(double), F->st2))) return;
+            if (hipEventRecord(F->ev_ready, F->st2) != hipSuccess) start_rc = SPEIGS_ERR_INTERNAL;
+        });
+    }
+    rc = factor_invert_l11(F);
+    if (start_th.joinable()) start_th.join();
+    if (rc) return rc;
+    if (start_rc) return start_rc;
+
+    // start block: the proposed block of the learned start (its first m
+    // columns), else column 0 = the reference's xorshift64* start vector (or
+    // the caller's) and the other columns from a counter-based generator
+    {
+        if (Xstart) {
+            rfrom_f32rows(Xstart, 32, std::min(m, 32), n, T2, ldT, st);
+        } else {
+            SEHIP_CHECK(hipStreamWaitEvent(st, F->ev_ready, 0));
             rfrom_cols(v0, 1, n, T2, ldT, st);
             rrandom(T2, ldT, 1, m, n, seed, st);
         }
