@@ -128,6 +128,13 @@ def main():
             last = r
             print(f"[bench] {path} step {s}: {dt:.3f}s rc={r.rc} flag={r.info['flag']} nops={r.info['nops']} "
                   f"maxres={r.resid.max():.2e}", file=sys.stderr, flush=True)
+            # a failed or unconverged call is not a measurement of the metric
+            bound = args.tol * max(r.info["anorm"], 1.0)
+            if r.rc != 0 or r.info["flag"] != 0 or not r.resid.max() <= bound:
+                print(json.dumps({"error": f"{path} step {s}: rc={r.rc} flag={r.info['flag']} "
+                                           f"max resid {r.resid.max():.3e} > {bound:.3e} "
+                                           f"({pkg.last_error()})"}))
+                sys.exit(3)
         results[path] = (times, last)
     chosen_key = "chosen" if "chosen" in results else next(iter(results))
     times, r = results[chosen_key]

@@ -83,6 +83,13 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
 /* number of visible HIP devices (0 when none); never fails */
 int speigs_hip_device_count(void);
 
+/* Where the calling thread's most recent SPEIGS_ERR_INTERNAL / _MEM came from:
+ * "file:line: failing step (code c)", with the HIP runtime's error name for a
+ * failed HIP call, or "" if the last speigs_ex / speigs_hip_op_* call on this
+ * thread recorded none.  No counterpart in the reference (whose only internal
+ * failures are CHOLMOD's and LAPACK's status codes); for diagnosis only. */
+const char *speigs_hip_last_error(void);
+
 /* Device memory freed at the end of a call is kept per device and reused by
  * later calls (up to SPEIGS_HIP_CACHE_MB megabytes per device, default a
  * quarter of its memory; 0 disables keeping).  This returns every kept block

@@ -122,6 +122,8 @@ def _bind_ext(lib: ctypes.CDLL) -> None:
                               POINTER(HipOpts), _F64P, _F64P, _F64P, POINTER(Info), POINTER(HipInfo)]
     lib.speigs_hip_device_count.restype = c_int
     lib.speigs_hip_device_count.argtypes = []
+    lib.speigs_hip_last_error.restype = ctypes.c_char_p
+    lib.speigs_hip_last_error.argtypes = []
     lib.speigs_hip_release_cache.restype = None
     lib.speigs_hip_release_cache.argtypes = []
     vp, u64, f32p = ctypes.c_void_p, ctypes.c_uint64, POINTER(ctypes.c_float)
@@ -335,6 +337,12 @@ def release_cache(lib=None) -> None:
     """Return the device memory the library keeps between calls to the HIP
     runtime (speigs_hip_release_cache)."""
     (lib or load()).speigs_hip_release_cache()
+
+
+def last_error(lib=None) -> str:
+    """Where this thread's most recent internal failure happened
+    (speigs_hip_last_error): "file:line: step (code c)", or ""."""
+    return (lib or load()).speigs_hip_last_error().decode()
 
 
 def errmsg(code: int, lib=None) -> str:

@@ -24,6 +24,7 @@
 #include "host_par.h"
 #include "batch.h"
 #include "mpnn.h"
+#include "op_impl.h"
 
 using namespace sehip;
 
@@ -291,8 +292,8 @@ struct HostSolveOp : MultOp {
     int apply(const double *x, double *y) override
     {
         std::vector<double> hx(n);
-        if (hipMemcpyAsync(hx.data(), x, n * 8, hipMemcpyDeviceToHost, stream) != hipSuccess) return SPEIGS_ERR_INTERNAL;
-        if (hipStreamSynchronize(stream) != hipSuccess) return SPEIGS_ERR_INTERNAL;
+        SEHIP_CHECK(hipMemcpyAsync(hx.data(), x, n * 8, hipMemcpyDeviceToHost, stream));
+        SEHIP_CHECK(hipStreamSynchronize(stream));
         if (kind == 0) {
             for (int64_t i = 0; i < n; i++) bw[i] = hx[perm[i]];
             band_solve_u(n, kd, fac.data(), kd + 1, bw.data());
@@ -300,8 +301,8 @@ struct HostSolveOp : MultOp {
         } else {
             lu_solve(n, fac.data(), piv.data(), hx.data());
         }
-        if (hipMemcpyAsync(y, hx.data(), n * 8, hipMemcpyHostToDevice, stream) != hipSuccess) return SPEIGS_ERR_INTERNAL;
-        if (hipStreamSynchronize(stream) != hipSuccess) return SPEIGS_ERR_INTERNAL;
+        SEHIP_CHECK(hipMemcpyAsync(y, hx.data(), n * 8, hipMemcpyHostToDevice, stream));
+        SEHIP_CHECK(hipStreamSynchronize(stream));
         return 0;
     }
 };
@@ -311,6 +312,7 @@ struct FactorOp : MultOp {
     Factor *F = nullptr;
     ~FactorOp() override { factor_destroy(F); }
     int apply(const double *x, double *y) override { return factor_apply(F, x, y); }
+    int check() override { return factor_check(F); }
 };
 
 struct StreamGuard {
@@ -442,6 +444,8 @@ const char *speigs_errmsg(int c)
     }
 }
 
+const char *speigs_hip_last_error(void) { return last_error_text(); }
+
 int speigs_hip_device_count(void)
 {
     int c = 0;
@@ -456,6 +460,7 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
               speigs_which which, const speigs_opts *opts, const speigs_hip_opts *hopts, double *lambda, double *V,
               double *resid, speigs_info *info, speigs_hip_info *hinfo)
 {
+    clear_error();
     // declared first, destroyed last: with SPEIGS_HIP_TRACE=1 it reports the
     // time the other locals' destructors take (device memory, streams)
     struct TraceRelease {
@@ -516,10 +521,10 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
     info->anorm = anorm;
 
     // ---- the GPU part: needs a device, no CPU fallback
-    if (speigs_hip_device_count() <= 0) return SPEIGS_ERR_INTERNAL;
+    if (speigs_hip_device_count() <= 0) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "no HIP device visible");
     if (hipSetDevice(hopts->device) != hipSuccess) return SPEIGS_ERR_ARG;
     StreamGuard sg;
-    if (hipStreamCreate(&sg.s) != hipSuccess) return SPEIGS_ERR_INTERNAL;
+    SEHIP_CHECK(hipStreamCreate(&sg.s));
     DevMem mem;
     // the SELL matrix of the SpMVs is built on the host and uploaded on a
     // stream of its own while the device factorizes (joined before use, and
@@ -562,7 +567,7 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
             rc = factor_create(n, Ap, Ai, Ax, fopt, sg.s, &fo->F, hinfo);
             if (rc) return rc;
             info->t_analyze = wall() - ta;
-            if (hipStreamCreate(&sell_sg.s) != hipSuccess) return SPEIGS_ERR_INTERNAL;
+            SEHIP_CHECK(hipStreamCreate(&sell_sg.s));
             sell_th = std::thread([&]() { sell_rc = sell_build(n, Ap, Ai, Ax, &sell_pre, mem, sell_sg.s); });
             // the block eigensolver's solve plan is built while the device factorizes
             if (hopts->precision == SPEIGS_HIP_FP32_BLOCK) rowsolve_prepare_async(fo->F);
@@ -628,7 +633,13 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
 
     double ti = wall();
     auto *fop = dynamic_cast<FactorOp *>(op.get());
-    const bool block = fop && fop->F && hopts->precision == SPEIGS_HIP_FP32_BLOCK && info->path == SPEIGS_PATH_SHIFTINV;
+    bool block = fop && fop->F && hopts->precision == SPEIGS_HIP_FP32_BLOCK && info->path == SPEIGS_PATH_SHIFTINV;
+    if (block && lobpcg_block_size(n, k) == 0) {
+        // k beyond one block (k > 24): Lanczos over the same FP32 factor, each
+        // apply refined to FP64 (the FP32_IR path)
+        block = false;
+        fop->F->opt.precision = SPEIGS_HIP_FP32_IR;
+    }
     // learned start: the network's block for the block eigensolver, the sum of
     // its first min(k, 32) columns as the Lanczos start vector
     float *xl = nullptr;
@@ -873,7 +884,7 @@ int blocks_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *A
     for (int64_t c = 0; c < ncomp; c++)
         if (!onhost[c]) devc.push_back(c);
     if (!devc.empty()) {
-        if (speigs_hip_device_count() <= 0) return SPEIGS_ERR_INTERNAL;
+        if (speigs_hip_device_count() <= 0) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "no HIP device visible");
         if (hipSetDevice(ho->device) != hipSuccess) return SPEIGS_ERR_ARG;
         std::vector<BatchComp> bc(devc.size());
         int64_t nb = 0, nnzb = 0;
@@ -912,7 +923,7 @@ int blocks_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *A
         }
         trace_phase("device block matrix", tp);
         StreamGuard sg;
-        if (hipStreamCreate(&sg.s) != hipSuccess) return SPEIGS_ERR_INTERNAL;
+        SEHIP_CHECK(hipStreamCreate(&sg.s));
         DevMem mem;
         std::unique_ptr<MultOp> op;
         int rc = SPEIGS_OK;
@@ -1009,33 +1020,18 @@ int blocks_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *A
 }  // namespace
 
 // ---------------------------------------------------------------- operator ABI
-struct speigs_hip_op {
-    int mode = 0;
-    int64_t n = 0;
-    hipStream_t stream = nullptr;
-    DevMem mem;
-    SellA A;
-    Factor *F = nullptr;
-    double *x = nullptr, *y = nullptr;
-    int64_t cap = 0;
-    int precision = 0;
-    ~speigs_hip_op()
-    {
-        factor_destroy(F);
-        mem.release();
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
+// (struct speigs_hip_op: op_impl.h)
 
 extern "C" {
 
 int speigs_hip_op_create(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const double *Ax, int mode,
                          int precision, double delta, speigs_hip_op **out, speigs_hip_info *hinfo)
 {
+    clear_error();
     if (!out || n <= 0 || !Ap || !Ai || !Ax) return SPEIGS_ERR_ARG;
     *out = nullptr;
     if (mode != SPEIGS_HIP_OP_MULT && mode != SPEIGS_HIP_OP_SOLVE) return SPEIGS_ERR_ARG;
-    if (speigs_hip_device_count() <= 0) return SPEIGS_ERR_INTERNAL;
+    if (speigs_hip_device_count() <= 0) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "no HIP device visible");
     speigs_hip_info lhi;
     if (!hinfo) hinfo = &lhi;
     std::memset(hinfo, 0, sizeof *hinfo);
@@ -1043,7 +1039,8 @@ int speigs_hip_op_create(speigs_int n, const speigs_int *Ap, const speigs_int *A
     op->mode = mode;
     op->n = n;
     op->precision = precision;
-    if (hipStreamCreate(&op->stream) != hipSuccess) return SPEIGS_ERR_INTERNAL;
+    SEHIP_CHECK(hipGetDevice(&op->dev));
+    SEHIP_CHECK(hipStreamCreate(&op->stream));
     int rc = sell_build(n, Ap, Ai, Ax, &op->A, op->mem, op->stream);
     if (rc) return rc;
     hinfo->bytes_spmv = spmv_bytes(op->A);
@@ -1065,6 +1062,7 @@ int speigs_hip_op_create(speigs_int n, const speigs_int *Ap, const speigs_int *A
 
 int speigs_hip_op_refactor(speigs_hip_op *op, double delta)
 {
+    clear_error();
     if (!op || !op->F) return SPEIGS_ERR_ARG;
     int st = factor_numeric(op->F, delta);
     if (st < 0) return st;
@@ -1086,6 +1084,7 @@ static int op_buffers(speigs_hip_op *op, int64_t nvec)
 
 int speigs_hip_op_multA(speigs_hip_op *op, const double *x, double *y, speigs_int nvec)
 {
+    clear_error();
     if (!op || !x || !y || nvec <= 0) return SPEIGS_ERR_ARG;
     int rc = op_buffers(op, nvec);
     if (rc) return rc;
@@ -1099,6 +1098,7 @@ int speigs_hip_op_multA(speigs_hip_op *op, const double *x, double *y, speigs_in
 
 int speigs_hip_op_apply(speigs_hip_op *op, const double *x, double *y, speigs_int nvec)
 {
+    clear_error();
     if (!op) return SPEIGS_ERR_ARG;
     if (op->mode == SPEIGS_HIP_OP_MULT) return speigs_hip_op_multA(op, x, y, nvec);
     if (!x || !y || nvec <= 0) return SPEIGS_ERR_ARG;
@@ -1127,7 +1127,7 @@ int speigs_hip_op_apply(speigs_hip_op *op, const double *x, double *y, speigs_in
     }
     SEHIP_CHECK(hipMemcpyAsync(y, op->y, bytes, hipMemcpyDeviceToHost, op->stream));
     SEHIP_CHECK(hipStreamSynchronize(op->stream));
-    return SPEIGS_OK;
+    return factor_check(op->F);
 }
 
 void speigs_hip_op_destroy(speigs_hip_op *op) { delete op; }

@@ -212,7 +212,7 @@ int lanczos_batched(DevOperator *op, int64_t n, std::vector<BatchComp> &comps, c
             double *seg = hv.data() + comps[c].off;
             uniform_vec(&cs[c].seed, comps[c].n, seg);
             const double nr = std::sqrt(host_sumsq(comps[c].n, seg));
-            if (nr == 0.0) return SPEIGS_ERR_INTERNAL;
+            if (nr == 0.0) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "re-injected start vector has norm 0");
             const double s = 1.0 / nr;
             for (int64_t i = 0; i < comps[c].n; i++) seg[i] *= s;
         }
@@ -358,6 +358,7 @@ int lanczos_batched(DevOperator *op, int64_t n, std::vector<BatchComp> &comps, c
         if ((rc = down_f64(W, lamh, W.lam))) return rc;
         if ((rc = down_f64(W, rrh, W.rr))) return rc;
         SEHIP_CHECK(hipStreamSynchronize(W.st));
+        if ((rc = op->check())) return rc;
         bool anyrestart = false;
         for (int64_t c = 0; c < nc; c++) {
             CompState &s = cs[c];

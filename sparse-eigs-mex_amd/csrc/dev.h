@@ -23,13 +23,27 @@
 
 namespace sehip {
 
+// Every internal failure records where it happened (source file and line, the
+// failing call and, for HIP calls, the runtime's error name) in a per-thread
+// record that speigs_hip_last_error() returns; the error code stays the
+// reference's (SPEIGS_ERR_*).
 #define SEHIP_CHECK(expr)                                                     \
     do {                                                                      \
         hipError_t e_ = (expr);                                               \
-        if (e_ != hipSuccess) return sehip::hip_error_code(e_);               \
+        if (e_ != hipSuccess) return sehip::hip_fail(e_, __FILE__, __LINE__, #expr); \
     } while (0)
+#define SEHIP_FAIL(code, what) sehip::fail((code), __FILE__, __LINE__, (what))
 
 int hip_error_code(hipError_t e);
+// records the site and returns the SPEIGS_ERR_* code of e (clears HIP's last error)
+int hip_fail(hipError_t e, const char *file, int line, const char *call);
+// records the site and returns code
+int fail(int code, const char *file, int line, const char *what);
+// the calling thread's record ("" if none); a helper thread's record is
+// handed to the thread that joins it with note_error()
+const char *last_error_text();
+void note_error(const char *text);
+void clear_error();
 double wall();
 // SPEIGS_HIP_TRACE=1: phase times of the set-up on stderr; trace_phase prints
 // the time since t and resets t to now
@@ -184,6 +198,9 @@ struct DevOperator {
     virtual int apply(const double *x, double *y) = 0;
     // y = A x, nv vectors
     virtual int multA(const double *X, int64_t ldx, double *Y, int64_t ldy, int nv) = 0;
+    // after the operator's stream was synchronized: a failure the device
+    // recorded during the applies since (0 if none)
+    virtual int check() { return 0; }
     int64_t nops = 0;
     int64_t n = 0;
     hipStream_t stream = nullptr;

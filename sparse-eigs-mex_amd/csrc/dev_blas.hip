@@ -24,6 +24,44 @@ int hip_error_code(hipError_t e)
     return SPEIGS_ERR_INTERNAL;
 }
 
+namespace {
+thread_local char g_last_error[512];
+
+const char *base_name(const char *path)
+{
+    const char *b = path;
+    for (const char *p = path; *p; p++)
+        if (*p == '/') b = p + 1;
+    return b;
+}
+}  // namespace
+
+int fail(int code, const char *file, int line, const char *what)
+{
+    snprintf(g_last_error, sizeof g_last_error, "%s:%d: %s (code %d)", base_name(file), line, what, code);
+    if (trace_on()) fprintf(stderr, "[speigs_hip] error %s\n", g_last_error);
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *file, int line, const char *call)
+{
+    (void)hipGetLastError();
+    const int code = hip_error_code(e);
+    snprintf(g_last_error, sizeof g_last_error, "%s:%d: %s -> %s (code %d)", base_name(file), line, call,
+             hipGetErrorName(e), code);
+    if (trace_on()) fprintf(stderr, "[speigs_hip] error %s\n", g_last_error);
+    return code;
+}
+
+const char *last_error_text() { return g_last_error; }
+
+void clear_error() { g_last_error[0] = 0; }
+
+void note_error(const char *text)
+{
+    if (text && text != g_last_error) snprintf(g_last_error, sizeof g_last_error, "%s", text);
+}
+
 double wall()
 {
     timespec t;

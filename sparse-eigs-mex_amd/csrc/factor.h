@@ -84,8 +84,14 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
 // 0 ok, 1 not positive definite, <0 error
 int factor_numeric(Factor *F, double delta);
 void factor_attach_matrix(Factor *F, const SellA *A);
-// y = (A + delta I)^-1 x on device vectors (FP64 in/out)
+// y = (A + delta I)^-1 x on device vectors (FP64 in/out).  A solve whose
+// bounded waits gave up is reported by factor_check once the factor's stream
+// has been synchronized after it (and by the next factor_apply).
 int factor_apply(Factor *F, const double *x, double *y);
+int factor_check(const Factor *F);
+// true when factor_apply allocates nothing: the single-vector solves, whose
+// workspace is allocated with the factor (not the inverse-based debug mode)
+bool factor_apply_preallocated(const Factor *F);
 void factor_stats(const Factor *F, speigs_hip_info *hinfo);
 void factor_destroy(Factor *F);
 const Symbolic *factor_symbolic(const Factor *F);

@@ -87,7 +87,8 @@ __global__ void __launch_bounds__(256) k_diag_delta(T *pool, const int32_t *__re
 }
 
 // pool entry of (i, j) = A value (+ delta on diagonals), one thread per stored
-// column; an entry outside the symbolic structure sets bit 2 of *flag
+// column; an entry outside the symbolic structure sets bit 2 of flag[0] and
+// the first one found leaves its (i, j) in flag[2], flag[3]
 template <class T>
 __global__ void __launch_bounds__(256) k_assemble(T *pool, const int64_t *__restrict__ Ap,
                                                   const int64_t *__restrict__ Ai, const double *__restrict__ Ax,
@@ -120,7 +121,11 @@ __global__ void __launch_bounds__(256) k_assemble(T *pool, const int64_t *__rest
                 else hi = mid;
             }
             if (lo == end || rows[lo] != row) {
-                atomicOr(flag, 2);
+                // the first such entry is kept for the error report
+                if ((atomicOr(flag, 2) & 2) == 0) {
+                    flag[2] = (int)i;
+                    flag[3] = (int)j;
+                }
                 continue;
             }
             lr = c + (lo - rptr[s]);
@@ -831,11 +836,21 @@ __global__ void __launch_bounds__(256) k_bwd(const T *__restrict__ pool, const i
 // continues, so a logic error shows up as SPEIGS_ERR_INTERNAL, never a hang.
 
 
-__device__ __forceinline__ void wait_epoch(const int *f, int epoch, int *err)
+// The wait's bound and the value it waits for come from the host (WaitSpec):
+// normally 2^22 polls for the solve's own epoch; a host test sets a value no
+// item ever publishes and a short bound, so that every wait gives up and the
+// error path runs with every kernel still exiting normally.
+struct WaitSpec {
+    int mask;    // awaited value = epoch ^ mask (0 in normal operation)
+    int spin;    // polls before giving up
+};
+
+__device__ __forceinline__ void wait_epoch(const int *f, int epoch, WaitSpec ws, int *err)
 {
+    epoch ^= ws.mask;
     // relaxed polling: an agent-scope acquire here would invalidate the L2 of
     // the XCD on every poll; the published data is read with coherent loads
-    for (int it = 0; it < (1 << 22); it++) {
+    for (int it = 0; it < ws.spin; it++) {
         if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch) return;
         if ((it & 63) == 63 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
         __builtin_amdgcn_s_sleep(1);
@@ -913,7 +928,7 @@ __global__ void __launch_bounds__(256) k_fwd_ea(const EaTask *__restrict__ tasks
 template <class T>
 __global__ void __launch_bounds__(256) k_fwd_blk(const T *__restrict__ pool, const BlkTask *__restrict__ tasks,
                                                  SolveArgs a, const int64_t *__restrict__ panbase, int *flags,
-                                                 int epoch, unsigned *ticket, int *err, double *y)
+                                                 int epoch, WaitSpec wsp, unsigned *ticket, int *err, double *y)
 {
     __shared__ double wl[64];
     __shared__ double yp[64];
@@ -959,7 +974,7 @@ __global__ void __launch_bounds__(256) k_fwd_blk(const T *__restrict__ pool, con
 #pragma unroll
         for (int j = 0; j < 16; j++) cur[j] = pre[j];
         if (p + 1 < pend) fetch(p + 1);
-        if (threadIdx.x == 0) wait_epoch(&fl[p], epoch, err);
+        if (threadIdx.x == 0) wait_epoch(&fl[p], epoch, wsp, err);
         __syncthreads();
         const int pw = min(64, c - p * 64);
         if (threadIdx.x < 64) yp[lane] = lane < pw ? ld_coherent(&y[c0 + p * 64 + lane]) : 0.0;
@@ -1043,7 +1058,8 @@ __global__ void __launch_bounds__(256) k_bwd_upd(const T *__restrict__ pool, con
 template <class T>
 __global__ void __launch_bounds__(256) k_bwd_blk(const T *__restrict__ pool, const BlkTask *__restrict__ tasks,
                                                  SolveArgs a, const int64_t *__restrict__ panbase, int *flags,
-                                                 int epoch, unsigned *ticket, int *err, const double *__restrict__ y,
+                                                 int epoch, WaitSpec wsp, unsigned *ticket, int *err,
+                                                 const double *__restrict__ y,
                                                  double *x, const double *__restrict__ part,
                                                  const int64_t *__restrict__ partoff)
 {
@@ -1101,7 +1117,7 @@ __global__ void __launch_bounds__(256) k_bwd_blk(const T *__restrict__ pool, con
         for (int i = 0; i < 16; i++) cur[i] = pre[i];
         if (pp - 1 > p) fetch(pp - 1);
         __syncthreads();
-        if (threadIdx.x == 0) wait_epoch(&fl[pp], epoch, err);
+        if (threadIdx.x == 0) wait_epoch(&fl[pp], epoch, wsp, err);
         __syncthreads();
         if (threadIdx.x < 64) xs[lane] = (pp * 64 + lane < c) ? ld_coherent(&x[c0 + pp * 64 + lane]) : 0.0;
         __syncthreads();
@@ -1202,8 +1218,8 @@ int run_numeric(Factor *F, double delta)
     hipStream_t st = F->st;
     T *pool = (T *)F->pool;
     const double piv_rel = F->fp32 ? 1e-5 : 1e-11;
-    SEHIP_CHECK(hipMemsetAsync(F->flag, 0, sizeof(int), st));
-    if ((uintptr_t)pool % 16) return SPEIGS_ERR_INTERNAL;
+    SEHIP_CHECK(hipMemsetAsync(F->flag, 0, 4 * sizeof(int), st));
+    if ((uintptr_t)pool % 16) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "front pool not 16-byte aligned");
     k_zero<T><<<dim3((unsigned)std::min<int64_t>(4096, (S.pool * (int64_t)sizeof(T) / 16 + 255) / 256 + 1)), dim3(256), 0,
                 st>>>(pool, S.pool);
     const dim3 gn((unsigned)((F->n + 255) / 256));
@@ -1290,11 +1306,15 @@ int run_numeric(Factor *F, double delta)
             far_pending = false;
         }
     }
-    int hflag = 0;
-    SEHIP_CHECK(hipMemcpyAsync(&hflag, F->flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    int hflag[4] = {0, 0, 0, 0};
+    SEHIP_CHECK(hipMemcpyAsync(hflag, F->flag, sizeof hflag, hipMemcpyDeviceToHost, st));
     SEHIP_CHECK(hipStreamSynchronize(st));
-    if (hflag & 2) return SPEIGS_ERR_INTERNAL;   // A outside the symbolic structure
-    return hflag ? 1 : 0;
+    if (hflag[0] & 2) {   // an entry of A outside the symbolic structure
+        char what[160];
+        snprintf(what, sizeof what, "assembly: A(%d, %d) has no place in the symbolic structure", hflag[2], hflag[3]);
+        return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, what);
+    }
+    return hflag[0] ? 1 : 0;
 }
 
 template <class T>
@@ -1312,12 +1332,13 @@ void run_solve(Factor *F, const double *b, double *x)
     k_gather_perm<<<dim3(nb), dim3(256), 0, st>>>(b, F->perm, F->xp, F->n);
     if (F->solve_mode == 1) {
         const int ep = ++F->epoch;
+        const WaitSpec wsp = F->wait_test ? WaitSpec{1 << 30, 64} : WaitSpec{0, 1 << 22};
         for (int32_t L = 0; L < S.nlevels; L++) {
             const LevelPlan &P = F->plan[L];
             if (!P.fr_cnt) continue;
             k_fwd_init<<<dim3((unsigned)P.fin_cnt), dim3(256), 0, st>>>(F->fin + P.fin_off, a, F->xp);
             k_fwd_blk<T><<<dim3((unsigned)P.fb_cnt), dim3(256), 0, st>>>(pool, F->fblk + P.fb_off, a, F->panbase,
-                                                                        F->fflag, ep, F->tick_f, F->err, F->yp);
+                                                                        F->fflag, ep, wsp, F->tick_f, F->err, F->yp);
         }
         for (int32_t L = S.nlevels - 1; L >= 0; L--) {
             const LevelPlan &P = F->plan[L];
@@ -1326,7 +1347,7 @@ void run_solve(Factor *F, const double *b, double *x)
                 k_bwd_upd<T><<<dim3((unsigned)P.ub_cnt), dim3(256), 0, st>>>(pool, F->upd + P.ub_off, a, F->xp,
                                                                             F->bpart, F->partoff);
             k_bwd_blk<T><<<dim3((unsigned)P.bb_cnt), dim3(256), 0, st>>>(pool, F->bblk + P.bb_off, a, F->panbase,
-                                                                        F->bflag, ep, F->tick_b, F->err, F->yp, F->xp,
+                                                                        F->bflag, ep, wsp, F->tick_b, F->err, F->yp, F->xp,
                                                                         F->bpart, F->partoff);
         }
         k_scatter_perm<<<dim3(nb), dim3(256), 0, st>>>(F->xp, F->perm, x, F->n);
@@ -1374,11 +1395,9 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     std::unique_ptr<Factor> F(new Factor());
     F->opt = o;
     F->st = st;
-    if (hipStreamCreate(&F->st2) != hipSuccess || hipEventCreateWithFlags(&F->ev_ready, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&F->ev_far, hipEventDisableTiming) != hipSuccess) {
-        (void)hipGetLastError();
-        return SPEIGS_ERR_INTERNAL;
-    }
+    SEHIP_CHECK(hipStreamCreate(&F->st2));
+    SEHIP_CHECK(hipEventCreateWithFlags(&F->ev_ready, hipEventDisableTiming));
+    SEHIP_CHECK(hipEventCreateWithFlags(&F->ev_far, hipEventDisableTiming));
     F->n = n;
     F->fp32 = o.precision == SPEIGS_HIP_FP32_IR || o.precision == SPEIGS_HIP_FP32_BLOCK;
     // debug knobs for A/B checks of kernel variants
@@ -1386,6 +1405,8 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     if (const char *e = getenv("SPEIGS_HIP_MFMA")) F->use_mfma = atoi(e) != 0;
     if (const char *e = getenv("SPEIGS_HIP_REG_PANEL")) F->use_reg_panel = atoi(e) != 0;
     if (const char *e = getenv("SPEIGS_HIP_POTRF_WAVES")) F->potrf_waves = atoi(e) == 1 ? 1 : 4;
+    // test knob: every bounded wait of the blocked single-vector solve gives up
+    if (const char *e = getenv("SPEIGS_HIP_WAIT_TEST")) F->wait_test = atoi(e) == 1;
     SymbolicOptions so;
     int rc = symbolic_analyze(n, Ap, Ai, so, &F->S);
     if (rc) return rc;
@@ -1667,7 +1688,7 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     *F->herr = 0;
     F->ev.resize(2 * kEvPairs);
     for (auto &e : F->ev)
-        if (hipEventCreate(&e) != hipSuccess) return SPEIGS_ERR_INTERNAL;
+        SEHIP_CHECK(hipEventCreate(&e));
     trace_phase("small uploads, events", tp);
     // uploads
     const size_t esz = F->fp32 ? sizeof(float) : sizeof(double);
@@ -1689,7 +1710,7 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     if (ufid.empty()) ufid.push_back(0);
     if ((rc = upload(F.get(), &F->ufid, ufid))) return rc;
     if ((rc = upload(F.get(), &F->upref, upref))) return rc;
-    if ((rc = F->mem.alloc(&F->flag, 1))) return rc;
+    if ((rc = F->mem.alloc(&F->flag, 4))) return rc;
     F->wssize = wsoff[ns];
     if ((rc = F->mem.alloc(&F->ws, (size_t)std::max<int64_t>(F->wssize, 1)))) return rc;
     if ((rc = F->mem.alloc(&F->xp, (size_t)n))) return rc;
@@ -1721,10 +1742,23 @@ int factor_numeric(Factor *F, double delta)
 
 void factor_attach_matrix(Factor *F, const SellA *A) { F->A = A; }
 
+bool factor_apply_preallocated(const Factor *F)
+{
+    return F->opt.precision != SPEIGS_HIP_FP32_BLOCK && F->solve_mode != 2;
+}
+
+int factor_check(const Factor *F)
+{
+    if (F->herr && *F->herr)
+        return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "solve: a bounded wait between work items gave up");
+    return 0;
+}
+
 int factor_apply(Factor *F, const double *x, double *y)
 {
     if (!F->factored) return SPEIGS_ERR_FACTOR;
-    if (F->herr && *F->herr) return SPEIGS_ERR_INTERNAL;   // a bounded wait gave up in an earlier solve
+    int rc0 = factor_check(F);   // a bounded wait gave up in an earlier solve
+    if (rc0) return rc0;
     if (F->solve_mode == 2 && !F->inverted) {
         const int rc = factor_invert_l11(F);
         if (rc) return rc;
@@ -1744,13 +1778,13 @@ int factor_apply(Factor *F, const double *x, double *y)
     }
     if (!F->A) return 0;
     const double bn = device_norm(F, x);
-    if (bn < 0) return SPEIGS_ERR_INTERNAL;
+    if (bn < 0) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "refinement: norm of the right-hand side");
     if (bn == 0.0) return 0;
     for (int it = 0; it < F->opt.ir_maxit; it++) {
         spmv(*F->A, y, F->ax, F->st);
         axpby_residual(x, F->ax, y, F->delta, F->r, F->n, F->st);
         const double rn = device_norm(F, F->r);
-        if (rn < 0) return SPEIGS_ERR_INTERNAL;
+        if (rn < 0) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "refinement: norm of the residual");
         if (rn <= F->opt.ir_tol * bn) break;
         const int e = ev_begin(F);
         run_solve<float>(F, F->r, F->d);

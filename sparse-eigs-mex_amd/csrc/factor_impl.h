@@ -94,6 +94,7 @@ struct Factor {
     bool use_reg_panel = true;       // register POTRF + TRSM as a product with L_kk^-1
     int potrf_waves = 4;             // register POTRF on 4 waves (columns split) or 1
     int solve_mode = 1;               // 0: one workgroup per front, 1: blocked multi-workgroup
+    bool wait_test = false;           // test knob: the blocked solve's waits all give up
     EaTask *sea = nullptr;
     PcbTask *fin = nullptr;
     EaBlk *eab = nullptr;             // factor extend-add blocks
@@ -146,7 +147,12 @@ void rowsolve_free(Factor *F);
 int factor_block_solve_d(Factor *F, const double *B, double *X, int nb);
 void factor_block_free(Factor *F);
 
-// block eigensolver preconditioned by the FP32 factor (lobpcg.cpp)
+// block eigensolver preconditioned by the FP32 factor (lobpcg.cpp): one
+// block of m = k + 8 vectors (fewer when n is small), m <= 32, the columns of
+// one multi-right-hand-side solve; lobpcg_block_size is m, or 0 when k does not
+// fit (the driver then runs Lanczos over the same FP32 factor with refinement)
+constexpr int kLobpcgMaxBlock = 32;
+int lobpcg_block_size(int64_t n, int64_t k);
 // Xstart: optional proposed start block (device, n x 32 row-major FP32)
 int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, const speigs_opts *o,
                const speigs_hip_opts *ho, double anorm, double *lambda, double *Vout, double *resid,

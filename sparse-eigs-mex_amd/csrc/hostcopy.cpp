@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstring>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -64,11 +65,11 @@ int acquire(int dev, int count, std::vector<Slot *> &out)
             delete s;
             return SPEIGS_ERR_MEM;
         }
-        if (hipEventCreateWithFlags(&s->ev, hipEventDisableTiming) != hipSuccess) {
-            (void)hipGetLastError();
+        const hipError_t e = hipEventCreateWithFlags(&s->ev, hipEventDisableTiming);
+        if (e != hipSuccess) {
             (void)hipHostFree(s->buf);
             delete s;
-            return SPEIGS_ERR_INTERNAL;
+            return hip_fail(e, __FILE__, __LINE__, "hipEventCreateWithFlags(staging slot)");
         }
         out.push_back(s);
     }
@@ -86,8 +87,7 @@ void release(int dev, std::vector<Slot *> &slots)
 int wait_slot(Slot *s)
 {
     if (!s->pending) return 0;
-    const hipError_t e = hipEventSynchronize(s->ev);
-    if (e != hipSuccess) return hip_error_code(e);
+    SEHIP_CHECK(hipEventSynchronize(s->ev));
     s->pending = false;
     return 0;
 }
@@ -109,7 +109,7 @@ struct Layout {
 // thread t's chunks t, t + T, ...: device -> slot -> host, two slots in flight
 int d2h_worker(const Layout &L, int t, int T, Slot *s0, Slot *s1, hipStream_t st, int dev)
 {
-    if (hipSetDevice(dev) != hipSuccess) return SPEIGS_ERR_INTERNAL;
+    SEHIP_CHECK(hipSetDevice(dev));
     Slot *sl[2] = {s0, s1};
     int rc = 0;
     for (Slot *s : sl)
@@ -138,7 +138,7 @@ int d2h_worker(const Layout &L, int t, int T, Slot *s0, Slot *s1, hipStream_t st
 // previous DMA has ended, the last two stay pending
 int h2d_worker(const Layout &L, int t, int T, Slot *s0, Slot *s1, hipStream_t st, int dev)
 {
-    if (hipSetDevice(dev) != hipSuccess) return SPEIGS_ERR_INTERNAL;
+    SEHIP_CHECK(hipSetDevice(dev));
     Slot *sl[2] = {s0, s1};
     int rc = 0;
     size_t j = 0;
@@ -180,17 +180,22 @@ int staged(bool to_host, void *dst, size_t dpitch, const void *src, size_t spitc
         return rc;
     }
     std::vector<int> rcs(T, 0);
+    std::vector<std::string> why(T);
     auto run = [&](int t) {
         rcs[t] = to_host ? d2h_worker(L, t, T, slots[2 * t], slots[2 * t + 1], st, dev)
                          : h2d_worker(L, t, T, slots[2 * t], slots[2 * t + 1], st, dev);
+        if (rcs[t]) why[t] = last_error_text();
     };
     std::vector<std::thread> th;
     for (int t = 1; t < T; t++) th.emplace_back(run, t);
     run(0);
     for (auto &x : th) x.join();
     release(dev, slots);
-    for (int r : rcs)
-        if (r) return r;
+    for (int t = 0; t < T; t++)
+        if (rcs[t]) {
+            note_error(why[t].c_str());
+            return rcs[t];
+        }
     return 0;
 }
 

@@ -217,7 +217,7 @@ int lanczos_dev(DevOperator *op, int64_t n, int64_t k, const speigs_opts *o, con
     }
     {
         double nr = std::sqrt(host_sumsq(n, hv.data()));
-        if (nr == 0.0) return SPEIGS_ERR_INTERNAL;
+        if (nr == 0.0) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "re-injected start vector has norm 0");
         double s = 1.0 / nr;
         for (int64_t i = 0; i < n; i++) hv[i] *= s;
         if ((rc = copy_to_device(W.V, hv.data(), (size_t)n * sizeof(double), W.st))) return rc;
@@ -233,6 +233,7 @@ int lanczos_dev(DevOperator *op, int64_t n, int64_t k, const speigs_opts *o, con
             op->nops++;
             if (rc) return rc;
             SEHIP_CHECK(hipStreamSynchronize(W.st));
+            if ((rc = op->check())) return rc;
             info->t_op += wall() - ts;
             ts = wall();
             double *hc = H.data() + (size_t)j * ldh;
@@ -360,6 +361,7 @@ int lanczos_dev(DevOperator *op, int64_t n, int64_t k, const speigs_opts *o, con
         if (rc2) return rc2;
     }
     SEHIP_CHECK(hipStreamSynchronize(W.st));
+    if ((rc = op->check())) return rc;
     info->nconv = nconv;
     info->restarts = restarts;
     info->nops = op->nops;

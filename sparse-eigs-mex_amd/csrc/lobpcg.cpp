@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -218,16 +219,25 @@ void symmetrize(std::vector<double> &H, int d)
 
 }  // namespace
 
+int lobpcg_block_size(int64_t n, int64_t k)
+{
+    // k + 8 guard vectors: at 100^3, k = 20, blocks of 28..32 all take 12
+    // iterations (26: 13, 24: 14), so the smallest of them is the cheapest
+    if (k + 8 > kLobpcgMaxBlock) return 0;
+    int64_t m = k + 8;
+    if (m > n / 2) m = std::max<int64_t>(k, n / 2);
+    return k > m ? 0 : (int)m;
+}
+
 int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, const speigs_opts *o,
                const speigs_hip_opts *ho, double anorm, double *lambda, double *Vout, double *resid,
                speigs_info *info, speigs_hip_info *hinfo, const float *Xstart)
 {
-    // k + 8 guard vectors: at 100^3, k = 20, blocks of 28..32 all take 12
-    // iterations (26: 13, 24: 14), so the smallest of them is the cheapest
-    int m = (int)std::min<int64_t>(32, k + 8);
-    if (m > n / 2) m = (int)std::max<int64_t>(k, n / 2);
-    if (k > m || m > 32) return SPEIGS_ERR_ARG;
-    const int64_t maxit = o->maxit ? std::max<int64_t>(o->maxit, 50) : 500;
+    const int m = lobpcg_block_size(n, k);
+    if (m == 0) return SEHIP_FAIL(SPEIGS_ERR_ARG, "LOBPCG: k too large for one block");
+    // an iteration plays the part of a restart cycle: the reference's cap
+    // (maxit restarts, 0 -> 300; src/speigs_core.c:156)
+    const int64_t maxit = o->maxit ? o->maxit : 300;
     const double tol = o->tol > 0 ? o->tol : 1e-14;
     const double rtol = tol * (anorm > 1.0 ? anorm : 1.0);
     // the caller's eigenvector array is faulted in on helper threads while the
@@ -259,6 +269,7 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
     uint64_t seed = o->seed ? o->seed : 88172645463325252ULL;
     std::vector<double> h;
     int start_rc = 0;
+    std::string start_msg;
     std::thread start_th;
     struct JoinStart {
         std::thread &t;
@@ -271,23 +282,29 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
         int dev = 0;
         SEHIP_CHECK(hipGetDevice(&dev));
         start_th = std::thread([&, dev] {
-            if (hipSetDevice(dev) != hipSuccess) {
-                start_rc = SPEIGS_ERR_INTERNAL;
-                return;
-            }
-            h.resize((size_t)n);
-            if (ho && ho->start == SPEIGS_HIP_START_VECTOR && ho->start_vec)
-                std::memcpy(h.data(), ho->start_vec, (size_t)n * sizeof(double));
-            else
-                uniform_vec(&seed, n, h.data());
-            if ((start_rc = copy_to_device(v0, h.data(), (size_t)n * sizeof(double), F->st2))) return;
-            if (hipEventRecord(F->ev_ready, F->st2) != hipSuccess) start_rc = SPEIGS_ERR_INTERNAL;
+            auto run = [&]() -> int {
+                SEHIP_CHECK(hipSetDevice(dev));
+                h.resize((size_t)n);
+                if (ho && ho->start == SPEIGS_HIP_START_VECTOR && ho->start_vec)
+                    std::memcpy(h.data(), ho->start_vec, (size_t)n * sizeof(double));
+                else
+                    uniform_vec(&seed, n, h.data());
+                const int r = copy_to_device(v0, h.data(), (size_t)n * sizeof(double), F->st2);
+                if (r) return r;
+                SEHIP_CHECK(hipEventRecord(F->ev_ready, F->st2));
+                return 0;
+            };
+            start_rc = run();
+            if (start_rc) start_msg = last_error_text();
         });
     }
     rc = factor_invert_l11(F);
     if (start_th.joinable()) start_th.join();
     if (rc) return rc;
-    if (start_rc) return start_rc;
+    if (start_rc) {
+        note_error(start_msg.c_str());
+        return start_rc;
+    }
 
     // start block: the proposed block of the learned start (its first m
     // columns), else column 0 = the reference's xorshift64* start vector (or
@@ -305,7 +322,7 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
             if ((rc = dgram(W, T2, ldT, m, T2, ldT, m, g))) return rc;
             int r = 0;
             if ((rc = svqb(W, T2, ldT, m, g, S, ldS, &r))) return rc;
-            if (r < m) return SPEIGS_ERR_INTERNAL;
+            if (r < m) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "LOBPCG: start block lost rank in orthonormalisation");
             if (pass == 0) rcopy(S, ldS, m, n, T2, ldT, st);
         }
     }
@@ -325,6 +342,18 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
     int64_t it = 0, nconv = 0, solves = 0;
     std::vector<double> res(m);
     double *lam_dev = W.red;
+    // the reference's iteration breakdown (include/speigs.h:57-68): t_op the
+    // preconditioner solves and A-products, t_ortho the projection and
+    // orthonormalisation of [W P], t_resid the residuals, t_ritz Rayleigh-Ritz
+    // and the new block; each phase ends at a stream synchronisation
+    double tmark = wall();
+    auto lap = [&](double &acc) -> int {
+        SEHIP_CHECK(hipStreamSynchronize(st));
+        const double now = wall();
+        acc += now - tmark;
+        tmark = now;
+        return 0;
+    };
     for (;; it++) {
         // S = [X | P | W] (no P in the first iteration): Z = [P W] is contiguous
         double *Z = S + m, *AZ = AS + m, *Wb = S + m + np;
@@ -338,10 +367,12 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
                 if (res[i] <= rtol) nconv++;
             }
         }
+        if ((rc = lap(info->t_resid))) return rc;
         if (nconv >= k || it >= maxit) break;
         // W = T R: one FP32 multi-right-hand-side solve, in place on the rows of S
         if ((rc = factor_block_solve_rows(F, Wb, ldS, 1, Wb, ldS, 1, m))) return rc;
         solves++;
+        if ((rc = lap(info->t_op))) return rc;
         int nz = m + np;
         double keep = 0.0, cond = 0.0;
         int r = 0;
@@ -359,7 +390,9 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
             if (r == 0) break;
             nz = r;
         }
+        if ((rc = lap(info->t_ortho))) return rc;
         rspmm(A, Z, ldS, AZ, ldS, nz, st);
+        if ((rc = lap(info->t_op))) return rc;
         // Rayleigh-Ritz on S = [X Z] (orthonormal): H = S^T A S.  X holds the
         // previous Ritz vectors, so X^T A X = diag(theta) (as in Knyazev's and
         // SciPy's LOBPCG) and only S^T A Z is formed: (m + nz) x nz
@@ -394,7 +427,11 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
         std::swap(S, Sn);
         std::swap(AS, ASn);
         np = m;
-        if ((it + 1) % 8 == 0) rspmm(A, S, ldS, AS, ldS, m, st);   // refresh A X against drift
+        if ((rc = lap(info->t_ritz))) return rc;
+        if ((it + 1) % 8 == 0) {
+            rspmm(A, S, ldS, AS, ldS, m, st);   // refresh A X against drift
+            if ((rc = lap(info->t_op))) return rc;
+        }
     }
     // final pairs: exact A X, Rayleigh quotients, residuals
     rspmm(A, S, ldS, AS, ldS, m, st);
@@ -405,6 +442,7 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
     for (int t = 0; t < m; t++) lam[t] = xax[(size_t)t * m + t] / xx[(size_t)t * m + t];
     SEHIP_CHECK(hipMemcpyAsync(lam_dev, lam.data(), m * sizeof(double), hipMemcpyHostToDevice, st));
     if ((rc = resid_norms(W, AS, S, ldS, lam_dev, m, (int)k, S + m, rr))) return rc;
+    if ((rc = lap(info->t_resid))) return rc;
     nconv = 0;
     for (int t = 0; t < k; t++) {
         const double r = std::sqrt(std::max(rr[t], 0.0)) / std::sqrt(xx[(size_t)t * m + t]);

@@ -732,7 +732,7 @@ int speigs_hip_model_create(int layers, uint64_t seed, speigs_hip_model **out)
 {
     if (!out || layers < 1 || layers > 8) return SPEIGS_ERR_ARG;
     *out = nullptr;
-    if (speigs_hip_device_count() <= 0) return SPEIGS_ERR_INTERNAL;
+    if (speigs_hip_device_count() <= 0) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "no HIP device visible");
     std::unique_ptr<speigs_hip_model> m(new speigs_hip_model());
     m->layers = layers;
     const int64_t np = mp_nparams(layers);
@@ -746,7 +746,7 @@ int speigs_hip_model_create(int layers, uint64_t seed, speigs_hip_model **out)
             const double u = (double)((z * 0x2545F4914F6CDD1DULL) >> 11) * (1.0 / 9007199254740992.0);
             m->params[l * LAYER_P + e] = (float)((2.0 * u - 1.0) * lim);
         }
-    if (hipStreamCreate(&m->st) != hipSuccess) return SPEIGS_ERR_INTERNAL;
+    SEHIP_CHECK(hipStreamCreate(&m->st));
     int rc;
     if ((rc = m->mem.alloc(&m->p, (size_t)np))) return rc;
     if ((rc = m->mem.alloc(&m->g, (size_t)np))) return rc;

@@ -31,6 +31,7 @@
 // coalesced and transpose each wave's 16 x 64 slice through LDS.
 #include <algorithm>
 #include <cstring>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -608,6 +609,7 @@ struct RsState {
     // waits for that stream
     std::thread th;
     int plan_rc = 0;
+    std::string plan_msg;     // the helper thread's error record
     bool ready = false;
     hipStream_t up_st = nullptr;
     RsHost host;
@@ -889,7 +891,10 @@ int factor_block_solve_rows(Factor *F, const double *Bin, int64_t brs, int64_t b
     RsState &B = *reinterpret_cast<RsState *>(F->rs);
     if (!B.ready) {
         if (B.th.joinable()) B.th.join();
-        if (B.plan_rc) return B.plan_rc;
+        if (B.plan_rc) {
+            note_error(B.plan_msg.c_str());
+            return B.plan_rc;
+        }
         SEHIP_CHECK(hipStreamSynchronize(B.up_st));
         B.ready = true;
     }
@@ -910,13 +915,15 @@ void rowsolve_prepare_async(Factor *F)
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) (void)hipGetLastError();
     B->th = std::thread([F, B, dev] {
-        if ((B->plan_rc = rs_plan(F, *B))) return;
-        if (hipSetDevice(dev) != hipSuccess || hipStreamCreate(&B->up_st) != hipSuccess) {
-            (void)hipGetLastError();
-            B->plan_rc = SPEIGS_ERR_INTERNAL;
-            return;
-        }
-        B->plan_rc = rs_upload(F, *B, B->up_st);
+        auto run = [&]() -> int {
+            int rc = rs_plan(F, *B);
+            if (rc) return rc;
+            SEHIP_CHECK(hipSetDevice(dev));
+            SEHIP_CHECK(hipStreamCreate(&B->up_st));
+            return rs_upload(F, *B, B->up_st);
+        };
+        B->plan_rc = run();
+        if (B->plan_rc) B->plan_msg = last_error_text();
     });
 }
 

@@ -551,7 +551,7 @@ int symbolic_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, const Symb
             S->rel[S->relptr[s] + (q - S->rptr[s])] = (int32_t)pos;
         }
     });
-    if (bad.load()) return SPEIGS_ERR_INTERNAL;
+    if (bad.load()) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "symbolic: a supernode row missing from its parent front");
     S->t_symbolic = wall() - t1;
     return SPEIGS_OK;
 }

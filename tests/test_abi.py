@@ -116,6 +116,9 @@ def test_gpu_paths_fail_loudly_without_device(hiplib):
         pytest.skip("a HIP device is present")
     A = pkg.gen.lap3d(12, True)
     assert _solve(A, 4, "lm").rc == -5          # SPEIGS_ERR_INTERNAL, no CPU fallback
+    assert "no HIP device visible" in pkg.last_error()
+    assert _solve(pkg.gen.lap3d(3, True), 2, "sm").rc == 0     # dense short-circuit: host only
+    assert pkg.last_error() == ""               # a successful call leaves no record
 
 
 def test_norm_matches_serial_sum_order(hiplib):

@@ -19,8 +19,24 @@ def _solve(A, k, which, opts=None, env=None, monkeypatch=None, **kw):
     if env and monkeypatch:
         for key, val in env.items():
             monkeypatch.setenv(key, val)
-    print(f"[case] n={len(A[0]) - 1} k={k} which={which} opts={opts} env={env}", flush=True)
-    return pkg.speigs(A, k, which, opts=opts, **kw)
+    print(f"[case] n={len(A[0]) - 1} k={k} which={which} opts={opts} env={env} {kw}", flush=True)
+    r = pkg.speigs(A, k, which, opts=opts, **kw)
+    if r.rc < 0:
+        print(f"[error] rc={r.rc} at {pkg.last_error()}", flush=True)
+    return r
+
+
+def _norm1(A):
+    """||A||_1 of the symmetric matrix whose upper triangle A stores (the
+    blocks path leaves info.anorm at 0, as the reference's does)"""
+    Ap, Ai, Ax = A
+    col = np.repeat(np.arange(len(Ap) - 1), np.diff(Ap))
+    s = np.zeros(len(Ap) - 1)
+    up = Ai <= col
+    np.add.at(s, col[up], np.abs(Ax[up]))
+    off = up & (Ai < col)
+    np.add.at(s, Ai[off], np.abs(Ax[off]))
+    return s.max()
 
 
 def _comps(sizes, seed):
@@ -54,14 +70,53 @@ def test_batched_matches_serial(hiplib, monkeypatch, which):
     assert res.max() <= 1e-9 * max(1.0, np.abs(rb.lam).max())
 
 
-def test_batched_vs_oracle(hiplib, oracle):
-    """12 components, k = 6 'sm': eigenvalues equal the CPU oracle's to 1e-10 relative."""
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_batched_vs_oracle(hiplib, oracle, precision):
+    """12 components, k = 6 'sm': eigenvalues equal the CPU oracle's to 1e-10
+    relative, residuals within tol * ||A||, with the shared factor in FP64
+    (precision 0) and in FP32 with every apply refined to FP64 (precision 1;
+    precision 2 asks for the FP32 factor, which the batched Lanczos path also
+    applies with refinement)."""
     A = _comps([1000, 1200, 1500, 2000, 1728, 2197, 900, 1100, 1331, 1600, 2744, 1024], seed=7)
-    r = pkg.speigs(A, 6, "sm", opts={"tol": 1e-11})
+    r = _solve(A, 6, "sm", opts={"tol": 1e-11}, hip_opts={"precision": precision})
     ro = pkg.speigs(A, 6, "sm", opts={"tol": 1e-11}, lib=oracle, prefix="oracle_")
-    print("gpu", r.lam, r.info, "oracle", ro.lam, ro.info)
-    assert r.rc == 0 and ro.rc == 0
+    print("gpu", r.lam, r.info, r.hip_info, "oracle", ro.lam, ro.info)
+    assert r.rc == 0 and ro.rc == 0 and r.info["path"] == 4 and r.info["flag"] == 0
+    assert r.hip_info["precision_used"] == (0 if precision == 0 else 1)
+    if precision:
+        assert r.hip_info["ir_steps"] > 0
     assert np.abs(r.lam - ro.lam).max() <= 1e-10 * np.abs(ro.lam).max()
+    R = gen.spmv_upper(*A, r.V) - r.V * r.lam
+    assert np.linalg.norm(R, axis=0).max() <= 1e-11 * _norm1(A)
+
+
+def test_config4_full_size(hiplib, oracle):
+    """The 512-component workload at its full size (n = 5.08e6, sizes
+    log-uniform in [500, 50,000], half 2-D, half 3-D), k = 6 'sm', with the
+    FP32 factor refined to FP64: every returned vector lies in one component,
+    every residual is within tol * ||A||_1, the values equal the FP64 batched
+    run's to 1e-10 relative, and each value is one of the oracle's k smallest
+    of the component that holds its vector."""
+    Ap, Ai, Ax, sizes = gen.batched_components(512, seed=4)
+    A = (Ap, Ai, Ax)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    r32 = _solve(A, 6, "sm", opts={"tol": 1e-10}, hip_opts={"precision": 2})
+    r64 = _solve(A, 6, "sm", opts={"tol": 1e-10}, hip_opts={"precision": 0})
+    print("fp32", r32.info, r32.hip_info, "fp64", r64.info)
+    for r in (r32, r64):
+        assert r.rc == 0 and r.info["flag"] == 0 and r.info["ncomp"] == 512 and r.info["path"] == 4
+    assert np.abs(r32.lam - r64.lam).max() <= 1e-10 * np.abs(r64.lam).max()
+    R = gen.spmv_upper(*A, r32.V) - r32.V * r32.lam
+    assert np.linalg.norm(R, axis=0).max() <= 1e-10 * _norm1(A)
+    for t in range(6):
+        mass = np.array([np.sum(r32.V[offs[c]:offs[c + 1], t] ** 2) for c in range(512)])
+        c = int(np.argmax(mass))
+        assert np.sort(mass)[-2] <= 1e-20
+        lo, hi = offs[c], offs[c + 1]
+        sub = (Ap[lo:hi + 1] - Ap[lo], Ai[Ap[lo]:Ap[hi]] - lo, Ax[Ap[lo]:Ap[hi]])
+        o = pkg.speigs(sub, 6, "sm", opts={"tol": 1e-10}, lib=oracle, prefix="oracle_")
+        assert o.rc == 0
+        assert np.min(np.abs(o.lam - r32.lam[t])) <= 1e-10 * abs(r32.lam[t]), (t, c, r32.lam[t], o.lam)
 
 
 def test_batched_closed_form(hiplib):

@@ -88,3 +88,41 @@ def test_learned_start_same_answer(hiplib, precision):
     assert np.abs(r1.lam - r0.lam).max() <= 1e-10 * np.abs(r0.lam).max()
     assert r1.resid.max() <= 1e-11 * r1.info["anorm"]
     m.close()
+
+
+def test_trained_start_gain(hiplib, oracle):
+    """The committed network (models/start_grf32.bin, trained on the GPU on
+    plain-solver eigenvectors of 32^3 GRF Laplacians, seeds 10,000..11,999) on
+    three held-out instances of its family (seeds 950,000..950,002), k = 16,
+    tol 1e-10, on the chosen path: the eigenspace part its block captures is
+    far above an untrained network's, the block eigensolver needs fewer
+    iterations in total than with the uniform start, and the answer is the
+    uniform start's and (first instance) the oracle's to 1e-10 relative."""
+    m = pkg.StartModel(3, seed=1)
+    m.load(ROOT / "sparse-eigs-mex_amd" / "models" / "start_grf32.bin")
+    u = pkg.StartModel(3, seed=1)                      # untrained, same architecture
+    it_u = it_l = 0
+    for j in range(3):
+        A = gen.grf_lap3d(32, seed=950_000 + j)
+        print(f"[case] grf_lap3d(32, seed={950_000 + j}) k=16", flush=True)
+        r0 = pkg.speigs(A, 16, "sm", opts={"tol": 1e-10}, hip_opts={"precision": 2})
+        r1 = pkg.speigs(A, 16, "sm", opts={"tol": 1e-10},
+                        hip_opts={"precision": 2, "start": 2, "model": m.handle.value})
+        assert r0.rc == 0 and r1.rc == 0 and r0.info["flag"] == 0 and r1.info["flag"] == 0
+        assert np.abs(r1.lam - r0.lam).max() <= 1e-10 * np.abs(r0.lam).max()
+        assert r1.resid.max() <= 1e-10 * r1.info["anorm"]
+        lt, _ = m.grad(A, r0.V, seed=88172645463325252)
+        lu, _ = u.grad(A, r0.V, seed=88172645463325252)
+        print(f"instance {j}: iterations uniform {r0.info['restarts']} learned {r1.info['restarts']}, "
+              f"loss trained {lt:.3f} untrained {lu:.3f}")
+        assert lt <= 0.5 and lu >= 0.7 and lt <= lu - 0.25
+        it_u += r0.info["restarts"]
+        it_l += r1.info["restarts"]
+        if j == 0:
+            o = pkg.speigs(A, 16, "sm", opts={"tol": 1e-10}, lib=oracle, prefix="oracle_")
+            assert o.rc == 0
+            assert np.abs(r1.lam - o.lam).max() <= 1e-10 * np.abs(o.lam).max()
+    print("total iterations: uniform", it_u, "learned", it_l)
+    assert it_l < it_u
+    m.close()
+    u.close()

@@ -15,7 +15,10 @@ pytestmark = pytest.mark.gpu
 
 def _solve(A, k, which, opts=None, vectors=True, resid=True, hip_opts=None):
     print(f"[case] n={len(A[0]) - 1} k={k} which={which} opts={opts} hip={hip_opts}", flush=True)
-    return pkg.speigs(A, k, which, opts=opts, vectors=vectors, resid=resid, hip_opts=hip_opts)
+    r = pkg.speigs(A, k, which, opts=opts, vectors=vectors, resid=resid, hip_opts=hip_opts)
+    if r.rc < 0:
+        print(f"[error] rc={r.rc} at {pkg.last_error()}", flush=True)
+    return r
 
 
 def _op(hiplib, A, mode, precision=0, delta=0.0):
@@ -29,7 +32,42 @@ def _op(hiplib, A, mode, precision=0, delta=0.0):
     hi = pkg.HipInfo()
     rc_ = hiplib.speigs_hip_op_create(len(Ap) - 1, Ap.ctypes.data, Ai.ctypes.data, Ax.ctypes.data, mode, precision,
                                       delta, ctypes.byref(op), ctypes.byref(hi))
+    if rc_ < 0:
+        print(f"[error] op_create rc={rc_} at {pkg.last_error()}", flush=True)
     return rc_, op, hi
+
+
+def _apply(hiplib, op, B, X):
+    rc_ = hiplib.speigs_hip_op_apply(op, B.ctypes.data, X.ctypes.data, 1 if B.ndim == 1 else B.shape[1])
+    if rc_ < 0:
+        print(f"[error] op_apply rc={rc_} at {pkg.last_error()}", flush=True)
+    return rc_
+
+
+def test_solve_wait_failure_is_reported(hiplib, monkeypatch):
+    """Host-forced failure of the blocked FP64 solve's bounded waits: with
+    SPEIGS_HIP_WAIT_TEST=1 every wait looks for a flag value no work item
+    publishes and gives up after 64 polls, so every kernel still exits.  The
+    failed solve must be reported by the apply that ran it (not only by a later
+    one) and by speigs(), with the site in speigs_hip_last_error(); a factor
+    made without the knob solves normally afterwards."""
+    A = gen.weighted_lap3d(24, seed=9, upper=True)
+    n = len(A[0]) - 1
+    B = np.asfortranarray(np.random.default_rng(5).standard_normal((n, 1)))
+    X = np.zeros((n, 1), order="F")
+    monkeypatch.setenv("SPEIGS_HIP_WAIT_TEST", "1")
+    rc_, op, _ = _op(hiplib, A, 1, 0, 0.0)
+    assert rc_ == 0
+    assert _apply(hiplib, op, B, X) == -5
+    assert "bounded wait" in pkg.last_error()
+    hiplib.speigs_hip_op_destroy(op)
+    r = _solve(A, 4, "sm")
+    assert r.rc == -5 and "bounded wait" in pkg.last_error()
+    monkeypatch.delenv("SPEIGS_HIP_WAIT_TEST")
+    rc_, op, _ = _op(hiplib, A, 1, 0, 0.0)
+    assert rc_ == 0 and _apply(hiplib, op, B, X) == 0 and pkg.last_error() == ""
+    hiplib.speigs_hip_op_destroy(op)
+    assert np.linalg.norm(gen.spmv_upper(*A, X) - B) <= 1e-11 * np.linalg.norm(B)
 
 
 @pytest.mark.parametrize("upper", [False, True])
@@ -43,7 +81,7 @@ def test_solve_vs_numpy(hiplib, upper, precision):
     D = gen.csc_to_dense(*A, symmetrize_upper=True) + delta * np.eye(n)
     B = np.asfortranarray(np.random.default_rng(2).standard_normal((n, 3)))
     X = np.zeros((n, 3), order="F")
-    assert hiplib.speigs_hip_op_apply(op, B.ctypes.data, X.ctypes.data, 3) == 0
+    assert _apply(hiplib, op, B, X) == 0
     Xref = np.linalg.solve(D, B)
     err = np.abs(X - Xref).max() / np.abs(Xref).max()
     print("solve rel err", err, "nnzL", hi.nnz_L, "nsuper", hi.nsuper)
@@ -59,7 +97,7 @@ def test_solve_known_answer(hiplib):
     v = np.ascontiguousarray(gen.dirichlet_mode3(N, 3, 1, 2))
     lam = sum(2 - 2 * np.cos(p * np.pi / (N + 1)) for p in (3, 1, 2))
     x = np.zeros_like(v)
-    assert hiplib.speigs_hip_op_apply(op, v.ctypes.data, x.ctypes.data, 1) == 0
+    assert _apply(hiplib, op, v, x) == 0
     assert np.abs(x - v / lam).max() <= 1e-12 / lam
     hiplib.speigs_hip_op_destroy(op)
 
@@ -148,7 +186,7 @@ def test_potrf_waves_bitwise(hiplib, monkeypatch, prec):
         rc_, op, hi = _op(hiplib, A, 1, prec, 0.0)
         assert rc_ == 0
         X = np.zeros((n, 3), order="F")
-        assert hiplib.speigs_hip_op_apply(op, B.ctypes.data, X.ctypes.data, 3) == 0
+        assert _apply(hiplib, op, B, X) == 0
         hiplib.speigs_hip_op_destroy(op)
         out.append(X)
     R = gen.spmv_upper(*A, out[1]) - B
@@ -169,7 +207,7 @@ def test_kernel_variants_agree(hiplib, monkeypatch, mode, mfma, prec):
     assert rc_ == 0
     B = np.asfortranarray(np.random.default_rng(4).standard_normal((n, 2)))
     X = np.zeros((n, 2), order="F")
-    assert hiplib.speigs_hip_op_apply(op, B.ctypes.data, X.ctypes.data, 2) == 0
+    assert _apply(hiplib, op, B, X) == 0
     R = gen.spmv_upper(*A, X) - B
     rel = np.linalg.norm(R) / np.linalg.norm(B)
     print("mode", mode, "mfma", mfma, "residual", rel)
@@ -186,7 +224,7 @@ def test_inverse_solve_weighted_100(hiplib, monkeypatch):
     assert rc_ == 0
     B = np.asfortranarray(np.random.default_rng(8).standard_normal((n, 1)))
     X = np.zeros((n, 1), order="F")
-    assert hiplib.speigs_hip_op_apply(op, B.ctypes.data, X.ctypes.data, 1) == 0
+    assert _apply(hiplib, op, B, X) == 0
     rel = np.linalg.norm(gen.spmv_upper(*A, X) - B) / np.linalg.norm(B)
     print("inverse-based FP64 solve residual", rel)
     assert rel <= 1e-11
@@ -217,16 +255,16 @@ def test_block_solve_fp32(hiplib, monkeypatch, N, nrhs):
     rc_, op64, _ = _op(hiplib, A, 1, 0, 0.0)
     assert rc_ == 0
     X64 = np.zeros((n, nrhs), order="F")
-    assert hiplib.speigs_hip_op_apply(op64, B.ctypes.data, X64.ctypes.data, nrhs) == 0
+    assert _apply(hiplib, op64, B, X64) == 0
     hiplib.speigs_hip_op_destroy(op64)
     rc_, op, _ = _op(hiplib, A, 1, 2, 0.0)
     assert rc_ == 0
     Xf = np.zeros((n, nrhs), order="F")
     Xg = np.zeros((n, nrhs), order="F")
     monkeypatch.setenv("SPEIGS_HIP_BLOCK_GENERIC", "0")
-    assert hiplib.speigs_hip_op_apply(op, B.ctypes.data, Xf.ctypes.data, nrhs) == 0
+    assert _apply(hiplib, op, B, Xf) == 0
     monkeypatch.setenv("SPEIGS_HIP_BLOCK_GENERIC", "1")
-    assert hiplib.speigs_hip_op_apply(op, B.ctypes.data, Xg.ctypes.data, nrhs) == 0
+    assert _apply(hiplib, op, B, Xg) == 0
     hiplib.speigs_hip_op_destroy(op)
     col = lambda D, R: (np.linalg.norm(D, axis=0) / np.linalg.norm(R, axis=0)).max()
     e_fg, e_g64, e_f64 = col(Xf - Xg, Xg), col(Xg - X64, X64), col(Xf - X64, X64)

@@ -136,6 +136,66 @@ def test_rng_known_answers(oracle):
     assert v.min() >= -0.5 and v.max() < 0.5
 
 
+def _gf2_mul(A, B):
+    """product of 64 x 64 bit matrices over GF(2), each a list of 64 column bitmasks"""
+    out = []
+    for col in B:
+        acc, i = 0, 0
+        while col:
+            if col & 1:
+                acc ^= A[i]
+            col >>= 1
+            i += 1
+        out.append(acc)
+    return out
+
+
+def _gf2_pow(A, e):
+    R = [1 << b for b in range(64)]
+    while e:
+        if e & 1:
+            R = _gf2_mul(R, A)
+        A = _gf2_mul(A, A)
+        e >>= 1
+    return R
+
+
+def test_rng_state_map_has_full_period(oracle):
+    """The oracle's generator state update (reference src/speigs_core.c:49-54,
+    xorshift64* after Vigna 2016) is linear over GF(2); what fixes it
+    mathematically is that its 64 x 64 bit matrix T has order exactly 2^64 - 1,
+    i.e. the state runs through every nonzero value (T^(2^64-1) = I and
+    T^((2^64-1)/p) != I for each prime p of 2^64 - 1).  A wrong shift, a
+    shift in the wrong direction or a dropped step breaks that.  The
+    multiplier of the output step is the published decimal constant."""
+    oracle.oracle_rng_next.restype = ctypes.c_uint64
+    oracle.oracle_rng_next.argtypes = [ctypes.POINTER(ctypes.c_uint64)]
+    T = []
+    for b in range(64):
+        s = ctypes.c_uint64(1 << b)
+        oracle.oracle_rng_next(ctypes.byref(s))
+        T.append(s.value)
+    primes = [3, 5, 17, 257, 641, 65537, 6700417]
+    order = (1 << 64) - 1
+    assert np.prod(np.array(primes, dtype=object)) == order
+    eye = [1 << b for b in range(64)]
+    assert _gf2_pow(T, order) == eye
+    for p in primes:
+        assert _gf2_pow(T, order // p) != eye, p
+    # linearity: the map of a sum of states is the sum of the maps
+    s = ctypes.c_uint64(0x9E3779B97F4A7C15)
+    oracle.oracle_rng_next(ctypes.byref(s))
+    acc = 0
+    for b in range(64):
+        if (0x9E3779B97F4A7C15 >> b) & 1:
+            acc ^= T[b]
+    assert s.value == acc
+    # output = state * M_32 mod 2^64, M_32 = 2685821657736338717 (Vigna 2016)
+    s = ctypes.c_uint64(1)
+    out = oracle.oracle_rng_next(ctypes.byref(s))
+    assert out == (s.value * 2685821657736338717) % (1 << 64)
+
+
 @pytest.mark.parametrize("fn", ["oracle_syev", "oracle_syev_jacobi"])
 def test_dense_eig_vs_numpy(oracle, fn):
     rng = np.random.default_rng(3)
