@@ -130,6 +130,16 @@ int  speigs_hip_op_refactor(speigs_hip_op *op, double delta);
 int  speigs_hip_op_apply(speigs_hip_op *op, const double *x, double *y, speigs_int nvec);
 /* y = A x (always A, never the shifted matrix) */
 int  speigs_hip_op_multA(speigs_hip_op *op, const double *x, double *y, speigs_int nvec);
+/* speigs_hip_op_apply without any device allocation.  No counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * host vectors are staged through a fixed 64 MiB array that every device holds
+ * until the process ends, so n * nvec <= 4,194,304 (else SPEIGS_ERR_ARG, with
+ * the reason in speigs_hip_last_error()); operators of precision
+ * SPEIGS_HIP_FP32_BLOCK, whose solve plan is allocated at first use, are
+ * refused with SPEIGS_ERR_ARG.  All calls on one device share that array:
+ * calls that overlap on one device race. */
+int  speigs_hip_op_apply_noalloc(speigs_hip_op *op, const double *x, double *y, speigs_int nvec);
 void speigs_hip_op_destroy(speigs_hip_op *op);
 
 /* ---- learned start: a message-passing network over the graph of A --------
