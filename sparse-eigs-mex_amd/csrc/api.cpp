@@ -41,13 +41,15 @@ int key_asc(const void *a, const void *b)
 }
 int key_desc(const void *a, const void *b) { return -key_asc(a, b); }
 
-// 1 full symmetric, 2 upper only, 3 lower only, 0 not symmetric (src/speigs_core.c:363-401)
+// Storage class of A (src/speigs_core.c:363-401): 1 full symmetric (or
+// diagonal), 2 upper triangle only, 3 lower triangle only, 0 not symmetric.
 int symmetry_kind(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax)
 {
+    const int nt = host_threads();
     // which triangles hold entries: a parallel scan (chunks stop once both
     // have been seen)
     std::atomic<int> seen{0};     // bit 0: above the diagonal, bit 1: below
-    parallel_for(n, host_threads(), 65536, [&](int64_t c, int) {
+    parallel_for(n, nt, 65536, [&](int64_t c, int) {
         if (seen.load(std::memory_order_relaxed) == 3) return;
         int f = 0;
         for (int64_t q = Ap[c]; q < Ap[c + 1]; q++) {
@@ -56,49 +58,74 @@ int symmetry_kind(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
         }
         if (f & ~seen.load(std::memory_order_relaxed)) seen.fetch_or(f, std::memory_order_relaxed);
     });
-    const bool up = seen.load() & 1, lo = seen.load() & 2;
-    if (up && !lo) return 2;
-    if (lo && !up) return 3;
-    if (!up && !lo) return 1;
-    const int64_t nnz = Ap[n];
-    std::vector<int64_t> Tp(n + 1, 0), Ti(nnz), nx(n, 0);
-    std::vector<double> Tx(nnz);
-    for (int64_t q = 0; q < nnz; q++) nx[Ai[q]]++;
-    for (int64_t i = 0; i < n; i++) { Tp[i + 1] = Tp[i] + nx[i]; nx[i] = Tp[i]; }
-    for (int64_t c = 0; c < n; c++)
-        for (int64_t q = Ap[c]; q < Ap[c + 1]; q++) {
-            int64_t d = nx[Ai[q]]++;
-            Ti[d] = c;
-            Tx[d] = Ax[q];
+    const int tri = seen.load();
+    if (tri == 1) return 2;
+    if (tri == 2) return 3;
+    if (tri == 0) return 1;
+    // Both triangles stored: the reference requires the CSC arrays of A to
+    // equal those of its transpose entry for entry (values to 1e-12 relative).
+    // The same test without forming the transpose, every column in parallel:
+    // a transpose lists each column's rows in non-decreasing order, and lists
+    // the entries (c, i) of column i -- a run of equal rows when c repeats --
+    // in the order the entries (i, c) stand in column c.  So every column must
+    // be non-decreasing, and every run of row i in column c must meet a run of
+    // row c of the same length in column i (found by binary search), with the
+    // values pairwise equal.  Applied to all runs in both columns, the run
+    // lengths also make the column counts equal the row counts.
+    std::atomic<bool> sym{true};
+    parallel_for(n, nt, 4096, [&](int64_t c, int) {
+        if (!sym.load(std::memory_order_relaxed)) return;
+        const int64_t e = Ap[c + 1];
+        for (int64_t q = Ap[c]; q < e;) {
+            const int64_t i = Ai[q];
+            int64_t r = q + 1;
+            while (r < e && Ai[r] == i) r++;
+            if (r < e && Ai[r] < i) { sym.store(false); return; }
+            const int64_t *col = Ai + Ap[i], *cend = Ai + Ap[i + 1];
+            const int64_t *lo = std::lower_bound(col, cend, c), *hi = std::upper_bound(lo, cend, c);
+            if (hi - lo != r - q) { sym.store(false); return; }
+            const double *mirror = Ax + (lo - Ai);
+            for (int64_t t = 0; t < r - q; t++) {
+                const double a = Ax[q + t], b = mirror[t];
+                const double mag = std::max(std::fabs(a), std::fabs(b));
+                if (std::fabs(a - b) > 1e-12 * (mag > 0.0 ? mag : 1.0)) { sym.store(false); return; }
+            }
+            q = r;
         }
-    for (int64_t i = 0; i <= n; i++)
-        if (Tp[i] != Ap[i]) return 0;
-    for (int64_t q = 0; q < nnz; q++) {
-        if (Ti[q] != Ai[q]) return 0;
-        double a = Ax[q], b = Tx[q], mx = std::max(std::fabs(a), std::fabs(b));
-        if (std::fabs(a - b) > 1e-12 * (mx > 0 ? mx : 1.0)) return 0;
-    }
-    return 1;
+    });
+    return sym.load() ? 1 : 0;
 }
 
+// Diagonal A (src/speigs_core.c:404-428): the eigenvalues are the diagonal
+// entries (the last stored one of a column, zero where none is stored) and the
+// eigenvectors unit vectors.  The k of smallest ('sm') or largest ('lm')
+// magnitude are selected without sorting all n, equal magnitudes in index
+// order (the order a stable sort of all n gives).
 int diag_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax, int64_t k, speigs_which which,
               double *lambda, double *V, double *resid, speigs_info *info)
 {
     std::vector<double> d(n, 0.0);
-    std::vector<KeyIdx> ord(n);
-    for (int64_t c = 0; c < n; c++)
-        for (int64_t p = Ap[c]; p < Ap[c + 1]; p++)
-            if (Ai[p] == c) d[c] = Ax[p];
-    for (int64_t i = 0; i < n; i++) ord[i] = {std::fabs(d[i]), i};
-    qsort(ord.data(), (size_t)n, sizeof(KeyIdx), which == SPEIGS_SM ? key_asc : key_desc);
+    parallel_for(n, host_threads(), 65536, [&](int64_t c, int) {
+        for (int64_t q = Ap[c]; q < Ap[c + 1]; q++)
+            if (Ai[q] == c) d[c] = Ax[q];
+    });
+    std::vector<int64_t> idx(n);
+    for (int64_t i = 0; i < n; i++) idx[i] = i;
+    const bool small = which == SPEIGS_SM;
+    std::partial_sort(idx.begin(), idx.begin() + k, idx.end(), [&](int64_t a, int64_t b) {
+        const double x = std::fabs(d[a]), y = std::fabs(d[b]);
+        if (x != y) return small ? x < y : x > y;
+        return a < b;
+    });
     for (int64_t t = 0; t < k; t++) {
-        int64_t i = ord[t].idx;
+        const int64_t i = idx[t];
         lambda[t] = d[i];
-        if (V) {
-            std::memset(V + t * n, 0, (size_t)n * sizeof(double));
-            V[t * n + i] = 1.0;
-        }
         if (resid) resid[t] = 0.0;
+        if (V) {
+            double *col = V + t * n;
+            std::fill(col, col + n, 0.0);
+            col[i] = 1.0;
+        }
     }
     info->nconv = k;
     info->flag = 0;
@@ -717,79 +744,7 @@ int speigs(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const doubl
     return speigs_ex(n, Ap, Ai, Ax, k, which, opts, nullptr, lambda, V, resid, info, nullptr);
 }
 
-// complex Hermitian through the real embedding [[R,-S],[S,R]] (src/speigs_core.c:660-768)
-int speigs_z(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const double *Axr, const double *Axi,
-             speigs_int k, speigs_which which, const speigs_opts *opts, double *lambda, double *Vr, double *Vi,
-             double *resid, speigs_info *info)
-{
-    if (!Axi) return speigs(n, Ap, Ai, Axr, k, which, opts, lambda, Vr, resid, info);
-    if (n <= 0 || k <= 0 || k > n || !Ap || !Ai || !Axr || !lambda) return SPEIGS_ERR_ARG;
-    const int64_t n2 = 2 * n, nnz = Ap[n];
-    std::vector<int64_t> Bp(n2 + 1), Bi;
-    std::vector<double> Bx;
-    Bi.reserve(4 * nnz);
-    Bx.reserve(4 * nnz);
-    for (int64_t j = 0; j < n; j++) {
-        Bp[j] = (int64_t)Bi.size();
-        for (int64_t q = Ap[j]; q < Ap[j + 1]; q++)
-            if (Axr[q] != 0.0) { Bi.push_back(Ai[q]); Bx.push_back(Axr[q]); }
-        for (int64_t q = Ap[j]; q < Ap[j + 1]; q++)
-            if (Axi[q] != 0.0) { Bi.push_back(n + Ai[q]); Bx.push_back(Axi[q]); }
-    }
-    for (int64_t j = 0; j < n; j++) {
-        Bp[n + j] = (int64_t)Bi.size();
-        for (int64_t q = Ap[j]; q < Ap[j + 1]; q++)
-            if (Axi[q] != 0.0) { Bi.push_back(Ai[q]); Bx.push_back(-Axi[q]); }
-        for (int64_t q = Ap[j]; q < Ap[j + 1]; q++)
-            if (Axr[q] != 0.0) { Bi.push_back(n + Ai[q]); Bx.push_back(Axr[q]); }
-    }
-    Bp[n2] = (int64_t)Bi.size();
-    if (Bi.empty()) { Bi.push_back(0); Bx.push_back(0.0); }
-    const int64_t kk = std::min<int64_t>(2 * k + 2, n2);
-    std::vector<double> l2(kk), r2(kk), V2((size_t)n2 * kk), ar((size_t)n * k, 0.0), ai((size_t)n * k, 0.0), wr(n), wi(n);
-    int rc = speigs(n2, Bp.data(), Bi.data(), Bx.data(), kk, which, opts, l2.data(), V2.data(), r2.data(), info);
-    if (rc) return rc;
-    int64_t nacc = 0;
-    for (int64_t c = 0; c < kk && nacc < k; c++) {
-        const double *a = V2.data() + (size_t)c * n2, *b = a + n;
-        double nn = 0.0;
-        for (int64_t i = 0; i < n; i++) nn += a[i] * a[i] + b[i] * b[i];
-        if (nn <= 0.0) continue;
-        nn = 1.0 / std::sqrt(nn);
-        for (int64_t i = 0; i < n; i++) { wr[i] = a[i] * nn; wi[i] = b[i] * nn; }
-        for (int pass = 0; pass < 2; pass++)
-            for (int64_t u = 0; u < nacc; u++) {
-                const double *ur = ar.data() + u * n, *ui = ai.data() + u * n;
-                double re = 0.0, im = 0.0;
-                for (int64_t i = 0; i < n; i++) {
-                    re += ur[i] * wr[i] + ui[i] * wi[i];
-                    im += ur[i] * wi[i] - ui[i] * wr[i];
-                }
-                for (int64_t i = 0; i < n; i++) {
-                    wr[i] -= re * ur[i] - im * ui[i];
-                    wi[i] -= re * ui[i] + im * ur[i];
-                }
-            }
-        double wn = 0.0;
-        for (int64_t i = 0; i < n; i++) wn += wr[i] * wr[i] + wi[i] * wi[i];
-        wn = std::sqrt(wn);
-        if (wn <= 0.1) continue;     // an i-multiple of an accepted direction
-        for (int64_t i = 0; i < n; i++) {
-            ar[nacc * n + i] = wr[i] / wn;
-            ai[nacc * n + i] = wi[i] / wn;
-        }
-        lambda[nacc] = l2[c];
-        if (resid) resid[nacc] = r2[c];
-        nacc++;
-    }
-    if (Vr) std::memcpy(Vr, ar.data(), (size_t)n * k * sizeof(double));
-    if (Vi) std::memcpy(Vi, ai.data(), (size_t)n * k * sizeof(double));
-    if (info) {
-        info->nconv = nacc;
-        if (nacc < k) info->flag = 1;
-    }
-    return SPEIGS_OK;
-}
+// speigs_z (complex Hermitian through the real embedding, src/speigs_core.c:660-768): hermitian.hip
 
 }  // extern "C"
 

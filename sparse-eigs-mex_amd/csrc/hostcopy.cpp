@@ -11,8 +11,9 @@
 // the host copy out of the other), so the host copies run in parallel while
 // the DMA engine stays busy.  The pinned buffers are allocated on first use
 // and kept, per device, until the process ends (2 kThreads kChunk bytes per
-// concurrent copy); a buffer whose last transfer may still be in flight
-// carries the event that ends it, and the next user waits for that event.
+// concurrent copy).  Within a copy a buffer whose transfer may still be in
+// flight carries the event that ends it; every transfer has ended before the
+// copy returns its buffers to the pool.
 #include <algorithm>
 #include <cstring>
 #include <mutex>
@@ -190,6 +191,17 @@ int staged(bool to_host, void *dst, size_t dpitch, const void *src, size_t spitc
     for (int t = 1; t < T; t++) th.emplace_back(run, t);
     run(0);
     for (auto &x : th) x.join();
+    // the last transfers of a host-to-device copy may still be in flight:
+    // wait for them here, so that no slot goes back to the pool holding an
+    // event of this call's stream (which the caller may destroy)
+    for (Slot *s : slots) {
+        const int r = wait_slot(s);
+        if (r && !rc) {
+            rc = r;
+            why[0] = last_error_text();
+            rcs[0] = r;
+        }
+    }
     release(dev, slots);
     for (int t = 0; t < T; t++)
         if (rcs[t]) {

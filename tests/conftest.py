@@ -46,3 +46,15 @@ def oracle(pkg):
 @pytest.fixture(scope="session")
 def hiplib(pkg):
     return pkg.load()
+
+
+@pytest.hookimpl(hookwrapper=True)
+def pytest_runtest_makereport(item, call):
+    """A failed test's report carries speigs_hip_last_error(): where the
+    library's most recent internal failure on this thread happened."""
+    outcome = yield
+    rep = outcome.get_result()
+    if rep.when == "call" and rep.failed:
+        mod = sys.modules.get("speigs_amd")
+        if mod is not None and getattr(mod, "_lib", None) is not None:
+            rep.sections.append(("speigs_hip_last_error", mod.last_error() or "(none recorded)"))

@@ -232,3 +232,93 @@ def test_analyze_ordering_unchanged(hiplib, N, seed, perm_sha, snode_sha, nnz_l,
     assert hashlib.sha256(perm.tobytes()).hexdigest()[:16] == perm_sha
     assert hashlib.sha256(snode.tobytes()).hexdigest()[:16] == snode_sha
     assert hi.nnz_L == nnz_l and hi.nsuper == nsuper
+
+
+def _csc_from_entries(n, entries):
+    """CSC arrays with the entries (row, col, value) kept in the given order
+    inside each column (so unsorted columns and duplicates can be built)"""
+    cols = [[] for _ in range(n)]
+    for r, c, v in entries:
+        cols[c].append((r, v))
+    Ap = np.zeros(n + 1, np.int64)
+    Ai, Ax = [], []
+    for c in range(n):
+        for r, v in cols[c]:
+            Ai.append(r)
+            Ax.append(v)
+        Ap[c + 1] = len(Ai)
+    return Ap, np.array(Ai, np.int64), np.array(Ax, float)
+
+
+def _sym_cases():
+    rng = np.random.default_rng(5)
+    n = 40
+    base = []
+    for c in range(n):
+        base.append((c, c, 4.0 + c))
+        for r in range(c + 1, n):
+            if rng.random() < 0.15:
+                v = float(rng.standard_normal())
+                base.append((r, c, v))
+                base.append((c, r, v))
+    base.sort(key=lambda e: (e[1], e[0]))
+    cases = {"symmetric": list(base)}
+    # a duplicated off-diagonal pair (both mirrors twice) and a duplicated diagonal
+    r0, c0, v0 = next(e for e in base if e[0] > e[1])
+    dup = list(base) + [(r0, c0, 0.5), (c0, r0, 0.5), (7, 7, 1.0)]
+    cases["duplicates, mirrored"] = sorted(dup, key=lambda e: (e[1], e[0]))
+    dup1 = list(base) + [(r0, c0, 0.5)]
+    cases["duplicate without mirror"] = sorted(dup1, key=lambda e: (e[1], e[0]))
+    # a symmetric pattern whose column 3 lists its rows out of order
+    uns = list(base)
+    col3 = [e for e in uns if e[1] == 3]
+    rest = [e for e in uns if e[1] != 3]
+    cases["unsorted column"] = rest[:0] + sorted(rest + col3[::-1], key=lambda e: e[1])
+    # values: within and beyond the 1e-12 relative tolerance
+    for name, eps in (("perturbed 1e-13", 1e-13), ("perturbed 1e-11", 1e-11)):
+        pert = [(r, c, v * (1 + eps) if (r == r0 and c == c0) else v) for r, c, v in base]
+        cases[name] = pert
+    cases["missing mirror"] = [e for e in base if not (e[0] == c0 and e[1] == r0)]
+    return n, cases
+
+
+def test_symmetry_check_matches_transpose_compare(hiplib, oracle):
+    """The full-storage symmetry test (runs matched by binary search, no
+    transpose formed) rejects exactly what the oracle's restatement of the
+    reference's transpose comparison (src/speigs_core.c:376-398) rejects:
+    duplicates, unsorted columns, value perturbations either side of the
+    1e-12 tolerance, a missing mirror entry.  n = 40 takes the host dense
+    short-circuit, so accepted inputs are also solved on the host."""
+    n, cases = _sym_cases()
+    want = {"symmetric": True, "duplicates, mirrored": True, "duplicate without mirror": False,
+            "unsorted column": False, "perturbed 1e-13": True, "perturbed 1e-11": False, "missing mirror": False}
+    for name, ents in cases.items():
+        A = _csc_from_entries(n, ents)
+        r = _solve(A, 3, "sm")
+        o = pkg.speigs(A, 3, "sm", lib=oracle, prefix="oracle_")
+        print(name, r.rc, o.rc)
+        assert (r.rc != -6) == want[name], name
+        assert (o.rc != -6) == want[name], name
+        if want[name]:
+            assert r.rc == 0 and o.rc == 0 and np.allclose(r.lam, o.lam, rtol=1e-12, atol=0)
+
+
+def test_diag_path_selection_order(hiplib, oracle):
+    """Diagonal matrices: the k values of smallest / largest magnitude with
+    equal magnitudes taken in index order, the unit eigenvectors of those
+    positions, a missing diagonal entry read as 0 -- the same as the oracle."""
+    d = np.array([3.0, -1.0, 1.0, 2.0, -1.0, 1.0, -3.0, 0.5])
+    n = len(d)
+    keep = np.arange(n) != 7
+    Ap = np.concatenate([[0], np.cumsum(keep)]).astype(np.int64)
+    Ai = np.arange(n, dtype=np.int64)[keep]
+    A = (Ap, Ai, d[keep])
+    dd = np.where(keep, d, 0.0)
+    for which, k in (("sm", 5), ("lm", 4)):
+        mag = np.abs(dd)
+        order = np.argsort(mag if which == "sm" else -mag, kind="stable")[:k]
+        r = _solve(A, k, which)
+        o = pkg.speigs(A, k, which, lib=oracle, prefix="oracle_")
+        assert r.rc == 0 and r.info["path"] == 0
+        assert np.array_equal(r.lam, dd[order]) and np.array_equal(o.lam, r.lam)
+        assert np.array_equal(r.V, np.eye(n)[:, order]) and np.array_equal(o.V, r.V)

@@ -302,3 +302,71 @@ def test_device_memory_reuse(hiplib, monkeypatch):
     monkeypatch.setenv("SPEIGS_HIP_CACHE_MB", "0")
     same(solve(B), rb)
     same(solve(A), ra)
+
+
+def _solve_z(Ap, Ai, Axr, Axi, k, which, opts=None):
+    print(f"[case] speigs_z n={len(Ap) - 1} k={k} which={which}", flush=True)
+    r = pkg.speigs_z(Ap, Ai, Axr, Axi, k, which, opts=opts)
+    if r.rc < 0:
+        print(f"[error] rc={r.rc} at {pkg.last_error()}", flush=True)
+    return r
+
+
+def test_ref11_hermitian(hiplib, oracle):
+    """Reference test 11 (test/test_speigs.c:331-389) on the GPU path: A = U M U*
+    with U unitary diagonal has the closed-form Laplacian spectrum; complex
+    residuals <= 1e-8 and no two returned vectors within an i-multiple.  The
+    values equal the oracle's, the returned vectors are orthonormal in C^n."""
+    r = rc.check_hermitian(_solve_z)
+    o = rc.check_hermitian(lambda *a: pkg.speigs_z(*a, lib=oracle, prefix="oracle_"))
+    assert np.abs(r.lam - o.lam).max() <= 1e-10 * np.abs(o.lam).max()
+    G = r.V.conj().T @ r.V
+    assert np.abs(G - np.eye(G.shape[0])).max() <= 1e-12
+    assert r.info["nconv"] == 5 and r.info["flag"] == 0
+
+
+@pytest.mark.parametrize("N,k,which", [(10, 8, "sm"), (9, 6, "lm")])
+def test_hermitian_multiplets(hiplib, oracle, N, k, which):
+    """k reaching into the triple eigenvalues: each complex multiplet appears
+    with its full multiplicity (the selection keeps C-linearly independent
+    directions only), values equal the closed form and the oracle, the
+    vectors of a multiplet span the closed-form eigenspace rotated by U."""
+    Ap, Ai, Ax = gen.lap3d(N, True)
+    n = N ** 3
+    i = np.arange(n, dtype=np.int64)
+    th = 0.7 * np.sin(0.37 * i) + 0.001 * i
+    col = np.repeat(np.arange(n), np.diff(Ap))
+    d = th[Ai] - th[col]
+    Ar, Aim = Ax * np.cos(d), Ax * np.sin(d)
+    r = _solve_z(Ap, Ai, Ar, Aim, k, which)
+    o = pkg.speigs_z(Ap, Ai, Ar, Aim, k, which, lib=oracle, prefix="oracle_")
+    assert r.rc == 0 and o.rc == 0 and r.info["flag"] == 0 and r.info["nconv"] == k
+    ex = gen.exact_spectrum3(N, True)
+    want = ex[:k] if which == "sm" else ex[::-1][:k]
+    assert np.abs(r.lam - want).max() <= 1e-9 * np.abs(want).max()
+    assert np.abs(r.lam - o.lam).max() <= 1e-10 * np.abs(o.lam).max()
+    Ac = np.zeros((n, n), complex)
+    Ac[Ai, col] = Ar + 1j * Aim
+    res = max(np.linalg.norm(Ac @ r.V[:, t] - r.lam[t] * r.V[:, t]) for t in range(k))
+    assert res <= 1e-8
+    G = r.V.conj().T @ r.V
+    assert np.abs(G - np.eye(k)).max() <= 1e-12
+    # the eigenspace of A for value lam is U times M's: compare projectors
+    U = np.exp(1j * th)
+    for lam in np.unique(np.round(r.lam, 9)):
+        sel = np.abs(r.lam - lam) <= 1e-8
+        Mb = rc.dirichlet_eigenspace(N, lam)
+        if Mb.shape[1] != sel.sum():
+            continue                         # a multiplet cut at k
+        Eb = U[:, None] * Mb
+        Q, _ = np.linalg.qr(Eb)
+        P = r.V[:, sel]
+        assert np.linalg.norm(P - Q @ (Q.conj().T @ P)) <= 1e-8
+
+
+def test_hermitian_real_input(hiplib):
+    """speigs_z without an imaginary part is speigs."""
+    A = gen.weighted_lap3d(12, seed=2, upper=True)
+    r = _solve_z(*A, None, 4, "sm")
+    s = _solve(A, 4, "sm")
+    assert r.rc == 0 and np.array_equal(r.lam, s.lam)
