@@ -12,12 +12,15 @@
 // plus Y and X (n rows each).  Front s (c columns, r update rows), forward:
 //     W  = [P_c; 0] + sum of the children's U rows at their places in s
 //          (a precomputed pull list per front row, children in order, so the
-//          sums are deterministic and need no atomics; one launch per level
-//          assembles W1 into T and W2 into U for every front with children
-//          or outside the wave-per-front kernels)
+//          sums are deterministic and need no atomics).  Each W row is summed
+//          where it is read: W2 rows in the epilogue that writes U_s, W1 rows
+//          in the loads of the fronts whose columns fit one tile; only the
+//          W1 of wider fronts (read by several output tiles) is assembled
+//          into T by one launch per level
 //     Y  = L11^-1 W1,   U_s = W2 - L21 Y
 // backward:
-//     Z  = Y - L21^T X_R,   X_c = L11^-T Z      (X_R = X at the update rows)
+//     Z  = Y - L21^T X_R,   X_c = L11^-T Z      (X_R = X at the update rows,
+//          read by index inside the product)
 // All four products run on v_mfma_f32_16x16x4f32 with the vector rows as the
 // second operand: lane l of a product holds column l%16 of the right-hand
 // sides, so every vector access is 16 consecutive floats.
@@ -80,6 +83,52 @@ __device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c)
 // compiler from moving the accesses across this point.  No memory fence: it
 // would also wait for the global loads kept in flight across it.
 __device__ __forceinline__ void wave_lds_sync() { __builtin_amdgcn_wave_barrier(); }
+
+// Row rid of the fronts' right-hand side W (rows of front s from frow[s]):
+// the sum, in pull-list order, of its P row (rows < c) and of the children's
+// update rows that land on it, for the W1 rows of the fronts whose columns
+// fit one tile (read by one product task only, so they are summed in that
+// task's loads rather than assembled).  The list bounds, the first kPull
+// indices and those rows are in flight together; longer lists continue one
+// entry at a time.  8 consecutive elements from col (two 16-byte loads per
+// list entry).
+constexpr int kPull = 4;
+
+__device__ __forceinline__ void pull_sum8(const RsArgs &a, const float *__restrict__ buf, int64_t rid, int col,
+                                          float4 (&o)[2])
+{
+    const int64_t q0 = a.lptr[rid], q1 = a.lptr[rid + 1];
+    int32_t id[kPull];
+#pragma unroll
+    for (int u = 0; u < kPull; u++) id[u] = q0 + u < q1 ? a.lst[q0 + u] : -1;
+    float4 u0[kPull], u1[kPull];
+#pragma unroll
+    for (int u = 0; u < kPull; u++) {
+        const float *src = buf + (int64_t)(id[u] < 0 ? 0 : id[u]) * RW + col;
+        u0[u] = id[u] < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : *reinterpret_cast<const float4 *>(src);
+        u1[u] = id[u] < 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : *reinterpret_cast<const float4 *>(src + 4);
+    }
+    float4 x0 = make_float4(0.f, 0.f, 0.f, 0.f), x1 = x0;
+    auto add = [&](const float4 &p, const float4 &q) {
+        x0.x += p.x;
+        x0.y += p.y;
+        x0.z += p.z;
+        x0.w += p.w;
+        x1.x += q.x;
+        x1.y += q.y;
+        x1.z += q.z;
+        x1.w += q.w;
+    };
+#pragma unroll
+    for (int u = 0; u < kPull; u++)
+        if (id[u] >= 0) add(u0[u], u1[u]);
+    for (int64_t q = q0 + kPull; q < q1; q++) {
+        const float *src = buf + (int64_t)a.lst[q] * RW + col;
+        add(*reinterpret_cast<const float4 *>(src), *reinterpret_cast<const float4 *>(src + 4));
+    }
+    o[0] = x0;
+    o[1] = x1;
+}
 
 // ---- tiny fronts (c <= 64, r <= 64): one wave per front ----------------------
 // Operand layout (MFMA step (kq, t), lane l = 16 g + lc): the product index is
@@ -284,7 +333,9 @@ __global__ void __launch_bounds__(256) k_rs_tiny_bwd(const T *__restrict__ pool,
 }
 
 // ---- tile products of the other fronts ---------------------------------------
-// W of the level's fronts: W1 rows to T (row t0 + c0 + i), W2 rows to U
+// W of the level's fronts with children: W1 rows to T (row t0 + c0 + i), W2
+// rows to U (the plan leaves out what is read in place: leaves, and the W1 of
+// fronts whose product task sums it from the pull lists)
 __global__ void __launch_bounds__(256) k_rs_assemble(const RsRow *__restrict__ tasks, RsArgs a, float *__restrict__ buf)
 {
     const RsRow t = tasks[blockIdx.x];
@@ -316,21 +367,6 @@ __global__ void __launch_bounds__(256) k_rs_assemble(const RsRow *__restrict__ t
     }
     const int64_t dst = i < c ? a.t0 + a.col0[s] + i : a.n + a.uoff[s] + (i - c);
     *(f32x4 *)(buf + dst * RW + 4 * q4) = acc;
-}
-
-// backward: X at the update rows of the level's larger fronts, gathered once
-// into their U rows (free after the forward sweep), so the transposed product
-// streams them like the forward one streams Y
-__global__ void __launch_bounds__(256) k_rs_gather(const RsRow *__restrict__ tasks, RsArgs a,
-                                                   const float *__restrict__ Xg, float *__restrict__ buf)
-{
-    const RsRow t = tasks[blockIdx.x];
-    const int s = t.s;
-    const int32_t r = a.Fs[s] - a.ncol[s];
-    const int i = t.I * 32 + (threadIdx.x >> 3), q4 = threadIdx.x & 7;
-    if (i >= r) return;
-    const int64_t g = a.rows[a.rptr[s] + i];
-    *(f32x4 *)(buf + (a.n + a.uoff[s] + i) * RW + 4 * q4) = *(const f32x4 *)(Xg + g * RW + 4 * q4);
 }
 
 // the four tile products (output rows of one 64-row tile, all right-hand sides):
@@ -383,19 +419,28 @@ __device__ __forceinline__ void rs_load_a(const T *__restrict__ L, int F, int c,
 }
 
 // the K tile's vector rows (64 x 32 floats), loaded by the whole workgroup:
-// thread e takes row e / 4, floats 8 (e % 4) .. +7
+// thread e takes row e / 4, floats 8 (e % 4) .. +7.  Mode 0: W1 from T, or
+// summed from the pull lists when the front's columns are one tile (pull; its
+// W1 rows are read by this task alone); mode 1: Y; mode 2: X at the update
+// rows, read by index; mode 3: Z from T.
 template <int MODE>
-__device__ __forceinline__ void rs_load_b(const RsArgs &a, int64_t c0, int64_t ub, int K, int kn,
-                                          const float *__restrict__ buf, const float *__restrict__ Yg, float4 (&bv)[2])
+__device__ __forceinline__ void rs_load_b(const RsArgs &a, int64_t c0, int64_t fr, const int32_t *__restrict__ R,
+                                          bool pull, bool leaf, int K, int kn, const float *__restrict__ buf,
+                                          const float *__restrict__ Yg, const float *__restrict__ Xg, float4 (&bv)[2])
 {
     const int row = threadIdx.x >> 2, col = 8 * (threadIdx.x & 3);
     const int64_t kk = (int64_t)K * RTILE + row;
     bv[0] = make_float4(0.f, 0.f, 0.f, 0.f);
     bv[1] = bv[0];
     if (row < kn) {
-        const float *src = (MODE == 0 || MODE == 3) ? buf + (a.t0 + c0 + kk) * RW
+        if (MODE == 0 && pull) {
+            pull_sum8(a, buf, fr + kk, col, bv);
+            return;
+        }
+        const float *src = (MODE == 0 && leaf) ? buf + (c0 + kk) * RW
+                         : (MODE == 0 || MODE == 3) ? buf + (a.t0 + c0 + kk) * RW
                          : MODE == 1                ? Yg + (c0 + kk) * RW
-                                                    : buf + (ub + kk) * RW;     // X_R, gathered into the U rows
+                                                    : Xg + (int64_t)R[kk] * RW;
         bv[0] = *reinterpret_cast<const float4 *>(src + col);
         bv[1] = *reinterpret_cast<const float4 *>(src + col + 4);
     }
@@ -418,15 +463,17 @@ __global__ void __launch_bounds__(256) k_rs_part(const T *__restrict__ pool, con
     // tiles and takes part in the barriers
     const bool active = 16 * wv < on;
     const T *L = pool + a.foff[s];
-    const int64_t c0 = a.col0[s];
-    const int64_t ub = a.n + a.uoff[s];      // mode 2: X at the update rows
+    const int64_t c0 = a.col0[s], fr = a.frow[s];
+    const int32_t *R = a.rows + a.rptr[s];   // mode 2: the update rows' indices
+    const bool leaf = a.chptr[s + 1] == a.chptr[s];   // W1 = P rows, W2 = 0
+    const bool pull = !leaf && c <= RTILE;            // mode 0: W1 from the pull lists
     const int kdim = MODE == 2 ? r : c;
     f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
     float av[16];
     float4 bv[2];
     auto kn_of = [&](int K) { return min(RTILE, kdim - K * RTILE); };
     if (active) rs_load_a<T, MODE>(L, F, c, I, tk.k0, kn_of(tk.k0), orow0, on, wv, g, lc, lane, av);
-    rs_load_b<MODE>(a, c0, ub, tk.k0, kn_of(tk.k0), buf, Yg, bv);
+    rs_load_b<MODE>(a, c0, fr, R, pull, leaf, tk.k0, kn_of(tk.k0), buf, Yg, Xg, bv);
     {
         float *dst = &Bs[0][threadIdx.x >> 2][8 * (threadIdx.x & 3)];
         *reinterpret_cast<float4 *>(dst) = bv[0];
@@ -441,7 +488,7 @@ __global__ void __launch_bounds__(256) k_rs_part(const T *__restrict__ pool, con
         const bool more = K + 1 < tk.k1;
         if (more) {     // next tile's loads in flight during this tile's products
             if (active) rs_load_a<T, MODE>(L, F, c, I, K + 1, kn_of(K + 1), orow0, on, wv, g, lc, lane, av);
-            rs_load_b<MODE>(a, c0, ub, K + 1, kn_of(K + 1), buf, Yg, bv);
+            rs_load_b<MODE>(a, c0, fr, R, pull, leaf, K + 1, kn_of(K + 1), buf, Yg, Xg, bv);
         }
         if (active) {
             if (MODE == 2 || MODE == 3) {
@@ -486,7 +533,7 @@ __global__ void __launch_bounds__(256) k_rs_part(const T *__restrict__ pool, con
             const int row = 16 * wv + 4 * g + v;
             o0[v] = 0.f;
             o1[v] = 0.f;
-            if ((MODE == 1 || MODE == 2) && row < on) {
+            if (((MODE == 1 && !leaf) || MODE == 2) && row < on) {
                 o0[v] = src[row * RW + lc];
                 o1[v] = src[row * RW + 16 + lc];
             }
@@ -522,6 +569,7 @@ __global__ void __launch_bounds__(256) k_rs_fin(const RsFin *__restrict__ tasks,
     const int orow0 = t.I * RTILE;
     const int on = min(RTILE, (MODE == 1 ? F - c : c) - orow0);
     const int64_t c0 = a.col0[s];
+    const bool leaf = a.chptr[s + 1] == a.chptr[s];   // W2 = 0
     // partial sums in a fixed order, each step's PER loads independent
     float sum[PER];
 #pragma unroll
@@ -538,7 +586,7 @@ __global__ void __launch_bounds__(256) k_rs_fin(const RsFin *__restrict__ tasks,
 #pragma unroll
     for (int u = 0; u < PER; u++) {
         const int e = threadIdx.x + 256 * u;
-        o[u] = ((MODE == 1 || MODE == 2) && (e >> 5) < on) ? src[e] : 0.f;
+        o[u] = (((MODE == 1 && !leaf) || MODE == 2) && (e >> 5) < on) ? src[e] : 0.f;
     }
 #pragma unroll
     for (int u = 0; u < PER; u++) {
@@ -588,7 +636,6 @@ struct RsLevel {
     int64_t sm_off = 0, sm_cnt = 0;          // tiny fronts, by class (tiny_class)
     int64_t tc_off[9] = {}, tc_cnt[9] = {};
     int64_t as_off = 0, as_cnt = 0;          // assembly chunks (every front of the level)
-    int64_t gx_off = 0, gx_cnt = 0;          // backward gathers of X_R (the larger fronts)
     int64_t pt_off[4] = {0, 0, 0, 0}, pt_cnt[4] = {0, 0, 0, 0};
     int64_t ft_off[4] = {0, 0, 0, 0}, ft_cnt[4] = {0, 0, 0, 0};
 };
@@ -596,7 +643,7 @@ struct RsLevel {
 // host-side plan of the solve (task lists, pull lists), uploaded at first use
 struct RsHost {
     std::vector<int32_t> small, lst;
-    std::vector<RsRow> asmt, gxt;
+    std::vector<RsRow> asmt;
     std::vector<RsPart> pt;
     std::vector<RsFin> ft;
     std::vector<int64_t> frow, uoff, lptr;
@@ -616,7 +663,7 @@ struct RsState {
     DevMem mem;
     std::vector<RsLevel> lev;
     int32_t *small = nullptr;
-    RsRow *asmt = nullptr, *gxt = nullptr;
+    RsRow *asmt = nullptr;
     RsPart *pt = nullptr;
     RsFin *ft = nullptr;
     float *part = nullptr, *buf = nullptr, *Yg = nullptr, *Xg = nullptr;
@@ -726,7 +773,7 @@ int rs_plan(const Factor *F, RsState &B)
     std::vector<std::vector<int64_t>> bylevel(S.nlevels);
     for (int64_t s = 0; s < ns; s++) bylevel[S.level[s]].push_back(s);
     std::vector<int32_t> &small = H.small;
-    std::vector<RsRow> &asmt = H.asmt, &gxt = H.gxt;
+    std::vector<RsRow> &asmt = H.asmt;
     std::vector<RsPart> &pt = H.pt;
     std::vector<RsFin> &ft = H.ft;
     int64_t &maxpart = H.maxpart;
@@ -747,19 +794,19 @@ int rs_plan(const Factor *F, RsState &B)
             small.insert(small.end(), cls[q].begin(), cls[q].end());
         }
         P.sm_cnt = (int64_t)small.size() - P.sm_off;
+        // assembled: every row of a front with children, except the W1 rows of
+        // the non-tiny fronts whose columns fit one tile (their product task
+        // sums them from the pull lists).  A leaf is read in place: its W1 is
+        // its P rows, its W2 zero.
         P.as_off = (int64_t)asmt.size();
         for (int64_t s : bylevel[L]) {
-            const int64_t c = S.c1[s] - S.c0[s];
-            if (is_tiny(c, S.F[s] - c) && S.chptr[s + 1] == S.chptr[s]) continue;   // leaf: read in place
-            for (int64_t I = 0; I * 32 < S.F[s]; I++) asmt.push_back({(int32_t)s, (int32_t)I});
+            const int64_t c = S.c1[s] - S.c0[s], r = S.F[s] - c;
+            const bool tiny = is_tiny(c, r), leaf = S.chptr[s + 1] == S.chptr[s];
+            if (leaf) continue;
+            const int64_t i0 = (!tiny && c <= RTILE) ? c : 0;      // first assembled row
+            for (int64_t I = i0 / 32; I * 32 < S.F[s]; I++) asmt.push_back({(int32_t)s, (int32_t)I});
         }
         P.as_cnt = (int64_t)asmt.size() - P.as_off;
-        P.gx_off = (int64_t)gxt.size();
-        for (int64_t s : big) {
-            const int64_t r = S.F[s] - (S.c1[s] - S.c0[s]);
-            for (int64_t I = 0; I * 32 < r; I++) gxt.push_back({(int32_t)s, (int32_t)I});
-        }
-        P.gx_cnt = (int64_t)gxt.size() - P.gx_off;
         for (int mode = 0; mode < 4; mode++) {
             const int64_t p0 = (int64_t)pt.size(), f0 = (int64_t)ft.size();
             plan_parts(S, big, mode, pt, ft, p0);
@@ -772,7 +819,6 @@ int rs_plan(const Factor *F, RsState &B)
     }
     if (small.empty()) small.push_back(0);
     if (asmt.empty()) asmt.push_back({0, 0});
-    if (gxt.empty()) gxt.push_back({0, 0});
     if (pt.empty()) pt.push_back({0, 0, 0, 0, 0, 0});
     if (ft.empty()) ft.push_back({0, 0, 0, 0, 0});
     return 0;
@@ -789,7 +835,7 @@ int rs_upload(Factor *F, RsState &B, hipStream_t st)
         if (e) return e;
         return copy_to_device(*dst, v.data(), v.size() * sizeof(v[0]), st);
     };
-    if ((rc = up(&B.small, H.small)) || (rc = up(&B.asmt, H.asmt)) || (rc = up(&B.gxt, H.gxt)) ||
+    if ((rc = up(&B.small, H.small)) || (rc = up(&B.asmt, H.asmt)) ||
         (rc = up(&B.pt, H.pt)) || (rc = up(&B.ft, H.ft)) || (rc = up(&B.frow, H.frow)) || (rc = up(&B.uoff, H.uoff)) ||
         (rc = up(&B.lptr, H.lptr)) || (rc = up(&B.lst, H.lst)))
         return rc;
@@ -872,7 +918,6 @@ void rs_run(Factor *F, RsState &B, const double *Bin, int64_t brs, int64_t bcs, 
     }
     for (int32_t L = S.nlevels - 1; L >= 0; L--) {
         const RsLevel &P = B.lev[L];
-        if (P.gx_cnt) k_rs_gather<<<dim3((unsigned)P.gx_cnt), blk, 0, st>>>(B.gxt + P.gx_off, a, B.Xg, B.buf);
         parts(2, P);
         parts(3, P);
         if (P.sm_cnt) tiny_launch<T, false>(P, B, pool, a, st);
