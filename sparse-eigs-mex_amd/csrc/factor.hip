@@ -20,6 +20,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <string>
+#include <thread>
 
 #include "factor_impl.h"
 #include "mfma.h"
@@ -1387,6 +1389,95 @@ int upload(Factor *F, D **dst, const std::vector<D, Al> &src)
     return 0;
 }
 
+// The plan of the blocked single-vector solves (the Lanczos paths): built
+// from the symbolic structure and uploaded once per factor.
+int single_solve_plan(Factor *F)
+{
+    if (F->plan_ready) return 0;
+    const Symbolic &S = F->S;
+    const int64_t ns = S.ns;
+    hipStream_t st = F->st;
+    int rc = 0;
+    double tp = wall();
+    std::vector<std::vector<int64_t>> bylevel(S.nlevels);
+    for (int64_t s = 0; s < ns; s++) bylevel[S.level[s]].push_back(s);
+    // blocked solve plans: forward row blocks in increasing order per front,
+    // backward panels in decreasing order per front (ticket = dependency order)
+    std::vector<EaTask> sea;
+    std::vector<BlkTask> fblk, bblk;
+    std::vector<int64_t> panbase(ns + 1, 0);
+    for (int64_t s2 = 0; s2 < ns; s2++) panbase[s2 + 1] = panbase[s2] + (S.c1[s2] - S.c0[s2] + 63) / 64;
+    for (int32_t L = 0; L < S.nlevels; L++) {
+        LevelPlan &P = F->plan[L];
+        int64_t maxch = 0;
+        for (int64_t s2 : bylevel[L]) maxch = std::max(maxch, S.chptr[s2 + 1] - S.chptr[s2]);
+        for (int64_t o2 = 0; o2 < maxch; o2++) {
+            P.sea_off.push_back((int64_t)sea.size());
+            for (int64_t s2 : bylevel[L])
+                if (S.chptr[s2] + o2 < S.chptr[s2 + 1])
+                    sea.push_back({(int32_t)S.chl[S.chptr[s2] + o2], (int32_t)s2, 0, 0});
+            P.sea_cnt.push_back((int64_t)sea.size() - P.sea_off.back());
+        }
+        P.fb_off = (int64_t)fblk.size();
+        for (int64_t s2 : bylevel[L])
+            for (int64_t b = 0; b * 64 < S.F[s2]; b++) fblk.push_back({(int32_t)s2, (int32_t)b});
+        P.fb_cnt = (int64_t)fblk.size() - P.fb_off;
+        P.bb_off = (int64_t)bblk.size();
+        for (int64_t s2 : bylevel[L])
+            for (int64_t pp = (S.c1[s2] - S.c0[s2] + 63) / 64 - 1; pp >= 0; pp--) bblk.push_back({(int32_t)s2, (int32_t)pp});
+        P.bb_cnt = (int64_t)bblk.size() - P.bb_off;
+    }
+    std::vector<PcbTask> fin;
+    for (int32_t L = 0; L < S.nlevels; L++) {
+        LevelPlan &P = F->plan[L];
+        P.fin_off = (int64_t)fin.size();
+        for (int64_t s2 : bylevel[L])
+            for (int64_t cb = 0; cb * 256 < S.F[s2]; cb++) fin.push_back({(int32_t)s2, (int32_t)cb});
+        P.fin_cnt = (int64_t)fin.size() - P.fin_off;
+    }
+    // backward update-row chunks and their partial-sum slots (reused across levels)
+    std::vector<UpdTask> updv;
+    std::vector<int64_t> partoff(ns + 1, 0);
+    int64_t partmax = 1;
+    for (int32_t L = 0; L < S.nlevels; L++) {
+        LevelPlan &P = F->plan[L];
+        P.ub_off = (int64_t)updv.size();
+        int64_t off = 0;
+        for (int64_t s2 : bylevel[L]) {
+            const int64_t c = S.c1[s2] - S.c0[s2], r = S.F[s2] - c;
+            const int64_t npan = (c + 63) / 64, nch = (r + 511) / 512;
+            partoff[s2] = off;
+            off += npan * nch * 64;
+            for (int64_t pp = 0; pp < npan; pp++)
+                for (int64_t j = 0; j < nch; j++) updv.push_back({(int32_t)s2, (int32_t)pp, (int32_t)j, 0});
+        }
+        partmax = std::max(partmax, off);
+        P.ub_cnt = (int64_t)updv.size() - P.ub_off;
+    }
+    if (updv.empty()) updv.push_back({0, 0, 0, 0});
+    if ((rc = upload(F, &F->upd, updv))) return rc;
+    if ((rc = upload(F, &F->partoff, partoff))) return rc;
+    if ((rc = F->mem.alloc(&F->bpart, (size_t)partmax))) return rc;
+    if ((rc = F->mem.alloc(&F->tick_f, (size_t)ns + 1))) return rc;
+    if ((rc = F->mem.alloc(&F->tick_b, (size_t)ns + 1))) return rc;
+    SEHIP_CHECK(hipMemsetAsync(F->tick_f, 0, ((size_t)ns + 1) * sizeof(unsigned), st));
+    SEHIP_CHECK(hipMemsetAsync(F->tick_b, 0, ((size_t)ns + 1) * sizeof(unsigned), st));
+    if (fin.empty()) fin.push_back({0, 0});
+    if ((rc = upload(F, &F->fin, fin))) return rc;
+    if (sea.empty()) sea.push_back({0, 0, 0, 0});
+    if ((rc = upload(F, &F->sea, sea))) return rc;
+    if ((rc = upload(F, &F->fblk, fblk))) return rc;
+    if ((rc = upload(F, &F->bblk, bblk))) return rc;
+    if ((rc = upload(F, &F->panbase, panbase))) return rc;
+    if ((rc = F->mem.alloc(&F->fflag, (size_t)panbase[ns] + 1))) return rc;
+    if ((rc = F->mem.alloc(&F->bflag, (size_t)panbase[ns] + 1))) return rc;
+    SEHIP_CHECK(hipMemsetAsync(F->fflag, 0, ((size_t)panbase[ns] + 1) * sizeof(int), st));
+    SEHIP_CHECK(hipMemsetAsync(F->bflag, 0, ((size_t)panbase[ns] + 1) * sizeof(int), st));
+    trace_phase("single-vector solve plan", tp);
+    F->plan_ready = true;
+    return 0;
+}
+
 }  // namespace
 
 int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *Ax, const FactorOptions &o,
@@ -1407,6 +1498,37 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     if (const char *e = getenv("SPEIGS_HIP_POTRF_WAVES")) F->potrf_waves = atoi(e) == 1 ? 1 : 4;
     // test knob: every bounded wait of the blocked single-vector solve gives up
     if (const char *e = getenv("SPEIGS_HIP_WAIT_TEST")) F->wait_test = atoi(e) == 1;
+    // A's CSC arrays do not depend on the ordering: a helper thread uploads
+    // them while this one orders (the top of the dissection leaves cores free)
+    int dev = 0;
+    SEHIP_CHECK(hipGetDevice(&dev));
+    {
+        int r = 0;
+        if ((r = F->mem.alloc(&F->Ax, std::max<size_t>((size_t)Ap[n], 1))) ||
+            (r = F->mem.alloc(&F->Apd, (size_t)n + 1)) || (r = F->mem.alloc(&F->Aid, std::max<size_t>((size_t)Ap[n], 1))))
+            return r;
+    }
+    int up_rc = 0;
+    std::string up_msg;
+    std::thread up_th([&] {
+        auto run = [&]() -> int {
+            SEHIP_CHECK(hipSetDevice(dev));
+            int r = 0;
+            if (Ap[n] > 0 && ((r = copy_to_device(F->Ax, Ax, (size_t)Ap[n] * sizeof(double), st)) ||
+                              (r = copy_to_device(F->Aid, Ai, (size_t)Ap[n] * sizeof(int64_t), st))))
+                return r;
+            return copy_to_device(F->Apd, Ap, ((size_t)n + 1) * sizeof(int64_t), st);
+        };
+        up_rc = run();
+        if (up_rc) up_msg = last_error_text();
+    });
+    struct Join {
+        std::thread &t;
+        ~Join()
+        {
+            if (t.joinable()) t.join();
+        }
+    } join_up{up_th};
     SymbolicOptions so;
     int rc = symbolic_analyze(n, Ap, Ai, so, &F->S);
     if (rc) return rc;
@@ -1442,9 +1564,6 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
         if (r == 0 && count > 0) early.push_back({(void *)*d, (const void *)h, count * sizeof(**d)});
         return r;
     };
-    if ((rc = stage(&F->Ax, Ax, (size_t)Ap[n]))) return rc;
-    if ((rc = stage(&F->Apd, Ap, (size_t)n + 1))) return rc;
-    if ((rc = stage(&F->Aid, Ai, (size_t)Ap[n]))) return rc;
     if ((rc = stage(&F->iperm, S.iperm.data(), S.iperm.size()))) return rc;
     if ((rc = stage(&F->foff, S.foff.data(), S.foff.size()))) return rc;
     if ((rc = stage(&F->wsoff, wsoff.data(), wsoff.size()))) return rc;
@@ -1458,6 +1577,11 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     if ((rc = stage(&F->rel, S.rel.data(), S.rel.size()))) return rc;
     if ((rc = stage(&F->chl, chl32.data(), chl32.size()))) return rc;
     if ((rc = stage(&F->perm, S.perm.data(), S.perm.size()))) return rc;
+    up_th.join();
+    if (up_rc) {
+        note_error(up_msg.c_str());
+        return up_rc;
+    }
     for (const Early &e : early)
         if ((rc = copy_to_device(e.d, e.h, e.bytes, st))) return rc;
     // the supernode of each column and A's diagonal, once
@@ -1596,33 +1720,6 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
             fprintf(stderr, "[speigs_hip] level %2d: %6zu fronts, sum c %8lld, sum r %9lld, max F %6lld\n", (int)L,
                     bylevel[L].size(), (long long)sc, (long long)sr, (long long)mf);
         }
-    // blocked solve plans: forward row blocks in increasing order per front,
-    // backward panels in decreasing order per front (ticket = dependency order)
-    std::vector<EaTask> sea;
-    std::vector<BlkTask> fblk, bblk;
-    std::vector<int64_t> panbase(ns + 1, 0);
-    for (int64_t s2 = 0; s2 < ns; s2++) panbase[s2 + 1] = panbase[s2] + (S.c1[s2] - S.c0[s2] + 63) / 64;
-    for (int32_t L = 0; L < S.nlevels; L++) {
-        LevelPlan &P = F->plan[L];
-        int64_t maxch = 0;
-        for (int64_t s2 : bylevel[L]) maxch = std::max(maxch, S.chptr[s2 + 1] - S.chptr[s2]);
-        for (int64_t o2 = 0; o2 < maxch; o2++) {
-            P.sea_off.push_back((int64_t)sea.size());
-            for (int64_t s2 : bylevel[L])
-                if (S.chptr[s2] + o2 < S.chptr[s2 + 1])
-                    sea.push_back({(int32_t)S.chl[S.chptr[s2] + o2], (int32_t)s2, 0, 0});
-            P.sea_cnt.push_back((int64_t)sea.size() - P.sea_off.back());
-        }
-        P.fb_off = (int64_t)fblk.size();
-        for (int64_t s2 : bylevel[L])
-            for (int64_t b = 0; b * 64 < S.F[s2]; b++) fblk.push_back({(int32_t)s2, (int32_t)b});
-        P.fb_cnt = (int64_t)fblk.size() - P.fb_off;
-        P.bb_off = (int64_t)bblk.size();
-        for (int64_t s2 : bylevel[L])
-            for (int64_t pp = (S.c1[s2] - S.c0[s2] + 63) / 64 - 1; pp >= 0; pp--) bblk.push_back({(int32_t)s2, (int32_t)pp});
-        P.bb_cnt = (int64_t)bblk.size() - P.bb_off;
-    }
-    std::vector<PcbTask> fin;
     std::vector<EaBlk> eab;
     for (int32_t L = 0; L < S.nlevels; L++) {
         LevelPlan &P = F->plan[L];
@@ -1633,54 +1730,15 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
                     for (int64_t rb = cb / 4; rb * 256 < S.F[s2]; rb++)   // the lower part: rows >= 64 cb
                         eab.push_back({(int32_t)s2, (int32_t)cb, (int32_t)rb, 0});
         P.pcb_cnt = (int64_t)eab.size() - P.pcb_off;
-        P.fin_off = (int64_t)fin.size();
-        for (int64_t s2 : bylevel[L])
-            for (int64_t cb = 0; cb * 256 < S.F[s2]; cb++) fin.push_back({(int32_t)s2, (int32_t)cb});
-        P.fin_cnt = (int64_t)fin.size() - P.fin_off;
     }
-    // backward update-row chunks and their partial-sum slots (reused across levels)
-    std::vector<UpdTask> updv;
-    std::vector<int64_t> partoff(ns + 1, 0);
-    int64_t partmax = 1;
-    for (int32_t L = 0; L < S.nlevels; L++) {
-        LevelPlan &P = F->plan[L];
-        P.ub_off = (int64_t)updv.size();
-        int64_t off = 0;
-        for (int64_t s2 : bylevel[L]) {
-            const int64_t c = S.c1[s2] - S.c0[s2], r = S.F[s2] - c;
-            const int64_t npan = (c + 63) / 64, nch = (r + 511) / 512;
-            partoff[s2] = off;
-            off += npan * nch * 64;
-            for (int64_t pp = 0; pp < npan; pp++)
-                for (int64_t j = 0; j < nch; j++) updv.push_back({(int32_t)s2, (int32_t)pp, (int32_t)j, 0});
-        }
-        partmax = std::max(partmax, off);
-        P.ub_cnt = (int64_t)updv.size() - P.ub_off;
-    }
-    trace_phase("solve task lists", tp);
-    if (updv.empty()) updv.push_back({0, 0, 0, 0});
-    if ((rc = upload(F.get(), &F->upd, updv))) return rc;
-    if ((rc = upload(F.get(), &F->partoff, partoff))) return rc;
-    if ((rc = F->mem.alloc(&F->bpart, (size_t)partmax))) return rc;
-    if ((rc = F->mem.alloc(&F->tick_f, (size_t)ns + 1))) return rc;
-    if ((rc = F->mem.alloc(&F->tick_b, (size_t)ns + 1))) return rc;
-    SEHIP_CHECK(hipMemsetAsync(F->tick_f, 0, ((size_t)ns + 1) * sizeof(unsigned), st));
-    SEHIP_CHECK(hipMemsetAsync(F->tick_b, 0, ((size_t)ns + 1) * sizeof(unsigned), st));
     if (eab.empty()) eab.push_back({0, 0, 0, 0});
-    if (fin.empty()) fin.push_back({0, 0});
     if ((rc = upload(F.get(), &F->eab, eab))) return rc;
-    if ((rc = upload(F.get(), &F->fin, fin))) return rc;
-    if (sea.empty()) sea.push_back({0, 0, 0, 0});
     if (ea.empty()) ea.push_back({0, 0, 0, 0});
     if (tiles.empty()) tiles.push_back({0, 0, 0, 0});
-    if ((rc = upload(F.get(), &F->sea, sea))) return rc;
-    if ((rc = upload(F.get(), &F->fblk, fblk))) return rc;
-    if ((rc = upload(F.get(), &F->bblk, bblk))) return rc;
-    if ((rc = upload(F.get(), &F->panbase, panbase))) return rc;
-    if ((rc = F->mem.alloc(&F->fflag, (size_t)panbase[ns] + 1))) return rc;
-    if ((rc = F->mem.alloc(&F->bflag, (size_t)panbase[ns] + 1))) return rc;
-    SEHIP_CHECK(hipMemsetAsync(F->fflag, 0, ((size_t)panbase[ns] + 1) * sizeof(int), st));
-    SEHIP_CHECK(hipMemsetAsync(F->bflag, 0, ((size_t)panbase[ns] + 1) * sizeof(int), st));
+    // the single-vector solves' plan: now, unless the factor is the block
+    // eigensolver's (whose solves have plans of their own); then at the first
+    // factor_apply, if any
+    if (o.precision != SPEIGS_HIP_FP32_BLOCK && (rc = single_solve_plan(F.get()))) return rc;
     if ((rc = F->mem.alloc(&F->counters, (size_t)2 * S.nlevels + 2))) return rc;
     if ((rc = F->mem.alloc(&F->err, 1))) return rc;
     SEHIP_CHECK(hipMemsetAsync(F->err, 0, sizeof(int), st));
@@ -1759,6 +1817,7 @@ int factor_apply(Factor *F, const double *x, double *y)
     if (!F->factored) return SPEIGS_ERR_FACTOR;
     int rc0 = factor_check(F);   // a bounded wait gave up in an earlier solve
     if (rc0) return rc0;
+    if ((rc0 = single_solve_plan(F))) return rc0;
     if (F->solve_mode == 2 && !F->inverted) {
         const int rc = factor_invert_l11(F);
         if (rc) return rc;

@@ -95,6 +95,7 @@ struct Factor {
     int potrf_waves = 4;             // register POTRF on 4 waves (columns split) or 1
     int solve_mode = 1;               // 0: one workgroup per front, 1: blocked multi-workgroup
     bool wait_test = false;           // test knob: the blocked solve's waits all give up
+    bool plan_ready = false;          // the single-vector solves' plan is on the device
     EaTask *sea = nullptr;
     PcbTask *fin = nullptr;
     EaBlk *eab = nullptr;             // factor extend-add blocks

@@ -95,7 +95,7 @@ typedef std::vector<std::unique_ptr<NdNode>> NdList;
 class Dissection {
 public:
     Dissection(const Graph &g, int64_t leaf, std::vector<int64_t> &perm, int threads)
-        : g_(g), leaf_(leaf), perm_(perm), rid_(g.n, 1), lev_(g.n, 0), free_threads_(threads - 1) {}
+        : g_(g), leaf_(leaf), perm_(perm), rid_(g.n, 1), lev_(g.n, 0), lev2_(g.n, 0), free_threads_(threads - 1) {}
 
     std::vector<int64_t> c0, c1;
     std::vector<std::vector<int64_t>> kids;
@@ -115,13 +115,13 @@ private:
     // region sizes below which a split is not handed to another thread
     static constexpr int64_t kSpawnMin = 16384;
     struct Scratch {
-        std::vector<int32_t> q, qs, ls;
+        std::vector<int32_t> q, qs;
     };
 
     const Graph &g_;
     int64_t leaf_;
     std::vector<int64_t> &perm_;
-    std::vector<int32_t> rid_, lev_;
+    std::vector<int32_t> rid_, lev_, lev2_;     // lev2_: a candidate's levels (pseudo-peripheral search)
     std::atomic<int32_t> next_rid_{2};
     std::atomic<int> free_threads_;
     int64_t next_pos_ = 0;
@@ -134,10 +134,10 @@ private:
         return sc;
     }
 
-    // BFS inside region r from s; fills q[0..cnt), lev_, returns cnt and height
-    int64_t bfs(int32_t r, int32_t s, int32_t &h, std::vector<int32_t> &q)
+    // BFS inside region r from s; fills q[0..cnt), lev, returns cnt and height
+    int64_t bfs(int32_t r, int32_t s, int32_t &h, std::vector<int32_t> &q, int32_t *lev)
     {
-        int32_t *rid = rid_.data(), *lev = lev_.data(), *qq = q.data();
+        int32_t *rid = rid_.data(), *qq = q.data();
         const int64_t *xadj = g_.xadj.data();
         const int32_t *adj = g_.adj.data();
         const int32_t vis = -r;       // temporary mark
@@ -211,7 +211,7 @@ private:
         std::vector<int32_t> &q = scratch().q;
         if (q.size() < nodes.size()) q.resize(nodes.size());
         int32_t h = 0;
-        int64_t cnt = bfs(r, nodes[0], h, q);
+        int64_t cnt = bfs(r, nodes[0], h, q, lev_.data());
         if (cnt == (int64_t)nodes.size()) {
             dissect(nodes, r, roots, h);    // q and lev_ hold the BFS from nodes[0]
             return;
@@ -224,7 +224,7 @@ private:
         for (;;) {
             while (start < nodes.size() && rid_ld(rid, nodes[start]) != r) start++;
             if (start == nodes.size()) break;
-            cnt = bfs(r, nodes[start], h, q);
+            cnt = bfs(r, nodes[start], h, q, lev_.data());
             const int32_t nr = next_rid_.fetch_add(1);
             comps.emplace_back(q.begin(), q.begin() + cnt);
             for (int32_t v : comps.back()) rid_st(rid, v, nr);
@@ -300,7 +300,7 @@ private:
         std::vector<int32_t> &q = sc.q;
         int32_t h = h0;
         int32_t s = nodes[0];
-        if (h0 < 0) bfs(r, s, h, q);
+        if (h0 < 0) bfs(r, s, h, q, lev_.data());
         for (int it = 0; it < 3; it++) {         // pseudo-peripheral root
             int32_t best = -1;
             int64_t bd = 0;
@@ -308,18 +308,17 @@ private:
                 const int32_t v = q[i];
                 if (best < 0 || deg(v) < bd) { best = v; bd = deg(v); }
             }
-            // keep the current level structure in case the candidate is no better
-            sc.qs.assign(q.begin(), q.begin() + cnt);
-            sc.ls.resize(cnt);
-            for (int64_t i = 0; i < cnt; i++) sc.ls[i] = lev_[q[i]];
+            // the candidate's level structure goes to a queue and level array
+            // of its own, so the current one stays in place if it is no better
+            if ((int64_t)sc.qs.size() < cnt) sc.qs.resize(cnt);
             int32_t h2 = 0;
-            bfs(r, best, h2, q);
+            bfs(r, best, h2, sc.qs, lev2_.data());
             if (h2 > h) {
                 s = best;
                 h = h2;
+                std::copy(sc.qs.begin(), sc.qs.begin() + cnt, q.begin());
+                for (int64_t i = 0; i < cnt; i++) lev_[q[i]] = lev2_[q[i]];
             } else {
-                std::copy(sc.qs.begin(), sc.qs.end(), q.begin());
-                for (int64_t i = 0; i < cnt; i++) lev_[q[i]] = sc.ls[i];
                 break;
             }
         }
