@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import ref_cases as rc
-from conftest import load_pkg
+from conftest import ORACLE_SO, load_pkg
 
 pkg = load_pkg()
 gen = pkg.gen
@@ -298,3 +298,62 @@ def test_weighted_laplacian_constant_mode(solve):
     assert abs(r.lam[0] - 1e-3) <= 1e-12
     v = r.V[:, 0] * np.sign(r.V[0, 0])
     assert np.abs(v - 1 / np.sqrt(len(v))).max() <= 1e-9
+
+
+MF_SO = ORACLE_SO.parent / "liboracle_mf.so"
+
+
+@pytest.fixture(scope="module")
+def oracle_mf(oracle):
+    if not MF_SO.exists():
+        pytest.skip("the multifrontal CPU baseline needs MKL's BLAS (oracle/Makefile)")
+    lib = ctypes.CDLL(str(MF_SO), mode=ctypes.RTLD_GLOBAL)
+    pkg.bind_api(lib, "oracle_")
+    lib.oracle_mf_factor.restype = ctypes.c_void_p
+    lib.oracle_mf_factor.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
+    lib.oracle_mf_solve.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    lib.oracle_mf_free.argtypes = [ctypes.c_void_p]
+    lib.oracle_mf_set_threads.argtypes = [ctypes.c_int]
+    return lib
+
+
+@pytest.mark.parametrize("case", ["weighted12", "disconnected", "neumann"])
+def test_multifrontal_baseline_solve_vs_numpy(oracle_mf, case):
+    """The CPU baseline's supernodal factor (oracle/o_mf.c): solves to
+    rounding against numpy's dense solve, on a weighted 3-D Laplacian, on two
+    disconnected blocks (several tree roots) and with the singular Neumann
+    matrix reported as not positive definite (status 1)."""
+    A = {"weighted12": pkg.gen.weighted_lap3d(12, seed=3, upper=True),
+         "disconnected": pkg.gen.block_diag([pkg.gen.lap3d(7, True), pkg.gen.lap3d(5, True)]),
+         "neumann": pkg.gen.lap3d(10, False)}[case]
+    Ap, Ai, Ax = (np.ascontiguousarray(a) for a in A)
+    n = len(Ap) - 1
+    st = ctypes.c_int(-9)
+    F = oracle_mf.oracle_mf_factor(n, Ap.ctypes.data, Ai.ctypes.data, Ax.ctypes.data, 0.0, ctypes.byref(st))
+    try:
+        if case == "neumann":
+            assert st.value == 1
+            return
+        assert st.value == 0
+        D = pkg.gen.csc_to_dense(Ap, Ai, Ax, symmetrize_upper=True)
+        b = np.random.default_rng(3).standard_normal(n)
+        x = np.zeros(n)
+        oracle_mf.oracle_mf_solve(F, b.ctypes.data, x.ctypes.data)
+        assert np.linalg.norm(D @ x - b) <= 1e-13 * np.linalg.norm(b) * np.linalg.cond(D)
+        assert np.abs(x - np.linalg.solve(D, b)).max() <= 1e-10 * np.abs(x).max()
+    finally:
+        oracle_mf.oracle_mf_free(F)
+
+
+@pytest.mark.parametrize("threads", [1, 4])
+def test_multifrontal_baseline_speigs_equals_oracle(oracle, oracle_mf, threads):
+    """The driver over the multifrontal factor takes the simplicial oracle's
+    path exactly: same operation counts and shift, eigenvalues to 1e-12."""
+    oracle_mf.oracle_mf_set_threads(threads)
+    for A, k in ((pkg.gen.weighted_lap3d(16, seed=3), 12), (pkg.gen.lap3d(14, False), 5)):
+        r = pkg.speigs(A, k, "sm", opts={"tol": 1e-10}, lib=oracle_mf, prefix="oracle_")
+        o = pkg.speigs(A, k, "sm", opts={"tol": 1e-10}, lib=oracle, prefix="oracle_")
+        assert r.rc == 0 and o.rc == 0
+        assert r.info["nops"] == o.info["nops"] and r.info["shift"] == o.info["shift"]
+        assert np.abs(r.lam - o.lam).max() <= 1e-12 * np.abs(o.lam).max()
