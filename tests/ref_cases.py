@@ -1,7 +1,8 @@
 """The reference validation suite (test/test_speigs.c, 13 tests / 73 checks),
 restated as functions over a `solve` callable so the same cases run against
 the CPU oracle (tests/test_oracle.py) and the MI355X library
-(tests/test_gpu_parity.py).  Tolerances are the reference's own.
+(tests/test_gpu_sm.py, tests/test_gpu_lm.py, tests/test_gpu_chosen.py).  Tolerances are the
+reference's own.
 
 solve(A, k, which, opts=None, vectors=True) -> speigs_amd.Result
 solve_z(Ap, Ai, Axr, Axi, k, which) -> speigs_amd.Result

@@ -215,7 +215,7 @@ def test_kernel_variants_agree(hiplib, monkeypatch, mode, mfma, prec):
     hiplib.speigs_hip_op_destroy(op)
 
 
-def test_inverse_solve_weighted_100(hiplib, monkeypatch):
+def test_inverse_solve_weighted_40(hiplib, monkeypatch):
     """inverse-based (chain-free) FP64 solve on a 40^3 weighted 3-D Laplacian: residual at rounding level."""
     monkeypatch.setenv("SPEIGS_HIP_SOLVE_MODE", "2")
     A = gen.weighted_lap3d(40, seed=3, upper=True)
