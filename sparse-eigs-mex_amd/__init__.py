@@ -351,3 +351,4 @@ def errmsg(code: int, lib=None) -> str:
 
 
 from . import gen  # noqa: E402  (workload generators; no library needed)
+from .mmread import MMMatrix, mmread, mmwrite  # noqa: E402  (Matrix Market files; no library needed)
