@@ -51,10 +51,10 @@ def workload(N: int, seed: int):
     return pkg.gen.weighted_lap3d(N, seed=seed, wlo=0.1, whi=10.0, shift=1e-3, upper=True)
 
 
-def run_once(A, k, tol, precision):
+def run_once(A, k, tol, precision, use_graph=0):
     pkg = _pkg()
     t0 = time.perf_counter()
-    r = pkg.speigs(A, k, "sm", opts={"tol": tol}, hip_opts={"precision": precision})
+    r = pkg.speigs(A, k, "sm", opts={"tol": tol}, hip_opts={"precision": precision, "use_graph": use_graph})
     dt = time.perf_counter() - t0
     return dt, r
 
@@ -126,6 +126,8 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--paths", default="fp64,chosen")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", type=int, default=0, choices=(0, 1),
+                    help="replay the block solve's level launches from a captured hipGraph")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES (reads BYTES, writes BYTES): the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -154,13 +156,13 @@ def main():
     for path in args.paths.split(","):
         prec = {"fp64": 0, "fp32ir": 1, "chosen": 2}[path]
         for w in range(args.warmup):
-            dt, r = run_once(A, args.k, args.tol, prec)
+            dt, r = run_once(A, args.k, args.tol, prec, args.graph)
             print(f"[bench] {path} warmup {w}: {dt:.3f}s rc={r.rc} flag={r.info['flag']} nops={r.info['nops']}",
                   file=sys.stderr, flush=True)
         times = []
         last = None
         for s in range(args.steps):
-            dt, r = run_once(A, args.k, args.tol, prec)
+            dt, r = run_once(A, args.k, args.tol, prec, args.graph)
             times.append(dt)
             last = r
             print(f"[bench] {path} step {s}: {dt:.3f}s rc={r.rc} flag={r.info['flag']} nops={r.info['nops']} "
@@ -196,7 +198,7 @@ def main():
                 "avg_ms": round(1e3 * hi["t_solve"] / hi["solves"], 3) if hi.get("solves") else None}
     cfg = {"workload": f"{args.N}^3 variable-coefficient 3-D Laplacian (weights log-uniform [0.1,10], seed "
                        f"{args.seed}) + 1e-3 I, n={n}, k={args.k} 'sm', tol {args.tol}",
-           "path": chosen_key, "nops": info["nops"], "restarts": info["restarts"], "flag": info["flag"],
+           "path": chosen_key, "use_graph": args.graph, "nops": info["nops"], "restarts": info["restarts"], "flag": info["flag"],
            "max_resid": float(r.resid.max()), "lambda0": float(r.lam[0]),
            "t_analyze": info["t_analyze"], "t_factor": info["t_factor"], "t_iter": info["t_iter"],
            "nnz_L": hi.get("nnz_L"), "factor_flops": hi.get("factor_flops"), "ir_steps": hi.get("ir_steps"),

@@ -670,9 +670,15 @@ struct RsState {
     int64_t *frow = nullptr, *uoff = nullptr, *lptr = nullptr;
     int32_t *lst = nullptr;
     int64_t n = 0, t0 = 0;
+    // the level launches of one solve captured as a graph (opt.use_graph):
+    // they read and write only the plan's own buffers and the factor's pool,
+    // so one capture serves every later solve of this factor
+    hipGraphExec_t gexec = nullptr;
+    const void *gpool = nullptr;
     ~RsState()
     {
         if (th.joinable()) th.join();
+        if (gexec) (void)hipGraphExecDestroy(gexec);
         if (up_st) (void)hipStreamDestroy(up_st);
     }
 };
@@ -684,6 +690,41 @@ int tiny_class(int64_t c, int64_t r)
 {
     auto b = [](int64_t x) { return x <= 16 ? 0 : x <= 32 ? 1 : 2; };
     return 3 * b(c) + b(r);
+}
+
+// A launch of a class with a handful of fronts costs one front's latency chain
+// (8-25 us at 100^3) however little it does, while the kernel of a larger
+// class (NQ, NR at least the front's) solves the same front exactly: its
+// extra blocks hold zeros.  So a class of fewer than kTinyMin fronts joins the
+// cheapest larger class of its level that has kTinyMin or more, and the small
+// classes left over join together into the smallest class covering them all.
+constexpr size_t kTinyMin = 1024;
+
+void promote_tiny_classes(std::vector<int32_t> (&cls)[9])
+{
+    auto covers = [](int a, int b) { return a / 3 >= b / 3 && a % 3 >= b % 3; };
+    auto cost = [](int q) { return (1 << (q / 3)) * (1 << (q % 3)); };
+    auto move = [&](int from, int to) {
+        cls[to].insert(cls[to].end(), cls[from].begin(), cls[from].end());
+        cls[from].clear();
+    };
+    for (int q = 0; q < 9; q++) {
+        if (cls[q].empty() || cls[q].size() >= kTinyMin) continue;
+        int best = -1;
+        for (int p = 0; p < 9; p++)
+            if (p != q && cls[p].size() >= kTinyMin && covers(p, q) && (best < 0 || cost(p) < cost(best))) best = p;
+        if (best >= 0) move(q, best);
+    }
+    int jc = 0, jr = 0, nsmall = 0;
+    for (int q = 0; q < 9; q++)
+        if (!cls[q].empty() && cls[q].size() < kTinyMin) {
+            jc = std::max(jc, q / 3);
+            jr = std::max(jr, q % 3);
+            nsmall++;
+        }
+    if (nsmall > 1)
+        for (int q = 0; q < 9; q++)
+            if (q != 3 * jc + jr && !cls[q].empty() && cls[q].size() < kTinyMin) move(q, 3 * jc + jr);
 }
 
 void plan_parts(const Symbolic &S, const std::vector<int64_t> &fronts, int mode, std::vector<RsPart> &pt,
@@ -788,6 +829,7 @@ int rs_plan(const Factor *F, RsState &B)
             if (is_tiny(c, S.F[s] - c)) cls[tiny_class(c, S.F[s] - c)].push_back((int32_t)s);
             else big.push_back(s);
         }
+        promote_tiny_classes(cls);
         for (int q = 0; q < 9; q++) {
             P.tc_off[q] = (int64_t)small.size();
             P.tc_cnt[q] = (int64_t)cls[q].size();
@@ -874,18 +916,15 @@ void tiny_launch(const RsLevel &P, RsState &B, const T *pool, const RsArgs &a, h
     tiny_one<T, FWD, 4, 4>(P, 8, B, pool, a, st);
 }
 
+// forward and backward over the levels, between the permutations
 template <class T>
-void rs_run(Factor *F, RsState &B, const double *Bin, int64_t brs, int64_t bcs, double *Xout, int64_t xrs,
-            int64_t xcs, int nb)
+void rs_levels(Factor *F, RsState &B, hipStream_t st)
 {
     const Symbolic &S = F->S;
-    hipStream_t st = F->st;
     const T *pool = (const T *)F->pool;
     const RsArgs a{F->foff, F->col0, F->rptr, F->chptr, B.frow, B.uoff, B.lptr, F->Fs, F->ncol, F->rows, B.lst, B.n,
                    B.t0};
-    const unsigned nrb = (unsigned)((F->n + RS_PERM_ROWS - 1) / RS_PERM_ROWS);
     const dim3 blk(256);
-    k_rs_in<<<dim3(nrb), blk, 0, st>>>(Bin, brs, bcs, F->perm, F->n, nb, B.buf);
     auto parts = [&](int mode, const RsLevel &P) {
         const int64_t pc = P.pt_cnt[mode], fc = P.ft_cnt[mode];
         const RsPart *pt = B.pt + P.pt_off[mode];
@@ -922,7 +961,41 @@ void rs_run(Factor *F, RsState &B, const double *Bin, int64_t brs, int64_t bcs, 
         parts(3, P);
         if (P.sm_cnt) tiny_launch<T, false>(P, B, pool, a, st);
     }
+}
+
+template <class T>
+int rs_run(Factor *F, RsState &B, const double *Bin, int64_t brs, int64_t bcs, double *Xout, int64_t xrs,
+           int64_t xcs, int nb)
+{
+    hipStream_t st = F->st;
+    const unsigned nrb = (unsigned)((F->n + RS_PERM_ROWS - 1) / RS_PERM_ROWS);
+    const dim3 blk(256);
+    k_rs_in<<<dim3(nrb), blk, 0, st>>>(Bin, brs, bcs, F->perm, F->n, nb, B.buf);
+    if (F->opt.use_graph) {
+        if (B.gexec && B.gpool != F->pool) {
+            SEHIP_CHECK(hipGraphExecDestroy(B.gexec));
+            B.gexec = nullptr;
+        }
+        if (!B.gexec) {
+            // thread-local capture: launches of other threads are not drawn in
+            SEHIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            rs_levels<T>(F, B, st);
+            const hipError_t le = hipGetLastError();
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(st, &g);
+            SEHIP_CHECK(le);
+            SEHIP_CHECK(ce);
+            const hipError_t ie = hipGraphInstantiate(&B.gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            SEHIP_CHECK(ie);
+            B.gpool = F->pool;
+        }
+        SEHIP_CHECK(hipGraphLaunch(B.gexec, st));
+    } else {
+        rs_levels<T>(F, B, st);
+    }
     k_rs_out<<<dim3((unsigned)((F->n + 7) / 8)), blk, 0, st>>>(B.Xg, F->perm, F->n, nb, Xout, xrs, xcs);
+    return 0;
 }
 
 }  // namespace
@@ -944,8 +1017,9 @@ int factor_block_solve_rows(Factor *F, const double *Bin, int64_t brs, int64_t b
         B.ready = true;
     }
     const int e = ev_begin(F);
-    if (F->fp32) rs_run<float>(F, B, Bin, brs, bcs, Xout, xrs, xcs, nb);
-    else rs_run<double>(F, B, Bin, brs, bcs, Xout, xrs, xcs, nb);
+    const int rc = F->fp32 ? rs_run<float>(F, B, Bin, brs, bcs, Xout, xrs, xcs, nb)
+                           : rs_run<double>(F, B, Bin, brs, bcs, Xout, xrs, xcs, nb);
+    if (rc) return rc;
     ev_end(F, e);
     F->block_solves++;
     F->block_rhs += nb;

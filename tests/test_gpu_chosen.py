@@ -143,3 +143,21 @@ def test_chosen_telemetry(hiplib):
     for key in ("t_op", "t_ortho", "t_resid", "t_ritz"):
         assert i[key] > 0.0
     assert i["t_op"] + i["t_ortho"] + i["t_resid"] + i["t_ritz"] <= i["t_iter"] * 1.05
+
+
+def test_chosen_graph_replay_equals_eager(hiplib):
+    """use_graph = 1: the block solve's level launches are captured once into a
+    hipGraph and replayed by every later solve, with the eigensolver's own
+    eager kernels and host synchronizations between the replays.  Every
+    replay must compute what the eager launches compute: the whole iteration
+    (each solve feeds the next) ends bit for bit where the eager run ends,
+    twice in a row (a second call captures again on a new factor)."""
+    A = gen.weighted_lap3d(24, seed=6)
+    e = _solve(A, 20, opts={"tol": 1e-10})
+    for rep in range(2):
+        g = _solve(A, 20, opts={"tol": 1e-10}, hip_opts={"precision": 2, "use_graph": 1})
+        assert g.rc == 0 and g.info["flag"] == 0 and g.hip_info["solves"] >= 3
+        assert g.info["nops"] == e.info["nops"] and g.hip_info["solves"] == e.hip_info["solves"]
+        assert np.array_equal(g.lam, e.lam)
+        assert np.array_equal(g.V, e.V)
+        assert np.array_equal(g.resid, e.resid)
