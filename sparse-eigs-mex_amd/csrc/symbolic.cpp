@@ -20,8 +20,12 @@
 // subset of its parent's front -- the extend-add of the multifrontal method.
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <functional>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <thread>
 
@@ -32,10 +36,22 @@ namespace sehip {
 
 namespace {
 
+// An array whose elements start uninitialized (each is written before it is
+// read): std::vector's fill would be a serial pass over megabytes.
+template <class T>
+struct RawArray {
+    std::unique_ptr<T[]> p;
+    void alloc(int64_t m) { p.reset(new T[std::max<int64_t>(m, 1)]); }
+    T &operator[](int64_t i) { return p[i]; }
+    const T &operator[](int64_t i) const { return p[i]; }
+    T *data() { return p.get(); }
+    const T *data() const { return p.get(); }
+};
+
 struct Graph {
     int64_t n = 0;
-    std::vector<int64_t> xadj;
-    std::vector<int32_t> adj;
+    RawArray<int64_t> xadj;
+    RawArray<int32_t> adj;
 };
 
 // Adjacency of the stored strict upper triangle, both directions.  Row x
@@ -45,7 +61,10 @@ struct Graph {
 // row filled through atomic cursors and then sorted.
 void build_graph(int64_t n, const int64_t *Ap, const int64_t *Ai, Graph &g, int nt)
 {
-    std::vector<int64_t> lo(n, 0), up(n, 0);
+    RawArray<int64_t> lo, up;
+    lo.alloc(n);
+    up.alloc(n);
+    parallel_for(n, nt, 65536, [&](int64_t i, int) { up[i] = 0; });
     parallel_for(n, nt, 8192, [&](int64_t j, int) {
         int64_t c = 0;
         for (int64_t q = Ap[j]; q < Ap[j + 1]; q++) {
@@ -58,9 +77,25 @@ void build_graph(int64_t n, const int64_t *Ap, const int64_t *Ai, Graph &g, int 
         lo[j] = c;
     });
     g.n = n;
-    g.xadj.assign(n + 1, 0);
-    for (int64_t i = 0; i < n; i++) g.xadj[i + 1] = g.xadj[i] + lo[i] + up[i];
-    g.adj.resize(std::max<int64_t>(g.xadj[n], 1));
+    g.xadj.alloc(n + 1);
+    // row starts: an exclusive scan of the row lengths, by blocks
+    const int nb = std::max(1, std::min(nt, (int)(n / 65536) + 1));
+    std::vector<int64_t> bsum(nb + 1, 0);
+    parallel_for(nb, nb, 1, [&](int64_t b, int) {
+        int64_t t = 0;
+        for (int64_t i = n * b / nb; i < n * (b + 1) / nb; i++) t += lo[i] + up[i];
+        bsum[b + 1] = t;
+    });
+    for (int b = 0; b < nb; b++) bsum[b + 1] += bsum[b];
+    parallel_for(nb, nb, 1, [&](int64_t b, int) {
+        int64_t t = bsum[b];
+        for (int64_t i = n * b / nb; i < n * (b + 1) / nb; i++) {
+            g.xadj[i] = t;
+            t += lo[i] + up[i];
+        }
+    });
+    g.xadj[n] = bsum[nb];
+    g.adj.alloc(g.xadj[n]);
     // cursors into the upper halves
     parallel_for(n, nt, 65536, [&](int64_t i, int) { up[i] = g.xadj[i] + lo[i]; });
     parallel_for(n, nt, 8192, [&](int64_t j, int) {
@@ -74,7 +109,7 @@ void build_graph(int64_t n, const int64_t *Ap, const int64_t *Ai, Graph &g, int 
         }
     });
     parallel_for(n, nt, 8192, [&](int64_t i, int) {
-        std::sort(g.adj.begin() + g.xadj[i] + lo[i], g.adj.begin() + g.xadj[i + 1]);
+        std::sort(g.adj.data() + g.xadj[i] + lo[i], g.adj.data() + g.xadj[i + 1]);
     });
 }
 
@@ -95,7 +130,11 @@ typedef std::vector<std::unique_ptr<NdNode>> NdList;
 class Dissection {
 public:
     Dissection(const Graph &g, int64_t leaf, std::vector<int64_t> &perm, int threads)
-        : g_(g), leaf_(leaf), perm_(perm), rid_(g.n, 1), lev_(g.n, 0), lev2_(g.n, 0), free_threads_(threads - 1) {}
+        : g_(g), leaf_(leaf), perm_(perm), rid_(g.n), lev_(g.n), lev2_(g.n), threads_(threads)
+    {
+        int32_t *rid = rid_.data();
+        parallel_for(g.n, threads, 65536, [&](int64_t i, int) { rid[i] = 1; });
+    }
 
     std::vector<int64_t> c0, c1;
     std::vector<std::vector<int64_t>> kids;
@@ -105,17 +144,99 @@ public:
         std::vector<int32_t> all(g_.n);
         std::iota(all.begin(), all.end(), 0);
         NdList roots;
-        split_components(all, 1, roots);
+        // thread 0 dissects from the top; the others take forked regions
+        auto work = [&](int t) {
+            if (t > 0) {
+                worker();
+                return;
+            }
+            split_components(all, 1, roots);
+            {
+                std::lock_guard<std::mutex> lk(jm_);
+                finished_ = true;
+            }
+            jcv_.notify_all();
+        };
+        if (threads_ <= 1) {
+            work(0);
+        } else if (!HostPool::get().run(threads_, work)) {
+            std::vector<std::thread> th;
+            for (int t = 1; t < threads_; t++) th.emplace_back(work, t);
+            work(0);
+            for (auto &x : th) x.join();
+        }
         // number the tree in postorder (children in order, then the node): the
         // columns of a node follow all of its descendants'
         for (auto &r : roots) number(*r);
     }
 
 private:
-    // region sizes below which a split is not handed to another thread
-    static constexpr int64_t kSpawnMin = 16384;
+    // Fork-join of the recursion: a forked region waits in a shared list until
+    // an idle thread takes it, or until its parent, done with its other side,
+    // runs it itself; a parent whose forked side is running elsewhere helps
+    // with other waiting regions meanwhile.  Regions smaller than kForkMin
+    // are not forked.  Which thread dissects a region does not change the
+    // result.
+    static constexpr int64_t kForkMin = 8192;
+    struct Job {
+        std::function<void()> fn;
+        int state = 0;               // 0 waiting, 1 running, 2 done (under jm_)
+    };
+    void fork(Job &j)
+    {
+        if (threads_ <= 1) return;   // joined (run) in place
+        {
+            std::lock_guard<std::mutex> lk(jm_);
+            jobs_.push_back(&j);
+        }
+        jcv_.notify_one();
+    }
+    void run_job(Job &j, std::unique_lock<std::mutex> &lk)
+    {
+        j.state = 1;
+        lk.unlock();
+        j.fn();
+        lk.lock();
+        j.state = 2;
+        jcv_.notify_all();
+    }
+    void join(Job &j)
+    {
+        std::unique_lock<std::mutex> lk(jm_);
+        if (j.state == 0) {
+            auto it = std::find(jobs_.begin(), jobs_.end(), &j);
+            if (it != jobs_.end()) jobs_.erase(it);
+            run_job(j, lk);
+            return;
+        }
+        while (j.state != 2) {
+            if (!jobs_.empty()) {
+                Job *o = jobs_.back();
+                jobs_.pop_back();
+                run_job(*o, lk);
+                continue;
+            }
+            jcv_.wait(lk);
+        }
+    }
+    void worker()
+    {
+        std::unique_lock<std::mutex> lk(jm_);
+        for (;;) {
+            if (!jobs_.empty()) {
+                Job *o = jobs_.front();      // the oldest: the largest regions
+                jobs_.pop_front();
+                run_job(*o, lk);
+                continue;
+            }
+            if (finished_) return;
+            jcv_.wait(lk);
+        }
+    }
+
     struct Scratch {
         std::vector<int32_t> q, qs;
+        std::vector<int64_t> lo, los;    // level starts of q, of qs
     };
 
     const Graph &g_;
@@ -123,7 +244,11 @@ private:
     std::vector<int64_t> &perm_;
     std::vector<int32_t> rid_, lev_, lev2_;     // lev2_: a candidate's levels (pseudo-peripheral search)
     std::atomic<int32_t> next_rid_{2};
-    std::atomic<int> free_threads_;
+    const int threads_;
+    std::mutex jm_;
+    std::condition_variable jcv_;
+    std::deque<Job *> jobs_;
+    bool finished_ = false;
     int64_t next_pos_ = 0;
 
     int64_t deg(int32_t v) const { return g_.xadj[v + 1] - g_.xadj[v]; }
@@ -134,8 +259,10 @@ private:
         return sc;
     }
 
-    // BFS inside region r from s; fills q[0..cnt), lev, returns cnt and height
-    int64_t bfs(int32_t r, int32_t s, int32_t &h, std::vector<int32_t> &q, int32_t *lev)
+    // BFS inside region r from s; fills q[0..cnt) in level order, lev, and
+    // lofs[l] = the first position of level l (lofs[h + 1] = cnt); returns cnt
+    // and the height h
+    int64_t bfs(int32_t r, int32_t s, int32_t &h, std::vector<int32_t> &q, int32_t *lev, std::vector<int64_t> &lofs)
     {
         int32_t *rid = rid_.data(), *qq = q.data();
         const int64_t *xadj = g_.xadj.data();
@@ -151,11 +278,14 @@ private:
         // neighbours' ids), which does not change the visiting order.
         int32_t cur = 0;
         int64_t lend = 1;
+        lofs.clear();
+        lofs.push_back(0);
         constexpr int64_t kA = 16;
         while (head < tail) {
             if (head == lend) {
                 cur++;
                 lend = tail;
+                lofs.push_back(head);
             }
             if (head + 3 * kA < tail) __builtin_prefetch(xadj + qq[head + 3 * kA]);
             if (head + 2 * kA < tail) __builtin_prefetch(adj + xadj[qq[head + 2 * kA]]);
@@ -174,7 +304,8 @@ private:
                 }
             }
         }
-        h = lev[qq[tail - 1]];        // the last node queued is on the last level
+        h = cur;                      // the level of the last node queued
+        lofs.push_back(tail);
         for (int64_t i = 0; i < tail; i++) rid_st(rid, qq[i], r);
         return tail;
     }
@@ -208,12 +339,13 @@ private:
             roots.push_back(leaf_node(nodes.data(), (int64_t)nodes.size()));
             return;
         }
-        std::vector<int32_t> &q = scratch().q;
+        Scratch &sc = scratch();
+        std::vector<int32_t> &q = sc.q;
         if (q.size() < nodes.size()) q.resize(nodes.size());
         int32_t h = 0;
-        int64_t cnt = bfs(r, nodes[0], h, q, lev_.data());
+        int64_t cnt = bfs(r, nodes[0], h, q, lev_.data(), sc.lo);
         if (cnt == (int64_t)nodes.size()) {
-            dissect(nodes, r, roots, h);    // q and lev_ hold the BFS from nodes[0]
+            dissect(nodes, r, roots, h);    // q, lev_ and sc.lo hold the BFS from nodes[0]
             return;
         }
         // peel components off one at a time
@@ -224,16 +356,15 @@ private:
         for (;;) {
             while (start < nodes.size() && rid_ld(rid, nodes[start]) != r) start++;
             if (start == nodes.size()) break;
-            cnt = bfs(r, nodes[start], h, q, lev_.data());
+            cnt = bfs(r, nodes[start], h, q, lev_.data(), sc.lo);
             const int32_t nr = next_rid_.fetch_add(1);
             comps.emplace_back(q.begin(), q.begin() + cnt);
             for (int32_t v : comps.back()) rid_st(rid, v, nr);
             rids.push_back(nr);
         }
         std::vector<int32_t>().swap(nodes);
-        // components are independent: the large ones are dissected on free
-        // threads (each into its own slot), the rest by this thread; the slots
-        // are collected in component order
+        // components are independent: the large ones are forked, the rest
+        // dissected here; the slots are collected in component order
         std::vector<NdList> slot(comps.size());
         auto one = [&](size_t c) {
             if ((int64_t)comps[c].size() <= leaf_)
@@ -242,69 +373,56 @@ private:
                 dissect(comps[c], rids[c], slot[c], -1);
             std::vector<int32_t>().swap(comps[c]);
         };
-        // many components with enough work in total (a block-diagonal input):
-        // all of them go to the pool; otherwise only the large ones
-        int64_t total = 0;
-        for (auto &cc : comps) total += (int64_t)cc.size();
-        const bool many = comps.size() >= 16 && total >= 2 * kSpawnMin;
-        std::vector<size_t> large, small;
+        // (which ones are forked is fixed first: a forked component's list is
+        // released by whichever thread dissects it)
+        std::vector<char> big(comps.size());
+        for (size_t c = 0; c < comps.size(); c++) big[c] = (int64_t)comps[c].size() >= kForkMin;
+        std::deque<Job> forked;
         for (size_t c = 0; c < comps.size(); c++)
-            (many || (int64_t)comps[c].size() >= kSpawnMin ? large : small).push_back(c);
-        std::atomic<size_t> next{0};
-        auto work = [&] {
-            for (;;) {
-                const size_t q = next.fetch_add(1);
-                if (q >= large.size()) break;
-                one(large[q]);
+            if (big[c]) {
+                forked.emplace_back();
+                forked.back().fn = [&one, c] { one(c); };
+                fork(forked.back());
             }
-        };
-        std::vector<std::thread> th;
-        for (size_t t = 1; t < large.size(); t++) {
-            int f = free_threads_.load();
-            while (f > 0 && !free_threads_.compare_exchange_weak(f, f - 1)) {}
-            if (f <= 0) break;
-            th.emplace_back([&] {
-                work();
-                free_threads_.fetch_add(1);
-            });
-        }
-        work();
-        for (size_t c : small) one(c);
-        for (auto &t : th) t.join();
+        for (size_t c = 0; c < comps.size(); c++)
+            if (!big[c]) one(c);
+        for (auto &j : forked) join(j);
         for (auto &sl : slot)
             for (auto &r : sl) roots.push_back(std::move(r));
     }
 
     // connected region r whose nodes are listed in the order of a BFS from
-    // nodes[0], lev_ holding the levels (height h): no component search, no
-    // first BFS
-    void split_ordered(std::vector<int32_t> &nodes, int32_t r, NdList &roots, int32_t h)
+    // nodes[0], lev_ holding the levels (height h, level starts lofs): no
+    // component search, no first BFS
+    void split_ordered(std::vector<int32_t> &nodes, int32_t r, NdList &roots, int32_t h, std::vector<int64_t> &lofs)
     {
         if ((int64_t)nodes.size() <= leaf_) {
             roots.push_back(leaf_node(nodes.data(), (int64_t)nodes.size()));
             return;
         }
-        std::vector<int32_t> &q = scratch().q;
-        if (q.size() < nodes.size()) q.resize(nodes.size());
-        std::copy(nodes.begin(), nodes.end(), q.begin());
+        Scratch &sc = scratch();
+        if (sc.q.size() < nodes.size()) sc.q.resize(nodes.size());
+        std::copy(nodes.begin(), nodes.end(), sc.q.begin());
+        sc.lo.swap(lofs);
         dissect(nodes, r, roots, h);
     }
 
-    // connected region r with nodes.size() > leaf; h0 >= 0: the thread's queue
-    // and lev_ already hold the BFS from nodes[0], of height h0
+    // connected region r with nodes.size() > leaf; h0 >= 0: the thread's queue,
+    // level starts and lev_ already hold the BFS from nodes[0], of height h0
     void dissect(std::vector<int32_t> &nodes, int32_t r, NdList &roots, int32_t h0)
     {
         Scratch &sc = scratch();
         const int64_t cnt = (int64_t)nodes.size();
         if ((int64_t)sc.q.size() < cnt) sc.q.resize(cnt);
         std::vector<int32_t> &q = sc.q;
+        std::vector<int64_t> &lo = sc.lo;
         int32_t h = h0;
         int32_t s = nodes[0];
-        if (h0 < 0) bfs(r, s, h, q, lev_.data());
+        if (h0 < 0) bfs(r, s, h, q, lev_.data(), lo);
         for (int it = 0; it < 3; it++) {         // pseudo-peripheral root
             int32_t best = -1;
             int64_t bd = 0;
-            for (int64_t i = cnt - 1; i >= 0 && lev_[q[i]] == h; i--) {
+            for (int64_t i = cnt - 1; i >= lo[h]; i--) {     // the last level
                 const int32_t v = q[i];
                 if (best < 0 || deg(v) < bd) { best = v; bd = deg(v); }
             }
@@ -312,12 +430,13 @@ private:
             // of its own, so the current one stays in place if it is no better
             if ((int64_t)sc.qs.size() < cnt) sc.qs.resize(cnt);
             int32_t h2 = 0;
-            bfs(r, best, h2, sc.qs, lev2_.data());
+            bfs(r, best, h2, sc.qs, lev2_.data(), sc.los);
             if (h2 > h) {
                 s = best;
                 h = h2;
                 std::copy(sc.qs.begin(), sc.qs.begin() + cnt, q.begin());
                 for (int64_t i = 0; i < cnt; i++) lev_[q[i]] = lev2_[q[i]];
+                lo.swap(sc.los);
             } else {
                 break;
             }
@@ -332,7 +451,7 @@ private:
         // 3-D grids a corner-rooted structure has near-equal middle levels;
         // balanced splits there cut factor flops by ~10%)
         std::vector<int64_t> lsz(h + 1, 0);
-        for (int64_t i = 0; i < cnt; i++) lsz[lev_[q[i]]]++;
+        for (int32_t l = 0; l <= h; l++) lsz[l] = lo[l + 1] - lo[l];
         int32_t sep = -1, med = -1;
         int64_t cum = 0, best = 0;
         for (int32_t l = 0; l <= h; l++) {
@@ -350,9 +469,8 @@ private:
         sep = std::min(std::max(sep, 1), h - 1);
         // thin the separator: no neighbour after it -> before it; then the reverse
         const int32_t *rid = rid_.data();
-        std::vector<int32_t> S;
-        for (int64_t i = 0; i < cnt; i++)
-            if (lev_[q[i]] == sep) S.push_back(q[i]);
+        const int64_t a0 = lo[sep], b0 = lo[sep + 1];     // the level structure: [A | S | B]
+        std::vector<int32_t> S(q.begin() + a0, q.begin() + b0);
         for (int32_t v : S) {
             bool touches_b = false;
             for (int64_t p = g_.xadj[v]; p < g_.xadj[v + 1] && !touches_b; p++) {
@@ -370,19 +488,22 @@ private:
             }
             if (!touches_a) lev_[v] = sep + 1;
         }
-        std::vector<int32_t> A, B;
+        // A = the levels before sep, then the separator nodes moved to it; B =
+        // the nodes moved to it, then the levels after sep (the order of a scan
+        // of the level structure).  A and B keep the region's id: they are not
+        // adjacent, so a search inside one never reaches the other; only the
+        // separator's nodes leave the region (id 0, no region's)
+        std::vector<int32_t> A(q.begin(), q.begin() + a0), B;
         std::unique_ptr<NdNode> nd(new NdNode());
-        A.reserve(cnt / 2 + 1);
-        B.reserve(cnt / 2 + 1);
-        const int32_t ra = next_rid_.fetch_add(2), rb = ra + 1;
-        for (int64_t i = 0; i < cnt; i++) {
-            const int32_t v = q[i], l = lev_[v];
+        B.reserve(cnt - b0 + S.size());
+        for (int32_t v : S) {
+            const int32_t l = lev_[v];
             if (l < sep) A.push_back(v);
             else if (l > sep) B.push_back(v);
             else nd->cols.push_back(v);
         }
-        for (int32_t v : A) rid_st(rid_.data(), v, ra);
-        for (int32_t v : B) rid_st(rid_.data(), v, rb);
+        B.insert(B.end(), q.begin() + b0, q.begin() + cnt);
+        for (int32_t v : nd->cols) rid_st(rid_.data(), v, 0);
         // A (levels below sep, and the separator nodes moved to it) lists its
         // nodes in the order of a BFS from s inside A, at the same levels --
         // a shortest path from s to a node below sep stays below sep, and a
@@ -394,24 +515,17 @@ private:
                 lev_[v] = sep;
                 ha = sep;
             }
+        std::vector<int64_t> alo(lo.begin(), lo.begin() + sep + 1);    // A's level starts
+        if (ha == sep) alo.push_back((int64_t)A.size());
         std::vector<int32_t>().swap(nodes);
         NdList ka, kb;
-        bool spawned = false;
-        std::thread th;
-        if ((int64_t)A.size() >= kSpawnMin && (int64_t)B.size() >= kSpawnMin) {
-            int f = free_threads_.load();
-            while (f > 0 && !free_threads_.compare_exchange_weak(f, f - 1)) {}
-            if (f > 0) {
-                th = std::thread([this, &A, &ka, ra, ha] {
-                    split_ordered(A, ra, ka, ha);
-                    free_threads_.fetch_add(1);
-                });
-                spawned = true;
-            }
-        }
-        if (!spawned) split_ordered(A, ra, ka, ha);
-        split_components(B, rb, kb);
-        if (spawned) th.join();
+        Job ja;
+        ja.fn = [this, &A, &ka, r, ha, &alo] { split_ordered(A, r, ka, ha, alo); };
+        const bool forked = (int64_t)A.size() >= kForkMin && (int64_t)B.size() >= kForkMin;
+        if (forked) fork(ja);
+        else ja.fn();
+        split_components(B, r, kb);
+        if (forked) join(ja);
         std::vector<int32_t>().swap(A);
         std::vector<int32_t>().swap(B);
         for (auto &k : ka) nd->kids.push_back(std::move(k));
@@ -474,7 +588,9 @@ int symbolic_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, const Symb
     parallel_for(n, nt, 65536, [&](int64_t i, int) { ip32[i] = (int32_t)S->iperm[i]; });
     for (int32_t L = 0; L < S->nlevels; L++) {
         const std::vector<int64_t> &nodes = bylevel[L];
-        parallel_for((int64_t)nodes.size(), nt, 16, [&](int64_t i, int tid) {
+        // a few large fronts at the top levels: one per task
+        const int64_t grain = std::max<int64_t>(1, std::min<int64_t>(16, (int64_t)nodes.size() / (4 * nt)));
+        parallel_for((int64_t)nodes.size(), nt, grain, [&](int64_t i, int tid) {
             const int64_t s = nodes[i];
             std::vector<int32_t> &mk = mark[tid];
             if (mk.empty()) mk.assign(n, -1);

@@ -183,8 +183,9 @@ def test_fast_rcm_equals_reference_scan(hiplib, oracle, case):
 @pytest.mark.parametrize("case", ["grid", "components"])
 def test_analyze_deterministic_across_threads(hiplib, monkeypatch, case):
     """host ordering + symbolic analysis (no device): the same permutation and
-    supernode partition with 1 and 8 threads, a valid permutation, and no empty
-    supernode (every supernode id owns at least one column)."""
+    supernode partition with 1, 3, 8 and 16 threads (regions are dissected by
+    whichever thread takes them), a valid permutation, and no empty supernode
+    (every supernode id owns at least one column)."""
     import ctypes
     if case == "grid":
         A = pkg.gen.weighted_lap3d(40, seed=3)
@@ -195,7 +196,7 @@ def test_analyze_deterministic_across_threads(hiplib, monkeypatch, case):
     n = len(Ap) - 1
     hiplib.speigs_hip_analyze.argtypes = [ctypes.c_int64] + [ctypes.c_void_p] * 4 + [ctypes.POINTER(pkg.HipInfo)]
     out = []
-    for t in ("1", "8"):
+    for t in ("1", "3", "8", "16"):
         monkeypatch.setenv("SPEIGS_HIP_THREADS", t)
         perm = np.zeros(n, np.int64)
         snode = np.zeros(n, np.int64)
@@ -203,8 +204,9 @@ def test_analyze_deterministic_across_threads(hiplib, monkeypatch, case):
         assert hiplib.speigs_hip_analyze(n, Ap.ctypes.data, Ai.ctypes.data, perm.ctypes.data, snode.ctypes.data,
                                          ctypes.byref(hi)) == 0
         out.append((perm, snode, hi.nsuper))
-    (p1, s1, ns1), (p8, s8, ns8) = out
-    assert np.array_equal(p1, p8) and np.array_equal(s1, s8) and ns1 == ns8
+    (p1, s1, ns1) = out[0]
+    for p8, s8, ns8 in out[1:]:
+        assert np.array_equal(p1, p8) and np.array_equal(s1, s8) and ns1 == ns8
     assert np.array_equal(np.sort(p1), np.arange(n))
     assert np.array_equal(np.unique(s1), np.arange(ns1))
 
