@@ -69,7 +69,10 @@ typedef struct {
     double  bytes_solve;    /* algorithmic bytes of one solve pair (factor + vectors)  */
     double  bytes_spmv;     /* algorithmic bytes of one SpMV                           */
     double  t_invert;       /* in-place inversion of the diagonal blocks (block path)  */
-    double  reserved[7];
+    double  perturbed_pivots; /* indefinite fallback (path 5): pivots of the signed
+                             * factorization A = L S L^T replaced by +-sqrt(eps)||A||_1
+                             * (the applies are refined against A)                  */
+    double  reserved[6];
 } speigs_hip_info;
 
 void speigs_hip_default_opts(speigs_hip_opts *o);

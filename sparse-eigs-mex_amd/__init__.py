@@ -63,7 +63,7 @@ class HipInfo(ctypes.Structure):
                 ("t_order", c_double), ("t_symbolic", c_double), ("t_upload", c_double),
                 ("t_numeric", c_double), ("t_solve", c_double), ("t_spmv", c_double),
                 ("bytes_solve", c_double), ("bytes_spmv", c_double),
-                ("t_invert", c_double), ("reserved", c_double * 7)]
+                ("t_invert", c_double), ("perturbed_pivots", c_double), ("reserved", c_double * 6)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}

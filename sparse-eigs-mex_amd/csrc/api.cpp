@@ -612,11 +612,29 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
                 }
                 if (st == 1) delta = 0.0;
             }
+            bool indefinite = false;
             if (st == 0) {
                 info->shift = delta;
                 info->path = SPEIGS_PATH_SHIFTINV;
             } else {
-                // indefinite: LU at sigma = 0, dense on the host for small n
+                indefinite = true;
+                // Indefinite: no shift up to the cap makes A + delta I positive
+                // definite.  The reference factors A itself at sigma = 0 with an
+                // LU (src/speigs_core.c:974-984); here the same supernodal
+                // structure holds the signed factorization A = L S L^T
+                // (S = diag(+-1), pivots below sqrt(eps) ||A||_1 replaced), and
+                // every apply is refined against A to FP64 accuracy.
+                st = factor_numeric_signed(fo->F, 1.4901161193847656e-08 * anorm);
+                if (st < 0 && st != SPEIGS_ERR_FACTOR) return st;
+            }
+            if (st == 0 && indefinite) {
+                info->shift = 0.0;
+                info->path = SPEIGS_PATH_LU;
+                hinfo->perturbed_pivots = (double)factor_perturbed(fo->F);
+            } else if (st != 0) {
+                // the signed factorization broke down (a pivot that is not a
+                // number): a dense host LU for small n, else the reference's
+                // factorization error
                 factor_destroy(fo->F);
                 fo->F = nullptr;
                 if (n > 6000) return SPEIGS_ERR_FACTOR;

@@ -83,6 +83,11 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
                   hipStream_t st, Factor **F, speigs_hip_info *hinfo);
 // 0 ok, 1 not positive definite, <0 error
 int factor_numeric(Factor *F, double delta);
+// The indefinite fallback: A = L S L^T (S = diag(+-1)) without pivoting, a pivot
+// of magnitude below tau replaced by +-tau; factor_apply then refines every
+// solve against A.  0 ok (factor_perturbed() pivots replaced), <0 error.
+int factor_numeric_signed(Factor *F, double tau);
+int factor_perturbed(const Factor *F);
 void factor_attach_matrix(Factor *F, const SellA *A);
 // y = (A + delta I)^-1 x on device vectors (FP64 in/out).  A solve whose
 // bounded waits gave up is reported by factor_check once the factor's stream

@@ -514,16 +514,26 @@ __global__ void __launch_bounds__(64) k_potrf_reg(T *pool, const TileTask *__res
 // updates its own columns q > p.  Each element sees the same operations in the
 // same order as in k_potrf_reg, so the factor and the inverse are the same to
 // the bit.
-template <class T>
+//
+// SG: the signed factorization of a symmetric indefinite block, A = L S L^T
+// with S = diag(+-1) and L's diagonal positive (the LDL^T factorization with
+// L scaled by |D|^1/2): step p's pivot d gives s_p = sign(d), L(p, p) =
+// sqrt|d|, L(r, p) = A(r, p) / (L(p, p) s_p), and the rank-1 update carries
+// s_p.  A pivot with |d| < tau is replaced by s_p tau (static pivoting: the
+// apply refines the solution against A, and flag[1] counts the
+// replacements).  X is S L^-1 (rows scaled by the signs), so that the TRSM's
+// B X^T is A21 L^-T S, the signed L21; the signs go to sgn[column].
+template <class T, bool SG>
 __global__ void __launch_bounds__(256) k_potrf_reg4(T *pool, const TileTask *__restrict__ tasks,
                                                     const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
                                                     const int32_t *__restrict__ ncol, const int64_t *__restrict__ col0,
                                                     const double *__restrict__ diagA, double delta, double piv_rel,
-                                                    int *flag, T *__restrict__ xinv)
+                                                    int *flag, T *__restrict__ xinv, T *__restrict__ sgn, double tau)
 {
     __shared__ __align__(16) T Lb[2][TB];
     __shared__ T Xb[2][TB];
     __shared__ T xs[TB][TB + 1];
+    __shared__ T Sb[TB];                      // SG: the pivots' signs
     const TileTask t = tasks[blockIdx.x];
     const int32_t F = Fs[t.s], c = ncol[t.s];
     const int k0 = t.k * TB, w = min(TB, c - k0);
@@ -538,6 +548,7 @@ __global__ void __launch_bounds__(256) k_potrf_reg4(T *pool, const TileTask *__r
         x[j] = T(q == r ? 1 : 0);
     }
     bool bad = false;
+    int npert = 0;                            // SG: replaced pivots (owner waves' lane 0)
     const double dAr = r < w ? diagA[col0[t.s] + k0 + r] : 0.0;
 #pragma unroll
     for (int p = 0; p < TB; p++) {
@@ -545,28 +556,48 @@ __global__ void __launch_bounds__(256) k_potrf_reg4(T *pool, const TileTask *__r
         const int jp = p & 15, b = p & 1;
         if (wv == (p >> 4)) {
             T d = bcast(a[jp], p);
-            const double ref = fabs(bcast(dAr, p) + delta);
-            if (!((double)d > piv_rel * ref) || !(d == d)) {
-                bad = true;
-                d = T(1);
+            if (SG) {
+                if (!(d == d)) {
+                    bad = true;
+                    d = T(1);
+                }
+                const T sp = d < T(0) ? T(-1) : T(1);
+                if (!(fabs((double)d) >= tau)) {
+                    d = sp * T(tau);
+                    npert++;
+                }
+                const T sd = sqrt(d * sp);
+                a[jp] = (r == p) ? sd : a[jp] / (sd * sp);
+                const T xp = x[jp] / sd;
+                x[jp] = xp;
+                Lb[b][r] = a[jp];
+                Xb[b][r] = xp;
+                if (r == 0) Sb[p] = sp;
+            } else {
+                const double ref = fabs(bcast(dAr, p) + delta);
+                if (!((double)d > piv_rel * ref) || !(d == d)) {
+                    bad = true;
+                    d = T(1);
+                }
+                // true divisions (not a reciprocal), as in k_potrf_reg
+                const T sd = sqrt(d);
+                a[jp] = (r == p) ? sd : a[jp] / sd;
+                const T xp = x[jp] / sd;
+                x[jp] = xp;
+                Lb[b][r] = a[jp];
+                Xb[b][r] = xp;
             }
-            // true divisions (not a reciprocal), as in k_potrf_reg
-            const T sd = sqrt(d);
-            a[jp] = (r == p) ? sd : a[jp] / sd;
-            const T xp = x[jp] / sd;
-            x[jp] = xp;
-            Lb[b][r] = a[jp];
-            Xb[b][r] = xp;
         }
         __syncthreads();
         if (q0 + 15 > p) {
             const T lr = Lb[b][r], xr = Xb[b][r];
+            const T lrs = SG ? lr * Sb[p] : lr;
             T lq[16];                                   // L(q, p), the same for every lane
 #pragma unroll
             for (int j = 0; j < 16; j++) lq[j] = Lb[b][q0 + j];
 #pragma unroll
             for (int j = 0; j < 16; j++) {
-                const T an = a[j] - lr * lq[j], xn = x[j] - lq[j] * xr;
+                const T an = a[j] - lrs * lq[j], xn = x[j] - lq[j] * xr;
                 const bool upd = q0 + j > p;
                 a[j] = upd ? an : a[j];
                 x[j] = upd ? xn : x[j];
@@ -574,6 +605,14 @@ __global__ void __launch_bounds__(256) k_potrf_reg4(T *pool, const TileTask *__r
         }
     }
     if (bad && r == 0) atomicOr(flag, 1);
+    if (SG) {
+        if (npert && r == 0) atomicAdd(flag + 1, npert);
+        // X = S L^-1: the stored X's row q (register column q of every lane)
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            if (q0 + j < w) x[j] *= Sb[q0 + j];
+        if (wv == 0 && r < w) sgn[col0[t.s] + k0 + r] = Sb[r];
+    }
 #pragma unroll
     for (int j = 0; j < 16; j++) {
         const int q = q0 + j;
@@ -643,13 +682,15 @@ __global__ void __launch_bounds__(256) k_trsm_mfma(T *pool, const TileTask *__re
 // in flight.  Lanes 0-15 of each result value hold 16 consecutive rows of one
 // column, so the read-modify-write of the column-major front is coalesced.
 // Every global load is issued before the first store (a store into the front
-// between loads would order them).
-template <class T>
+// between loads would order them).  SG: the signed factorization's update
+// C -= L_i S L_j^T, the column signs applied to the j operand as it is loaded.
+template <class T, bool SG>
 __global__ void __launch_bounds__(256) k_update_mfma(T *pool, const TileTask *__restrict__ pt,
                                                      const int32_t *__restrict__ ufid,
                                                      const int64_t *__restrict__ upref, int outer,
                                                      const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
-                                                     const int32_t *__restrict__ ncol)
+                                                     const int32_t *__restrict__ ncol, const int64_t *__restrict__ col0,
+                                                     const T *__restrict__ sgn)
 {
     __shared__ T Ai[TB][TB + 4];   // [kk][row]
     __shared__ T Aj[TB][TB + 4];
@@ -663,6 +704,7 @@ __global__ void __launch_bounds__(256) k_update_mfma(T *pool, const TileTask *__
     const int rb = (wv & 1) * 32, cb = (wv >> 1) * 32;
     const int lr = lane & 15, lk = lane >> 4;
     T va[16], vb[16], cv[2][2][4];
+    const T *sg = SG ? sgn + col0[u.s] : nullptr;
     auto load = [&](int kc, int w) {
 #pragma unroll
         for (int q = 0; q < 16; q++) {
@@ -670,6 +712,7 @@ __global__ void __launch_bounds__(256) k_update_mfma(T *pool, const TileTask *__
             const bool in = kk < w;
             va[q] = (in && lane < nr) ? fr[(int64_t)(kc + kk) * F + r0 + lane] : T(0);
             vb[q] = (in && lane < nq) ? fr[(int64_t)(kc + kk) * F + q0 + lane] : T(0);
+            if (SG) vb[q] = in ? vb[q] * sg[kc + kk] : vb[q];
         }
     };
     load(u.k0, min(TB, u.kw));
@@ -736,6 +779,14 @@ __global__ void k_scatter_perm(const double *__restrict__ x, const int64_t *__re
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) y[perm[i]] = x[i];
+}
+
+// between the two sweeps of a signed factor's solve: y <- S y
+template <class T>
+__global__ void k_sign_scale(double *__restrict__ y, const T *__restrict__ sgn, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] *= (double)sgn[i];
 }
 
 
@@ -1220,6 +1271,9 @@ int run_numeric(Factor *F, double delta)
     hipStream_t st = F->st;
     T *pool = (T *)F->pool;
     const double piv_rel = F->fp32 ? 1e-5 : 1e-11;
+    // the signed factorization runs on the register panel and the MFMA update
+    const bool sg = F->signed_fac;
+    if (sg && !F->use_reg_panel) return SEHIP_FAIL(SPEIGS_ERR_ARG, "signed factorization needs the register panel");
     SEHIP_CHECK(hipMemsetAsync(F->flag, 0, 4 * sizeof(int), st));
     if ((uintptr_t)pool % 16) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "front pool not 16-byte aligned");
     k_zero<T><<<dim3((unsigned)std::min<int64_t>(4096, (S.pool * (int64_t)sizeof(T) / 16 + 255) / 256 + 1)), dim3(256), 0,
@@ -1243,10 +1297,14 @@ int run_numeric(Factor *F, double delta)
                 for (size_t g = 0; g + 1 < cut.size(); g++) {
                     const int64_t p0 = (int64_t)g * kXinvSlots;
                     const int64_t pc = std::min(kXinvSlots, P.k_potrf_cnt[k] - p0);
-                    if (F->potrf_waves == 4)
-                        k_potrf_reg4<T><<<dim3((unsigned)pc), dim3(256), 0, st>>>(
+                    if (sg)
+                        k_potrf_reg4<T, true><<<dim3((unsigned)pc), dim3(256), 0, st>>>(
                             pool, F->tiles + P.k_potrf_off[k] + p0, F->foff, F->Fs, F->ncol, F->col0, F->diagA,
-                            delta, piv_rel, F->flag, xinv);
+                            delta, piv_rel, F->flag, xinv, (T *)F->sgn, F->tau);
+                    else if (F->potrf_waves == 4)
+                        k_potrf_reg4<T, false><<<dim3((unsigned)pc), dim3(256), 0, st>>>(
+                            pool, F->tiles + P.k_potrf_off[k] + p0, F->foff, F->Fs, F->ncol, F->col0, F->diagA,
+                            delta, piv_rel, F->flag, xinv, nullptr, 0.0);
                     else
                         k_potrf_reg<T><<<dim3((unsigned)pc), dim3(64), 0, st>>>(
                             pool, F->tiles + P.k_potrf_off[k] + p0, F->foff, F->Fs, F->ncol, F->col0, F->diagA,
@@ -1270,9 +1328,12 @@ int run_numeric(Factor *F, double delta)
                 if (cnt <= 0) return;
                 const int32_t *uf = F->ufid + off;
                 const int64_t *up = F->upref + refoff;
-                if (F->use_mfma)
-                    k_update_mfma<T><<<dim3((unsigned)cnt), dim3(256), 0, s2>>>(pool, pt, uf, up, outer, F->foff,
-                                                                                F->Fs, F->ncol);
+                if (sg)
+                    k_update_mfma<T, true><<<dim3((unsigned)cnt), dim3(256), 0, s2>>>(
+                        pool, pt, uf, up, outer, F->foff, F->Fs, F->ncol, F->col0, (const T *)F->sgn);
+                else if (F->use_mfma)
+                    k_update_mfma<T, false><<<dim3((unsigned)cnt), dim3(256), 0, s2>>>(
+                        pool, pt, uf, up, outer, F->foff, F->Fs, F->ncol, nullptr, nullptr);
                 else
                     k_update<T><<<dim3((unsigned)cnt), dim3(256), 0, s2>>>(pool, pt, uf, up, outer, F->foff, F->Fs,
                                                                            F->ncol);
@@ -1316,6 +1377,7 @@ int run_numeric(Factor *F, double delta)
         snprintf(what, sizeof what, "assembly: A(%d, %d) has no place in the symbolic structure", hflag[2], hflag[3]);
         return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, what);
     }
+    F->perturbed = sg ? hflag[1] : 0;
     return hflag[0] ? 1 : 0;
 }
 
@@ -1342,6 +1404,7 @@ void run_solve(Factor *F, const double *b, double *x)
             k_fwd_blk<T><<<dim3((unsigned)P.fb_cnt), dim3(256), 0, st>>>(pool, F->fblk + P.fb_off, a, F->panbase,
                                                                         F->fflag, ep, wsp, F->tick_f, F->err, F->yp);
         }
+        if (F->signed_fac) k_sign_scale<T><<<dim3(nb), dim3(256), 0, st>>>(F->yp, (const T *)F->sgn, F->n);
         for (int32_t L = S.nlevels - 1; L >= 0; L--) {
             const LevelPlan &P = F->plan[L];
             if (!P.fr_cnt) continue;
@@ -1361,6 +1424,7 @@ void run_solve(Factor *F, const double *b, double *x)
         const LevelPlan &P = F->plan[L];
         if (P.fr_cnt) k_fwd<T><<<dim3((unsigned)P.fr_cnt), dim3(256), 0, st>>>(pool, F->fronts + P.fr_off, a, F->xp, F->yp);
     }
+    if (F->signed_fac) k_sign_scale<T><<<dim3(nb), dim3(256), 0, st>>>(F->yp, (const T *)F->sgn, F->n);
     for (int32_t L = S.nlevels - 1; L >= 0; L--) {
         const LevelPlan &P = F->plan[L];
         if (P.fr_cnt) k_bwd<T><<<dim3((unsigned)P.fr_cnt), dim3(256), 0, st>>>(pool, F->fronts + P.fr_off, a, F->yp, F->xp);
@@ -1793,10 +1857,47 @@ int factor_numeric(Factor *F, double delta)
 {
     F->delta = delta;
     F->inverted = false;
+    F->signed_fac = false;
     int st = F->fp32 ? run_numeric<float>(F, delta) : run_numeric<double>(F, delta);
     F->factored = (st == 0);
     return st;
 }
+
+int factor_numeric_signed(Factor *F, double tau)
+{
+    F->delta = 0.0;
+    F->inverted = false;
+    F->factored = false;
+    if (!(tau > 0.0)) return SPEIGS_ERR_ARG;
+    if (!F->sgn) {
+        const size_t cnt = (size_t)std::max<int64_t>(F->n, 1);
+        int rc;
+        if (F->fp32) {
+            float *p = nullptr;
+            rc = F->mem.alloc(&p, cnt);
+            F->sgn = p;
+        } else {
+            double *p = nullptr;
+            rc = F->mem.alloc(&p, cnt);
+            F->sgn = p;
+        }
+        if (rc) return rc;
+    }
+    // the signed kernels: register panel (4 waves), MFMA update, blocked solve
+    F->use_reg_panel = true;
+    F->use_mfma = true;
+    F->potrf_waves = 4;
+    if (F->solve_mode == 2) F->solve_mode = 1;
+    F->signed_fac = true;
+    F->tau = tau;
+    int st = F->fp32 ? run_numeric<float>(F, 0.0) : run_numeric<double>(F, 0.0);
+    if (st == 1) return SEHIP_FAIL(SPEIGS_ERR_FACTOR, "signed factorization: a pivot is not a number");
+    if (st < 0) return st;
+    F->factored = true;
+    return 0;
+}
+
+int factor_perturbed(const Factor *F) { return F->perturbed; }
 
 void factor_attach_matrix(Factor *F, const SellA *A) { F->A = A; }
 
@@ -1823,31 +1924,40 @@ int factor_apply(Factor *F, const double *x, double *y)
         if (rc) return rc;
     }
     ev_harvest(F, false);
-    if (!F->fp32) {
+    if (!F->fp32 && !F->signed_fac) {
         const int e = ev_begin(F);
         run_solve<double>(F, x, y);
         ev_end(F, e);
         return 0;
     }
-    // FP32 factor: iterative refinement on the FP64 residual of (A + delta I)
-    {
+    // FP32 factor, or a signed factor (pivots may have been replaced):
+    // iterative refinement on the FP64 residual of (A + delta I)
+    auto solve = [&](const double *b, double *out) {
         const int e = ev_begin(F);
-        run_solve<float>(F, x, y);
+        if (F->fp32) run_solve<float>(F, b, out);
+        else run_solve<double>(F, b, out);
         ev_end(F, e);
-    }
+    };
+    solve(x, y);
     if (!F->A) return 0;
     const double bn = device_norm(F, x);
     if (bn < 0) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "refinement: norm of the right-hand side");
     if (bn == 0.0) return 0;
+    // The signed factor stands in for the reference's LU with its refinement
+    // steps, whose eigenpairs are judged at tol = 1e-14 ||A||: its applies are
+    // refined until the residual stops falling (to the rounding level), not
+    // only to ir_tol.  A singular A stops after one step.
+    const double tol = F->signed_fac ? F->opt.ir_tol * 1e-3 : F->opt.ir_tol;
+    double prev = -1.0;
     for (int it = 0; it < F->opt.ir_maxit; it++) {
         spmv(*F->A, y, F->ax, F->st);
         axpby_residual(x, F->ax, y, F->delta, F->r, F->n, F->st);
         const double rn = device_norm(F, F->r);
         if (rn < 0) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "refinement: norm of the residual");
-        if (rn <= F->opt.ir_tol * bn) break;
-        const int e = ev_begin(F);
-        run_solve<float>(F, F->r, F->d);
-        ev_end(F, e);
+        if (rn <= tol * bn) break;
+        if (F->signed_fac && prev >= 0.0 && rn > 0.5 * prev) break;
+        prev = rn;
+        solve(F->r, F->d);
         vadd(y, F->d, F->n, F->st);
         F->ir_steps++;
     }

@@ -116,6 +116,13 @@ struct Factor {
     double t_solve = 0.0, t_invert = 0.0;
     int64_t block_solves = 0, block_rhs = 0;   // multi-right-hand-side solves (FP32 vectors)
     bool inverted = false;            // L11 of every front holds its inverse
+    // signed factorization A = L S L^T of an indefinite matrix (the fallback
+    // where no shift makes A + delta I positive definite): the column signs,
+    // the pivot floor and how many pivots it replaced
+    bool signed_fac = false;
+    void *sgn = nullptr;
+    double tau = 0.0;
+    int perturbed = 0;
     void *xinv = nullptr;             // per panel launch: inverses of the diagonal tiles (TRSM)
     void *rs = nullptr, *bs_d = nullptr;     // block-solve state (row-major FP32 / generic FP64 vectors)
     ~Factor()

@@ -116,6 +116,71 @@ def test_ref2_neumann_singular(hiplib):
     rc.check_neumann_singular(_solve)
 
 
+def test_ref6_indefinite(hiplib):
+    """reference test 6 on the product library: A - cI with c between two
+    eigenvalues is indefinite, no shift makes it positive definite, and the call
+    falls back to an LU at sigma = 0 (path 5) and returns the k eigenvalues
+    nearest zero (test/test_speigs.c:177-205)."""
+    r = rc.check_indefinite(_solve)
+    assert r.rc == 0
+
+
+def test_ref7_pathological(hiplib):
+    """reference test 7: sigma exactly on an eigenvalue (A - cI singular): the
+    call returns cleanly, finds the zero eigenvalue and reports partial
+    convergence (test/test_speigs.c:211-227)."""
+    rc.check_pathological(_solve)
+
+
+def _between(ex, mid):
+    while mid + 1 < len(ex) and ex[mid + 1] - ex[mid] < 1e-6:
+        mid += 1
+    return 0.5 * (ex[mid] + ex[mid + 1])
+
+
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_indefinite_beyond_host_lu(hiplib, precision):
+    """n = 13,824, past the 6,000 of the dense host LU (which returned
+    SPEIGS_ERR_FACTOR here before): A - cI with c between two eigenvalues in
+    the middle of the spectrum is factored on the GPU as A = L S L^T and every
+    apply is refined against A; path 5 (the reference's LU path), converged at
+    the reference's default tol (1e-14 ||A||), the 6 eigenvalues nearest zero
+    equal to the closed form, residuals checked here against A.  The FP32
+    precisions (1, and 2, whose block eigensolver needs a definite operator)
+    run the same fallback over an FP32 signed factor."""
+    N, k = 24, 6
+    ex = gen.exact_spectrum3(N, True)
+    c = _between(ex, len(ex) // 2)
+    A = rc._shift_diag(gen.lap3d(N, True), c)
+    want = np.sort(np.abs(ex - c))[:k]
+    r = _solve(A, k, "sm", hip_opts={"precision": precision})
+    print(r.info, r.hip_info)
+    assert r.rc == 0 and r.info["path"] == 5 and r.info["flag"] == 0
+    assert np.abs(np.abs(r.lam) - want).max() <= 1e-10 * max(1.0, want.max())
+    res = np.linalg.norm(gen.spmv_upper(*A, r.V) - r.V * r.lam, axis=0)
+    assert res.max() <= 1e-12 * r.info["anorm"]
+    assert r.hip_info["ir_steps"] > 0
+
+
+def test_indefinite_weighted_vs_dense(hiplib):
+    """the variable-coefficient family shifted into its spectrum (n = 1,728):
+    the 8 eigenvalues nearest zero equal numpy's dense ones to 1e-10, and
+    their eigenvectors span the same spaces."""
+    A0 = gen.weighted_lap3d(12, seed=5)
+    D = gen.csc_to_dense(*A0, symmetrize_upper=True)
+    ev, U = np.linalg.eigh(D)
+    c = _between(ev, len(ev) // 3)
+    A = rc._shift_diag(A0, c)
+    order = np.argsort(np.abs(ev - c), kind="stable")[:8]
+    want = ev[order] - c
+    r = _solve(A, 8, "sm")
+    assert r.rc == 0 and r.info["path"] == 5 and r.info["flag"] == 0
+    got = r.lam[np.argsort(np.abs(r.lam), kind="stable")]
+    assert np.abs(np.sort(got) - np.sort(want)).max() <= 1e-10 * np.abs(ev).max()
+    P = U[:, order]
+    assert np.linalg.norm(r.V - P @ (P.T @ r.V)) <= 1e-8
+
+
 def test_ref3_multiplicity(hiplib):
     r = rc.check_multiplicity(_solve)
     ex = gen.exact_spectrum3(16, True)
