@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -826,13 +827,96 @@ int blocks_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *A
     std::vector<std::vector<double>> cv(ncomp), cl(ncomp), cr(ncomp);
     std::vector<int> crc(ncomp, 0);
     std::vector<speigs_info> cinfo(ncomp);
+    // Components the recursion would send to the dense short-circuit
+    // (src/speigs_core.c:318-353) and that are not diagonal go to the device
+    // in one batch (dense_batch.hip) on a helper thread of their own, beside
+    // the other host components and the device batch; the tiny ones (below
+    // kDenseDevMin rows, microseconds on the host) stay in the recursion.
+    // SPEIGS_HIP_DENSE_DEVICE=0 keeps them all on the host.
+    constexpr int64_t kDenseDevMin = 64;
+    const char *dde = getenv("SPEIGS_HIP_DENSE_DEVICE");
+    const bool dense_dev = batch && !(dde && atoi(dde) == 0);
+    std::vector<int64_t> densec;
+    std::vector<std::vector<double>> dmat;
+    std::vector<DenseJob> djobs;
+    if (dense_dev) {
+        std::vector<int64_t> cand;
+        for (int64_t c = 0; c < ncomp; c++)
+            if (onhost[c] && dmax > 0 && (sz[c] <= dmax || 4 * kc[c] > sz[c]) && sz[c] >= kDenseDevMin &&
+                sz[c] <= kDenseBatchMax)
+                cand.push_back(c);
+        std::vector<char> isdiag(cand.size(), 0);
+        dmat.resize(cand.size());
+        parallel_for((int64_t)cand.size(), host_threads(), 1, [&](int64_t i, int) {
+            const int64_t c = cand[i], nc = sz[c];
+            CompMatrix M;
+            extract_comp(c, nc, gidx, off[c], comp, loc, Ap, Ai, Ax, M);
+            bool diag = true;
+            for (int64_t jj = 0; jj < nc && diag; jj++)
+                for (int64_t q = M.Ap[jj]; q < M.Ap[jj + 1]; q++)
+                    if (M.Ai[q] != jj) {
+                        diag = false;
+                        break;
+                    }
+            isdiag[i] = diag && opts->detect;
+            if (isdiag[i]) return;
+            std::vector<double> &D = dmat[i];
+            D.assign((size_t)nc * nc, 0.0);
+            for (int64_t jj = 0; jj < nc; jj++)
+                for (int64_t q = M.Ap[jj]; q < M.Ap[jj + 1]; q++) {
+                    D[M.Ai[q] + jj * nc] = M.Ax[q];
+                    D[jj + M.Ai[q] * nc] = M.Ax[q];
+                }
+        });
+        for (size_t i = 0; i < cand.size(); i++) {
+            if (isdiag[i]) continue;
+            const int64_t c = cand[i];
+            densec.push_back(c);
+            onhost[c] = 0;
+            cl[c].assign(kc[c], 0.0);
+            cr[c].assign(kc[c], 0.0);
+            if (V) cv[c].assign((size_t)sz[c] * kc[c], 0.0);
+            DenseJob J;
+            J.n = sz[c];
+            J.k = kc[c];
+            J.A = dmat[i].data();
+            J.largest = which == SPEIGS_LM;
+            J.lambda = cl[c].data();
+            J.V = V ? cv[c].data() : nullptr;
+            djobs.push_back(std::move(J));
+        }
+    }
+    int dense_rc = 0;
+    std::string dense_msg;
+    std::thread dense_th;
+    struct DenseJoin {
+        std::thread &t;
+        ~DenseJoin()
+        {
+            if (t.joinable()) t.join();
+        }
+    } dense_join{dense_th};
+    if (!djobs.empty()) {
+        if (speigs_hip_device_count() <= 0) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "no HIP device visible");
+        const int dev = ho->device;
+        dense_th = std::thread([&, dev] {
+            auto run = [&]() -> int {
+                SEHIP_CHECK(hipSetDevice(dev));
+                StreamGuard ds;
+                SEHIP_CHECK(hipStreamCreate(&ds.s));
+                return dense_batch(djobs, ds.s);
+            };
+            dense_rc = run();
+            if (dense_rc) dense_msg = last_error_text();
+        });
+    }
     // host components (each a recursive call: diagonal, dense, or -- when
     // batching is off -- the full single-component device path), in parallel
     // when they stay on the host
     {
         std::vector<int64_t> hostc;
         for (int64_t c = 0; c < ncomp; c++)
-            if (onhost[c]) hostc.push_back(c);
+            if (onhost[c] && std::find(densec.begin(), densec.end(), c) == densec.end()) hostc.push_back(c);
         const int nt = batch ? host_threads() : 1;
         std::vector<CompMatrix> mats(nt);
         parallel_for((int64_t)hostc.size(), nt, 1, [&](int64_t i, int tid) {
@@ -855,7 +939,7 @@ int blocks_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *A
     // device components: one block-diagonal matrix
     std::vector<int64_t> devc;
     for (int64_t c = 0; c < ncomp; c++)
-        if (!onhost[c]) devc.push_back(c);
+        if (!onhost[c] && std::find(densec.begin(), densec.end(), c) == densec.end()) devc.push_back(c);
     if (!devc.empty()) {
         if (speigs_hip_device_count() <= 0) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "no HIP device visible");
         if (hipSetDevice(ho->device) != hipSuccess) return SPEIGS_ERR_ARG;
@@ -959,6 +1043,14 @@ int blocks_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *A
                 if (si.flag) info->flag = 1;
             }
         }
+    }
+    if (dense_th.joinable()) {
+        dense_th.join();
+        if (dense_rc) {
+            note_error(dense_msg.c_str());
+            return dense_rc;
+        }
+        trace_phase("dense components (device batch)", tp);
     }
     // candidates in component order, then the k best by |lambda|
     std::vector<double> clam, cres;

@@ -221,7 +221,25 @@ struct DenseTri {
     std::vector<double> A, d0, e0, w;   // reflectors, tridiagonal, eigenvalues
 };
 int dense_tri_values(int64_t n, const double *Afull, DenseTri &T);
+// the same from a tridiagonal already formed (d, e; e[n - 1] unused): T.A empty
+int dense_tri_values_from(int64_t n, const double *d, const double *e, DenseTri &T);
 int dense_tri_vectors(const DenseTri &T, int64_t k, const int64_t *sel, double *Vsel, int64_t ldv);
+// the selected eigenvectors of T's tridiagonal only (before the back-transformation)
+int dense_tri_zvectors(const DenseTri &T, int64_t k, const int64_t *sel, double *Z, int64_t ldz);
+
+// The dense short-circuit for a batch of small matrices (dense_batch.hip):
+// each job's k eigenpairs of smallest (largest) |lambda|, the
+// tridiagonalisation and back-transformation on the device (stream st).
+constexpr int64_t kDenseBatchMax = 1024;    // largest n of a batched matrix
+struct DenseJob {
+    int64_t n = 0, k = 0;
+    const double *A = nullptr;   // host, n x n column-major, full symmetric
+    bool largest = false;        // 'lm': largest |lambda|
+    double *lambda = nullptr;    // out: k values, in selection order
+    double *V = nullptr;         // out: n x k vectors, or null
+    std::vector<int64_t> sel;    // the selected eigenvalue indices (ascending order of T's values)
+};
+int dense_batch(std::vector<DenseJob> &jobs, hipStream_t st);
 int band_chol_u(int64_t n, int64_t kd, double *AB, int64_t ld);
 void band_solve_u(int64_t n, int64_t kd, const double *AB, int64_t ld, double *b);
 int lu_factor(int64_t n, double *A, int64_t *piv);

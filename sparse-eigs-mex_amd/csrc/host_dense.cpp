@@ -344,6 +344,18 @@ void tri_shift_solve(int64_t n, const double *d, const double *e, double sigma, 
 
 }  // namespace
 
+int dense_tri_values_from(int64_t n, const double *d, const double *e, DenseTri &T)
+{
+    T.n = n;
+    T.A.clear();
+    T.d0.assign(d, d + n);
+    T.e0.assign(e, e + n);
+    T.w = T.d0;
+    if (n <= 1) return 0;
+    std::vector<double> e2 = T.e0;
+    return ql_values(n, T.w.data(), e2.data());
+}
+
 int dense_tri_values(int64_t n, const double *Afull, DenseTri &T)
 {
     T.n = n;
@@ -364,14 +376,10 @@ int dense_tri_values(int64_t n, const double *Afull, DenseTri &T)
     return ql_values(n, T.w.data(), e.data());
 }
 
-int dense_tri_vectors(const DenseTri &T, int64_t k, const int64_t *sel, double *Vsel, int64_t ldv)
+// the selected eigenvectors of the tridiagonal (d0, e0): X[t] for sel[t]
+static int tri_vectors(const DenseTri &T, int64_t k, const int64_t *sel, std::vector<std::vector<double>> &X)
 {
     const int64_t n = T.n;
-    if (k <= 0) return 0;
-    if (n == 1) {
-        Vsel[0] = 1.0;
-        return 0;
-    }
     const std::vector<double> &d0 = T.d0, &e0 = T.e0, &w = T.w;
     double tnorm = 0.0;
     for (int64_t i = 0; i < n; i++)
@@ -382,7 +390,7 @@ int dense_tri_vectors(const DenseTri &T, int64_t k, const int64_t *sel, double *
     std::vector<int64_t> pos(k);
     std::iota(pos.begin(), pos.end(), 0);
     std::sort(pos.begin(), pos.end(), [&](int64_t a, int64_t b) { return sel[a] < sel[b]; });
-    std::vector<std::vector<double>> X(k, std::vector<double>(n));
+    X.assign(k, std::vector<double>(n));
     TriWork tw(n);
     int64_t cstart = 0;
     double prev_sigma = 0.0;
@@ -414,6 +422,35 @@ int dense_tri_vectors(const DenseTri &T, int64_t k, const int64_t *sel, double *
             for (int64_t i = 0; i < n; i++) x[i] /= nr;
         }
     }
+    return 0;
+}
+
+int dense_tri_zvectors(const DenseTri &T, int64_t k, const int64_t *sel, double *Z, int64_t ldz)
+{
+    const int64_t n = T.n;
+    if (k <= 0) return 0;
+    if (n == 1) {
+        Z[0] = 1.0;
+        return 0;
+    }
+    std::vector<std::vector<double>> X;
+    const int rc = tri_vectors(T, k, sel, X);
+    if (rc) return rc;
+    for (int64_t t = 0; t < k; t++) std::copy(X[t].begin(), X[t].end(), Z + (size_t)t * ldz);
+    return 0;
+}
+
+int dense_tri_vectors(const DenseTri &T, int64_t k, const int64_t *sel, double *Vsel, int64_t ldv)
+{
+    const int64_t n = T.n;
+    if (k <= 0) return 0;
+    if (n == 1) {
+        Vsel[0] = 1.0;
+        return 0;
+    }
+    std::vector<std::vector<double>> X;
+    const int rc = tri_vectors(T, k, sel, X);
+    if (rc) return rc;
     // back-transformation: x <- H_0 H_1 ... H_{n-3} x for all k vectors at
     // once, the vectors interleaved (Xr[i k + t]) so that the loops over t
     // vectorize; each vector's dot product still sums over i in order

@@ -46,7 +46,17 @@ namespace sehip {
 namespace {
 
 constexpr int RW = 32;            // floats per vector row
-constexpr int RTILE = 64;         // output rows per tile
+constexpr int RTILE = 64;         // K tile depth; output rows per tile of the X11 products
+// The two L21 products (modes 1, 2) run with 64- or 128-row output tiles,
+// chosen per level: with 128 (each wave 32 rows) a K tile's vector rows,
+// staged once per workgroup, are read half as often, but there are half the
+// workgroups and the transposed product's LDS allows two per CU -- all 128 was
+// 3.64 -> 4.20 ms per solve at 100^3.  Both give the same result to the bit
+// (the same K ranges and MFMA order per output element), so the choice can be
+// made by timing (the autotuner below) without changing any answer; at 100^3
+// it kept 64 rows at every level, and its two timed solves cost ~1.5 ms, so
+// 64 is the default and the autotuner runs on request.
+constexpr int RT_WIDE = 128;
 constexpr int RKCHUNK = 4;        // K tiles per part task
 constexpr int RBP = 36;           // LDS pitch of a staged vector row (rows k and k + 4 on other banks)
 constexpr int TINY = 64;          // wave-per-front limit on c and r
@@ -382,31 +392,34 @@ __global__ void __launch_bounds__(256) k_rs_assemble(const RsRow *__restrict__ t
 // the wave's A operands of K tile K: modes 0, 1 lane (g, lc) holds
 // A(row 16 wv + lc, k = 16 kq + 4 g + t) at x = 4 kq + t; modes 2, 3 the
 // wave's 16 output columns of L along k (lane = k), transposed through LDS later
-template <class T, int MODE>
+// NB = 16-row blocks of output rows per wave (tile rows / 64)
+template <class T, int MODE, int NB>
 __device__ __forceinline__ void rs_load_a(const T *__restrict__ L, int F, int c, int I, int K, int kn, int orow0,
-                                          int on, int wv, int g, int lc, int lane, float (&av)[16])
+                                          int on, int wv, int g, int lc, int lane, float (&av)[16 * NB])
 {
     if (MODE == 0 || MODE == 1) {
 #pragma unroll
-        for (int kq = 0; kq < 4; kq++)
+        for (int nb = 0; nb < NB; nb++)
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const int k = 16 * kq + 4 * g + t, kk = K * RTILE + k, x = 4 * kq + t;
-                const int row = 16 * wv + lc;
-                float v = 0.f;
-                if (row < on && k < kn) {
-                    if (MODE == 0) {
-                        if (K < I || k <= row) v = (float)L[(int64_t)kk * F + orow0 + row];
-                    } else {
-                        v = (float)L[(int64_t)kk * F + c + orow0 + row];
+            for (int kq = 0; kq < 4; kq++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int k = 16 * kq + 4 * g + t, kk = K * RTILE + k, x = 16 * nb + 4 * kq + t;
+                    const int row = 16 * NB * wv + 16 * nb + lc;
+                    float v = 0.f;
+                    if (row < on && k < kn) {
+                        if (MODE == 0) {
+                            if (K < I || k <= row) v = (float)L[(int64_t)kk * F + orow0 + row];
+                        } else {
+                            v = (float)L[(int64_t)kk * F + c + orow0 + row];
+                        }
                     }
+                    av[x] = v;
                 }
-                av[x] = v;
-            }
     } else {
 #pragma unroll
-        for (int jj = 0; jj < 16; jj++) {
-            const int j = 16 * wv + jj;      // tile-local output row = front column orow0 + j
+        for (int jj = 0; jj < 16 * NB; jj++) {
+            const int j = 16 * NB * wv + jj;   // tile-local output row = front column orow0 + j
             float v = 0.f;
             if (j < on && lane < kn) {
                 const int64_t col = (int64_t)(orow0 + j) * F;
@@ -446,33 +459,40 @@ __device__ __forceinline__ void rs_load_b(const RsArgs &a, int64_t c0, int64_t f
     }
 }
 
-template <class T, int MODE>
+template <class T, int MODE, int RT>
 __global__ void __launch_bounds__(256) k_rs_part(const T *__restrict__ pool, const RsPart *__restrict__ tasks, RsArgs a,
                                                  float *__restrict__ buf, float *__restrict__ Yg,
                                                  float *__restrict__ Xg, float *__restrict__ part)
 {
-    __shared__ float At[4][RTILE][20];             // modes 2, 3: [k][j] of the wave's A tile
+    constexpr int NB = RT / 64;                       // 16-row blocks of output rows per wave
+    constexpr bool TR = MODE == 2 || MODE == 3;       // A read along columns, transposed through LDS
+    __shared__ float At[TR ? 4 : 1][TR ? RTILE : 1][TR ? 16 * NB + 4 : 1];   // [k][j] of the wave's A tile
     __shared__ __align__(16) float Bs[2][RTILE][RBP];   // the K tile's vector rows, double-buffered
     const RsPart tk = tasks[blockIdx.x];
     const int s = tk.s, I = tk.I;
     const int32_t F = a.Fs[s], c = a.ncol[s], r = F - c;
-    const int orow0 = I * RTILE;
-    const int on = min(RTILE, (MODE == 1 ? r : c) - orow0);
+    const int orow0 = I * RT;
+    const int on = min(RT, (MODE == 1 ? r : c) - orow0);
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, lc = lane & 15;
     // a wave whose rows are past the tile still loads its share of the vector
     // tiles and takes part in the barriers
-    const bool active = 16 * wv < on;
+    const bool active = 16 * NB * wv < on;
     const T *L = pool + a.foff[s];
     const int64_t c0 = a.col0[s], fr = a.frow[s];
     const int32_t *R = a.rows + a.rptr[s];   // mode 2: the update rows' indices
     const bool leaf = a.chptr[s + 1] == a.chptr[s];   // W1 = P rows, W2 = 0
     const bool pull = !leaf && c <= RTILE;            // mode 0: W1 from the pull lists
     const int kdim = MODE == 2 ? r : c;
-    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
-    float av[16];
+    f32x4 acc0[NB], acc1[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) {
+        acc0[nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+        acc1[nb] = acc0[nb];
+    }
+    float av[16 * NB];
     float4 bv[2];
     auto kn_of = [&](int K) { return min(RTILE, kdim - K * RTILE); };
-    if (active) rs_load_a<T, MODE>(L, F, c, I, tk.k0, kn_of(tk.k0), orow0, on, wv, g, lc, lane, av);
+    if (active) rs_load_a<T, MODE, NB>(L, F, c, I, tk.k0, kn_of(tk.k0), orow0, on, wv, g, lc, lane, av);
     rs_load_b<MODE>(a, c0, fr, R, pull, leaf, tk.k0, kn_of(tk.k0), buf, Yg, Xg, bv);
     {
         float *dst = &Bs[0][threadIdx.x >> 2][8 * (threadIdx.x & 3)];
@@ -482,32 +502,39 @@ __global__ void __launch_bounds__(256) k_rs_part(const T *__restrict__ pool, con
     __syncthreads();
     for (int K = tk.k0; K < tk.k1; K++) {
         const int bb = (K - tk.k0) & 1;
-        float ca[16];
+        float ca[16 * NB];
 #pragma unroll
-        for (int x = 0; x < 16; x++) ca[x] = av[x];
+        for (int x = 0; x < 16 * NB; x++) ca[x] = av[x];
         const bool more = K + 1 < tk.k1;
         if (more) {     // next tile's loads in flight during this tile's products
-            if (active) rs_load_a<T, MODE>(L, F, c, I, K + 1, kn_of(K + 1), orow0, on, wv, g, lc, lane, av);
+            if (active) rs_load_a<T, MODE, NB>(L, F, c, I, K + 1, kn_of(K + 1), orow0, on, wv, g, lc, lane, av);
             rs_load_b<MODE>(a, c0, fr, R, pull, leaf, K + 1, kn_of(K + 1), buf, Yg, Xg, bv);
         }
         if (active) {
-            if (MODE == 2 || MODE == 3) {
+            if constexpr (TR) {
 #pragma unroll
-                for (int jj = 0; jj < 16; jj++) At[wv][lane][jj] = ca[jj];
+                for (int jj = 0; jj < 16 * NB; jj++) At[wv][lane][jj] = ca[jj];
                 wave_lds_sync();
 #pragma unroll
-                for (int kq = 0; kq < 4; kq++)
+                for (int nb = 0; nb < NB; nb++)
 #pragma unroll
-                    for (int t = 0; t < 4; t++) ca[4 * kq + t] = At[wv][16 * kq + 4 * g + t][lc];
+                    for (int kq = 0; kq < 4; kq++)
+#pragma unroll
+                        for (int t = 0; t < 4; t++) ca[16 * nb + 4 * kq + t] = At[wv][16 * kq + 4 * g + t][16 * nb + lc];
                 wave_lds_sync();
             }
 #pragma unroll
             for (int kq = 0; kq < 4; kq++)
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
-                    const int x = 4 * kq + t, k = 16 * kq + 4 * g + t;
-                    acc0 = mfma4(ca[x], Bs[bb][k][lc], acc0);
-                    acc1 = mfma4(ca[x], Bs[bb][k][16 + lc], acc1);
+                    const int k = 16 * kq + 4 * g + t;
+                    const float b0 = Bs[bb][k][lc], b1 = Bs[bb][k][16 + lc];
+#pragma unroll
+                    for (int nb = 0; nb < NB; nb++) {
+                        const int x = 16 * nb + 4 * kq + t;
+                        acc0[nb] = mfma4(ca[x], b0, acc0[nb]);
+                        acc1[nb] = mfma4(ca[x], b1, acc1[nb]);
+                    }
                 }
         }
         if (more) {
@@ -523,61 +550,67 @@ __global__ void __launch_bounds__(256) k_rs_part(const T *__restrict__ pool, con
     if (tk.direct) {
         // modes 1, 2 read before they write: all reads first (a store between
         // them would serialize the loads)
-        float o0[4], o1[4];
+        float o0[NB][4], o1[NB][4];
         const float *src = MODE == 1 ? buf + (a.n + a.uoff[s] + orow0) * RW : Yg + (c0 + orow0) * RW;
         float *dst = MODE == 0 ? Yg + (c0 + orow0) * RW
                    : MODE == 1 ? buf + (a.n + a.uoff[s] + orow0) * RW
                    : MODE == 2 ? buf + (a.t0 + c0 + orow0) * RW : Xg + (c0 + orow0) * RW;
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int row = 16 * wv + 4 * g + v;
-            o0[v] = 0.f;
-            o1[v] = 0.f;
-            if (((MODE == 1 && !leaf) || MODE == 2) && row < on) {
-                o0[v] = src[row * RW + lc];
-                o1[v] = src[row * RW + 16 + lc];
-            }
-        }
+        for (int nb = 0; nb < NB; nb++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int row = 16 * wv + 4 * g + v;
-            if (row < on) {
-                dst[row * RW + lc] = (MODE == 1 || MODE == 2) ? o0[v] - acc0[v] : acc0[v];
-                dst[row * RW + 16 + lc] = (MODE == 1 || MODE == 2) ? o1[v] - acc1[v] : acc1[v];
+            for (int v = 0; v < 4; v++) {
+                const int row = 16 * NB * wv + 16 * nb + 4 * g + v;
+                o0[nb][v] = 0.f;
+                o1[nb][v] = 0.f;
+                if (((MODE == 1 && !leaf) || MODE == 2) && row < on) {
+                    o0[nb][v] = src[row * RW + lc];
+                    o1[nb][v] = src[row * RW + 16 + lc];
+                }
             }
-        }
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = 16 * NB * wv + 16 * nb + 4 * g + v;
+                if (row < on) {
+                    dst[row * RW + lc] = (MODE == 1 || MODE == 2) ? o0[nb][v] - acc0[nb][v] : acc0[nb][v];
+                    dst[row * RW + 16 + lc] = (MODE == 1 || MODE == 2) ? o1[nb][v] - acc1[nb][v] : acc1[nb][v];
+                }
+            }
     } else {
-        float *out = part + (int64_t)blockIdx.x * (RTILE * RW);
+        float *out = part + (int64_t)blockIdx.x * (RT * RW);
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int row = 16 * wv + 4 * g + v;
-            out[row * RW + lc] = acc0[v];
-            out[row * RW + 16 + lc] = acc1[v];
-        }
+        for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = 16 * NB * wv + 16 * nb + 4 * g + v;
+                out[row * RW + lc] = acc0[nb][v];
+                out[row * RW + 16 + lc] = acc1[nb][v];
+            }
     }
 }
 
-template <int MODE>
+template <int MODE, int RT>
 __global__ void __launch_bounds__(256) k_rs_fin(const RsFin *__restrict__ tasks, RsArgs a, float *__restrict__ buf,
                                                 float *__restrict__ Yg, float *__restrict__ Xg,
                                                 const float *__restrict__ part)
 {
-    constexpr int PER = RTILE * RW / 256;      // elements per thread
+    constexpr int PER = RT * RW / 256;                // elements per thread
     const RsFin t = tasks[blockIdx.x];
     const int s = t.s;
     const int32_t F = a.Fs[s], c = a.ncol[s];
-    const int orow0 = t.I * RTILE;
-    const int on = min(RTILE, (MODE == 1 ? F - c : c) - orow0);
+    const int orow0 = t.I * RT;
+    const int on = min(RT, (MODE == 1 ? F - c : c) - orow0);
     const int64_t c0 = a.col0[s];
     const bool leaf = a.chptr[s + 1] == a.chptr[s];   // W2 = 0
     // partial sums in a fixed order, each step's PER loads independent
     float sum[PER];
 #pragma unroll
     for (int u = 0; u < PER; u++) sum[u] = 0.f;
-    const float *p = part + t.p0 * (RTILE * RW) + threadIdx.x;
+    const float *p = part + t.p0 * (RT * RW) + threadIdx.x;
     for (int q = 0; q < t.np; q++)
 #pragma unroll
-        for (int u = 0; u < PER; u++) sum[u] += p[(int64_t)q * (RTILE * RW) + 256 * u];
+        for (int u = 0; u < PER; u++) sum[u] += p[(int64_t)q * (RT * RW) + 256 * u];
     const float *src = MODE == 1 ? buf + (a.n + a.uoff[s] + orow0) * RW : Yg + (c0 + orow0) * RW;
     float *dst = MODE == 0 ? Yg + (c0 + orow0) * RW
                : MODE == 1 ? buf + (a.n + a.uoff[s] + orow0) * RW
@@ -636,8 +669,11 @@ struct RsLevel {
     int64_t sm_off = 0, sm_cnt = 0;          // tiny fronts, by class (tiny_class)
     int64_t tc_off[9] = {}, tc_cnt[9] = {};
     int64_t as_off = 0, as_cnt = 0;          // assembly chunks (every front of the level)
-    int64_t pt_off[4] = {0, 0, 0, 0}, pt_cnt[4] = {0, 0, 0, 0};
-    int64_t ft_off[4] = {0, 0, 0, 0}, ft_cnt[4] = {0, 0, 0, 0};
+    // product task lists by variant: modes 0-3 with 64-row tiles, then modes
+    // 1 and 2 with 128-row tiles (variants 4, 5); wide[m - 1]: mode m runs wide
+    int64_t pt_off[6] = {}, pt_cnt[6] = {};
+    int64_t ft_off[6] = {}, ft_cnt[6] = {};
+    bool wide[2] = {false, false};
 };
 
 // host-side plan of the solve (task lists, pull lists), uploaded at first use
@@ -675,10 +711,17 @@ struct RsState {
     // so one capture serves every later solve of this factor
     hipGraphExec_t gexec = nullptr;
     const void *gpool = nullptr;
+    // tile-height autotuner: 0 not started, 1 / 2 timing the narrow / wide
+    // run, 3 decided; tune_run = the run being timed (-1 none)
+    int tune_state = 0, tune_run = -1;
+    std::vector<hipEvent_t> tune_ev[2];
     ~RsState()
     {
         if (th.joinable()) th.join();
         if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (auto &v : tune_ev)
+            for (hipEvent_t x : v)
+                if (x) (void)hipEventDestroy(x);
         if (up_st) (void)hipStreamDestroy(up_st);
     }
 };
@@ -727,13 +770,13 @@ void promote_tiny_classes(std::vector<int32_t> (&cls)[9])
             if (q != 3 * jc + jr && !cls[q].empty() && cls[q].size() < kTinyMin) move(q, 3 * jc + jr);
 }
 
-void plan_parts(const Symbolic &S, const std::vector<int64_t> &fronts, int mode, std::vector<RsPart> &pt,
+void plan_parts(const Symbolic &S, const std::vector<int64_t> &fronts, int mode, int64_t RT, std::vector<RsPart> &pt,
                 std::vector<RsFin> &ft, int64_t pbase)
 {
     for (int64_t s : fronts) {
         const int64_t c = S.c1[s] - S.c0[s], r = S.F[s] - c;
         const int64_t nt11 = (c + RTILE - 1) / RTILE, nupd = (r + RTILE - 1) / RTILE;
-        const int64_t nout = mode == 1 ? nupd : nt11;
+        const int64_t nout = mode == 1 ? (r + RT - 1) / RT : (c + RT - 1) / RT;
         for (int64_t I = 0; I < nout; I++) {
             int64_t kb, ke;
             if (mode == 0) { kb = 0; ke = I + 1; }
@@ -849,14 +892,15 @@ int rs_plan(const Factor *F, RsState &B)
             for (int64_t I = i0 / 32; I * 32 < S.F[s]; I++) asmt.push_back({(int32_t)s, (int32_t)I});
         }
         P.as_cnt = (int64_t)asmt.size() - P.as_off;
-        for (int mode = 0; mode < 4; mode++) {
+        for (int v = 0; v < 6; v++) {
+            const int mode = v < 4 ? v : v - 3;
             const int64_t p0 = (int64_t)pt.size(), f0 = (int64_t)ft.size();
-            plan_parts(S, big, mode, pt, ft, p0);
-            P.pt_off[mode] = p0;
-            P.pt_cnt[mode] = (int64_t)pt.size() - p0;
-            P.ft_off[mode] = f0;
-            P.ft_cnt[mode] = (int64_t)ft.size() - f0;
-            maxpart = std::max<int64_t>(maxpart, P.pt_cnt[mode]);
+            plan_parts(S, big, mode, v < 4 ? RTILE : RT_WIDE, pt, ft, p0);
+            P.pt_off[v] = p0;
+            P.pt_cnt[v] = (int64_t)pt.size() - p0;
+            P.ft_off[v] = f0;
+            P.ft_cnt[v] = (int64_t)ft.size() - f0;
+            maxpart = std::max<int64_t>(maxpart, P.pt_cnt[v]);
         }
     }
     if (small.empty()) small.push_back(0);
@@ -881,7 +925,7 @@ int rs_upload(Factor *F, RsState &B, hipStream_t st)
         (rc = up(&B.pt, H.pt)) || (rc = up(&B.ft, H.ft)) || (rc = up(&B.frow, H.frow)) || (rc = up(&B.uoff, H.uoff)) ||
         (rc = up(&B.lptr, H.lptr)) || (rc = up(&B.lst, H.lst)))
         return rc;
-    if ((rc = B.mem.alloc(&B.part, (size_t)H.maxpart * RTILE * RW))) return rc;
+    if ((rc = B.mem.alloc(&B.part, (size_t)H.maxpart * RT_WIDE * RW))) return rc;
     if ((rc = B.mem.alloc(&B.buf, (size_t)(2 * n + H.sumr) * RW))) return rc;
     if ((rc = B.mem.alloc(&B.Yg, (size_t)n * RW))) return rc;
     if ((rc = B.mem.alloc(&B.Xg, (size_t)n * RW))) return rc;
@@ -925,42 +969,109 @@ void rs_levels(Factor *F, RsState &B, hipStream_t st)
     const RsArgs a{F->foff, F->col0, F->rptr, F->chptr, B.frow, B.uoff, B.lptr, F->Fs, F->ncol, F->rows, B.lst, B.n,
                    B.t0};
     const dim3 blk(256);
-    auto parts = [&](int mode, const RsLevel &P) {
-        const int64_t pc = P.pt_cnt[mode], fc = P.ft_cnt[mode];
-        const RsPart *pt = B.pt + P.pt_off[mode];
-        const RsFin *ft = B.ft + P.ft_off[mode];
+    // wide: -1 the level's choice, else 0 / 1 for every level (autotuner runs);
+    // tune: events around each level's L21 products (mode 1 at 2 L, mode 2 at 2 L + 1)
+    auto parts = [&](int mode, const RsLevel &P, int wide, hipEvent_t e0, hipEvent_t e1) {
+        const bool w = (mode == 1 || mode == 2) && (wide < 0 ? P.wide[mode - 1] : wide == 1);
+        const int v = w ? mode + 3 : mode;
+        const int64_t pc = P.pt_cnt[v], fc = P.ft_cnt[v];
+        const RsPart *pt = B.pt + P.pt_off[v];
+        const RsFin *ft = B.ft + P.ft_off[v];
         const dim3 gp((unsigned)pc), gf((unsigned)fc);
-        switch (mode) {
+        if (e0) (void)hipEventRecord(e0, st);
+        switch (v) {
         case 0:
-            if (pc) k_rs_part<T, 0><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
-            if (fc) k_rs_fin<0><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
+            if (pc) k_rs_part<T, 0, RTILE><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
+            if (fc) k_rs_fin<0, RTILE><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
             break;
         case 1:
-            if (pc) k_rs_part<T, 1><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
-            if (fc) k_rs_fin<1><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
+            if (pc) k_rs_part<T, 1, RTILE><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
+            if (fc) k_rs_fin<1, RTILE><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
             break;
         case 2:
-            if (pc) k_rs_part<T, 2><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
-            if (fc) k_rs_fin<2><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
+            if (pc) k_rs_part<T, 2, RTILE><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
+            if (fc) k_rs_fin<2, RTILE><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
+            break;
+        case 3:
+            if (pc) k_rs_part<T, 3, RTILE><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
+            if (fc) k_rs_fin<3, RTILE><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
+            break;
+        case 4:
+            if (pc) k_rs_part<T, 1, RT_WIDE><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
+            if (fc) k_rs_fin<1, RT_WIDE><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
             break;
         default:
-            if (pc) k_rs_part<T, 3><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
-            if (fc) k_rs_fin<3><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
+            if (pc) k_rs_part<T, 2, RT_WIDE><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
+            if (fc) k_rs_fin<2, RT_WIDE><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
         }
+        if (e1) (void)hipEventRecord(e1, st);
     };
+    const int wide = B.tune_run;             // -1 outside the autotuner's runs
+    hipEvent_t *ev = wide >= 0 ? B.tune_ev[wide].data() : nullptr;
+    auto e_at = [&](int i) -> hipEvent_t { return ev ? ev[i] : nullptr; };
     for (int32_t L = 0; L < S.nlevels; L++) {
         const RsLevel &P = B.lev[L];
         if (P.as_cnt) k_rs_assemble<<<dim3((unsigned)P.as_cnt), blk, 0, st>>>(B.asmt + P.as_off, a, B.buf);
         if (P.sm_cnt) tiny_launch<T, true>(P, B, pool, a, st);
-        parts(0, P);
-        parts(1, P);
+        parts(0, P, wide, nullptr, nullptr);
+        parts(1, P, wide, e_at(4 * L), e_at(4 * L + 1));
     }
     for (int32_t L = S.nlevels - 1; L >= 0; L--) {
         const RsLevel &P = B.lev[L];
-        parts(2, P);
-        parts(3, P);
+        parts(2, P, wide, e_at(4 * L + 2), e_at(4 * L + 3));
+        parts(3, P, wide, nullptr, nullptr);
         if (P.sm_cnt) tiny_launch<T, false>(P, B, pool, a, st);
     }
+}
+
+// The autotuner of the L21 products' tile height (SPEIGS_HIP_RS_WIDE=auto):
+// the first solve of a factor runs every level with 64-row tiles, the second
+// with 128, each level's mode-1 and mode-2 launches bracketed by events; from
+// the third solve on each level and mode takes the faster (128 only if at
+// least 3 % faster).  The results are the same to the bit either way.
+// SPEIGS_HIP_RS_WIDE=1 runs every level with 128 rows; otherwise 64.
+int rs_tune_begin(RsState &B, int nlevels)
+{
+    if (B.tune_state == 0) {
+        const char *e = getenv("SPEIGS_HIP_RS_WIDE");
+        if (!e || strcmp(e, "auto") != 0) {
+            const bool w = e && e[0] == '1';
+            for (RsLevel &P : B.lev) P.wide[0] = P.wide[1] = w;
+            B.tune_state = 3;
+        } else {
+            for (int w = 0; w < 2; w++) {
+                B.tune_ev[w].assign((size_t)4 * nlevels, nullptr);
+                for (hipEvent_t &x : B.tune_ev[w]) SEHIP_CHECK(hipEventCreate(&x));
+            }
+            B.tune_state = 1;
+        }
+    }
+    B.tune_run = (B.tune_state == 1 || B.tune_state == 2) ? B.tune_state - 1 : -1;
+    return 0;
+}
+
+int rs_tune_end(RsState &B, int nlevels)
+{
+    if (B.tune_state == 1) {
+        B.tune_state = 2;
+    } else if (B.tune_state == 2) {
+        for (int w = 0; w < 2; w++)
+            for (hipEvent_t x : B.tune_ev[w]) SEHIP_CHECK(hipEventSynchronize(x));
+        for (int L = 0; L < nlevels; L++)
+            for (int m = 0; m < 2; m++) {
+                float t[2] = {0.f, 0.f};
+                for (int w = 0; w < 2; w++)
+                    SEHIP_CHECK(hipEventElapsedTime(&t[w], B.tune_ev[w][4 * L + 2 * m], B.tune_ev[w][4 * L + 2 * m + 1]));
+                B.lev[L].wide[m] = t[1] < 0.97f * t[0];
+            }
+        for (int w = 0; w < 2; w++) {
+            for (hipEvent_t x : B.tune_ev[w]) (void)hipEventDestroy(x);
+            B.tune_ev[w].clear();
+        }
+        B.tune_state = 3;
+    }
+    B.tune_run = -1;
+    return 0;
 }
 
 template <class T>
@@ -971,7 +1082,12 @@ int rs_run(Factor *F, RsState &B, const double *Bin, int64_t brs, int64_t bcs, d
     const unsigned nrb = (unsigned)((F->n + RS_PERM_ROWS - 1) / RS_PERM_ROWS);
     const dim3 blk(256);
     k_rs_in<<<dim3(nrb), blk, 0, st>>>(Bin, brs, bcs, F->perm, F->n, nb, B.buf);
-    if (F->opt.use_graph) {
+    int rc = rs_tune_begin(B, F->S.nlevels);
+    if (rc) return rc;
+    if (B.tune_state != 3) {
+        rs_levels<T>(F, B, st);   // the autotuner's timed runs (not captured)
+        if ((rc = rs_tune_end(B, F->S.nlevels))) return rc;
+    } else if (F->opt.use_graph) {
         if (B.gexec && B.gpool != F->pool) {
             SEHIP_CHECK(hipGraphExecDestroy(B.gexec));
             B.gexec = nullptr;

@@ -166,3 +166,27 @@ def test_many_components_batched_small(hiplib, monkeypatch):
     for t in range(6):
         mass = np.array([np.sum(rb.V[offs[c]:offs[c + 1], t] ** 2) for c in range(64)])
         assert np.sort(mass)[-2] <= 1e-20
+
+
+@pytest.mark.parametrize("which", ["sm", "lm"])
+def test_dense_components_on_device(hiplib, oracle, monkeypatch, which):
+    """the components the recursion sends to the dense short-circuit
+    (n <= 800, src/speigs_core.c:318-353) are tridiagonalised and
+    back-transformed on the device in one batch: with a 9-row (host), five
+    81-729-row (device dense batch) and a 1,728-row (device sparse batch)
+    component, the k = 10 values equal the oracle's to 1e-12 relative and the
+    host dense path's (SPEIGS_HIP_DENSE_DEVICE=0) to 1e-13, and every returned
+    vector's residual against A is within 1e-11 ||A||_1."""
+    sizes = [9, 81, 125, 400, 343, 729, 1728]
+    A = _comps(sizes, 11)
+    k = 10
+    r = _solve(A, k, which)
+    o = pkg.speigs(A, k, which, lib=oracle, prefix="oracle_")
+    h = _solve(A, k, which, env={"SPEIGS_HIP_DENSE_DEVICE": "0"}, monkeypatch=monkeypatch)
+    assert r.rc == 0 and o.rc == 0 and h.rc == 0 and r.info["path"] == 4 and r.info["ncomp"] == len(sizes)
+    scale = np.abs(o.lam).max()
+    assert np.abs(r.lam - o.lam).max() <= 1e-12 * scale
+    assert np.abs(r.lam - h.lam).max() <= 1e-13 * scale
+    R = gen.spmv_upper(*A, r.V) - r.V * r.lam
+    assert np.linalg.norm(R, axis=0).max() <= 1e-11 * _norm1(A)
+    assert np.abs(np.linalg.norm(r.V, axis=0) - 1.0).max() <= 1e-12

@@ -340,6 +340,35 @@ def test_block_solve_fp32(hiplib, monkeypatch, N, nrhs):
     assert e_fg <= 1e-4
 
 
+def test_block_solve_tile_heights_bitwise(hiplib, monkeypatch):
+    """the L21 products with 64- and 128-row output tiles (every level fixed by
+    SPEIGS_HIP_RS_WIDE = 0 / 1, and the autotuner's per-level choice) give the
+    same solution to the bit, on four solves in a row per operator (the
+    autotuner times the first two and picks per level from the third on)."""
+    A = gen.weighted_lap3d(40, seed=9, upper=True)
+    n = len(A[0]) - 1
+    B = np.asfortranarray(np.random.default_rng(7).standard_normal((n, 28)))
+    out = {}
+    for mode in ("0", "1", "auto", None):
+        if mode is None:
+            monkeypatch.delenv("SPEIGS_HIP_RS_WIDE", raising=False)
+        else:
+            monkeypatch.setenv("SPEIGS_HIP_RS_WIDE", mode)
+        rc_, op, _ = _op(hiplib, A, 1, 2, 0.0)
+        assert rc_ == 0
+        xs = []
+        for rep in range(4):
+            X = np.zeros((n, 28), order="F")
+            assert _apply(hiplib, op, B, X) == 0
+            xs.append(X)
+        hiplib.speigs_hip_op_destroy(op)
+        for X in xs[1:]:
+            assert np.array_equal(X, xs[0])
+        out[mode] = xs[0]
+    for m in ("1", "auto", None):
+        assert np.array_equal(out["0"], out[m])
+
+
 def test_device_memory_reuse(hiplib, monkeypatch):
     """Device blocks kept between calls (and reused by calls of other sizes)
     leave every result bitwise unchanged: a solve repeated after a smaller one,
