@@ -75,15 +75,15 @@ def test_batched_vs_oracle(hiplib, oracle, precision):
     """12 components, k = 6 'sm': eigenvalues equal the CPU oracle's to 1e-10
     relative, residuals within tol * ||A||, with the shared factor in FP64
     (precision 0) and in FP32 with every apply refined to FP64 (precision 1;
-    precision 2 asks for the FP32 factor, which the batched Lanczos path also
-    applies with refinement)."""
+    precision 2, the fast path, runs the batch on the FP64 factor: the block
+    eigensolver that its FP32 factor serves is not batched)."""
     A = _comps([1000, 1200, 1500, 2000, 1728, 2197, 900, 1100, 1331, 1600, 2744, 1024], seed=7)
     r = _solve(A, 6, "sm", opts={"tol": 1e-11}, hip_opts={"precision": precision})
     ro = pkg.speigs(A, 6, "sm", opts={"tol": 1e-11}, lib=oracle, prefix="oracle_")
     print("gpu", r.lam, r.info, r.hip_info, "oracle", ro.lam, ro.info)
     assert r.rc == 0 and ro.rc == 0 and r.info["path"] == 4 and r.info["flag"] == 0
-    assert r.hip_info["precision_used"] == (0 if precision == 0 else 1)
-    if precision:
+    assert r.hip_info["precision_used"] == (1 if precision == 1 else 0)
+    if precision == 1:
         assert r.hip_info["ir_steps"] > 0
     assert np.abs(r.lam - ro.lam).max() <= 1e-10 * np.abs(ro.lam).max()
     R = gen.spmv_upper(*A, r.V) - r.V * r.lam
@@ -93,16 +93,20 @@ def test_batched_vs_oracle(hiplib, oracle, precision):
 def test_config4_full_size(hiplib, oracle):
     """The 512-component workload at its full size (n = 5.08e6, sizes
     log-uniform in [500, 50,000], half 2-D, half 3-D), k = 6 'sm', with the
-    FP32 factor refined to FP64: every returned vector lies in one component,
-    every residual is within tol * ||A||_1, the values equal the FP64 batched
-    run's to 1e-10 relative, and each value is one of the oracle's k smallest
-    of the component that holds its vector."""
+    FP32 factor refined to FP64 (precision 1): every returned vector lies in
+    one component, every residual is within tol * ||A||_1, the values equal the
+    FP64 batched run's to 1e-10 relative, and each value is one of the oracle's
+    k smallest of the component that holds its vector.  Precision 2 (the fast
+    path) runs the batch in FP64, the faster operator for lockstep Lanczos."""
     Ap, Ai, Ax, sizes = gen.batched_components(512, seed=4)
     A = (Ap, Ai, Ax)
     offs = np.concatenate([[0], np.cumsum(sizes)])
-    r32 = _solve(A, 6, "sm", opts={"tol": 1e-10}, hip_opts={"precision": 2})
+    r32 = _solve(A, 6, "sm", opts={"tol": 1e-10}, hip_opts={"precision": 1})
     r64 = _solve(A, 6, "sm", opts={"tol": 1e-10}, hip_opts={"precision": 0})
+    rf = _solve(A, 6, "sm", opts={"tol": 1e-10}, hip_opts={"precision": 2})
     print("fp32", r32.info, r32.hip_info, "fp64", r64.info)
+    assert r32.hip_info["precision_used"] == 1 and r32.hip_info["ir_steps"] > 0
+    assert rf.rc == 0 and rf.hip_info["precision_used"] == 0 and np.array_equal(rf.lam, r64.lam)
     for r in (r32, r64):
         assert r.rc == 0 and r.info["flag"] == 0 and r.info["ncomp"] == 512 and r.info["path"] == 4
     assert np.abs(r32.lam - r64.lam).max() <= 1e-10 * np.abs(r64.lam).max()
