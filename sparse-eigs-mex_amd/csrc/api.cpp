@@ -591,6 +591,9 @@ int speigs_ex(speigs_int n, const speigs_int *Ap, const speigs_int *Ai, const do
             fopt.ir_maxit = hopts->ir_maxit > 0 ? hopts->ir_maxit : 12;
             fopt.ordering = opts->ordering;
             fopt.use_graph = hopts->use_graph;
+            // an apply error e makes the Krylov residual floor ~ e ||A||: the
+            // FP64 solves by inverted blocks are refined to 1 % of the tolerance
+            fopt.apply_tol = 1e-2 * (opts->tol > 0 ? opts->tol : 1e-14);
             double ta = wall();
             rc = factor_create(n, Ap, Ai, Ax, fopt, sg.s, &fo->F, hinfo);
             if (rc) return rc;
@@ -1000,6 +1003,7 @@ int blocks_path(int64_t n, const int64_t *Ap, const int64_t *Ai, const double *A
             fopt.ir_tol = ho->ir_tol > 0 ? ho->ir_tol : 1e-13;
             fopt.ir_maxit = ho->ir_maxit > 0 ? ho->ir_maxit : 12;
             fopt.ordering = opts->ordering;
+            fopt.apply_tol = 1e-2 * (opts->tol > 0 ? opts->tol : 1e-14);
             double ta = wall();
             if ((rc = factor_create(nb, bAp.data(), bAi.data(), bAx.data(), fopt, sg.s, &fo->F, hinfo))) return rc;
             info->t_analyze += wall() - ta;

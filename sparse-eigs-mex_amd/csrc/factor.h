@@ -74,6 +74,10 @@ struct FactorOptions {
     int precision = SPEIGS_HIP_FP64;
     double ir_tol = 1e-13;
     int ir_maxit = 12;
+    // FP64 solves by the inverted diagonal blocks (solve mode 3): refine an
+    // apply against A until ||b - (A + delta I) y|| <= apply_tol ||b|| (or the
+    // residual stops falling); the solve's own error grows with cond(L11)
+    double apply_tol = 1e-16;
     int ordering = 0;
     int use_graph = 0;
 };

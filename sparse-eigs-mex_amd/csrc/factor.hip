@@ -1555,6 +1555,12 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     SEHIP_CHECK(hipEventCreateWithFlags(&F->ev_far, hipEventDisableTiming));
     F->n = n;
     F->fp32 = o.precision == SPEIGS_HIP_FP32_IR || o.precision == SPEIGS_HIP_FP32_BLOCK;
+    // Single-vector solves: by the inverted diagonal blocks (mode 3) when the
+    // caller's tolerance leaves room for one such solve per apply (apply_tol
+    // >= 1e-14, i.e. an eigensolver tolerance >= 1e-12: ~3x faster per apply);
+    // for tighter ones each apply would need a refinement step, and the
+    // chained blocked solve (mode 1), exact to rounding in one pass, is kept
+    F->solve_mode = o.apply_tol >= 1e-14 ? 3 : 1;
     // debug knobs for A/B checks of kernel variants
     if (const char *e = getenv("SPEIGS_HIP_SOLVE_MODE")) F->solve_mode = atoi(e);
     if (const char *e = getenv("SPEIGS_HIP_MFMA")) F->use_mfma = atoi(e) != 0;
@@ -1799,10 +1805,10 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     if ((rc = upload(F.get(), &F->eab, eab))) return rc;
     if (ea.empty()) ea.push_back({0, 0, 0, 0});
     if (tiles.empty()) tiles.push_back({0, 0, 0, 0});
-    // the single-vector solves' plan: now, unless the factor is the block
-    // eigensolver's (whose solves have plans of their own); then at the first
-    // factor_apply, if any
-    if (o.precision != SPEIGS_HIP_FP32_BLOCK && (rc = single_solve_plan(F.get()))) return rc;
+    // the chained single-vector solves' plan (modes 0, 1): now, unless the
+    // factor is the block eigensolver's or solves by the inverted L11 (mode 3),
+    // whose solves have plans of their own; then at the first factor_apply, if any
+    if (o.precision != SPEIGS_HIP_FP32_BLOCK && F->solve_mode != 3 && (rc = single_solve_plan(F.get()))) return rc;
     if ((rc = F->mem.alloc(&F->counters, (size_t)2 * S.nlevels + 2))) return rc;
     if ((rc = F->mem.alloc(&F->err, 1))) return rc;
     SEHIP_CHECK(hipMemsetAsync(F->err, 0, sizeof(int), st));
@@ -1849,9 +1855,36 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
         hinfo->t_symbolic = S.t_symbolic;
         hinfo->t_upload = wall() - tu;
     }
+    // the single-vector solve's plan is built by a helper thread while the
+    // device factorizes (solve mode 3; the block path starts the same plan)
+    if (F->solve_mode == 3) rowsolve_prepare_async(F.get());
     *out = F.release();
     return SPEIGS_OK;
 }
+
+namespace {
+// a solve mode 3 factor of the single-vector paths: plan joined and L11
+// inverted right after the factorization, so that no apply allocates
+int vec_after_numeric(Factor *F)
+{
+    if (F->solve_mode != 3 || F->opt.precision == SPEIGS_HIP_FP32_BLOCK) return 0;
+    return factor_vec_prepare(F);
+}
+
+// one solve with the factor: the inverse-based one of solve mode 3 (also for
+// any factor whose L11 is inverted), else the blocked or per-front chains
+int one_solve(Factor *F, const double *b, double *x)
+{
+    if (F->solve_mode == 3 || (F->inverted && F->solve_mode != 2)) {
+        const int rc = factor_vec_solve(F, b, x);
+        if (rc == 0) F->solves++;
+        return rc;
+    }
+    if (F->fp32) run_solve<float>(F, b, x);
+    else run_solve<double>(F, b, x);
+    return 0;
+}
+}  // namespace
 
 int factor_numeric(Factor *F, double delta)
 {
@@ -1860,6 +1893,10 @@ int factor_numeric(Factor *F, double delta)
     F->signed_fac = false;
     int st = F->fp32 ? run_numeric<float>(F, delta) : run_numeric<double>(F, delta);
     F->factored = (st == 0);
+    if (st == 0) {
+        const int rc = vec_after_numeric(F);
+        if (rc) return rc;
+    }
     return st;
 }
 
@@ -1883,18 +1920,19 @@ int factor_numeric_signed(Factor *F, double tau)
         }
         if (rc) return rc;
     }
-    // the signed kernels: register panel (4 waves), MFMA update, blocked solve
+    // the signed kernels: register panel (4 waves), MFMA update; the solve
+    // scales by S between its sweeps (modes 1 and 3)
     F->use_reg_panel = true;
     F->use_mfma = true;
     F->potrf_waves = 4;
-    if (F->solve_mode == 2) F->solve_mode = 1;
+    if (F->solve_mode == 2 || F->solve_mode == 0) F->solve_mode = 1;
     F->signed_fac = true;
     F->tau = tau;
     int st = F->fp32 ? run_numeric<float>(F, 0.0) : run_numeric<double>(F, 0.0);
     if (st == 1) return SEHIP_FAIL(SPEIGS_ERR_FACTOR, "signed factorization: a pivot is not a number");
     if (st < 0) return st;
     F->factored = true;
-    return 0;
+    return vec_after_numeric(F);
 }
 
 int factor_perturbed(const Factor *F) { return F->perturbed; }
@@ -1903,7 +1941,8 @@ void factor_attach_matrix(Factor *F, const SellA *A) { F->A = A; }
 
 bool factor_apply_preallocated(const Factor *F)
 {
-    return F->opt.precision != SPEIGS_HIP_FP32_BLOCK && F->solve_mode != 2;
+    if (F->opt.precision == SPEIGS_HIP_FP32_BLOCK || F->solve_mode == 2) return false;
+    return F->solve_mode != 3 || factor_vec_prepared(F);
 }
 
 int factor_check(const Factor *F)
@@ -1918,27 +1957,26 @@ int factor_apply(Factor *F, const double *x, double *y)
     if (!F->factored) return SPEIGS_ERR_FACTOR;
     int rc0 = factor_check(F);   // a bounded wait gave up in an earlier solve
     if (rc0) return rc0;
-    if ((rc0 = single_solve_plan(F))) return rc0;
+    const bool vec = F->solve_mode == 3 || (F->inverted && F->solve_mode != 2);
+    if (!vec && (rc0 = single_solve_plan(F))) return rc0;
     if (F->solve_mode == 2 && !F->inverted) {
         const int rc = factor_invert_l11(F);
         if (rc) return rc;
     }
     ev_harvest(F, false);
-    if (!F->fp32 && !F->signed_fac) {
+    auto solve = [&](const double *b, double *out) -> int {
         const int e = ev_begin(F);
-        run_solve<double>(F, x, y);
+        const int rc = one_solve(F, b, out);
         ev_end(F, e);
-        return 0;
-    }
-    // FP32 factor, or a signed factor (pivots may have been replaced):
-    // iterative refinement on the FP64 residual of (A + delta I)
-    auto solve = [&](const double *b, double *out) {
-        const int e = ev_begin(F);
-        if (F->fp32) run_solve<float>(F, b, out);
-        else run_solve<double>(F, b, out);
-        ev_end(F, e);
+        return rc;
     };
-    solve(x, y);
+    // FP64 solves by the inverted L11 blocks: refined against A to apply_tol
+    const bool vec64 = vec && !F->fp32 && !F->signed_fac;
+    if (!F->fp32 && !F->signed_fac && !vec64) return solve(x, y);
+    // FP32 factor, a signed factor (pivots may have been replaced) or the
+    // inverse-based FP64 solve: iterative refinement on the FP64 residual of
+    // (A + delta I)
+    if ((rc0 = solve(x, y))) return rc0;
     if (!F->A) return 0;
     const double bn = device_norm(F, x);
     if (bn < 0) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "refinement: norm of the right-hand side");
@@ -1946,8 +1984,11 @@ int factor_apply(Factor *F, const double *x, double *y)
     // The signed factor stands in for the reference's LU with its refinement
     // steps, whose eigenpairs are judged at tol = 1e-14 ||A||: its applies are
     // refined until the residual stops falling (to the rounding level), not
-    // only to ir_tol.  A singular A stops after one step.
-    const double tol = F->signed_fac ? F->opt.ir_tol * 1e-3 : F->opt.ir_tol;
+    // only to ir_tol.  A singular A stops after one step.  The FP64 solve by
+    // inverted blocks is refined to apply_tol, which the eigensolver sets from
+    // its own tolerance, the same way.
+    const double tol = vec64 ? F->opt.apply_tol : F->signed_fac ? F->opt.ir_tol * 1e-3 : F->opt.ir_tol;
+    const bool stall_stop = F->signed_fac || vec64;
     double prev = -1.0;
     for (int it = 0; it < F->opt.ir_maxit; it++) {
         spmv(*F->A, y, F->ax, F->st);
@@ -1955,9 +1996,9 @@ int factor_apply(Factor *F, const double *x, double *y)
         const double rn = device_norm(F, F->r);
         if (rn < 0) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "refinement: norm of the residual");
         if (rn <= tol * bn) break;
-        if (F->signed_fac && prev >= 0.0 && rn > 0.5 * prev) break;
+        if (stall_stop && prev >= 0.0 && rn > 0.5 * prev) break;
         prev = rn;
-        solve(F->r, F->d);
+        if ((rc0 = solve(F->r, F->d))) return rc0;
         vadd(y, F->d, F->n, F->st);
         F->ir_steps++;
     }

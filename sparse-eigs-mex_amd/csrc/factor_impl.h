@@ -93,7 +93,8 @@ struct Factor {
     bool use_mfma = true;
     bool use_reg_panel = true;       // register POTRF + TRSM as a product with L_kk^-1
     int potrf_waves = 4;             // register POTRF on 4 waves (columns split) or 1
-    int solve_mode = 1;               // 0: one workgroup per front, 1: blocked multi-workgroup
+    int solve_mode = 3;               // 0: one workgroup per front, 1: blocked multi-workgroup,
+                                      // 2: generic inverse-based, 3: inverse-based on the block plan
     bool wait_test = false;           // test knob: the blocked solve's waits all give up
     bool plan_ready = false;          // the single-vector solves' plan is on the device
     EaTask *sea = nullptr;
@@ -151,6 +152,12 @@ int factor_block_solve_rows(Factor *F, const double *B, int64_t brs, int64_t bcs
 // starts building the row solve's plan on a helper thread (it reads only the
 // symbolic structure); the first factor_block_solve_rows joins it
 void rowsolve_prepare_async(Factor *F);
+// the single-right-hand-side solve over the same plan and the inverted L11
+// (solve mode 3): prepare joins the plan and inverts (allocates nothing after
+// the plan's upload); solve takes device FP64 vectors in the original order
+int factor_vec_prepare(Factor *F);
+bool factor_vec_prepared(const Factor *F);
+int factor_vec_solve(Factor *F, const double *b, double *x);
 void rowsolve_free(Factor *F);
 int factor_block_solve_d(Factor *F, const double *B, double *X, int nb);
 void factor_block_free(Factor *F);

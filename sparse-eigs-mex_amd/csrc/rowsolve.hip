@@ -664,6 +664,248 @@ __global__ void __launch_bounds__(256) k_rs_out(const float *__restrict__ Xg, co
     X[perm[i] * rs + (int64_t)v * cs] = (double)Xg[i * RW + v];
 }
 
+// ---- one right-hand side: the Lanczos paths' solves ----------------------
+// The same plan and the same inverted diagonal blocks as the block solve, one
+// FP64 value per buffer row: Y = X11 W1, U = W2 - L21 Y forward, Z = Y -
+// L21^T X_R, X_c = X11^T Z backward -- matrix-vector products with no chain
+// inside a front, only the tree levels.  Every sum runs in a fixed order.
+
+constexpr int VS_ASM = 8;        // assembly chunks (32 rows) per workgroup
+
+__global__ void __launch_bounds__(256) k_vs_in(const double *__restrict__ B, const int64_t *__restrict__ perm, int64_t n,
+                                               double *__restrict__ buf)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) buf[i] = B[perm[i]];
+}
+
+__global__ void __launch_bounds__(256) k_vs_out(const double *__restrict__ X, const int64_t *__restrict__ perm, int64_t n,
+                                                double *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[perm[i]] = X[i];
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_vs_sign(double *__restrict__ y, const T *__restrict__ sgn, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) y[i] *= (double)sgn[i];
+}
+
+__device__ __forceinline__ double vs_pull(const RsArgs &a, const double *__restrict__ buf, int64_t rid)
+{
+    double x = 0.0;
+    for (int64_t q = a.lptr[rid]; q < a.lptr[rid + 1]; q++) x += buf[a.lst[q]];
+    return x;
+}
+
+// the assembled rows of a level (the plan's 32-row chunks, eight per workgroup)
+__global__ void __launch_bounds__(256) k_vs_assemble(const RsRow *__restrict__ tasks, int64_t cnt, RsArgs a,
+                                                     double *__restrict__ buf)
+{
+    const int64_t tix = (int64_t)blockIdx.x * VS_ASM + (threadIdx.x >> 5);
+    if (tix >= cnt) return;
+    const RsRow t = tasks[tix];
+    const int s = t.s;
+    const int32_t F = a.Fs[s], c = a.ncol[s];
+    const int i = t.I * 32 + (threadIdx.x & 31);
+    if (i >= F) return;
+    const double x = vs_pull(a, buf, a.frow[s] + i);
+    buf[i < c ? a.t0 + a.col0[s] + i : a.n + a.uoff[s] + (i - c)] = x;
+}
+
+// tiny fronts (c, r <= 64), one wave per front, lane = row
+template <class T>
+__global__ void __launch_bounds__(256) k_vs_tiny_fwd(const T *__restrict__ pool, const int32_t *__restrict__ fronts,
+                                                     int nf, RsArgs a, double *__restrict__ buf, double *__restrict__ Y)
+{
+    __shared__ double sw[4][64], sy[4][64];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int fi = blockIdx.x * 4 + wv;
+    if (fi >= nf) return;
+    const int s = fronts[fi];
+    const int32_t F = a.Fs[s], c = a.ncol[s], r = F - c;
+    const T *L = pool + a.foff[s];
+    const int64_t c0 = a.col0[s], ub = a.n + a.uoff[s];
+    const bool leaf = a.chptr[s + 1] == a.chptr[s];
+    const int64_t tb = leaf ? c0 : a.t0 + c0;
+    sw[wv][lane] = lane < c ? buf[tb + lane] : 0.0;
+    const double w2 = (!leaf && lane < r) ? buf[ub + lane] : 0.0;
+    wave_lds_sync();
+    double y = 0.0;
+    for (int k = 0; k < c; k++)
+        if (k <= lane && lane < c) y += (double)L[(int64_t)k * F + lane] * sw[wv][k];
+    if (lane < c) Y[c0 + lane] = y;
+    sy[wv][lane] = lane < c ? y : 0.0;
+    wave_lds_sync();
+    if (lane < r) {
+        double u = 0.0;
+        for (int k = 0; k < c; k++) u += (double)L[(int64_t)k * F + c + lane] * sy[wv][k];
+        buf[ub + lane] = w2 - u;
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_vs_tiny_bwd(const T *__restrict__ pool, const int32_t *__restrict__ fronts,
+                                                     int nf, RsArgs a, const double *__restrict__ Y,
+                                                     double *__restrict__ X)
+{
+    __shared__ double sx[4][64], sz[4][64];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int fi = blockIdx.x * 4 + wv;
+    if (fi >= nf) return;
+    const int s = fronts[fi];
+    const int32_t F = a.Fs[s], c = a.ncol[s], r = F - c;
+    const T *L = pool + a.foff[s];
+    const int64_t c0 = a.col0[s];
+    const int32_t *R = a.rows + a.rptr[s];
+    sx[wv][lane] = lane < r ? X[R[lane]] : 0.0;
+    wave_lds_sync();
+    // lane j: its column of L21 and of X11, read down the column
+    double z = 0.0;
+    if (lane < c) {
+        const T *col = L + (int64_t)lane * F;
+        for (int k = 0; k < r; k++) z += (double)col[c + k] * sx[wv][k];
+        z = Y[c0 + lane] - z;
+    }
+    sz[wv][lane] = lane < c ? z : 0.0;
+    wave_lds_sync();
+    if (lane < c) {
+        const T *col = L + (int64_t)lane * F;
+        double x = 0.0;
+        for (int k = lane; k < c; k++) x += (double)col[k] * sz[wv][k];
+        X[c0 + lane] = x;
+    }
+}
+
+// The product tasks of the other fronts (the plan's 64-row output tiles and
+// K ranges of at most RKCHUNK 64-row tiles).  The task's vector segment (<=
+// 256 values: W1, Y, X at the update rows, or Z) is staged in LDS first.
+// Modes 0, 1: lane = output row, wave w takes K tile k0 + w, the four waves'
+// sums added in wave order.  Modes 2, 3: the output rows are columns of L;
+// wave w takes columns 16 w .. 16 w + 15, lane = row within each K tile, and
+// each column's sum is reduced over the lanes at the end.
+template <class T, int MODE>
+__global__ void __launch_bounds__(256) k_vs_part(const T *__restrict__ pool, const RsPart *__restrict__ tasks, RsArgs a,
+                                                 double *__restrict__ buf, double *__restrict__ Y,
+                                                 double *__restrict__ X, double *__restrict__ part)
+{
+    __shared__ double sv[RKCHUNK * RTILE];
+    __shared__ double sacc[4][RTILE];
+    const RsPart tk = tasks[blockIdx.x];
+    const int s = tk.s, I = tk.I;
+    const int32_t F = a.Fs[s], c = a.ncol[s], r = F - c;
+    const int orow0 = I * RTILE;
+    const int on = min(RTILE, (MODE == 1 ? r : c) - orow0);
+    const int kdim = MODE == 2 ? r : c;
+    const int kbeg = tk.k0 * RTILE, kend = min(kdim, tk.k1 * RTILE);
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const T *L = pool + a.foff[s];
+    const int64_t c0 = a.col0[s], fr = a.frow[s];
+    const bool leaf = a.chptr[s + 1] == a.chptr[s];
+    const bool pull = !leaf && c <= RTILE;
+    {
+        const int kk = kbeg + (int)threadIdx.x;
+        double v = 0.0;
+        if (kk < kend) {
+            if (MODE == 0) v = pull ? vs_pull(a, buf, fr + kk) : buf[(leaf ? c0 : a.t0 + c0) + kk];
+            else if (MODE == 1) v = Y[c0 + kk];
+            else if (MODE == 2) v = X[a.rows[a.rptr[s] + kk]];
+            else v = buf[a.t0 + c0 + kk];
+        }
+        sv[threadIdx.x] = v;
+    }
+    __syncthreads();
+    double res = 0.0;              // modes 0, 1: this thread's row (lane), wave 0 after the sum
+    if (MODE == 0 || MODE == 1) {
+        const int K = tk.k0 + wv;
+        double acc = 0.0;
+        if (K < tk.k1 && lane < on) {
+            const int row = orow0 + lane;
+            const int kn = min(RTILE, kdim - K * RTILE);
+            const T *base = L + (MODE == 1 ? c + row : row);
+            for (int k = 0; k < kn; k++) {
+                const int kk = K * RTILE + k;
+                if (MODE == 0 && K == I && k > lane) break;     // X11 lower triangular
+                acc += (double)base[(int64_t)kk * F] * sv[kk - kbeg];
+            }
+        }
+        sacc[wv][lane] = acc;
+        __syncthreads();
+        if (wv != 0) return;
+        res = sacc[0][lane] + sacc[1][lane] + sacc[2][lane] + sacc[3][lane];
+    } else {
+        double acc[16];
+#pragma unroll
+        for (int jj = 0; jj < 16; jj++) acc[jj] = 0.0;
+        for (int K = tk.k0; K < tk.k1; K++) {
+            const int kk = K * RTILE + lane;
+            if (kk >= kend) continue;
+            const double xv = sv[kk - kbeg];
+#pragma unroll
+            for (int jj = 0; jj < 16; jj++) {
+                const int j = 16 * wv + jj;
+                if (j >= on) continue;
+                const T *col = L + (int64_t)(orow0 + j) * F;
+                if (MODE == 2) acc[jj] += (double)col[c + kk] * xv;
+                else if (kk >= orow0 + j) acc[jj] += (double)col[kk] * xv;   // X11^T: k >= j
+            }
+        }
+#pragma unroll
+        for (int jj = 0; jj < 16; jj++) {
+            double v = acc[jj];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+            acc[jj] = v;
+        }
+        // lane jj of each wave now holds column 16 wv + jj's sum
+        double mine = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < 16; jj++)
+            if (lane == jj) mine = acc[jj];
+        if (lane < 16) sacc[0][16 * wv + lane] = mine;
+        __syncthreads();
+        if (wv != 0) return;
+        res = sacc[0][lane];
+    }
+    // wave 0, lane = output row
+    if (lane >= on) return;
+    if (tk.direct) {
+        if (MODE == 0) Y[c0 + orow0 + lane] = res;
+        else if (MODE == 1) {
+            const int64_t u = a.n + a.uoff[s] + orow0 + lane;
+            buf[u] = (leaf ? 0.0 : buf[u]) - res;
+        } else if (MODE == 2) buf[a.t0 + c0 + orow0 + lane] = Y[c0 + orow0 + lane] - res;
+        else X[c0 + orow0 + lane] = res;
+    } else {
+        part[(int64_t)blockIdx.x * RTILE + lane] = res;
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(64) k_vs_fin(const RsFin *__restrict__ tasks, RsArgs a, double *__restrict__ buf,
+                                               double *__restrict__ Y, double *__restrict__ X,
+                                               const double *__restrict__ part)
+{
+    const RsFin t = tasks[blockIdx.x];
+    const int s = t.s, lane = threadIdx.x;
+    const int32_t F = a.Fs[s], c = a.ncol[s];
+    const int orow0 = t.I * RTILE;
+    const int on = min(RTILE, (MODE == 1 ? F - c : c) - orow0);
+    if (lane >= on) return;
+    double res = 0.0;
+    for (int q = 0; q < t.np; q++) res += part[(t.p0 + q) * RTILE + lane];
+    const int64_t c0 = a.col0[s];
+    const bool leaf = a.chptr[s + 1] == a.chptr[s];
+    if (MODE == 0) Y[c0 + orow0 + lane] = res;
+    else if (MODE == 1) {
+        const int64_t u = a.n + a.uoff[s] + orow0 + lane;
+        buf[u] = (leaf ? 0.0 : buf[u]) - res;
+    } else if (MODE == 2) buf[a.t0 + c0 + orow0 + lane] = Y[c0 + orow0 + lane] - res;
+    else X[c0 + orow0 + lane] = res;
+}
+
 // ---- plan -----------------------------------------------------------------
 struct RsLevel {
     int64_t sm_off = 0, sm_cnt = 0;          // tiny fronts, by class (tiny_class)
@@ -706,6 +948,9 @@ struct RsState {
     int64_t *frow = nullptr, *uoff = nullptr, *lptr = nullptr;
     int32_t *lst = nullptr;
     int64_t n = 0, t0 = 0;
+    // the single-right-hand-side solve's buffers (one FP64 value per row)
+    bool rows = false, vec = false;            // which buffers the plan's upload allocates
+    double *vbuf = nullptr, *vY = nullptr, *vX = nullptr, *vpart = nullptr;
     // the level launches of one solve captured as a graph (opt.use_graph):
     // they read and write only the plan's own buffers and the factor's pool,
     // so one capture serves every later solve of this factor
@@ -925,10 +1170,18 @@ int rs_upload(Factor *F, RsState &B, hipStream_t st)
         (rc = up(&B.pt, H.pt)) || (rc = up(&B.ft, H.ft)) || (rc = up(&B.frow, H.frow)) || (rc = up(&B.uoff, H.uoff)) ||
         (rc = up(&B.lptr, H.lptr)) || (rc = up(&B.lst, H.lst)))
         return rc;
-    if ((rc = B.mem.alloc(&B.part, (size_t)H.maxpart * RT_WIDE * RW))) return rc;
-    if ((rc = B.mem.alloc(&B.buf, (size_t)(2 * n + H.sumr) * RW))) return rc;
-    if ((rc = B.mem.alloc(&B.Yg, (size_t)n * RW))) return rc;
-    if ((rc = B.mem.alloc(&B.Xg, (size_t)n * RW))) return rc;
+    if (B.rows) {
+        if ((rc = B.mem.alloc(&B.part, (size_t)H.maxpart * RT_WIDE * RW))) return rc;
+        if ((rc = B.mem.alloc(&B.buf, (size_t)(2 * n + H.sumr) * RW))) return rc;
+        if ((rc = B.mem.alloc(&B.Yg, (size_t)n * RW))) return rc;
+        if ((rc = B.mem.alloc(&B.Xg, (size_t)n * RW))) return rc;
+    }
+    if (B.vec) {
+        if ((rc = B.mem.alloc(&B.vpart, (size_t)H.maxpart * RTILE))) return rc;
+        if ((rc = B.mem.alloc(&B.vbuf, (size_t)(2 * n + H.sumr)))) return rc;
+        if ((rc = B.mem.alloc(&B.vY, (size_t)n))) return rc;
+        if ((rc = B.mem.alloc(&B.vX, (size_t)n))) return rc;
+    }
     B.host = RsHost();
     return 0;
 }
@@ -1114,7 +1367,105 @@ int rs_run(Factor *F, RsState &B, const double *Bin, int64_t brs, int64_t bcs, d
     return 0;
 }
 
+// one right-hand side: b, x device FP64 vectors of n (original order)
+template <class T>
+void vs_run(Factor *F, RsState &B, const double *b, double *x)
+{
+    const Symbolic &S = F->S;
+    hipStream_t st = F->st;
+    const T *pool = (const T *)F->pool;
+    const RsArgs a{F->foff, F->col0, F->rptr, F->chptr, B.frow, B.uoff, B.lptr, F->Fs, F->ncol, F->rows, B.lst, B.n,
+                   B.t0};
+    const dim3 blk(256);
+    const unsigned nb = (unsigned)((F->n + 255) / 256);
+    k_vs_in<<<dim3(nb), blk, 0, st>>>(b, F->perm, F->n, B.vbuf);
+    auto parts = [&](int mode, const RsLevel &P) {
+        const int64_t pc = P.pt_cnt[mode], fc = P.ft_cnt[mode];
+        const RsPart *pt = B.pt + P.pt_off[mode];
+        const RsFin *ft = B.ft + P.ft_off[mode];
+        const dim3 gp((unsigned)pc), gf((unsigned)fc), b64(64);
+        switch (mode) {
+        case 0:
+            if (pc) k_vs_part<T, 0><<<gp, blk, 0, st>>>(pool, pt, a, B.vbuf, B.vY, B.vX, B.vpart);
+            if (fc) k_vs_fin<0><<<gf, b64, 0, st>>>(ft, a, B.vbuf, B.vY, B.vX, B.vpart);
+            break;
+        case 1:
+            if (pc) k_vs_part<T, 1><<<gp, blk, 0, st>>>(pool, pt, a, B.vbuf, B.vY, B.vX, B.vpart);
+            if (fc) k_vs_fin<1><<<gf, b64, 0, st>>>(ft, a, B.vbuf, B.vY, B.vX, B.vpart);
+            break;
+        case 2:
+            if (pc) k_vs_part<T, 2><<<gp, blk, 0, st>>>(pool, pt, a, B.vbuf, B.vY, B.vX, B.vpart);
+            if (fc) k_vs_fin<2><<<gf, b64, 0, st>>>(ft, a, B.vbuf, B.vY, B.vX, B.vpart);
+            break;
+        default:
+            if (pc) k_vs_part<T, 3><<<gp, blk, 0, st>>>(pool, pt, a, B.vbuf, B.vY, B.vX, B.vpart);
+            if (fc) k_vs_fin<3><<<gf, b64, 0, st>>>(ft, a, B.vbuf, B.vY, B.vX, B.vpart);
+        }
+    };
+    for (int32_t L = 0; L < S.nlevels; L++) {
+        const RsLevel &P = B.lev[L];
+        if (P.as_cnt)
+            k_vs_assemble<<<dim3((unsigned)((P.as_cnt + VS_ASM - 1) / VS_ASM)), blk, 0, st>>>(B.asmt + P.as_off,
+                                                                                            P.as_cnt, a, B.vbuf);
+        if (P.sm_cnt)
+            k_vs_tiny_fwd<T><<<dim3((unsigned)((P.sm_cnt + 3) / 4)), blk, 0, st>>>(pool, B.small + P.sm_off,
+                                                                                  (int)P.sm_cnt, a, B.vbuf, B.vY);
+        parts(0, P);
+        parts(1, P);
+    }
+    if (F->signed_fac) k_vs_sign<T><<<dim3(nb), blk, 0, st>>>(B.vY, (const T *)F->sgn, F->n);
+    for (int32_t L = S.nlevels - 1; L >= 0; L--) {
+        const RsLevel &P = B.lev[L];
+        parts(2, P);
+        parts(3, P);
+        if (P.sm_cnt)
+            k_vs_tiny_bwd<T><<<dim3((unsigned)((P.sm_cnt + 3) / 4)), blk, 0, st>>>(pool, B.small + P.sm_off,
+                                                                                  (int)P.sm_cnt, a, B.vY, B.vX);
+    }
+    k_vs_out<<<dim3(nb), blk, 0, st>>>(B.vX, F->perm, F->n, x);
+}
+
+// join the plan's helper thread and its upload stream
+int rs_ready(RsState &B)
+{
+    if (B.ready) return 0;
+    if (B.th.joinable()) B.th.join();
+    if (B.plan_rc) {
+        note_error(B.plan_msg.c_str());
+        return B.plan_rc;
+    }
+    SEHIP_CHECK(hipStreamSynchronize(B.up_st));
+    B.ready = true;
+    return 0;
+}
+
 }  // namespace
+
+int factor_vec_prepare(Factor *F)
+{
+    rowsolve_prepare_async(F);
+    RsState &B = *reinterpret_cast<RsState *>(F->rs);
+    if (!B.vec) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "vector solve: the plan was made without its buffers");
+    int rc = rs_ready(B);
+    if (rc) return rc;
+    return factor_invert_l11(F);
+}
+
+bool factor_vec_prepared(const Factor *F)
+{
+    const RsState *B = reinterpret_cast<const RsState *>(F->rs);
+    return B && B->ready && B->vec && F->inverted;
+}
+
+int factor_vec_solve(Factor *F, const double *b, double *x)
+{
+    int rc = factor_vec_prepare(F);
+    if (rc) return rc;
+    RsState &B = *reinterpret_cast<RsState *>(F->rs);
+    if (F->fp32) vs_run<float>(F, B, b, x);
+    else vs_run<double>(F, B, b, x);
+    return 0;
+}
 
 int factor_block_solve_rows(Factor *F, const double *Bin, int64_t brs, int64_t bcs, double *Xout, int64_t xrs,
                             int64_t xcs, int nb)
@@ -1123,15 +1474,9 @@ int factor_block_solve_rows(Factor *F, const double *Bin, int64_t brs, int64_t b
     if (!F->inverted) return SPEIGS_ERR_FACTOR;
     rowsolve_prepare_async(F);
     RsState &B = *reinterpret_cast<RsState *>(F->rs);
-    if (!B.ready) {
-        if (B.th.joinable()) B.th.join();
-        if (B.plan_rc) {
-            note_error(B.plan_msg.c_str());
-            return B.plan_rc;
-        }
-        SEHIP_CHECK(hipStreamSynchronize(B.up_st));
-        B.ready = true;
-    }
+    if (!B.rows) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "block solve: the plan was made without its buffers");
+    int rc0 = rs_ready(B);
+    if (rc0) return rc0;
     const int e = ev_begin(F);
     const int rc = F->fp32 ? rs_run<float>(F, B, Bin, brs, bcs, Xout, xrs, xcs, nb)
                            : rs_run<double>(F, B, Bin, brs, bcs, Xout, xrs, xcs, nb);
@@ -1147,6 +1492,9 @@ void rowsolve_prepare_async(Factor *F)
     if (F->rs) return;
     RsState *B = new RsState();
     F->rs = B;
+    // the block path's row buffers, the single-vector path's value buffers
+    B->rows = F->opt.precision == SPEIGS_HIP_FP32_BLOCK;
+    B->vec = F->solve_mode == 3;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) (void)hipGetLastError();
     B->th = std::thread([F, B, dev] {

@@ -55,6 +55,7 @@ def test_solve_wait_failure_is_reported(hiplib, monkeypatch):
     n = len(A[0]) - 1
     B = np.asfortranarray(np.random.default_rng(5).standard_normal((n, 1)))
     X = np.zeros((n, 1), order="F")
+    monkeypatch.setenv("SPEIGS_HIP_SOLVE_MODE", "1")      # the blocked solve has the waits
     monkeypatch.setenv("SPEIGS_HIP_WAIT_TEST", "1")
     rc_, op, _ = _op(hiplib, A, 1, 0, 0.0)
     assert rc_ == 0
@@ -260,7 +261,7 @@ def test_potrf_waves_bitwise(hiplib, monkeypatch, prec):
 
 
 @pytest.mark.parametrize("mode,mfma,prec", [("0", "0", 0), ("1", "0", 0), ("1", "1", 0), ("1", "0", 1), ("1", "1", 1),
-                                            ("2", "1", 0), ("2", "1", 1)])
+                                            ("2", "1", 0), ("2", "1", 1), ("3", "1", 0), ("3", "1", 1)])
 def test_kernel_variants_agree(hiplib, monkeypatch, mode, mfma, prec):
     """one-workgroup-per-front vs blocked multi-workgroup solves, FMA vs MFMA
     Schur updates: the same solution to rounding (46^3 has fronts of ~2000)."""
@@ -278,6 +279,29 @@ def test_kernel_variants_agree(hiplib, monkeypatch, mode, mfma, prec):
     print("mode", mode, "mfma", mfma, "residual", rel)
     assert rel <= 1e-12
     hiplib.speigs_hip_op_destroy(op)
+
+
+def test_vector_solve_lanczos_matches_chained(hiplib, monkeypatch):
+    """the single-vector solve by the inverted diagonal blocks over the block
+    solve's plan (mode 3: chosen by default at tolerances >= 1e-12, where one
+    such solve per apply suffices) and the blocked chained solve (mode 1, the
+    default at tighter tolerances) give the same eigenpairs: at the reference's
+    default tolerance (1e-14 ||A||, mode 3 forced, its applies refined against
+    A) the values agree to 1e-12 relative, at 1e-10 to 1e-9."""
+    A = gen.weighted_lap3d(30, seed=12, upper=True)
+    h = {"precision": 0}
+    r3l = _solve(A, 8, "sm", opts={"tol": 1e-10}, hip_opts=h)
+    monkeypatch.setenv("SPEIGS_HIP_SOLVE_MODE", "3")
+    r3 = _solve(A, 8, "sm", hip_opts=h)
+    monkeypatch.setenv("SPEIGS_HIP_SOLVE_MODE", "1")
+    r1 = _solve(A, 8, "sm", hip_opts=h)
+    r1l = _solve(A, 8, "sm", opts={"tol": 1e-10}, hip_opts=h)
+    for r in (r3, r3l, r1, r1l):
+        assert r.rc == 0 and r.info["flag"] == 0
+    assert np.abs(r3.lam - r1.lam).max() <= 1e-12 * np.abs(r1.lam).max()
+    assert np.abs(r3l.lam - r1l.lam).max() <= 1e-9 * np.abs(r1.lam).max()
+    print("ir steps", r3.hip_info["ir_steps"], r3l.hip_info["ir_steps"], r1.hip_info["ir_steps"])
+    assert r3.hip_info["ir_steps"] > 0 and r1.hip_info["ir_steps"] == 0
 
 
 def test_inverse_solve_weighted_40(hiplib, monkeypatch):
