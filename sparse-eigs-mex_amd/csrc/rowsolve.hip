@@ -342,6 +342,358 @@ __global__ void __launch_bounds__(256) k_rs_tiny_bwd(const T *__restrict__ pool,
     }
 }
 
+// ---- narrow fronts of the lowest levels (c <= 64 columns, r <= 512) --------
+// A front whose columns fit one 64-row tile is solved by waves that read its
+// right-hand side W where it is needed: the W rows of a front with children
+// are summed from their pull lists inside the loads (no assembly pass), a
+// leaf's W1 is its P rows and its W2 zero.  Vector rows go through the wave's
+// own LDS in row layout (lane l: row l / 4, floats 8 (l % 4) .. +7, two
+// 16-byte loads per source row) and are read back in MFMA operand layout.
+// NQ = 16-row blocks of c (1, 2 or 4: the launch's class); blocks past c hold
+// zeros, so their products add exact zeros.  Operand layout as for every
+// product here (MFMA step (kq, t), lane l = 16 g + lc): the product index is
+// k = 16 kq + 4 g + t, the first operand A[16 rb + lc][k], the second
+// V[k][16 h + lc]; result register v of lane l is row 16 rb + 4 g + v,
+// right-hand side 16 h + lc.
+// A narrow front's task carries everything its first loads need, read by
+// scalar loads (the task index is wave-uniform): the chain before the first
+// operand load is one load instead of an index, then the front's fields.
+struct RsNTask {                 // forward: update rows [u0, u0 + NTR) of a front
+    int64_t foff, c0, fr, ub;    // factor panel, first column, first row id, first U row of the task
+    int32_t F, c, u0, leaf;
+};
+struct RsNFront {                // backward: a front
+    int64_t foff, c0, rptr;      // factor panel, first column, first update-row index
+    int32_t F, c;
+};
+
+constexpr int NTR = 64;          // update rows per forward task
+
+// row i of front s's W (zero when i >= lim): a leaf's P row, else the sum of
+// the row's pull list; 8 floats from col
+__device__ __forceinline__ void w_row8(const RsArgs &a, const float *__restrict__ buf, int64_t fr, int64_t c0,
+                                       bool leaf, int i, int lim, int col, float4 (&o)[2])
+{
+    o[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+    o[1] = o[0];
+    if (i >= lim) return;
+    if (leaf) {
+        const float *src = buf + (c0 + i) * RW + col;
+        o[0] = *reinterpret_cast<const float4 *>(src);
+        o[1] = *reinterpret_cast<const float4 *>(src + 4);
+        return;
+    }
+    pull_sum8(a, buf, fr + i, col, o);
+}
+
+__device__ __forceinline__ void stage8(float (*S)[RP], int row, int col, const float4 (&v)[2])
+{
+    *reinterpret_cast<float4 *>(&S[row][col]) = v[0];
+    *reinterpret_cast<float4 *>(&S[row][col + 4]) = v[1];
+}
+
+// forward, one wave per task (front s, update rows [u0, u0 + NTR)):
+//   Y = X11 W1 (every task; the task with u0 = 0 stores it), U = W2 - L21 Y
+//   for the task's update rows, 16 at a time with the next 16 rows' L21 and
+//   W2 loads in flight during the current products
+template <class T, int NQ>
+__global__ void __launch_bounds__(256) k_rs_nfwd(const T *__restrict__ pool, const RsNTask *__restrict__ tasks, int nt,
+                                                 RsArgs a, float *__restrict__ buf, float *__restrict__ Yg)
+{
+    __shared__ __align__(16) float Ws[4][16 * NQ][RP];   // the wave's W1, then Y
+    __shared__ __align__(16) float Us[4][16][RP];        // W2 of the current 16 update rows
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, lc = lane & 15;
+    const int ti = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
+    if (ti >= nt) return;       // per wave; no workgroup barrier below
+    const RsNTask tk = tasks[ti];
+    const int u0 = tk.u0;
+    const int32_t F = tk.F, c = tk.c, r = F - c;
+    const int nq = (c + 15) >> 4;                     // 16-row blocks of c (<= NQ)
+    const int un = min(NTR, r - u0);                  // the task's update rows (<= 0: none)
+    const int nbk = un > 0 ? (un + 15) >> 4 : 0;
+    const T *L = pool + tk.foff;
+    const int64_t c0 = tk.c0, fr = tk.fr, ub = tk.ub;
+    const bool leaf = tk.leaf != 0;
+    float (*Wq)[RP] = Ws[wv];
+    float (*Uq)[RP] = Us[wv];
+    const int srow = lane >> 2, scol = 8 * (lane & 3);
+
+    // every load that does not need Y first: W1, X11, the first 16 rows' L21 and W2
+    float4 w1[NQ][2];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) w_row8(a, buf, fr, c0, leaf, 16 * q + srow, c, scol, w1[q]);
+    float a11[NQ * (NQ + 1) / 2][4];
+#pragma unroll
+    for (int rb = 0; rb < NQ; rb++)
+#pragma unroll
+        for (int kq = 0; kq <= rb; kq++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int row = 16 * rb + lc, k = 16 * kq + 4 * g + t;
+                a11[rb * (rb + 1) / 2 + kq][t] = (row < c && k <= row) ? (float)L[(int64_t)k * F + row] : 0.f;
+            }
+    float a21[NQ][4];
+    float4 w2[2];
+    auto load_blk = [&](int b) {
+        const int lrow = u0 + 16 * b + lc;            // update row of this lane's L21 operand
+#pragma unroll
+        for (int kq = 0; kq < NQ; kq++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int k = 16 * kq + 4 * g + t;
+                a21[kq][t] = (lrow < r && k < c) ? (float)L[(int64_t)k * F + c + lrow] : 0.f;
+            }
+        w_row8(a, buf, fr, c0, leaf, c + u0 + 16 * b + srow, leaf ? 0 : F, scol, w2);
+    };
+    if (nbk > 0) load_blk(0);
+
+    // Y = X11 W1
+#pragma unroll
+    for (int q = 0; q < NQ; q++) stage8(Wq, 16 * q + srow, scol, w1[q]);
+    wave_lds_sync();
+    float wb[NQ][4][2];
+#pragma unroll
+    for (int kq = 0; kq < NQ; kq++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            wb[kq][t][0] = Wq[16 * kq + 4 * g + t][lc];
+            wb[kq][t][1] = Wq[16 * kq + 4 * g + t][16 + lc];
+        }
+    wave_lds_sync();            // W1 read before Y overwrites it
+#pragma unroll
+    for (int rb = 0; rb < NQ; rb++) {
+        if (rb >= nq) break;          // blocks past c: zero rows, never read below
+        f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
+#pragma unroll
+        for (int kq = 0; kq <= rb; kq++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                acc0 = mfma4(a11[rb * (rb + 1) / 2 + kq][t], wb[kq][t][0], acc0);
+                acc1 = mfma4(a11[rb * (rb + 1) / 2 + kq][t], wb[kq][t][1], acc1);
+            }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int yr = 16 * rb + 4 * g + v;
+            Wq[yr][lc] = acc0[v];
+            Wq[yr][16 + lc] = acc1[v];
+            if (u0 == 0 && yr < c) {
+                Yg[(c0 + yr) * RW + lc] = acc0[v];
+                Yg[(c0 + yr) * RW + 16 + lc] = acc1[v];
+            }
+        }
+    }
+    if (nbk == 0) return;
+    wave_lds_sync();
+    float yb[NQ][4][2];
+#pragma unroll
+    for (int kq = 0; kq < NQ; kq++)
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            yb[kq][t][0] = kq < nq ? Wq[16 * kq + 4 * g + t][lc] : 0.f;
+            yb[kq][t][1] = kq < nq ? Wq[16 * kq + 4 * g + t][16 + lc] : 0.f;
+        }
+
+    // U = W2 - L21 Y, 16 update rows at a time
+#pragma unroll
+    for (int b = 0; b < NTR / 16; b++) {
+        if (b >= nbk) break;
+        // this block's operands, then the next block's loads (into the same
+        // names: the compiler keeps both sets live, so no wait comes between)
+        float ca[NQ][4];
+#pragma unroll
+        for (int kq = 0; kq < NQ; kq++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) ca[kq][t] = a21[kq][t];
+        const float4 cw[2] = {w2[0], w2[1]};
+        if (b + 1 < nbk) load_blk(b + 1);
+        stage8(Uq, srow, scol, cw);
+        wave_lds_sync();
+        float o0[4], o1[4];
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            o0[v] = Uq[4 * g + v][lc];
+            o1[v] = Uq[4 * g + v][16 + lc];
+        }
+        f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
+#pragma unroll
+        for (int kq = 0; kq < NQ; kq++) {
+            if (kq >= nq) break;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                acc0 = mfma4(ca[kq][t], yb[kq][t][0], acc0);
+                acc1 = mfma4(ca[kq][t], yb[kq][t][1], acc1);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int ur = 16 * b + 4 * g + v;
+            if (ur < un) {
+                buf[(ub + ur) * RW + lc] = o0[v] - acc0[v];
+                buf[(ub + ur) * RW + 16 + lc] = o1[v] - acc1[v];
+            }
+        }
+        wave_lds_sync();        // this block's W2 read before the next block's is staged
+    }
+}
+
+// backward: Z = Y - L21^T X_R over the update rows in 64-row chunks, then
+// X_c = X11^T Z.  WPF waves per front: 1 (the wave takes every chunk, four
+// fronts per workgroup) or 4 (wave w takes chunks w, w + 4, ...; the partial
+// sums are added in wave order through LDS, and wave rb forms output block rb).
+template <class T, int NQ, int WPF>
+__global__ void __launch_bounds__(256) k_rs_nbwd(const T *__restrict__ pool, const RsNFront *__restrict__ fronts,
+                                                 int nf, RsArgs a, const float *__restrict__ Yg,
+                                                 float *__restrict__ Xg)
+{
+    __shared__ __align__(16) float Xs[4][64][RP];   // the wave's X_R chunk, then its (partial) Z
+    __shared__ __align__(16) float Zs[WPF == 4 ? 16 * NQ : 1][RP];   // WPF 4: the front's Z
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 4, lc = lane & 15;
+    const int fi = __builtin_amdgcn_readfirstlane(WPF == 1 ? blockIdx.x * 4 + wv : (int)blockIdx.x);
+    if (fi >= nf) return;       // WPF 4: uniform over the workgroup (one front each)
+    const int sub = WPF == 1 ? 0 : wv;
+    const RsNFront fd = fronts[fi];
+    const int32_t F = fd.F, c = fd.c, r = F - c;
+    const T *L = pool + fd.foff;
+    const int64_t c0 = fd.c0;
+    const int32_t *R = a.rows + fd.rptr;
+    float (*Xq)[RP] = Xs[wv];
+    const int srow = lane >> 2, scol = 8 * (lane & 3);
+
+    f32x4 acc[NQ][2];
+#pragma unroll
+    for (int rb = 0; rb < NQ; rb++) {
+        acc[rb][0] = f32x4{0.f, 0.f, 0.f, 0.f};
+        acc[rb][1] = acc[rb][0];
+    }
+    const int nch = (r + 63) >> 6;
+    for (int ch = sub; ch < nch; ch += WPF) {
+        const int k0 = ch * 64;
+        int32_t gi[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int k = k0 + 16 * q + srow;
+            gi[q] = k < r ? R[k] : -1;
+        }
+        // L21^T operand: A[j][k] = L(c + k, j)
+        float at[NQ][4][4];
+#pragma unroll
+        for (int rb = 0; rb < NQ; rb++)
+#pragma unroll
+            for (int kq = 0; kq < 4; kq++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int j = 16 * rb + lc, k = k0 + 16 * kq + 4 * g + t;
+                    at[rb][kq][t] = (j < c && k < r) ? (float)L[(int64_t)j * F + c + k] : 0.f;
+                }
+        float4 xr[4][2];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            xr[q][0] = make_float4(0.f, 0.f, 0.f, 0.f);
+            xr[q][1] = xr[q][0];
+            if (gi[q] >= 0) {
+                const float *src = Xg + (int64_t)gi[q] * RW + scol;
+                xr[q][0] = *reinterpret_cast<const float4 *>(src);
+                xr[q][1] = *reinterpret_cast<const float4 *>(src + 4);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) stage8(Xq, 16 * q + srow, scol, xr[q]);
+        wave_lds_sync();
+#pragma unroll
+        for (int kq = 0; kq < 4; kq++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const float b0 = Xq[16 * kq + 4 * g + t][lc], b1 = Xq[16 * kq + 4 * g + t][16 + lc];
+#pragma unroll
+                for (int rb = 0; rb < NQ; rb++) {
+                    acc[rb][0] = mfma4(at[rb][kq][t], b0, acc[rb][0]);
+                    acc[rb][1] = mfma4(at[rb][kq][t], b1, acc[rb][1]);
+                }
+            }
+        wave_lds_sync();        // the chunk read before the next one is staged
+    }
+
+    float (*Zq)[RP] = WPF == 1 ? Xq : Zs;
+    if (WPF == 1) {
+        // Z = Y - L21^T X_R, into the wave's LDS
+#pragma unroll
+        for (int rb = 0; rb < NQ; rb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int zr = 16 * rb + 4 * g + v;
+                const float y0 = zr < c ? Yg[(c0 + zr) * RW + lc] : 0.f;
+                const float y1 = zr < c ? Yg[(c0 + zr) * RW + 16 + lc] : 0.f;
+                Xq[zr][lc] = y0 - acc[rb][0][v];
+                Xq[zr][16 + lc] = y1 - acc[rb][1][v];
+            }
+        wave_lds_sync();
+    } else {
+        // the partial sums to LDS, then 16 rows of Z per wave: Y minus the four
+        // partials added in wave order
+#pragma unroll
+        for (int rb = 0; rb < NQ; rb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int zr = 16 * rb + 4 * g + v;
+                Xq[zr][lc] = acc[rb][0][v];
+                Xq[zr][16 + lc] = acc[rb][1][v];
+            }
+        __syncthreads();
+        if (wv < NQ) {
+            const int zr = 16 * wv + srow;
+            float4 y[2];
+            y[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+            y[1] = y[0];
+            if (zr < c) {
+                const float *src = Yg + (c0 + zr) * RW + scol;
+                y[0] = *reinterpret_cast<const float4 *>(src);
+                y[1] = *reinterpret_cast<const float4 *>(src + 4);
+            }
+            float4 z[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                float4 p = *reinterpret_cast<const float4 *>(&Xs[0][zr][scol + 4 * h]);
+#pragma unroll
+                for (int w = 1; w < 4; w++) {
+                    const float4 q = *reinterpret_cast<const float4 *>(&Xs[w][zr][scol + 4 * h]);
+                    p.x += q.x;
+                    p.y += q.y;
+                    p.z += q.z;
+                    p.w += q.w;
+                }
+                z[h] = make_float4(y[h].x - p.x, y[h].y - p.y, y[h].z - p.z, y[h].w - p.w);
+            }
+            stage8(Zs, zr, scol, z);
+        }
+        __syncthreads();
+    }
+
+    // X_c = X11^T Z (A[j][k] = L(k, j), k >= j); WPF 4: wave rb forms block rb
+#pragma unroll
+    for (int rb = 0; rb < NQ; rb++) {
+        if (WPF == 4 && rb != wv) continue;
+        f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0;
+        const int j = 16 * rb + lc;
+#pragma unroll
+        for (int kq = rb; kq < NQ; kq++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int k = 16 * kq + 4 * g + t;
+                const float av = (j < c && k < c && k >= j) ? (float)L[(int64_t)j * F + k] : 0.f;
+                acc0 = mfma4(av, Zq[k][lc], acc0);
+                acc1 = mfma4(av, Zq[k][16 + lc], acc1);
+            }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int xr = 16 * rb + 4 * g + v;
+            if (xr < c) {
+                Xg[(c0 + xr) * RW + lc] = acc0[v];
+                Xg[(c0 + xr) * RW + 16 + lc] = acc1[v];
+            }
+        }
+    }
+}
+
 // ---- tile products of the other fronts ---------------------------------------
 // W of the level's fronts with children: W1 rows to T (row t0 + c0 + i), W2
 // rows to U (the plan leaves out what is read in place: leaves, and the W1 of
@@ -459,6 +811,47 @@ __device__ __forceinline__ void rs_load_b(const RsArgs &a, int64_t c0, int64_t f
     }
 }
 
+// The finish of a split output tile: the partial sums of its tasks added in
+// a fixed order, then the tile written as a direct task would.
+template <int MODE, int RT>
+__device__ __forceinline__ void fin_tile(const RsFin &t, const RsArgs &a, float *__restrict__ buf,
+                                         float *__restrict__ Yg, float *__restrict__ Xg,
+                                         const float *__restrict__ part)
+{
+    constexpr int PER = RT * RW / 256;                // elements per thread
+    const int s = t.s;
+    const int32_t F = a.Fs[s], c = a.ncol[s];
+    const int orow0 = t.I * RT;
+    const int on = min(RT, (MODE == 1 ? F - c : c) - orow0);
+    const int64_t c0 = a.col0[s];
+    const bool leaf = a.chptr[s + 1] == a.chptr[s];   // W2 = 0
+    // partial sums in a fixed order, each step's PER loads independent
+    float sum[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) sum[u] = 0.f;
+    const float *p = part + t.p0 * (RT * RW) + threadIdx.x;
+    for (int q = 0; q < t.np; q++)
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            sum[u] += p[(int64_t)q * (RT * RW) + 256 * u];
+        }
+    const float *src = MODE == 1 ? buf + (a.n + a.uoff[s] + orow0) * RW : Yg + (c0 + orow0) * RW;
+    float *dst = MODE == 0 ? Yg + (c0 + orow0) * RW
+               : MODE == 1 ? buf + (a.n + a.uoff[s] + orow0) * RW
+               : MODE == 2 ? buf + (a.t0 + c0 + orow0) * RW : Xg + (c0 + orow0) * RW;
+    float o[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int e = threadIdx.x + 256 * u;
+        o[u] = (((MODE == 1 && !leaf) || MODE == 2) && (e >> 5) < on) ? src[e] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const int e = threadIdx.x + 256 * u;
+        if ((e >> 5) < on) dst[e] = (MODE == 1 || MODE == 2) ? o[u] - sum[u] : sum[u];
+    }
+}
+
 template <class T, int MODE, int RT>
 __global__ void __launch_bounds__(256) k_rs_part(const T *__restrict__ pool, const RsPart *__restrict__ tasks, RsArgs a,
                                                  float *__restrict__ buf, float *__restrict__ Yg,
@@ -546,8 +939,8 @@ __global__ void __launch_bounds__(256) k_rs_part(const T *__restrict__ pool, con
         }
         __syncthreads();
     }
-    if (!active) return;
     if (tk.direct) {
+        if (!active) return;
         // modes 1, 2 read before they write: all reads first (a store between
         // them would serialize the loads)
         float o0[NB][4], o1[NB][4];
@@ -578,6 +971,7 @@ __global__ void __launch_bounds__(256) k_rs_part(const T *__restrict__ pool, con
                 }
             }
     } else {
+        if (!active) return;
         float *out = part + (int64_t)blockIdx.x * (RT * RW);
 #pragma unroll
         for (int nb = 0; nb < NB; nb++)
@@ -595,37 +989,7 @@ __global__ void __launch_bounds__(256) k_rs_fin(const RsFin *__restrict__ tasks,
                                                 float *__restrict__ Yg, float *__restrict__ Xg,
                                                 const float *__restrict__ part)
 {
-    constexpr int PER = RT * RW / 256;                // elements per thread
-    const RsFin t = tasks[blockIdx.x];
-    const int s = t.s;
-    const int32_t F = a.Fs[s], c = a.ncol[s];
-    const int orow0 = t.I * RT;
-    const int on = min(RT, (MODE == 1 ? F - c : c) - orow0);
-    const int64_t c0 = a.col0[s];
-    const bool leaf = a.chptr[s + 1] == a.chptr[s];   // W2 = 0
-    // partial sums in a fixed order, each step's PER loads independent
-    float sum[PER];
-#pragma unroll
-    for (int u = 0; u < PER; u++) sum[u] = 0.f;
-    const float *p = part + t.p0 * (RT * RW) + threadIdx.x;
-    for (int q = 0; q < t.np; q++)
-#pragma unroll
-        for (int u = 0; u < PER; u++) sum[u] += p[(int64_t)q * (RT * RW) + 256 * u];
-    const float *src = MODE == 1 ? buf + (a.n + a.uoff[s] + orow0) * RW : Yg + (c0 + orow0) * RW;
-    float *dst = MODE == 0 ? Yg + (c0 + orow0) * RW
-               : MODE == 1 ? buf + (a.n + a.uoff[s] + orow0) * RW
-               : MODE == 2 ? buf + (a.t0 + c0 + orow0) * RW : Xg + (c0 + orow0) * RW;
-    float o[PER];
-#pragma unroll
-    for (int u = 0; u < PER; u++) {
-        const int e = threadIdx.x + 256 * u;
-        o[u] = (((MODE == 1 && !leaf) || MODE == 2) && (e >> 5) < on) ? src[e] : 0.f;
-    }
-#pragma unroll
-    for (int u = 0; u < PER; u++) {
-        const int e = threadIdx.x + 256 * u;
-        if ((e >> 5) < on) dst[e] = (MODE == 1 || MODE == 2) ? o[u] - sum[u] : sum[u];
-    }
+    fin_tile<MODE, RT>(tasks[blockIdx.x], a, buf, Yg, Xg, part);
 }
 
 // ---- permutation + precision change at the boundary -------------------------
@@ -916,11 +1280,24 @@ struct RsLevel {
     int64_t pt_off[6] = {}, pt_cnt[6] = {};
     int64_t ft_off[6] = {}, ft_cnt[6] = {};
     bool wide[2] = {false, false};
+    // block solve at the lowest levels (L < kNarrowLevels): the narrow fronts
+    // by class of c (16, 32, 64 columns) -- forward tasks, backward fronts
+    // with one wave or four -- and the lists of the level's other fronts (tiny
+    // classes, assembly chunks, product / finish tasks) without them
+    bool narrow = false;
+    int64_t nf_off[3] = {}, nf_cnt[3] = {};
+    int64_t n1_off[3] = {}, n1_cnt[3] = {}, n4_off[3] = {}, n4_cnt[3] = {};
+    int64_t rtc_off[9] = {}, rtc_cnt[9] = {};
+    int64_t ras_off = 0, ras_cnt = 0;
+    int64_t rpt_off[6] = {}, rpt_cnt[6] = {};
+    int64_t rft_off[6] = {}, rft_cnt[6] = {};
 };
 
 // host-side plan of the solve (task lists, pull lists), uploaded at first use
 struct RsHost {
     std::vector<int32_t> small, lst;
+    std::vector<RsNTask> ntask;
+    std::vector<RsNFront> nfront;
     std::vector<RsRow> asmt;
     std::vector<RsPart> pt;
     std::vector<RsFin> ft;
@@ -941,6 +1318,8 @@ struct RsState {
     DevMem mem;
     std::vector<RsLevel> lev;
     int32_t *small = nullptr;
+    RsNTask *ntask = nullptr;        // narrow fronts' forward tasks and backward records
+    RsNFront *nfront = nullptr;
     RsRow *asmt = nullptr;
     RsPart *pt = nullptr;
     RsFin *ft = nullptr;
@@ -1015,6 +1394,38 @@ void promote_tiny_classes(std::vector<int32_t> (&cls)[9])
             if (q != 3 * jc + jr && !cls[q].empty() && cls[q].size() < kTinyMin) move(q, 3 * jc + jr);
 }
 
+// Narrow fronts: the block solve's wave-per-front path at the lowest levels,
+// where fronts are many and small and a level's separate assembly, product
+// and finish launches each cost more than their work (measured at 100^3:
+// levels 0-2 went from 1.05 to 0.81 ms per solve; from level 3 on the tile
+// products are as fast or faster)
+constexpr int kNarrowLevels = 3;
+constexpr int64_t kNarrowRmax = 512;
+
+bool is_narrow(int64_t c, int64_t r) { return c <= RTILE && r <= kNarrowRmax; }
+
+int narrow_class(int64_t c) { return c <= 16 ? 0 : c <= 32 ? 1 : 2; }   // 16, 32, 64 columns
+
+// a class of fewer than kTinyMin entries joins the smallest larger class
+// present (its kernel solves the same front exactly)
+template <class E>
+void promote_narrow(std::vector<E> (&cls)[3])
+{
+    for (int q = 0; q < 2; q++) {
+        if (cls[q].empty() || cls[q].size() >= kTinyMin) continue;
+        for (int p = q + 1; p < 3; p++)
+            if (!cls[p].empty()) {
+                cls[p].insert(cls[p].end(), cls[q].begin(), cls[q].end());
+                cls[q].clear();
+                break;
+            }
+    }
+}
+
+// product tasks of one launch (first task pbase): a tile whose K range is one
+// chunk is a direct task; a longer one is split into chunks that leave
+// partial sums for a finish record; a mode-2 tile of a front without update
+// rows is a direct task with an empty K range (Z = Y)
 void plan_parts(const Symbolic &S, const std::vector<int64_t> &fronts, int mode, int64_t RT, std::vector<RsPart> &pt,
                 std::vector<RsFin> &ft, int64_t pbase)
 {
@@ -1029,11 +1440,10 @@ void plan_parts(const Symbolic &S, const std::vector<int64_t> &fronts, int mode,
             else if (mode == 2) { kb = 0; ke = nupd; }
             else { kb = I; ke = nt11; }
             const int64_t np = (ke - kb + RKCHUNK - 1) / RKCHUNK;
-            if (np == 1) {
+            if (np <= 1) {
                 pt.push_back({(int32_t)s, (int32_t)I, (int32_t)kb, (int32_t)ke, 1, 0});
                 continue;
             }
-            // np == 0 (mode 2 without update rows): the finish writes Z = Y
             RsFin f{(int32_t)s, (int32_t)I, (int64_t)pt.size() - pbase, (int32_t)np, 0};
             for (int64_t k0 = kb; k0 < ke; k0 += RKCHUNK)
                 pt.push_back({(int32_t)s, (int32_t)I, (int32_t)k0, (int32_t)std::min<int64_t>(ke, k0 + RKCHUNK), 0, 0});
@@ -1148,6 +1558,75 @@ int rs_plan(const Factor *F, RsState &B)
             maxpart = std::max<int64_t>(maxpart, P.pt_cnt[v]);
         }
     }
+    // the block solve's lists at the lowest levels: narrow fronts by class, the
+    // level's other fronts as above
+    for (int32_t L = 0; B.rows && L < std::min<int32_t>(kNarrowLevels, S.nlevels); L++) {
+        RsLevel &P = B.lev[L];
+        P.narrow = true;
+        std::vector<int64_t> big;
+        std::vector<RsNTask> fw[3];
+        std::vector<RsNFront> b1[3], b4[3];
+        std::vector<int32_t> cls[9];
+        for (int64_t s : bylevel[L]) {
+            const int64_t c = S.c1[s] - S.c0[s], r = S.F[s] - c;
+            if (is_narrow(c, r)) {
+                const int q = narrow_class(c);
+                const int32_t leaf = S.chptr[s + 1] == S.chptr[s];
+                for (int64_t I = 0; I == 0 || I * NTR < r; I++)
+                    fw[q].push_back({S.foff[s], S.c0[s], frow[s], n + uoff[s] + I * NTR, (int32_t)S.F[s], (int32_t)c,
+                                     (int32_t)(I * NTR), leaf});
+                (r <= 2 * 64 ? b1 : b4)[q].push_back({S.foff[s], S.c0[s], S.rptr[s], (int32_t)S.F[s], (int32_t)c});
+            } else if (is_tiny(c, r)) {
+                cls[tiny_class(c, r)].push_back((int32_t)s);
+            } else {
+                big.push_back(s);
+            }
+        }
+        promote_narrow(fw);
+        promote_narrow(b1);
+        promote_narrow(b4);
+        promote_tiny_classes(cls);
+        for (int q = 0; q < 3; q++) {
+            P.nf_off[q] = (int64_t)H.ntask.size();
+            P.nf_cnt[q] = (int64_t)fw[q].size();
+            H.ntask.insert(H.ntask.end(), fw[q].begin(), fw[q].end());
+            P.n1_off[q] = (int64_t)H.nfront.size();
+            P.n1_cnt[q] = (int64_t)b1[q].size();
+            H.nfront.insert(H.nfront.end(), b1[q].begin(), b1[q].end());
+            P.n4_off[q] = (int64_t)H.nfront.size();
+            P.n4_cnt[q] = (int64_t)b4[q].size();
+            H.nfront.insert(H.nfront.end(), b4[q].begin(), b4[q].end());
+        }
+        for (int q = 0; q < 9; q++) {
+            P.rtc_off[q] = (int64_t)small.size();
+            P.rtc_cnt[q] = (int64_t)cls[q].size();
+            small.insert(small.end(), cls[q].begin(), cls[q].end());
+        }
+        P.ras_off = (int64_t)asmt.size();
+        for (int32_t q = 0; q < 9; q++)
+            for (int32_t s : cls[q])
+                if (S.chptr[s + 1] != S.chptr[s])
+                    for (int64_t I = 0; I * 32 < S.F[s]; I++) asmt.push_back({s, (int32_t)I});
+        for (int64_t s : big) {
+            const int64_t c = S.c1[s] - S.c0[s];
+            if (S.chptr[s + 1] == S.chptr[s]) continue;
+            const int64_t i0 = c <= RTILE ? c : 0;      // first assembled row (as above)
+            for (int64_t I = i0 / 32; I * 32 < S.F[s]; I++) asmt.push_back({(int32_t)s, (int32_t)I});
+        }
+        P.ras_cnt = (int64_t)asmt.size() - P.ras_off;
+        for (int v = 0; v < 6; v++) {
+            const int mode = v < 4 ? v : v - 3;
+            const int64_t p0 = (int64_t)pt.size(), f0 = (int64_t)ft.size();
+            plan_parts(S, big, mode, v < 4 ? RTILE : RT_WIDE, pt, ft, p0);
+            P.rpt_off[v] = p0;
+            P.rpt_cnt[v] = (int64_t)pt.size() - p0;
+            P.rft_off[v] = f0;
+            P.rft_cnt[v] = (int64_t)ft.size() - f0;
+            maxpart = std::max<int64_t>(maxpart, P.rpt_cnt[v]);
+        }
+    }
+    if (H.ntask.empty()) H.ntask.push_back(RsNTask{});
+    if (H.nfront.empty()) H.nfront.push_back(RsNFront{});
     if (small.empty()) small.push_back(0);
     if (asmt.empty()) asmt.push_back({0, 0});
     if (pt.empty()) pt.push_back({0, 0, 0, 0, 0, 0});
@@ -1168,7 +1647,8 @@ int rs_upload(Factor *F, RsState &B, hipStream_t st)
     };
     if ((rc = up(&B.small, H.small)) || (rc = up(&B.asmt, H.asmt)) ||
         (rc = up(&B.pt, H.pt)) || (rc = up(&B.ft, H.ft)) || (rc = up(&B.frow, H.frow)) || (rc = up(&B.uoff, H.uoff)) ||
-        (rc = up(&B.lptr, H.lptr)) || (rc = up(&B.lst, H.lst)))
+        (rc = up(&B.lptr, H.lptr)) || (rc = up(&B.lst, H.lst)) || (rc = up(&B.ntask, H.ntask)) ||
+        (rc = up(&B.nfront, H.nfront)))
         return rc;
     if (B.rows) {
         if ((rc = B.mem.alloc(&B.part, (size_t)H.maxpart * RT_WIDE * RW))) return rc;
@@ -1190,13 +1670,14 @@ int rs_upload(Factor *F, RsState &B, hipStream_t st)
 template <class T, bool FWD, int NQ, int NR>
 void tiny_one(const RsLevel &P, int q, RsState &B, const T *pool, const RsArgs &a, hipStream_t st)
 {
-    const int64_t cnt = P.tc_cnt[q];
+    const int64_t cnt = P.narrow ? P.rtc_cnt[q] : P.tc_cnt[q];
     if (cnt <= 0) return;
+    const int64_t off = P.narrow ? P.rtc_off[q] : P.tc_off[q];
     const dim3 g((unsigned)((cnt + 3) / 4)), blk(256);
     if (FWD)
-        k_rs_tiny_fwd<T, NQ, NR><<<g, blk, 0, st>>>(pool, B.small + P.tc_off[q], (int)cnt, a, B.buf, B.Yg);
+        k_rs_tiny_fwd<T, NQ, NR><<<g, blk, 0, st>>>(pool, B.small + off, (int)cnt, a, B.buf, B.Yg);
     else
-        k_rs_tiny_bwd<T, NQ, NR><<<g, blk, 0, st>>>(pool, B.small + P.tc_off[q], (int)cnt, a, B.Yg, B.Xg);
+        k_rs_tiny_bwd<T, NQ, NR><<<g, blk, 0, st>>>(pool, B.small + off, (int)cnt, a, B.Yg, B.Xg);
 }
 
 template <class T, bool FWD>
@@ -1213,6 +1694,43 @@ void tiny_launch(const RsLevel &P, RsState &B, const T *pool, const RsArgs &a, h
     tiny_one<T, FWD, 4, 4>(P, 8, B, pool, a, st);
 }
 
+// the narrow fronts of a level: one launch per class present
+template <class T>
+void narrow_fwd(const RsLevel &P, RsState &B, const T *pool, const RsArgs &a, hipStream_t st)
+{
+    const dim3 blk(256);
+    for (int q = 0; q < 3; q++) {
+        const int64_t cnt = P.nf_cnt[q];
+        if (cnt <= 0) continue;
+        const dim3 g((unsigned)((cnt + 3) / 4));
+        const RsNTask *t = B.ntask + P.nf_off[q];
+        if (q == 0) k_rs_nfwd<T, 1><<<g, blk, 0, st>>>(pool, t, (int)cnt, a, B.buf, B.Yg);
+        else if (q == 1) k_rs_nfwd<T, 2><<<g, blk, 0, st>>>(pool, t, (int)cnt, a, B.buf, B.Yg);
+        else k_rs_nfwd<T, 4><<<g, blk, 0, st>>>(pool, t, (int)cnt, a, B.buf, B.Yg);
+    }
+}
+
+template <class T, int NQ>
+void narrow_bwd_class(int64_t c1, const RsNFront *f1, int64_t c4, const RsNFront *f4, const T *pool, const RsArgs &a,
+                      RsState &B, hipStream_t st)
+{
+    const dim3 blk(256);
+    if (c1 > 0) k_rs_nbwd<T, NQ, 1><<<dim3((unsigned)((c1 + 3) / 4)), blk, 0, st>>>(pool, f1, (int)c1, a, B.Yg, B.Xg);
+    if (c4 > 0) k_rs_nbwd<T, NQ, 4><<<dim3((unsigned)c4), blk, 0, st>>>(pool, f4, (int)c4, a, B.Yg, B.Xg);
+}
+
+template <class T>
+void narrow_bwd(const RsLevel &P, RsState &B, const T *pool, const RsArgs &a, hipStream_t st)
+{
+    for (int q = 0; q < 3; q++) {
+        const int64_t c1 = P.n1_cnt[q], c4 = P.n4_cnt[q];
+        const RsNFront *f1 = B.nfront + P.n1_off[q], *f4 = B.nfront + P.n4_off[q];
+        if (q == 0) narrow_bwd_class<T, 1>(c1, f1, c4, f4, pool, a, B, st);
+        else if (q == 1) narrow_bwd_class<T, 2>(c1, f1, c4, f4, pool, a, B, st);
+        else narrow_bwd_class<T, 4>(c1, f1, c4, f4, pool, a, B, st);
+    }
+}
+
 // forward and backward over the levels, between the permutations
 template <class T>
 void rs_levels(Factor *F, RsState &B, hipStream_t st)
@@ -1227,9 +1745,9 @@ void rs_levels(Factor *F, RsState &B, hipStream_t st)
     auto parts = [&](int mode, const RsLevel &P, int wide, hipEvent_t e0, hipEvent_t e1) {
         const bool w = (mode == 1 || mode == 2) && (wide < 0 ? P.wide[mode - 1] : wide == 1);
         const int v = w ? mode + 3 : mode;
-        const int64_t pc = P.pt_cnt[v], fc = P.ft_cnt[v];
-        const RsPart *pt = B.pt + P.pt_off[v];
-        const RsFin *ft = B.ft + P.ft_off[v];
+        const int64_t pc = P.narrow ? P.rpt_cnt[v] : P.pt_cnt[v], fc = P.narrow ? P.rft_cnt[v] : P.ft_cnt[v];
+        const RsPart *pt = B.pt + (P.narrow ? P.rpt_off[v] : P.pt_off[v]);
+        const RsFin *ft = B.ft + (P.narrow ? P.rft_off[v] : P.ft_off[v]);
         const dim3 gp((unsigned)pc), gf((unsigned)fc);
         if (e0) (void)hipEventRecord(e0, st);
         switch (v) {
@@ -1264,7 +1782,9 @@ void rs_levels(Factor *F, RsState &B, hipStream_t st)
     auto e_at = [&](int i) -> hipEvent_t { return ev ? ev[i] : nullptr; };
     for (int32_t L = 0; L < S.nlevels; L++) {
         const RsLevel &P = B.lev[L];
-        if (P.as_cnt) k_rs_assemble<<<dim3((unsigned)P.as_cnt), blk, 0, st>>>(B.asmt + P.as_off, a, B.buf);
+        if (P.narrow) narrow_fwd<T>(P, B, pool, a, st);
+        const int64_t asc = P.narrow ? P.ras_cnt : P.as_cnt;
+        if (asc) k_rs_assemble<<<dim3((unsigned)asc), blk, 0, st>>>(B.asmt + (P.narrow ? P.ras_off : P.as_off), a, B.buf);
         if (P.sm_cnt) tiny_launch<T, true>(P, B, pool, a, st);
         parts(0, P, wide, nullptr, nullptr);
         parts(1, P, wide, e_at(4 * L), e_at(4 * L + 1));
@@ -1274,6 +1794,7 @@ void rs_levels(Factor *F, RsState &B, hipStream_t st)
         parts(2, P, wide, e_at(4 * L + 2), e_at(4 * L + 3));
         parts(3, P, wide, nullptr, nullptr);
         if (P.sm_cnt) tiny_launch<T, false>(P, B, pool, a, st);
+        if (P.narrow) narrow_bwd<T>(P, B, pool, a, st);
     }
 }
 
