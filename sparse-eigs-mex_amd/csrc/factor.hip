@@ -627,6 +627,221 @@ __global__ void __launch_bounds__(256) k_potrf_reg4(T *pool, const TileTask *__r
     for (int j = 0; j < 16; j++) X[(int64_t)(q0 + j) * TB + r] = xs[r][q0 + j];
 }
 
+// 1 / sqrt(d) for a pivot: the hardware reciprocal square root in FP32 (about
+// 1 ulp), a rounded square root and division in FP64
+__device__ __forceinline__ float rsqrt_piv(float d) { return __builtin_amdgcn_rsqf(d); }
+__device__ __forceinline__ double rsqrt_piv(double d) { return 1.0 / sqrt(d); }
+
+// Cholesky of the 16 x 16 diagonal block at (o, o) of the LDS tile, with its
+// inverse, on one wave: lane 16g + r holds row r, columns 4g .. 4g+3 (a) and
+// X(4g+j, r) (x, the right-looking forward substitution t <- e_r of the
+// unblocked kernels).  Group gp factors its four columns alone (pivots and
+// multipliers by readlane inside the group), leaves them with its x in LDS,
+// and the groups to its right apply the rank-4 update: four LDS round trips
+// per block instead of one per column, each element updated column by column
+// in the same order.  thr (lane l) is the pivot threshold of tile row l; a
+// failed pivot is replaced by 1 and sets bad.
+template <class T>
+__device__ __forceinline__ void potrf_diag16(T (*S)[TB + 1], T (*XS)[TB + 1], T (*Cb)[16][8], int o, int w,
+                                             double thr, bool &bad)
+{
+    const int lane = threadIdx.x & 63, g = lane >> 4, r = lane & 15;
+    T a[4], x[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        a[j] = S[o + 4 * g + j][o + r];
+        x[j] = T(4 * g + j == r ? 1 : 0);
+    }
+#pragma unroll
+    for (int gp = 0; gp < 4; gp++) {
+        const int b = gp & 1;
+        if (g == gp) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int p = 4 * gp + i;
+                T d = bcast(a[i], 16 * gp + p);
+                if (o + p < w) {
+                    const double th = bcast(thr, o + p);
+                    if (!((double)d > th) || !(d == d)) {
+                        bad = true;
+                        d = T(1);
+                    }
+                }
+                const T ri = rsqrt_piv(d);
+                a[i] = (r == p) ? d * ri : a[i] * ri;
+                x[i] = x[i] * ri;
+#pragma unroll
+                for (int i2 = i + 1; i2 < 4; i2++) {
+                    const T lq = bcast(a[i], 16 * gp + 4 * gp + i2);     // L(4 gp + i2, p)
+                    a[i2] -= a[i] * lq;
+                    x[i2] -= lq * x[i];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                Cb[b][r][i] = a[i];
+                Cb[b][r][4 + i] = x[i];
+            }
+        }
+        if (gp == 3) break;
+        // the group's columns are read by the other lanes of this wave
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (g > gp) {
+            T lrow[4], xrow[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                lrow[i] = Cb[b][r][i];
+                xrow[i] = Cb[b][r][4 + i];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                T lq[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) lq[i] = Cb[b][4 * g + j][i];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    a[j] -= lrow[i] * lq[i];
+                    x[j] -= lq[i] * xrow[i];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int q = 4 * g + j;
+        if (q <= r) S[o + q][o + r] = a[j];
+        XS[o + r][o + q] = x[j];
+    }
+}
+
+// Blocked panel factorization and inverse: the 64 x 64 tile as 4 x 4 blocks of
+// 16, held in LDS (S[c][r] = A(r, c), overwritten by L; XS[c][r] = X(r, c)).
+// Block column kb: wave 0 factors the 16 x 16 diagonal block with its inverse
+// (potrf_diag16); the blocks below it are solved as products with that inverse
+// (L_ib = A_ib X_bb^T, one wave per block) and the trailing lower blocks
+// updated (S_ij -= L_ib L_jb^T, one wave per block pair), both on the matrix
+// cores.  Wave 0 updates the next diagonal block first and factors it while
+// waves 1-3 update the rest.  The off-diagonal blocks of X = L^-1 then follow
+// one block diagonal at a time by X_ij = -X_ii sum_{m=j}^{i-1} L_im X_mj.  The
+// factor agrees with the unblocked kernels' to rounding (a reciprocal square
+// root per pivot, products summed by the matrix cores).  Rows and columns
+// beyond w are the identity, so their pivots are 1 and their couplings 0; the
+// pivot test covers columns < w only.
+template <class T>
+__global__ void __launch_bounds__(256) k_potrf_blk(T *pool, const TileTask *__restrict__ tasks,
+                                                   const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
+                                                   const int32_t *__restrict__ ncol, const int64_t *__restrict__ col0,
+                                                   const double *__restrict__ diagA, double delta, double piv_rel,
+                                                   int *flag, T *__restrict__ xinv)
+{
+    typedef typename MfmaAcc<T>::V V;
+    __shared__ T S[TB][TB + 1];
+    __shared__ T XS[TB][TB + 1];
+    __shared__ T TS[4][16][17];     // per wave: TS[w][c][r] = T(r, c) of the X recurrence
+    __shared__ __align__(16) T Cb[2][16][8];
+    const TileTask t = tasks[blockIdx.x];
+    const int32_t F = Fs[t.s], c = ncol[t.s];
+    const int k0 = t.k * TB, w = min(TB, c - k0), wl = w - 1;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, lr = lane & 15, lk = lane >> 4;
+    T *base = pool + foff[t.s] + (int64_t)k0 * F + k0;
+    {
+        // every load in flight before the LDS stores (a select after each load
+        // would wait for it); entries outside the tile's lower part read a
+        // lower entry inside it and are replaced by the identity's
+        T v[TB * TB / 256];
+#pragma unroll
+        for (int e = 0; e < TB * TB / 256; e++) {
+            const int i = tid + 256 * e, cc = min(i >> 6, wl), rr = max(min(i & 63, wl), cc);
+            v[e] = base[(int64_t)cc * F + rr];
+        }
+#pragma unroll
+        for (int e = 0; e < TB * TB / 256; e++) {
+            const int i = tid + 256 * e, cc = i >> 6, rr = i & 63;
+            S[cc][rr] = (rr < w && cc < w && cc <= rr) ? v[e] : T(cc == rr ? 1 : 0);
+            XS[cc][rr] = T(0);
+        }
+    }
+    // lane l of wave 0: the pivot threshold of tile row l (A(k0 + l, k0 + l))
+    const double thr = piv_rel * fabs(diagA[col0[t.s] + k0 + min(lane, wl)] + delta);
+    bool bad = false;
+    __syncthreads();
+    if (wv == 0) potrf_diag16<T>(S, XS, Cb, 0, w, thr, bad);
+    __syncthreads();
+#pragma unroll 1
+    for (int kb = 0; kb < 3; kb++) {
+        const int o = 16 * kb, nb = 3 - kb;
+        if (wv < nb) {
+            const int ri = 16 * (kb + 1 + wv);
+            V acc = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+            for (int kk = 0; kk < 16; kk += 4)
+                acc = MfmaAcc<T>::blk(S[o + kk + lk][ri + lr], XS[o + kk + lk][o + lr], acc);
+#pragma unroll
+            for (int v = 0; v < 4; v++) S[o + MfmaAcc<T>::colof(lk, v)][ri + lr] = acc[v];
+        }
+        __syncthreads();
+        // block pairs (kb + ib, kb + 1 + jb), jb < ib: pair 0 (the next diagonal
+        // block) on wave 0, which then factors it; the others on waves 1-3
+        const int np = nb * (nb + 1) / 2;
+        for (int pi = wv == 0 ? 0 : wv; pi < (wv == 0 ? 1 : np); pi += 3) {
+            int ib = 1, jb = pi;
+            while (jb >= ib) {
+                jb -= ib;
+                ib++;
+            }
+            const int ri = 16 * (kb + ib), rj = 16 * (kb + 1 + jb);
+            V acc = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+            for (int kk = 0; kk < 16; kk += 4)
+                acc = MfmaAcc<T>::blk(S[o + kk + lk][ri + lr], S[o + kk + lk][rj + lr], acc);
+#pragma unroll
+            for (int v = 0; v < 4; v++) S[rj + MfmaAcc<T>::colof(lk, v)][ri + lr] -= acc[v];
+        }
+        if (wv == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            potrf_diag16<T>(S, XS, Cb, o + 16, w, thr, bad);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int dd = 1; dd < 4; dd++) {
+        const bool act = wv < 4 - dd;
+        const int ri = 16 * (dd + wv), rj = 16 * wv;     // block (i, j = i - dd)
+        if (act) {
+            V acc = {T(0), T(0), T(0), T(0)};
+            for (int rm = rj; rm < ri; rm += 16)
+#pragma unroll
+                for (int kk = 0; kk < 16; kk += 4)
+                    acc = MfmaAcc<T>::blk(S[rm + kk + lk][ri + lr], XS[rj + lr][rm + kk + lk], acc);
+#pragma unroll
+            for (int v = 0; v < 4; v++) TS[wv][MfmaAcc<T>::colof(lk, v)][lr] = acc[v];
+        }
+        __syncthreads();
+        if (act) {
+            V acc = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+            for (int kk = 0; kk < 16; kk += 4)
+                acc = MfmaAcc<T>::blk(XS[ri + kk + lk][ri + lr], TS[wv][lr][kk + lk], acc);
+#pragma unroll
+            for (int v = 0; v < 4; v++) XS[rj + MfmaAcc<T>::colof(lk, v)][ri + lr] = -acc[v];
+        }
+        __syncthreads();
+    }
+    // a failed pivot is seen by the lanes of its column's group
+    if (__syncthreads_or(bad) && tid == 0) atomicOr(flag, 1);
+    T *X = xinv + (int64_t)blockIdx.x * (TB * TB);
+#pragma unroll
+    for (int e = 0; e < TB * TB / 256; e++) {
+        const int i = tid + 256 * e, cc = i >> 6, rr = i & 63;
+        if (rr < w && cc < w && cc <= rr) base[(int64_t)cc * F + rr] = S[cc][rr];
+        X[i] = XS[cc][rr];
+    }
+}
+
 // tile row i below the diagonal block: B <- B X^T with X = L_kk^-1 from the
 // panel's scratch slot t.j (MFMA, the product formed as for the Schur update)
 template <class T>
@@ -1301,6 +1516,10 @@ int run_numeric(Factor *F, double delta)
                         k_potrf_reg4<T, true><<<dim3((unsigned)pc), dim3(256), 0, st>>>(
                             pool, F->tiles + P.k_potrf_off[k] + p0, F->foff, F->Fs, F->ncol, F->col0, F->diagA,
                             delta, piv_rel, F->flag, xinv, (T *)F->sgn, F->tau);
+                    else if (F->potrf_blocked)
+                        k_potrf_blk<T><<<dim3((unsigned)pc), dim3(256), 0, st>>>(
+                            pool, F->tiles + P.k_potrf_off[k] + p0, F->foff, F->Fs, F->ncol, F->col0, F->diagA,
+                            delta, piv_rel, F->flag, xinv);
                     else if (F->potrf_waves == 4)
                         k_potrf_reg4<T, false><<<dim3((unsigned)pc), dim3(256), 0, st>>>(
                             pool, F->tiles + P.k_potrf_off[k] + p0, F->foff, F->Fs, F->ncol, F->col0, F->diagA,
@@ -1565,7 +1784,11 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     if (const char *e = getenv("SPEIGS_HIP_SOLVE_MODE")) F->solve_mode = atoi(e);
     if (const char *e = getenv("SPEIGS_HIP_MFMA")) F->use_mfma = atoi(e) != 0;
     if (const char *e = getenv("SPEIGS_HIP_REG_PANEL")) F->use_reg_panel = atoi(e) != 0;
-    if (const char *e = getenv("SPEIGS_HIP_POTRF_WAVES")) F->potrf_waves = atoi(e) == 1 ? 1 : 4;
+    // 1 or 4: the unblocked panel kernel on that many waves; anything else: blocked
+    if (const char *e = getenv("SPEIGS_HIP_POTRF_WAVES")) {
+        F->potrf_waves = atoi(e) == 1 ? 1 : 4;
+        F->potrf_blocked = atoi(e) != 1 && atoi(e) != 4;
+    }
     // test knob: every bounded wait of the blocked single-vector solve gives up
     if (const char *e = getenv("SPEIGS_HIP_WAIT_TEST")) F->wait_test = atoi(e) == 1;
     // A's CSC arrays do not depend on the ordering: a helper thread uploads

@@ -93,6 +93,7 @@ struct Factor {
     bool use_mfma = true;
     bool use_reg_panel = true;       // register POTRF + TRSM as a product with L_kk^-1
     int potrf_waves = 4;             // register POTRF on 4 waves (columns split) or 1
+    bool potrf_blocked = true;       // blocked register POTRF (16-column blocks, MFMA updates)
     int solve_mode = 3;               // 0: one workgroup per front, 1: blocked multi-workgroup,
                                       // 2: generic inverse-based, 3: inverse-based on the block plan
     bool wait_test = false;           // test knob: the blocked solve's waits all give up

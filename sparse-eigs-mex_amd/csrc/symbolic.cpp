@@ -541,14 +541,16 @@ inline double sum_sq(double x) { return x * (x + 1.0) * (2.0 * x + 1.0) / 6.0; }
 int symbolic_analyze(int64_t n, const int64_t *Ap, const int64_t *Ai, const SymbolicOptions &o, Symbolic *S)
 {
     if (n >= (int64_t)1 << 31) return SPEIGS_ERR_ARG;
-    double t0 = wall();
+    double t0 = wall(), tp = t0;
     const int nt = host_threads();
     Graph g;
     build_graph(n, Ap, Ai, g, nt);
+    trace_phase("ordering: graph", tp);
     S->n = n;
     S->perm.assign(n, 0);
     Dissection nd(g, std::max<int64_t>(o.leaf, 1), S->perm, nt);
     nd.run();
+    trace_phase("ordering: dissection", tp);
     S->iperm.resize(n);
     parallel_for(n, nt, 65536, [&](int64_t i, int) { S->iperm[S->perm[i]] = i; });
     double t1 = wall();
