@@ -728,7 +728,8 @@ __device__ __forceinline__ void potrf_diag16(T (*S)[TB + 1], T (*XS)[TB + 1], T 
 // factor agrees with the unblocked kernels' to rounding (a reciprocal square
 // root per pivot, products summed by the matrix cores).  Rows and columns
 // beyond w are the identity, so their pivots are 1 and their couplings 0; the
-// pivot test covers columns < w only.
+// pivot test covers columns < w only, and the blocks wholly beyond w are
+// skipped (a leaf front of 16 columns factors one diagonal block).
 template <class T>
 __global__ void __launch_bounds__(256) k_potrf_blk(T *pool, const TileTask *__restrict__ tasks,
                                                    const int64_t *__restrict__ foff, const int32_t *__restrict__ Fs,
@@ -744,23 +745,28 @@ __global__ void __launch_bounds__(256) k_potrf_blk(T *pool, const TileTask *__re
     const TileTask t = tasks[blockIdx.x];
     const int32_t F = Fs[t.s], c = ncol[t.s];
     const int k0 = t.k * TB, w = min(TB, c - k0), wl = w - 1;
+    // the 16-column blocks holding columns < w; those after them are the
+    // identity in L and in X and are not factored
+    const int nbk = (w + 15) >> 4;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, lr = lane & 15, lk = lane >> 4;
     T *base = pool + foff[t.s] + (int64_t)k0 * F + k0;
     {
         // every load in flight before the LDS stores (a select after each load
         // would wait for it); entries outside the tile's lower part read a
         // lower entry inside it and are replaced by the identity's
+        // (column groups 4e .. 4e+3 at or beyond w are not loaded: e and w are
+        // uniform, so that test is a scalar branch)
         T v[TB * TB / 256];
 #pragma unroll
         for (int e = 0; e < TB * TB / 256; e++) {
             const int i = tid + 256 * e, cc = min(i >> 6, wl), rr = max(min(i & 63, wl), cc);
-            v[e] = base[(int64_t)cc * F + rr];
+            v[e] = 4 * e < w ? base[(int64_t)cc * F + rr] : T(0);
         }
 #pragma unroll
         for (int e = 0; e < TB * TB / 256; e++) {
             const int i = tid + 256 * e, cc = i >> 6, rr = i & 63;
             S[cc][rr] = (rr < w && cc < w && cc <= rr) ? v[e] : T(cc == rr ? 1 : 0);
-            XS[cc][rr] = T(0);
+            XS[cc][rr] = (cc == rr && cc >= 16 * nbk) ? T(1) : T(0);
         }
     }
     // lane l of wave 0: the pivot threshold of tile row l (A(k0 + l, k0 + l))
@@ -770,8 +776,8 @@ __global__ void __launch_bounds__(256) k_potrf_blk(T *pool, const TileTask *__re
     if (wv == 0) potrf_diag16<T>(S, XS, Cb, 0, w, thr, bad);
     __syncthreads();
 #pragma unroll 1
-    for (int kb = 0; kb < 3; kb++) {
-        const int o = 16 * kb, nb = 3 - kb;
+    for (int kb = 0; kb < nbk - 1; kb++) {
+        const int o = 16 * kb, nb = nbk - 1 - kb;
         if (wv < nb) {
             const int ri = 16 * (kb + 1 + wv);
             V acc = {T(0), T(0), T(0), T(0)};
@@ -809,7 +815,8 @@ __global__ void __launch_bounds__(256) k_potrf_blk(T *pool, const TileTask *__re
     }
 #pragma unroll
     for (int dd = 1; dd < 4; dd++) {
-        const bool act = wv < 4 - dd;
+        if (dd >= nbk) break;
+        const bool act = wv < nbk - dd;
         const int ri = 16 * (dd + wv), rj = 16 * wv;     // block (i, j = i - dd)
         if (act) {
             V acc = {T(0), T(0), T(0), T(0)};
@@ -1491,8 +1498,14 @@ int run_numeric(Factor *F, double delta)
     if (sg && !F->use_reg_panel) return SEHIP_FAIL(SPEIGS_ERR_ARG, "signed factorization needs the register panel");
     SEHIP_CHECK(hipMemsetAsync(F->flag, 0, 4 * sizeof(int), st));
     if ((uintptr_t)pool % 16) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "front pool not 16-byte aligned");
-    k_zero<T><<<dim3((unsigned)std::min<int64_t>(4096, (S.pool * (int64_t)sizeof(T) / 16 + 255) / 256 + 1)), dim3(256), 0,
-                st>>>(pool, S.pool);
+    if (F->pool_zeroed) {
+        // the first factorization: the fill queued at set-up
+        SEHIP_CHECK(hipStreamWaitEvent(st, F->ev_zero, 0));
+        F->pool_zeroed = false;
+    } else {
+        k_zero<T><<<dim3((unsigned)std::min<int64_t>(4096, (S.pool * (int64_t)sizeof(T) / 16 + 255) / 256 + 1)),
+                    dim3(256), 0, st>>>(pool, S.pool);
+    }
     const dim3 gn((unsigned)((F->n + 255) / 256));
     k_diag_delta<T><<<gn, dim3(256), 0, st>>>(pool, F->snode, F->col0, F->Fs, F->foff, F->n, delta);
     k_assemble<T><<<gn, dim3(256), 0, st>>>(pool, F->Apd, F->Aid, F->Ax, F->iperm, F->snode, F->col0, F->ncol, F->Fs,
@@ -1772,6 +1785,7 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     SEHIP_CHECK(hipStreamCreate(&F->st2));
     SEHIP_CHECK(hipEventCreateWithFlags(&F->ev_ready, hipEventDisableTiming));
     SEHIP_CHECK(hipEventCreateWithFlags(&F->ev_far, hipEventDisableTiming));
+    SEHIP_CHECK(hipEventCreateWithFlags(&F->ev_zero, hipEventDisableTiming));
     F->n = n;
     F->fp32 = o.precision == SPEIGS_HIP_FP32_IR || o.precision == SPEIGS_HIP_FP32_BLOCK;
     // Single-vector solves: by the inverted diagonal blocks (mode 3) when the
@@ -1827,6 +1841,22 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     if (rc) return rc;
     const Symbolic &S = F->S;
     double tu = wall(), tp = tu;
+    // the front pool, zeroed on the second stream while this thread builds the
+    // assembly data and task lists and the arrays go up on the first (the
+    // first numeric phase waits for it)
+    if (F->fp32) rc = F->mem.alloc((float **)&F->pool, (size_t)S.pool);
+    else rc = F->mem.alloc((double **)&F->pool, (size_t)S.pool);
+    if (rc) return rc;
+    if ((uintptr_t)F->pool % 16) return SEHIP_FAIL(SPEIGS_ERR_INTERNAL, "front pool not 16-byte aligned");
+    {
+        const unsigned zg = (unsigned)std::min<int64_t>(4096, (S.pool * (F->fp32 ? 4 : 8) / 16 + 255) / 256 + 1);
+        if (F->fp32) k_zero<float><<<dim3(zg), dim3(256), 0, F->st2>>>((float *)F->pool, S.pool);
+        else k_zero<double><<<dim3(zg), dim3(256), 0, F->st2>>>((double *)F->pool, S.pool);
+        SEHIP_CHECK(hipGetLastError());
+        SEHIP_CHECK(hipEventRecord(F->ev_zero, F->st2));
+        F->pool_zeroed = true;
+    }
+    trace_phase("pool allocation, zero fill queued", tp);
     // per-supernode arrays
     const int64_t ns = S.ns;
     std::vector<int32_t> Fs(ns), nc(ns), rows32(std::max<size_t>(S.rows.size(), 1)), chl32(std::max<size_t>(S.chl.size(), 1));
@@ -2042,19 +2072,12 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
         SEHIP_CHECK(hipEventCreate(&e));
     trace_phase("small uploads, events", tp);
     // uploads
-    const size_t esz = F->fp32 ? sizeof(float) : sizeof(double);
-    if (F->fp32) {
-        if ((rc = F->mem.alloc((float **)&F->pool, (size_t)S.pool))) return rc;
-    } else {
-        if ((rc = F->mem.alloc((double **)&F->pool, (size_t)S.pool))) return rc;
-    }
-    (void)esz;
     if (F->fp32) {
         if ((rc = F->mem.alloc((float **)&F->xinv, (size_t)xinv_max * TB * TB))) return rc;
     } else {
         if ((rc = F->mem.alloc((double **)&F->xinv, (size_t)xinv_max * TB * TB))) return rc;
     }
-    trace_phase("pool allocation", tp);
+    trace_phase("inverse slots", tp);
     if ((rc = upload(F.get(), &F->fronts, fronts))) return rc;
     if ((rc = upload(F.get(), &F->ea, ea))) return rc;
     if ((rc = upload(F.get(), &F->tiles, tiles))) return rc;

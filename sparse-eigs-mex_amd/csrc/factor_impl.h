@@ -67,6 +67,8 @@ struct Factor {
     hipStream_t st = nullptr;
     hipStream_t st2 = nullptr;              // look-ahead: the far part of block-end updates
     hipEvent_t ev_ready = nullptr, ev_far = nullptr;
+    hipEvent_t ev_zero = nullptr;    // the pool's zero fill queued at set-up (on st2)
+    bool pool_zeroed = false;        // the next numeric phase waits for ev_zero instead of zeroing
     DevMem mem;
     int64_t n = 0, nnzA = 0;
     bool fp32 = false;
@@ -132,6 +134,7 @@ struct Factor {
         if (st2) (void)hipStreamSynchronize(st2);
         if (ev_ready) (void)hipEventDestroy(ev_ready);
         if (ev_far) (void)hipEventDestroy(ev_far);
+        if (ev_zero) (void)hipEventDestroy(ev_zero);
         if (st2) (void)hipStreamDestroy(st2);
         if (hpart) (void)hipHostFree(hpart);
         if (herr) (void)hipHostFree(herr);
