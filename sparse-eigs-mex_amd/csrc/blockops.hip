@@ -216,6 +216,197 @@ void rgemm(const double *V, int64_t ldv, int m, const double *C, int p, double *
     k_rgemm<<<dim3((unsigned)((n + rows - 1) / rows)), dim3(256), 0, st>>>(V, ldv, m, C, p, X, ldx, n, beta);
 }
 
+// ------------------------------------------------------------------ Ritz step
+// The block eigensolver's Rayleigh-Ritz update and the next residuals in one
+// pass over S and AS (row-major, row stride ld, d <= 96 columns): with
+// C2 = [C | C_P] (d x 2m, column-major, m <= 32) on the device,
+//   Sn[:, 0:2m) = S C2            (new X, new P)
+//   ASn[:, 0:m) = AS C            (new A X)
+//   Sn[:, 2m:3m) = R = ASn[:, 0:m) - Sn[:, 0:m) diag(lam)
+// and, for the first k columns, per-workgroup sums of R's squares (part,
+// 32 per workgroup), summed in a fixed order by rritz_norms_reduce.  The
+// products are k_rgemm's (the same 16-row groups, inner steps and tiles), so
+// Sn and ASn are bitwise those of two rgemm calls, and R is formed from the
+// same doubles as a separate residual pass would read back.  Replaces two
+// tall products and the residual pass (three reads of n x m blocks).
+constexpr int RR_GROUPS = 4;   // 16-row groups per wave
+
+// PT = ceil(2m / 16) tiles of the S product, MT = ceil(m / 16) of the AS product
+template <int PT, int MT>
+__global__ void __launch_bounds__(256) k_rritz(const double *__restrict__ S, const double *__restrict__ AS, int64_t ld,
+                                               int d, const double *__restrict__ C2, int m,
+                                               const double *__restrict__ lam, int k, double *__restrict__ Sn,
+                                               double *__restrict__ ASn, int64_t n, double *__restrict__ part)
+{
+    __shared__ double Cs[96][RM_CP];   // Cs[kk][c] = C2[kk + c d]
+    __shared__ double red[4][32];
+    const int p = 2 * m;
+    for (int e = threadIdx.x; e < d * p; e += 256) {
+        const int c = e / d, kk = e - c * d;
+        Cs[kk][c] = C2[e];
+    }
+    const int d4 = (d + 3) & ~3, p16 = (p + 15) & ~15;
+    for (int e = threadIdx.x; e < d4 * p16; e += 256) {
+        const int kk = e / p16, c = e - kk * p16;
+        if (kk >= d || c >= p) Cs[kk][c] = 0.0;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lr = lane & 15, lk = lane >> 4;
+    const double l0 = lr < m ? lam[lr] : 0.0, l1 = 16 + lr < m ? lam[16 + lr] : 0.0;
+    double ns0 = 0.0, ns1 = 0.0;      // this lane's sums of squares, columns lr and 16 + lr
+    const int64_t rbase = ((int64_t)blockIdx.x * 4 + wv) * RR_GROUPS * 16;
+    for (int g = 0; g < RR_GROUPS; g++) {
+        const int64_t r0 = rbase + g * 16;
+        if (r0 >= n) break;
+        const int64_t arow = r0 + lr;
+        const bool rin = arow < n;
+        double va[24];
+        rd4 acc[PT], acc2[MT];
+#pragma unroll
+        for (int s2 = 0; s2 < 24; s2++) {
+            const int kk = 4 * s2 + lk;
+            va[s2] = (rin && kk < d) ? S[arow * ld + kk] : 0.0;
+        }
+#pragma unroll
+        for (int b = 0; b < PT; b++) acc[b] = rd4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s2 = 0; s2 < 24; s2++) {
+            if (4 * s2 >= d) break;
+#pragma unroll
+            for (int b = 0; b < PT; b++)
+                acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[s2], Cs[4 * s2 + lk][16 * b + lr], acc[b], 0, 0, 0);
+        }
+        // the new P (tiles 2, 3) leaves now
+#pragma unroll
+        for (int b = MT; b < PT; b++) {
+            const int c = 16 * b + lr;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int64_t row = r0 + lk + 4 * v;
+                if (row < n && c < p) Sn[row * ld + c] = acc[b][v];
+            }
+        }
+        // (C's first two tiles are read from LDS again below: keeping the
+        // values read above would hold 48 more doubles per lane)
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int s2 = 0; s2 < 24; s2++) {
+            const int kk = 4 * s2 + lk;
+            va[s2] = (rin && kk < d) ? AS[arow * ld + kk] : 0.0;
+        }
+#pragma unroll
+        for (int b = 0; b < MT; b++) acc2[b] = rd4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s2 = 0; s2 < 24; s2++) {
+            if (4 * s2 >= d) break;
+#pragma unroll
+            for (int b = 0; b < MT; b++)
+                acc2[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[s2], Cs[4 * s2 + lk][16 * b + lr], acc2[b], 0, 0, 0);
+        }
+        // lane (lr, lk), register v: row r0 + lk + 4v, column 16 b + lr
+#pragma unroll
+        for (int b = 0; b < MT; b++) {
+            const int c = 16 * b + lr;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int64_t row = r0 + lk + 4 * v;
+                if (row < n && c < p) Sn[row * ld + c] = acc[b][v];
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < MT; b++) {
+            const int c = 16 * b + lr;
+            const double lc = b == 0 ? l0 : l1;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int64_t row = r0 + lk + 4 * v;
+                if (row < n && c < m) {
+                    const double ax = acc2[b][v], x = acc[b][v];
+                    ASn[row * ld + c] = ax;
+                    const double rv = ax - lc * x;
+                    Sn[row * ld + p + c] = rv;
+                    if (c < k) {
+                        if (b == 0) ns0 = fma(rv, rv, ns0);
+                        else ns1 = fma(rv, rv, ns1);
+                    }
+                }
+            }
+        }
+    }
+    // column sums: the four lane groups of a wave, then the four waves, in order
+    ns0 += __shfl_xor(ns0, 16);
+    ns0 += __shfl_xor(ns0, 32);
+    ns1 += __shfl_xor(ns1, 16);
+    ns1 += __shfl_xor(ns1, 32);
+    if (lk == 0) {
+        red[wv][lr] = ns0;
+        red[wv][16 + lr] = ns1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 32)
+        part[(int64_t)blockIdx.x * 32 + threadIdx.x] =
+            (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+
+// sums of per-workgroup column partials (32 per workgroup) in a fixed order:
+// workgroup j of the launch sums partials [64 j, 64 j + 64) -- eight
+// interleaved partial sums per column, then their sum -- into out[j][c]
+__global__ void __launch_bounds__(256) k_part_reduce(const double *__restrict__ part, int64_t nb,
+                                                     double *__restrict__ out)
+{
+    __shared__ double red[8][32];
+    const int c = threadIdx.x & 31, sl = threadIdx.x >> 5;
+    const int64_t b0 = (int64_t)blockIdx.x * 64, b1 = min(nb, b0 + 64);
+    double t = 0.0;
+    for (int64_t b = b0 + sl; b < b1; b += 8) t += part[b * 32 + c];
+    red[sl][c] = t;
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        double u = 0.0;
+        for (int i = 0; i < 8; i++) u += red[i][c];
+        out[(int64_t)blockIdx.x * 32 + c] = u;
+    }
+}
+
+int64_t rritz_blocks(int64_t n) { return (n + 64 * RR_GROUPS - 1) / (64 * RR_GROUPS); }
+
+// per-workgroup partials and the levels of their sums
+int64_t rritz_part(int64_t n)
+{
+    int64_t cnt = rritz_blocks(n), tot = cnt;
+    while (cnt > 1) {
+        cnt = (cnt + 63) / 64;
+        tot += cnt;
+    }
+    return 32 * tot;
+}
+
+void rritz(const double *S, const double *AS, int64_t ld, int d, const double *C2, int m, const double *lam, int k,
+           double *Sn, double *ASn, int64_t n, double *part, double *out, hipStream_t st)
+{
+    const int64_t nb = rritz_blocks(n);
+    const dim3 g((unsigned)nb), t(256);
+    if (m <= 8) k_rritz<1, 1><<<g, t, 0, st>>>(S, AS, ld, d, C2, m, lam, k, Sn, ASn, n, part);
+    else if (m <= 16) k_rritz<2, 1><<<g, t, 0, st>>>(S, AS, ld, d, C2, m, lam, k, Sn, ASn, n, part);
+    else if (m <= 24) k_rritz<3, 2><<<g, t, 0, st>>>(S, AS, ld, d, C2, m, lam, k, Sn, ASn, n, part);
+    else k_rritz<4, 2><<<g, t, 0, st>>>(S, AS, ld, d, C2, m, lam, k, Sn, ASn, n, part);
+    // groups of 64 partials summed level by level (each level after the last
+    // in part), the last level into out
+    const double *src = part;
+    double *nxt_at = part + 32 * nb;
+    int64_t cnt = nb;
+    for (;;) {
+        const int64_t nxt = (cnt + 63) / 64;
+        double *dst = nxt == 1 ? out : nxt_at;
+        k_part_reduce<<<dim3((unsigned)nxt), t, 0, st>>>(src, cnt, dst);
+        if (nxt == 1) break;
+        src = dst;
+        nxt_at = dst + 32 * nxt;
+        cnt = nxt;
+    }
+}
+
 // ------------------------------------------------------------------ SpMM
 // Y = A X for row-major X, Y (nv <= 96 columns): SELL-64 slices; a 16-lane
 // group per row, lane j of the group on columns j, 16 + j, ..., so each

@@ -172,6 +172,14 @@ void rgram(const double *A, int64_t lda, int p, const double *B, int64_t ldb, in
 // itself (same stride, p <= m) but must not overlap it otherwise
 void rgemm(const double *V, int64_t ldv, int m, const double *C, int p, double *X, int64_t ldx, int64_t n,
            double beta, hipStream_t st);
+// the block eigensolver's Ritz step: Sn[:, 0:2m) = S C2, ASn[:, 0:m) = AS C2[:, 0:m),
+// Sn[:, 2m:3m) = ASn[:, 0:m) - Sn[:, 0:m) diag(lam), out[c] = ||Sn[:, 2m + c]||^2
+// for c < k (deterministic); S, AS, Sn, ASn share the row stride ld >= 3m; d <= 96,
+// m <= 32; part must hold rritz_part(n) doubles, out 32
+int64_t rritz_blocks(int64_t n);
+int64_t rritz_part(int64_t n);
+void rritz(const double *S, const double *AS, int64_t ld, int d, const double *C2, int m, const double *lam, int k,
+           double *Sn, double *ASn, int64_t n, double *part, double *out, hipStream_t st);
 // Y = A X (nv <= 96)
 void rspmm(const SellA &S, const double *X, int64_t ldx, double *Y, int64_t ldy, int nv, hipStream_t st);
 // R[:, c] = AX[:, c] - lam[c] X[:, c]

@@ -43,6 +43,7 @@ struct LWork {
     hipStream_t st = nullptr;
     DevMem mem;
     double *partial = nullptr, *G = nullptr, *C = nullptr, *red = nullptr, *npart = nullptr, *nrm = nullptr;
+    double *rpart = nullptr;    // the Ritz step's per-workgroup residual sums
 };
 
 // R = AX - X theta into R (m columns) and the squared norms of its first k columns
@@ -263,6 +264,7 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
     if ((rc = W.mem.alloc(&W.red, 4 * 96))) return rc;
     if ((rc = W.mem.alloc(&W.npart, (size_t)rresid_norms_part()))) return rc;
     if ((rc = W.mem.alloc(&W.nrm, 32))) return rc;
+    if ((rc = W.mem.alloc(&W.rpart, (size_t)rritz_part(n)))) return rc;
 
     // The start vector is generated and uploaded (on the factor's second
     // stream) by a helper thread while the device inverts the diagonal blocks.
@@ -354,13 +356,22 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
         tmark = now;
         return 0;
     };
+    // the residual block W and its norms come from the previous Ritz step's
+    // fused pass, except at the start and after a refresh of A X
+    bool have_res = false;
     for (;; it++) {
         // S = [X | P | W] (no P in the first iteration): Z = [P W] is contiguous
         double *Z = S + m, *AZ = AS + m, *Wb = S + m + np;
-        SEHIP_CHECK(hipMemcpyAsync(lam_dev, theta.data(), m * sizeof(double), hipMemcpyHostToDevice, st));
         {
             std::vector<double> g;
-            if ((rc = resid_norms(W, AS, S, ldS, lam_dev, m, (int)k, Wb, g))) return rc;
+            if (have_res) {
+                g.resize(k);
+                SEHIP_CHECK(hipMemcpyAsync(g.data(), W.nrm, (size_t)k * sizeof(double), hipMemcpyDeviceToHost, st));
+                SEHIP_CHECK(hipStreamSynchronize(st));
+            } else {
+                SEHIP_CHECK(hipMemcpyAsync(lam_dev, theta.data(), m * sizeof(double), hipMemcpyHostToDevice, st));
+                if ((rc = resid_norms(W, AS, S, ldS, lam_dev, m, (int)k, Wb, g))) return rc;
+            }
             nconv = 0;
             for (int i = 0; i < k; i++) {
                 res[i] = std::sqrt(std::max(g[i], 0.0));
@@ -421,15 +432,19 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
             theta[t] = tri.w[t];
             for (int i = m; i < d; i++) C2[(size_t)(m + t) * d + i] = C2[(size_t)t * d + i];
         }
-        std::vector<double> C(C2.begin(), C2.begin() + (size_t)d * m);
-        if ((rc = dgemm_tall(W, S, ldS, d, C2, 2 * m, Sn, ldS, 0.0))) return rc;   // new X, new P
-        if ((rc = dgemm_tall(W, AS, ldS, d, C, m, ASn, ldS, 0.0))) return rc;       // new A X
+        // new X, new P, new A X and the next residual block R = A X - X theta
+        // (into the next W slot, columns 2m .. 3m-1) in one pass over S and AS
+        SEHIP_CHECK(hipMemcpyAsync(W.C, C2.data(), (size_t)d * 2 * m * sizeof(double), hipMemcpyHostToDevice, st));
+        SEHIP_CHECK(hipMemcpyAsync(lam_dev, theta.data(), m * sizeof(double), hipMemcpyHostToDevice, st));
+        rritz(S, AS, ldS, d, W.C, m, lam_dev, (int)k, Sn, ASn, n, W.rpart, W.nrm, st);
         std::swap(S, Sn);
         std::swap(AS, ASn);
         np = m;
+        have_res = true;
         if ((rc = lap(info->t_ritz))) return rc;
         if ((it + 1) % 8 == 0) {
             rspmm(A, S, ldS, AS, ldS, m, st);   // refresh A X against drift
+            have_res = false;                   // and the residuals with it
             if ((rc = lap(info->t_op))) return rc;
         }
     }
