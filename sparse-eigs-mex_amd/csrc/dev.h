@@ -227,8 +227,15 @@ int dense_syev(int64_t n, double *A, double *w);   // Householder + implicit QL,
 struct DenseTri {
     int64_t n = 0;
     std::vector<double> A, d0, e0, w;   // reflectors, tridiagonal, eigenvalues
+    int iters = 4;                      // inverse-iteration steps per vector
+    bool fast = false;                  // their back substitutions by reciprocal pivots
 };
 int dense_tri_values(int64_t n, const double *Afull, DenseTri &T);
+// the same for the block eigensolver's projected matrices: the reduction on
+// the lower triangle, QL rotations normed by sqrt where no scaling is needed,
+// three inverse-iteration steps per vector with reciprocal pivots (values and
+// vectors agree with dense_tri_values' to rounding, not bitwise)
+int dense_tri_values_rr(int64_t n, const double *Afull, DenseTri &T);
 // the same from a tridiagonal already formed (d, e; e[n - 1] unused): T.A empty
 int dense_tri_values_from(int64_t n, const double *d, const double *e, DenseTri &T);
 int dense_tri_vectors(const DenseTri &T, int64_t k, const int64_t *sel, double *Vsel, int64_t ldv);

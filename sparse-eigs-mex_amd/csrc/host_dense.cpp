@@ -245,8 +245,79 @@ void tridiag_inplace(int64_t n, double *A, double *d, double *e)
     e[n - 1] = 0.0;
 }
 
-// eigenvalues of the tridiagonal (d, e) by implicit QL, no vectors; ascending
-int ql_values(int64_t n, double *d, double *e)
+// the same reduction on the lower triangle of A (column-major, entries above
+// the diagonal unused): p = A v as a symmetric product over the lower part and
+// the rank-2 update on the lower part only, half the flops of the full one;
+// the same reflectors (H_k = I - 2 v v^T, v in A[k+1.., k]) up to rounding
+void tridiag_lower(int64_t n, double *A, double *d, double *e)
+{
+    std::vector<double> vv(n), pv(n);
+    double *__restrict__ v = vv.data();
+    double *__restrict__ p = pv.data();
+    for (int64_t k = 0; k + 2 < n; k++) {
+        double *__restrict__ ak = A + k * n;
+        double xn = 0.0;
+        for (int64_t i = k + 1; i < n; i++) xn += ak[i] * ak[i];
+        xn = std::sqrt(xn);
+        const double x0 = ak[k + 1];
+        if (xn == 0.0) {
+            e[k] = 0.0;
+            for (int64_t i = k + 1; i < n; i++) ak[i] = 0.0;
+            continue;
+        }
+        const double alpha = x0 > 0 ? -xn : xn;
+        for (int64_t i = k + 1; i < n; i++) v[i] = ak[i];
+        v[k + 1] -= alpha;
+        double vn = 0.0;
+        for (int64_t i = k + 1; i < n; i++) vn += v[i] * v[i];
+        vn = std::sqrt(vn);
+        if (vn == 0.0) {
+            e[k] = x0;
+            for (int64_t i = k + 1; i < n; i++) ak[i] = 0.0;
+            continue;
+        }
+        const double iv = 1.0 / vn;
+        for (int64_t i = k + 1; i < n; i++) v[i] *= iv;
+        for (int64_t i = k + 1; i < n; i++) p[i] = 0.0;
+        for (int64_t j = k + 1; j < n; j++) {
+            const double *__restrict__ aj = A + j * n;
+            const double vj = v[j];
+            double s = aj[j] * vj;
+            for (int64_t i = j + 1; i < n; i++) {
+                p[i] += aj[i] * vj;
+                s += aj[i] * v[i];
+            }
+            p[j] += s;
+        }
+        double K = 0.0;
+        for (int64_t i = k + 1; i < n; i++) K += v[i] * p[i];
+        for (int64_t i = k + 1; i < n; i++) p[i] = 2.0 * p[i] - 2.0 * K * v[i];
+        for (int64_t j = k + 1; j < n; j++) {
+            double *__restrict__ aj = A + j * n;
+            const double vj = v[j], pj = p[j];
+            for (int64_t i = j; i < n; i++) aj[i] -= v[i] * pj + p[i] * vj;
+        }
+        e[k] = alpha;
+        for (int64_t i = k + 1; i < n; i++) ak[i] = v[i];
+    }
+    for (int64_t i = 0; i < n; i++) d[i] = A[i + i * n];
+    if (n >= 2) e[n - 2] = A[(n - 1) + (n - 2) * n];
+    e[n - 1] = 0.0;
+}
+
+// sqrt(f^2 + g^2): the plain formula where neither square can overflow or
+// underflow to zero, hypot (scaled) otherwise
+inline double rot_norm(double f, double g)
+{
+    const double m = std::max(std::fabs(f), std::fabs(g));
+    if (m > 1e-150 && m < 1e150) return std::sqrt(f * f + g * g);
+    return std::hypot(f, g);
+}
+
+// eigenvalues of the tridiagonal (d, e) by implicit QL, no vectors; ascending.
+// FAST: the rotations' norms by rot_norm instead of hypot
+template <bool FAST>
+int ql_values_t(int64_t n, double *d, double *e)
 {
     for (int64_t l = 0; l < n; l++) {
         int iter = 0;
@@ -257,17 +328,23 @@ int ql_values(int64_t n, double *d, double *e)
             if (mm == l) break;
             if (++iter > 60) return (int)(l + 1);
             double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-            double r = std::hypot(g, 1.0);
+            double r = FAST ? rot_norm(g, 1.0) : std::hypot(g, 1.0);
             g = d[mm] - d[l] + e[l] / (g + (g >= 0 ? r : -r));
             double s = 1.0, c = 1.0, pp = 0.0;
             bool under = false;
             for (int64_t i = mm - 1; i >= l; i--) {
                 const double f = s * e[i], b = c * e[i];
-                r = std::hypot(f, g);
+                r = FAST ? rot_norm(f, g) : std::hypot(f, g);
                 e[i + 1] = r;
                 if (r == 0.0) { d[i + 1] -= pp; e[mm] = 0.0; under = true; break; }
-                s = f / r;
-                c = g / r;
+                if (FAST) {
+                    const double ir = 1.0 / r;
+                    s = f * ir;
+                    c = g * ir;
+                } else {
+                    s = f / r;
+                    c = g / r;
+                }
                 g = d[i + 1] - pp;
                 r = (d[i] - g) * s + 2.0 * c * b;
                 pp = s * r;
@@ -284,16 +361,21 @@ int ql_values(int64_t n, double *d, double *e)
     return 0;
 }
 
-// workspace of tri_shift_solve, allocated once per dense_tri_vectors call
+int ql_values(int64_t n, double *d, double *e) { return ql_values_t<false>(n, d, e); }
+
+// workspace of the shifted tridiagonal solves, allocated once per
+// dense_tri_vectors call: the factorization of T - sigma I (rows u0, u1, u2 of
+// U, multipliers l, row swaps sw) and, for the fast solves, U's reciprocal
+// pivots
 struct TriWork {
-    std::vector<double> u0, u1, u2, l;
+    std::vector<double> u0, u1, u2, l, rp;
     std::vector<char> sw;
-    explicit TriWork(int64_t n) : u0(n), u1(n), u2(n), l(n), sw(n) {}
+    explicit TriWork(int64_t n) : u0(n), u1(n), u2(n), l(n), rp(n), sw(n) {}
 };
 
-// solve (T - sigma I) x = b in place, T = tridiag(e, d, e), Gaussian
-// elimination with partial pivoting; tiny pivots replaced by eps3
-void tri_shift_solve(int64_t n, const double *d, const double *e, double sigma, double eps3, double *x, TriWork &w)
+// T - sigma I = P L U, T = tridiag(e, d, e), Gaussian elimination with partial
+// pivoting; tiny pivots replaced by eps3
+void tri_shift_factor(int64_t n, const double *d, const double *e, double sigma, double eps3, TriWork &w)
 {
     double *u0 = w.u0.data(), *u1 = w.u1.data(), *u2 = w.u2.data(), *l = w.l.data();
     char *sw = w.sw.data();
@@ -323,6 +405,19 @@ void tri_shift_solve(int64_t n, const double *d, const double *e, double sigma, 
     }
     if (a0 == 0.0) a0 = eps3;
     u0[n - 1] = a0; u1[n - 1] = 0.0; u2[n - 1] = 0.0;
+    for (int64_t i = 0; i < n; i++) {
+        double piv = u0[i];
+        if (std::fabs(piv) < eps3) piv = piv < 0 ? -eps3 : eps3;
+        w.rp[i] = 1.0 / piv;
+    }
+}
+
+// x <- (T - sigma I)^-1 x with the factorization in w; FAST: the back
+// substitution multiplies by the reciprocal pivots instead of dividing
+void tri_shift_apply(int64_t n, double eps3, double *x, const TriWork &w, bool fast)
+{
+    const double *u0 = w.u0.data(), *u1 = w.u1.data(), *u2 = w.u2.data(), *l = w.l.data();
+    const char *sw = w.sw.data();
     for (int64_t i = 0; i + 1 < n; i++) {       // forward: apply the row operations to x
         if (sw[i]) {
             const double t = x[i];
@@ -336,9 +431,13 @@ void tri_shift_solve(int64_t n, const double *d, const double *e, double sigma, 
         double s = x[i];
         if (i + 1 < n) s -= u1[i] * x[i + 1];
         if (i + 2 < n) s -= u2[i] * x[i + 2];
-        double piv = u0[i];
-        if (std::fabs(piv) < eps3) piv = piv < 0 ? -eps3 : eps3;
-        x[i] = s / piv;
+        if (fast) {
+            x[i] = s * w.rp[i];
+        } else {
+            double piv = u0[i];
+            if (std::fabs(piv) < eps3) piv = piv < 0 ? -eps3 : eps3;
+            x[i] = s / piv;
+        }
     }
 }
 
@@ -376,6 +475,26 @@ int dense_tri_values(int64_t n, const double *Afull, DenseTri &T)
     return ql_values(n, T.w.data(), e.data());
 }
 
+int dense_tri_values_rr(int64_t n, const double *Afull, DenseTri &T)
+{
+    T.n = n;
+    T.A.assign(Afull, Afull + (size_t)n * n);
+    T.w.assign(n, 0.0);
+    T.iters = 3;
+    T.fast = true;
+    if (n <= 0) return 0;
+    if (n == 1) {
+        T.w[0] = T.A[0];
+        return 0;
+    }
+    T.d0.assign(n, 0.0);
+    T.e0.assign(n, 0.0);
+    tridiag_lower(n, T.A.data(), T.d0.data(), T.e0.data());
+    std::vector<double> e = T.e0;
+    T.w = T.d0;
+    return ql_values_t<true>(n, T.w.data(), e.data());
+}
+
 // the selected eigenvectors of the tridiagonal (d0, e0): X[t] for sel[t]
 static int tri_vectors(const DenseTri &T, int64_t k, const int64_t *sel, std::vector<std::vector<double>> &X)
 {
@@ -407,8 +526,10 @@ static int tri_vectors(const DenseTri &T, int64_t k, const int64_t *sel, std::ve
             z ^= z >> 12; z ^= z << 25; z ^= z >> 27;
             x[i] = (double)((z * 0x2545F4914F6CDD1DULL) >> 11) * (1.0 / 9007199254740992.0) - 0.5;
         }
-        for (int it = 0; it < 4; it++) {
-            tri_shift_solve(n, d0.data(), e0.data(), sigma, eps3, x.data(), tw);
+        // one factorization of T - sigma I for all of the vector's steps
+        tri_shift_factor(n, d0.data(), e0.data(), sigma, eps3, tw);
+        for (int it = 0; it < T.iters; it++) {
+            tri_shift_apply(n, eps3, x.data(), tw, T.fast);
             for (int64_t p2 = cstart; p2 < q; p2++) {     // reorthogonalise within the cluster
                 const std::vector<double> &y = X[pos[p2]];
                 double dot = 0.0;
@@ -470,7 +591,7 @@ int dense_tri_vectors(const DenseTri &T, int64_t k, const int64_t *sel, double *
             const double vi = v[i];
             double *xr = Xr.data() + (size_t)i * k;
             for (int64_t t = 0; t < k; t++)
-                if (sv[t] != 0.0) xr[t] -= sv[t] * vi;
+                xr[t] -= sv[t] * vi;     // exact for sv[t] = 0
         }
     }
     for (int64_t t = 0; t < k; t++)

@@ -421,7 +421,7 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
         // vectors by inverse iteration on the tridiagonal (O(d^2 m), not the
         // O(d^3) of accumulating every rotation)
         DenseTri tri;
-        if (dense_tri_values(d, H.data(), tri)) return SPEIGS_ERR_LAPACK;
+        if (dense_tri_values_rr(d, H.data(), tri)) return SPEIGS_ERR_LAPACK;
         std::vector<int64_t> sel(m);
         for (int t = 0; t < m; t++) sel[t] = t;
         // C2 = [C | C_P]: the Ritz coefficients and those of the new P (the Z
