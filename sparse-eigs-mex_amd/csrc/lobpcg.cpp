@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -44,7 +45,49 @@ struct LWork {
     DevMem mem;
     double *partial = nullptr, *G = nullptr, *C = nullptr, *red = nullptr, *npart = nullptr, *nrm = nullptr;
     double *rpart = nullptr;    // the Ritz step's per-workgroup residual sums
+    // pinned host staging of the small transfers (Gram matrices, coefficient
+    // blocks, Ritz values, norms): a pageable copy of a size not seen before
+    // can stall the host for milliseconds in the runtime's staging set-up
+    double *hbuf = nullptr;
+    LWork() = default;
+    LWork(const LWork &) = delete;
+    LWork &operator=(const LWork &) = delete;
+    ~LWork()
+    {
+        if (hbuf) (void)hipHostFree(hbuf);
+    }
 };
+constexpr size_t kHostIn = 96 * 96, kHostOut = 96 * 96;   // hbuf: [device -> host | host -> device]
+
+// device -> host through the pinned buffer (count <= kHostIn doubles); waits
+int to_host(LWork &W, double *h, const double *dev, size_t count)
+{
+    double tc = 0.0;
+    if (trace_on()) {
+        SEHIP_CHECK(hipStreamSynchronize(W.st));
+        tc = wall();
+    }
+    SEHIP_CHECK(hipMemcpyAsync(W.hbuf, dev, count * sizeof(double), hipMemcpyDeviceToHost, W.st));
+    SEHIP_CHECK(hipStreamSynchronize(W.st));
+    if (trace_on() && wall() - tc > 5e-4) {
+        char msg[96];
+        snprintf(msg, sizeof msg, "  copy back of %zu bytes (slow)", count * sizeof(double));
+        trace_phase(msg, tc);
+    }
+    std::memcpy(h, W.hbuf, count * sizeof(double));
+    return 0;
+}
+
+// host -> device through the pinned buffer (count <= kHostOut doubles): the
+// stream is drained first, so that no copy still reads the buffer (the device
+// is idle at every call site: it waits for these coefficients)
+int to_dev(LWork &W, double *dev, const double *h, size_t count)
+{
+    SEHIP_CHECK(hipStreamSynchronize(W.st));
+    std::memcpy(W.hbuf + kHostIn, h, count * sizeof(double));
+    SEHIP_CHECK(hipMemcpyAsync(dev, W.hbuf + kHostIn, count * sizeof(double), hipMemcpyHostToDevice, W.st));
+    return 0;
+}
 
 // R = AX - X theta into R (m columns) and the squared norms of its first k columns
 int resid_norms(LWork &W, const double *AX, const double *X, int64_t ld, const double *lam_dev, int m, int k,
@@ -52,9 +95,7 @@ int resid_norms(LWork &W, const double *AX, const double *X, int64_t ld, const d
 {
     rresid_norms(AX, ld, X, ld, lam_dev, m, k, W.n, R, ld, W.npart, W.nrm, W.st);
     out.resize(k);
-    SEHIP_CHECK(hipMemcpyAsync(out.data(), W.nrm, (size_t)k * sizeof(double), hipMemcpyDeviceToHost, W.st));
-    SEHIP_CHECK(hipStreamSynchronize(W.st));
-    return 0;
+    return to_host(W, out.data(), W.nrm, (size_t)k);
 }
 
 int dgram(LWork &W, const double *A, int64_t lda, int p, const double *B, int64_t ldb, int q,
@@ -62,16 +103,15 @@ int dgram(LWork &W, const double *A, int64_t lda, int p, const double *B, int64_
 {
     rgram(A, lda, p, B, ldb, q, W.n, W.partial, W.G, W.st);
     out.resize((size_t)p * q);
-    SEHIP_CHECK(hipMemcpyAsync(out.data(), W.G, (size_t)p * q * sizeof(double), hipMemcpyDeviceToHost, W.st));
-    SEHIP_CHECK(hipStreamSynchronize(W.st));
-    return 0;
+    return to_host(W, out.data(), W.G, (size_t)p * q);
 }
 
 // Y = V Cm + beta Y  (V n x m, Cm m x p on the host, column-major; Y may be V itself)
 int dgemm_tall(LWork &W, const double *V, int64_t ldv, int m, const std::vector<double> &Cm, int p, double *Y,
                int64_t ldy, double beta)
 {
-    SEHIP_CHECK(hipMemcpyAsync(W.C, Cm.data(), (size_t)m * p * sizeof(double), hipMemcpyHostToDevice, W.st));
+    const int rc = to_dev(W, W.C, Cm.data(), (size_t)m * p);
+    if (rc) return rc;
     rgemm(V, ldv, m, W.C, p, Y, ldy, W.n, beta, W.st);
     return 0;
 }
@@ -171,8 +211,10 @@ int project_svqb(LWork &W, double *S, int64_t ldS, int m, int nz, int *r, double
 {
     double *Z = S + m;
     std::vector<double> g;
+    double tp = wall();
     int rc = dgram(W, S, ldS, m + nz, Z, ldS, nz, g);   // rows 0..m-1: X^T Z, rows m..: Z^T Z
     if (rc) return rc;
+    trace_phase("  projection: Gram (device, copy back)", tp);
     const int ld = m + nz;
     std::vector<double> zz((size_t)nz * nz);
     for (int j = 0; j < nz; j++)
@@ -190,10 +232,14 @@ int project_svqb(LWork &W, double *S, int64_t ldS, int m, int nz, int *r, double
     // the Cholesky form when the projected block is well conditioned, SVQB
     // (which can drop directions) otherwise
     std::vector<double> T;
-    if (chol_transform(nz, zz, T, cond) == 0)
+    trace_phase("  projection: projected Gram (host)", tp);
+    if (chol_transform(nz, zz, T, cond) == 0) {
         *r = nz;
-    else if ((rc = svqb_transform(nz, zz, T, r, cond)))
-        return rc;
+        trace_phase("  projection: Cholesky form (host)", tp);
+    } else {
+        if ((rc = svqb_transform(nz, zz, T, r, cond))) return rc;
+        trace_phase("  projection: SVQB (host)", tp);
+    }
     const int rr = *r;
     if (rr == 0) return 0;
     // C = [-(X^T Z) T ; T], (m + nz) x rr column-major
@@ -206,7 +252,10 @@ int project_svqb(LWork &W, double *S, int64_t ldS, int m, int nz, int *r, double
         }
         for (int j = 0; j < nz; j++) C[(size_t)c * ld + m + j] = T[(size_t)c * nz + j];
     }
-    return dgemm_tall(W, S, ldS, ld, C, rr, Z, ldS, 0.0);
+    trace_phase("  projection: coefficients (host)", tp);
+    rc = dgemm_tall(W, S, ldS, ld, C, rr, Z, ldS, 0.0);
+    trace_phase("  projection: product queued", tp);
+    return rc;
 }
 
 void symmetrize(std::vector<double> &H, int d)
@@ -264,6 +313,8 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
     if ((rc = W.mem.alloc(&W.red, 4 * 96))) return rc;
     if ((rc = W.mem.alloc(&W.npart, (size_t)rresid_norms_part()))) return rc;
     if ((rc = W.mem.alloc(&W.nrm, 32))) return rc;
+    if (hipHostMalloc((void **)&W.hbuf, (kHostIn + kHostOut) * sizeof(double)) != hipSuccess)
+        return SEHIP_FAIL(SPEIGS_ERR_MEM, "LOBPCG: pinned host staging");
     if ((rc = W.mem.alloc(&W.rpart, (size_t)rritz_part(n)))) return rc;
 
     // The start vector is generated and uploaded (on the factor's second
@@ -366,10 +417,9 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
             std::vector<double> g;
             if (have_res) {
                 g.resize(k);
-                SEHIP_CHECK(hipMemcpyAsync(g.data(), W.nrm, (size_t)k * sizeof(double), hipMemcpyDeviceToHost, st));
-                SEHIP_CHECK(hipStreamSynchronize(st));
+                if ((rc = to_host(W, g.data(), W.nrm, (size_t)k))) return rc;
             } else {
-                SEHIP_CHECK(hipMemcpyAsync(lam_dev, theta.data(), m * sizeof(double), hipMemcpyHostToDevice, st));
+                if ((rc = to_dev(W, lam_dev, theta.data(), (size_t)m))) return rc;
                 if ((rc = resid_norms(W, AS, S, ldS, lam_dev, m, (int)k, Wb, g))) return rc;
             }
             nconv = 0;
@@ -409,7 +459,9 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
         // SciPy's LOBPCG) and only S^T A Z is formed: (m + nz) x nz
         const int d = m + nz;
         std::vector<double> G2, H((size_t)d * d, 0.0);
+        double tr = wall();
         if ((rc = dgram(W, S, ldS, d, AZ, ldS, nz, G2))) return rc;
+        trace_phase("  Rayleigh-Ritz: Gram (device, copy back)", tr);
         for (int t = 0; t < m; t++) H[(size_t)t * d + t] = theta[t];
         for (int j = 0; j < nz; j++)
             for (int t = 0; t < d; t++) {
@@ -428,15 +480,19 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
         // rows of C), so [X_new P_new] = S C2 is one pass over S
         std::vector<double> C2((size_t)d * 2 * m, 0.0);
         if (dense_tri_vectors(tri, m, sel.data(), C2.data(), d)) return SPEIGS_ERR_LAPACK;
+        trace_phase("  Rayleigh-Ritz: projected eigenproblem (host)", tr);
         for (int t = 0; t < m; t++) {
             theta[t] = tri.w[t];
             for (int i = m; i < d; i++) C2[(size_t)(m + t) * d + i] = C2[(size_t)t * d + i];
         }
         // new X, new P, new A X and the next residual block R = A X - X theta
         // (into the next W slot, columns 2m .. 3m-1) in one pass over S and AS
-        SEHIP_CHECK(hipMemcpyAsync(W.C, C2.data(), (size_t)d * 2 * m * sizeof(double), hipMemcpyHostToDevice, st));
-        SEHIP_CHECK(hipMemcpyAsync(lam_dev, theta.data(), m * sizeof(double), hipMemcpyHostToDevice, st));
-        rritz(S, AS, ldS, d, W.C, m, lam_dev, (int)k, Sn, ASn, n, W.rpart, W.nrm, st);
+        // theta rides after C2 in one upload
+        C2.resize((size_t)d * 2 * m + m);
+        std::copy(theta.begin(), theta.end(), C2.begin() + (size_t)d * 2 * m);
+        if ((rc = to_dev(W, W.C, C2.data(), C2.size()))) return rc;
+        rritz(S, AS, ldS, d, W.C, m, W.C + (size_t)d * 2 * m, (int)k, Sn, ASn, n, W.rpart, W.nrm, st);
+        trace_phase("  Rayleigh-Ritz: step queued", tr);
         std::swap(S, Sn);
         std::swap(AS, ASn);
         np = m;
@@ -455,7 +511,7 @@ int lobpcg_dev(const SellA &A, Factor *F, hipStream_t st, int64_t n, int64_t k, 
     if ((rc = dgram(W, S, ldS, m, S, ldS, m, xx))) return rc;
     std::vector<double> lam(m);
     for (int t = 0; t < m; t++) lam[t] = xax[(size_t)t * m + t] / xx[(size_t)t * m + t];
-    SEHIP_CHECK(hipMemcpyAsync(lam_dev, lam.data(), m * sizeof(double), hipMemcpyHostToDevice, st));
+    if ((rc = to_dev(W, lam_dev, lam.data(), (size_t)m))) return rc;
     if ((rc = resid_norms(W, AS, S, ldS, lam_dev, m, (int)k, S + m, rr))) return rc;
     if ((rc = lap(info->t_resid))) return rc;
     nconv = 0;
