@@ -1317,19 +1317,25 @@ struct RsState {
     RsHost host;
     DevMem mem;
     std::vector<RsLevel> lev;
+    // the two halves below the root (split): lev holds the first half's lists
+    // at levels < ljoin (all lists from ljoin on), levR the second half's
+    bool split = false, one_stream = false;
+    int32_t ljoin = 0;
+    std::vector<RsLevel> levR;
+    hipEvent_t sev[4] = {};        // fork / join of the halves' streams (up_st runs the second)
     int32_t *small = nullptr;
     RsNTask *ntask = nullptr;        // narrow fronts' forward tasks and backward records
     RsNFront *nfront = nullptr;
     RsRow *asmt = nullptr;
     RsPart *pt = nullptr;
     RsFin *ft = nullptr;
-    float *part = nullptr, *buf = nullptr, *Yg = nullptr, *Xg = nullptr;
+    float *part = nullptr, *part2 = nullptr, *buf = nullptr, *Yg = nullptr, *Xg = nullptr;
     int64_t *frow = nullptr, *uoff = nullptr, *lptr = nullptr;
     int32_t *lst = nullptr;
     int64_t n = 0, t0 = 0;
     // the single-right-hand-side solve's buffers (one FP64 value per row)
     bool rows = false, vec = false;            // which buffers the plan's upload allocates
-    double *vbuf = nullptr, *vY = nullptr, *vX = nullptr, *vpart = nullptr;
+    double *vbuf = nullptr, *vY = nullptr, *vX = nullptr, *vpart = nullptr, *vpart2 = nullptr;
     // the level launches of one solve captured as a graph (opt.use_graph):
     // they read and write only the plan's own buffers and the factor's pool,
     // so one capture serves every later solve of this factor
@@ -1346,6 +1352,8 @@ struct RsState {
         for (auto &v : tune_ev)
             for (hipEvent_t x : v)
                 if (x) (void)hipEventDestroy(x);
+        for (hipEvent_t x : sev)
+            if (x) (void)hipEventDestroy(x);
         if (up_st) (void)hipStreamDestroy(up_st);
     }
 };
@@ -1516,13 +1524,38 @@ int rs_plan(const Factor *F, RsState &B)
     std::vector<RsPart> &pt = H.pt;
     std::vector<RsFin> &ft = H.ft;
     int64_t &maxpart = H.maxpart;
-    B.lev.assign(S.nlevels, RsLevel());
-    for (int32_t L = 0; L < S.nlevels; L++) {
-        RsLevel &P = B.lev[L];
+    // The tree below a single root splits into two halves that share no front
+    // until the root: its first children's subtrees (about half of the fronts'
+    // F^2) and the rest.  Every level below the root is planned once per half,
+    // and a solve runs the halves' level chains on two streams (their launches
+    // are latency-bound, so the chains overlap), joining at the root.  Each
+    // front's work is the same either way.
+    B.split = false;
+    std::vector<char> left(ns, 1);
+    // test knob: 0 plans no split, 1 plans it but runs both halves on one stream
+    const char *ks = getenv("SPEIGS_HIP_RS_SPLIT");
+    B.one_stream = ks && atoi(ks) == 1;
+    if (S.nlevels >= 3 && bylevel[S.nlevels - 1].size() == 1 && !(ks && atoi(ks) == 0)) {
+        const int64_t R = bylevel[S.nlevels - 1][0], nk = S.chptr[R + 1] - S.chptr[R];
+        if (nk >= 2) {
+            std::vector<double> wsum(ns + 1, 0.0);
+            for (int64_t s = 0; s < ns; s++) wsum[s + 1] = wsum[s] + (double)S.F[s] * (double)S.F[s];
+            const double half = 0.5 * (wsum[ns] - (double)S.F[R] * (double)S.F[R]);
+            int64_t kj = S.chl[S.chptr[R]];
+            for (int64_t q = S.chptr[R]; q + 1 < S.chptr[R + 1]; q++) {   // the last child stays right
+                kj = S.chl[q];
+                if (wsum[kj + 1] >= half) break;
+            }
+            for (int64_t s = 0; s < ns; s++) left[s] = s <= kj;
+            B.split = true;
+            B.ljoin = S.nlevels - 1;
+        }
+    }
+    auto build_level = [&](int32_t L, const std::vector<int64_t> &fronts, RsLevel &P) {
         std::vector<int64_t> big;
         P.sm_off = (int64_t)small.size();
         std::vector<int32_t> cls[9];
-        for (int64_t s : bylevel[L]) {
+        for (int64_t s : fronts) {
             const int64_t c = S.c1[s] - S.c0[s];
             if (is_tiny(c, S.F[s] - c)) cls[tiny_class(c, S.F[s] - c)].push_back((int32_t)s);
             else big.push_back(s);
@@ -1539,7 +1572,7 @@ int rs_plan(const Factor *F, RsState &B)
         // sums them from the pull lists).  A leaf is read in place: its W1 is
         // its P rows, its W2 zero.
         P.as_off = (int64_t)asmt.size();
-        for (int64_t s : bylevel[L]) {
+        for (int64_t s : fronts) {
             const int64_t c = S.c1[s] - S.c0[s], r = S.F[s] - c;
             const bool tiny = is_tiny(c, r), leaf = S.chptr[s + 1] == S.chptr[s];
             if (leaf) continue;
@@ -1557,17 +1590,15 @@ int rs_plan(const Factor *F, RsState &B)
             P.ft_cnt[v] = (int64_t)ft.size() - f0;
             maxpart = std::max<int64_t>(maxpart, P.pt_cnt[v]);
         }
-    }
-    // the block solve's lists at the lowest levels: narrow fronts by class, the
-    // level's other fronts as above
-    for (int32_t L = 0; B.rows && L < std::min<int32_t>(kNarrowLevels, S.nlevels); L++) {
-        RsLevel &P = B.lev[L];
+        // the block solve's lists at the lowest levels: narrow fronts by class,
+        // the level's other fronts as above
+        if (!B.rows || L >= kNarrowLevels) return;
         P.narrow = true;
-        std::vector<int64_t> big;
+        big.clear();
         std::vector<RsNTask> fw[3];
         std::vector<RsNFront> b1[3], b4[3];
-        std::vector<int32_t> cls[9];
-        for (int64_t s : bylevel[L]) {
+        for (int q = 0; q < 9; q++) cls[q].clear();
+        for (int64_t s : fronts) {
             const int64_t c = S.c1[s] - S.c0[s], r = S.F[s] - c;
             if (is_narrow(c, r)) {
                 const int q = narrow_class(c);
@@ -1624,6 +1655,18 @@ int rs_plan(const Factor *F, RsState &B)
             P.rft_cnt[v] = (int64_t)ft.size() - f0;
             maxpart = std::max<int64_t>(maxpart, P.rpt_cnt[v]);
         }
+    };
+    B.lev.assign(S.nlevels, RsLevel());
+    B.levR.assign(S.nlevels, RsLevel());
+    for (int32_t L = 0; L < S.nlevels; L++) {
+        if (B.split && L < B.ljoin) {
+            std::vector<int64_t> fl, fr;
+            for (int64_t s : bylevel[L]) (left[s] ? fl : fr).push_back(s);
+            build_level(L, fl, B.lev[L]);
+            build_level(L, fr, B.levR[L]);
+        } else {
+            build_level(L, bylevel[L], B.lev[L]);
+        }
     }
     if (H.ntask.empty()) H.ntask.push_back(RsNTask{});
     if (H.nfront.empty()) H.nfront.push_back(RsNFront{});
@@ -1650,6 +1693,12 @@ int rs_upload(Factor *F, RsState &B, hipStream_t st)
         (rc = up(&B.lptr, H.lptr)) || (rc = up(&B.lst, H.lst)) || (rc = up(&B.ntask, H.ntask)) ||
         (rc = up(&B.nfront, H.nfront)))
         return rc;
+    if (B.split) {
+        for (hipEvent_t &e : B.sev)
+            SEHIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        if (B.rows && (rc = B.mem.alloc(&B.part2, (size_t)H.maxpart * RT_WIDE * RW))) return rc;
+        if (B.vec && (rc = B.mem.alloc(&B.vpart2, (size_t)H.maxpart * RTILE))) return rc;
+    }
     if (B.rows) {
         if ((rc = B.mem.alloc(&B.part, (size_t)H.maxpart * RT_WIDE * RW))) return rc;
         if ((rc = B.mem.alloc(&B.buf, (size_t)(2 * n + H.sumr) * RW))) return rc;
@@ -1733,7 +1782,7 @@ void narrow_bwd(const RsLevel &P, RsState &B, const T *pool, const RsArgs &a, hi
 
 // forward and backward over the levels, between the permutations
 template <class T>
-void rs_levels(Factor *F, RsState &B, hipStream_t st)
+int rs_levels(Factor *F, RsState &B, hipStream_t st)
 {
     const Symbolic &S = F->S;
     const T *pool = (const T *)F->pool;
@@ -1742,7 +1791,8 @@ void rs_levels(Factor *F, RsState &B, hipStream_t st)
     const dim3 blk(256);
     // wide: -1 the level's choice, else 0 / 1 for every level (autotuner runs);
     // tune: events around each level's L21 products (mode 1 at 2 L, mode 2 at 2 L + 1)
-    auto parts = [&](int mode, const RsLevel &P, int wide, hipEvent_t e0, hipEvent_t e1) {
+    auto parts = [&](int mode, const RsLevel &P, int wide, hipEvent_t e0, hipEvent_t e1, hipStream_t st,
+                     float *part) {
         const bool w = (mode == 1 || mode == 2) && (wide < 0 ? P.wide[mode - 1] : wide == 1);
         const int v = w ? mode + 3 : mode;
         const int64_t pc = P.narrow ? P.rpt_cnt[v] : P.pt_cnt[v], fc = P.narrow ? P.rft_cnt[v] : P.ft_cnt[v];
@@ -1752,50 +1802,92 @@ void rs_levels(Factor *F, RsState &B, hipStream_t st)
         if (e0) (void)hipEventRecord(e0, st);
         switch (v) {
         case 0:
-            if (pc) k_rs_part<T, 0, RTILE><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
-            if (fc) k_rs_fin<0, RTILE><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
+            if (pc) k_rs_part<T, 0, RTILE><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, part);
+            if (fc) k_rs_fin<0, RTILE><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, part);
             break;
         case 1:
-            if (pc) k_rs_part<T, 1, RTILE><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
-            if (fc) k_rs_fin<1, RTILE><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
+            if (pc) k_rs_part<T, 1, RTILE><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, part);
+            if (fc) k_rs_fin<1, RTILE><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, part);
             break;
         case 2:
-            if (pc) k_rs_part<T, 2, RTILE><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
-            if (fc) k_rs_fin<2, RTILE><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
+            if (pc) k_rs_part<T, 2, RTILE><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, part);
+            if (fc) k_rs_fin<2, RTILE><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, part);
             break;
         case 3:
-            if (pc) k_rs_part<T, 3, RTILE><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
-            if (fc) k_rs_fin<3, RTILE><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
+            if (pc) k_rs_part<T, 3, RTILE><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, part);
+            if (fc) k_rs_fin<3, RTILE><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, part);
             break;
         case 4:
-            if (pc) k_rs_part<T, 1, RT_WIDE><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
-            if (fc) k_rs_fin<1, RT_WIDE><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
+            if (pc) k_rs_part<T, 1, RT_WIDE><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, part);
+            if (fc) k_rs_fin<1, RT_WIDE><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, part);
             break;
         default:
-            if (pc) k_rs_part<T, 2, RT_WIDE><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, B.part);
-            if (fc) k_rs_fin<2, RT_WIDE><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, B.part);
+            if (pc) k_rs_part<T, 2, RT_WIDE><<<gp, blk, 0, st>>>(pool, pt, a, B.buf, B.Yg, B.Xg, part);
+            if (fc) k_rs_fin<2, RT_WIDE><<<gf, blk, 0, st>>>(ft, a, B.buf, B.Yg, B.Xg, part);
         }
         if (e1) (void)hipEventRecord(e1, st);
     };
     const int wide = B.tune_run;             // -1 outside the autotuner's runs
     hipEvent_t *ev = wide >= 0 ? B.tune_ev[wide].data() : nullptr;
     auto e_at = [&](int i) -> hipEvent_t { return ev ? ev[i] : nullptr; };
-    for (int32_t L = 0; L < S.nlevels; L++) {
-        const RsLevel &P = B.lev[L];
-        if (P.narrow) narrow_fwd<T>(P, B, pool, a, st);
+    // ea, eb: the autotuner's events before and after the level's L21 product
+    auto fwd = [&](const RsLevel &P, hipStream_t s, float *part, hipEvent_t ea, hipEvent_t eb) {
+        if (P.narrow) narrow_fwd<T>(P, B, pool, a, s);
         const int64_t asc = P.narrow ? P.ras_cnt : P.as_cnt;
-        if (asc) k_rs_assemble<<<dim3((unsigned)asc), blk, 0, st>>>(B.asmt + (P.narrow ? P.ras_off : P.as_off), a, B.buf);
-        if (P.sm_cnt) tiny_launch<T, true>(P, B, pool, a, st);
-        parts(0, P, wide, nullptr, nullptr);
-        parts(1, P, wide, e_at(4 * L), e_at(4 * L + 1));
+        if (asc) k_rs_assemble<<<dim3((unsigned)asc), blk, 0, s>>>(B.asmt + (P.narrow ? P.ras_off : P.as_off), a, B.buf);
+        if (P.sm_cnt) tiny_launch<T, true>(P, B, pool, a, s);
+        parts(0, P, wide, nullptr, nullptr, s, part);
+        parts(1, P, wide, ea, eb, s, part);
+    };
+    auto bwd = [&](const RsLevel &P, hipStream_t s, float *part, hipEvent_t ea, hipEvent_t eb) {
+        parts(2, P, wide, ea, eb, s, part);
+        parts(3, P, wide, nullptr, nullptr, s, part);
+        if (P.sm_cnt) tiny_launch<T, false>(P, B, pool, a, s);
+        if (P.narrow) narrow_bwd<T>(P, B, pool, a, s);
+    };
+    // The halves below the root on two streams.  The autotuner's timed runs
+    // use one, the first half's product opening the level's timing and the
+    // second's closing it.
+    const bool two = B.split && wide < 0 && !B.one_stream;
+    hipStream_t s2 = two ? B.up_st : st;
+    float *part2 = two ? B.part2 : B.part;
+    const int32_t lj = B.split ? B.ljoin : S.nlevels;
+    if (two) {
+        SEHIP_CHECK(hipEventRecord(B.sev[0], st));
+        SEHIP_CHECK(hipStreamWaitEvent(s2, B.sev[0], 0));
     }
-    for (int32_t L = S.nlevels - 1; L >= 0; L--) {
-        const RsLevel &P = B.lev[L];
-        parts(2, P, wide, e_at(4 * L + 2), e_at(4 * L + 3));
-        parts(3, P, wide, nullptr, nullptr);
-        if (P.sm_cnt) tiny_launch<T, false>(P, B, pool, a, st);
-        if (P.narrow) narrow_bwd<T>(P, B, pool, a, st);
+    for (int32_t L = 0; L < lj; L++) {
+        if (B.split) {
+            fwd(B.lev[L], st, B.part, e_at(4 * L), nullptr);
+            fwd(B.levR[L], s2, part2, nullptr, e_at(4 * L + 1));
+        } else {
+            fwd(B.lev[L], st, B.part, e_at(4 * L), e_at(4 * L + 1));
+        }
     }
+    // the root's level pulls both halves' update rows
+    if (two) {
+        SEHIP_CHECK(hipEventRecord(B.sev[1], s2));
+        SEHIP_CHECK(hipStreamWaitEvent(st, B.sev[1], 0));
+    }
+    for (int32_t L = lj; L < S.nlevels; L++) fwd(B.lev[L], st, B.part, e_at(4 * L), e_at(4 * L + 1));
+    for (int32_t L = S.nlevels - 1; L >= lj; L--) bwd(B.lev[L], st, B.part, e_at(4 * L + 2), e_at(4 * L + 3));
+    if (two) {
+        SEHIP_CHECK(hipEventRecord(B.sev[2], st));
+        SEHIP_CHECK(hipStreamWaitEvent(s2, B.sev[2], 0));
+    }
+    for (int32_t L = std::min(lj, S.nlevels) - 1; L >= 0; L--) {
+        if (B.split) {
+            bwd(B.lev[L], st, B.part, e_at(4 * L + 2), nullptr);
+            bwd(B.levR[L], s2, part2, nullptr, e_at(4 * L + 3));
+        } else {
+            bwd(B.lev[L], st, B.part, e_at(4 * L + 2), e_at(4 * L + 3));
+        }
+    }
+    if (two) {
+        SEHIP_CHECK(hipEventRecord(B.sev[3], s2));
+        SEHIP_CHECK(hipStreamWaitEvent(st, B.sev[3], 0));
+    }
+    return 0;
 }
 
 // The autotuner of the L21 products' tile height (SPEIGS_HIP_RS_WIDE=auto):
@@ -1859,7 +1951,7 @@ int rs_run(Factor *F, RsState &B, const double *Bin, int64_t brs, int64_t bcs, d
     int rc = rs_tune_begin(B, F->S.nlevels);
     if (rc) return rc;
     if (B.tune_state != 3) {
-        rs_levels<T>(F, B, st);   // the autotuner's timed runs (not captured)
+        if ((rc = rs_levels<T>(F, B, st))) return rc;   // the autotuner's timed runs (not captured)
         if ((rc = rs_tune_end(B, F->S.nlevels))) return rc;
     } else if (F->opt.use_graph) {
         if (B.gexec && B.gpool != F->pool) {
@@ -1869,10 +1961,14 @@ int rs_run(Factor *F, RsState &B, const double *Bin, int64_t brs, int64_t bcs, d
         if (!B.gexec) {
             // thread-local capture: launches of other threads are not drawn in
             SEHIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            rs_levels<T>(F, B, st);
+            const int lr = rs_levels<T>(F, B, st);
             const hipError_t le = hipGetLastError();
             hipGraph_t g = nullptr;
             const hipError_t ce = hipStreamEndCapture(st, &g);
+            if (lr) {
+                if (g) (void)hipGraphDestroy(g);
+                return lr;
+            }
             SEHIP_CHECK(le);
             SEHIP_CHECK(ce);
             const hipError_t ie = hipGraphInstantiate(&B.gexec, g, nullptr, nullptr, 0);
@@ -1881,8 +1977,8 @@ int rs_run(Factor *F, RsState &B, const double *Bin, int64_t brs, int64_t bcs, d
             B.gpool = F->pool;
         }
         SEHIP_CHECK(hipGraphLaunch(B.gexec, st));
-    } else {
-        rs_levels<T>(F, B, st);
+    } else if ((rc = rs_levels<T>(F, B, st))) {
+        return rc;
     }
     k_rs_out<<<dim3((unsigned)((F->n + 7) / 8)), blk, 0, st>>>(B.Xg, F->perm, F->n, nb, Xout, xrs, xcs);
     return 0;
@@ -1890,7 +1986,7 @@ int rs_run(Factor *F, RsState &B, const double *Bin, int64_t brs, int64_t bcs, d
 
 // one right-hand side: b, x device FP64 vectors of n (original order)
 template <class T>
-void vs_run(Factor *F, RsState &B, const double *b, double *x)
+int vs_run(Factor *F, RsState &B, const double *b, double *x)
 {
     const Symbolic &S = F->S;
     hipStream_t st = F->st;
@@ -1900,50 +1996,80 @@ void vs_run(Factor *F, RsState &B, const double *b, double *x)
     const dim3 blk(256);
     const unsigned nb = (unsigned)((F->n + 255) / 256);
     k_vs_in<<<dim3(nb), blk, 0, st>>>(b, F->perm, F->n, B.vbuf);
-    auto parts = [&](int mode, const RsLevel &P) {
+    auto parts = [&](int mode, const RsLevel &P, hipStream_t st, double *vpart) {
         const int64_t pc = P.pt_cnt[mode], fc = P.ft_cnt[mode];
         const RsPart *pt = B.pt + P.pt_off[mode];
         const RsFin *ft = B.ft + P.ft_off[mode];
         const dim3 gp((unsigned)pc), gf((unsigned)fc), b64(64);
         switch (mode) {
         case 0:
-            if (pc) k_vs_part<T, 0><<<gp, blk, 0, st>>>(pool, pt, a, B.vbuf, B.vY, B.vX, B.vpart);
-            if (fc) k_vs_fin<0><<<gf, b64, 0, st>>>(ft, a, B.vbuf, B.vY, B.vX, B.vpart);
+            if (pc) k_vs_part<T, 0><<<gp, blk, 0, st>>>(pool, pt, a, B.vbuf, B.vY, B.vX, vpart);
+            if (fc) k_vs_fin<0><<<gf, b64, 0, st>>>(ft, a, B.vbuf, B.vY, B.vX, vpart);
             break;
         case 1:
-            if (pc) k_vs_part<T, 1><<<gp, blk, 0, st>>>(pool, pt, a, B.vbuf, B.vY, B.vX, B.vpart);
-            if (fc) k_vs_fin<1><<<gf, b64, 0, st>>>(ft, a, B.vbuf, B.vY, B.vX, B.vpart);
+            if (pc) k_vs_part<T, 1><<<gp, blk, 0, st>>>(pool, pt, a, B.vbuf, B.vY, B.vX, vpart);
+            if (fc) k_vs_fin<1><<<gf, b64, 0, st>>>(ft, a, B.vbuf, B.vY, B.vX, vpart);
             break;
         case 2:
-            if (pc) k_vs_part<T, 2><<<gp, blk, 0, st>>>(pool, pt, a, B.vbuf, B.vY, B.vX, B.vpart);
-            if (fc) k_vs_fin<2><<<gf, b64, 0, st>>>(ft, a, B.vbuf, B.vY, B.vX, B.vpart);
+            if (pc) k_vs_part<T, 2><<<gp, blk, 0, st>>>(pool, pt, a, B.vbuf, B.vY, B.vX, vpart);
+            if (fc) k_vs_fin<2><<<gf, b64, 0, st>>>(ft, a, B.vbuf, B.vY, B.vX, vpart);
             break;
         default:
-            if (pc) k_vs_part<T, 3><<<gp, blk, 0, st>>>(pool, pt, a, B.vbuf, B.vY, B.vX, B.vpart);
-            if (fc) k_vs_fin<3><<<gf, b64, 0, st>>>(ft, a, B.vbuf, B.vY, B.vX, B.vpart);
+            if (pc) k_vs_part<T, 3><<<gp, blk, 0, st>>>(pool, pt, a, B.vbuf, B.vY, B.vX, vpart);
+            if (fc) k_vs_fin<3><<<gf, b64, 0, st>>>(ft, a, B.vbuf, B.vY, B.vX, vpart);
         }
     };
-    for (int32_t L = 0; L < S.nlevels; L++) {
-        const RsLevel &P = B.lev[L];
+    auto fwd = [&](const RsLevel &P, hipStream_t st, double *vpart) {
         if (P.as_cnt)
             k_vs_assemble<<<dim3((unsigned)((P.as_cnt + VS_ASM - 1) / VS_ASM)), blk, 0, st>>>(B.asmt + P.as_off,
                                                                                             P.as_cnt, a, B.vbuf);
         if (P.sm_cnt)
             k_vs_tiny_fwd<T><<<dim3((unsigned)((P.sm_cnt + 3) / 4)), blk, 0, st>>>(pool, B.small + P.sm_off,
                                                                                   (int)P.sm_cnt, a, B.vbuf, B.vY);
-        parts(0, P);
-        parts(1, P);
-    }
-    if (F->signed_fac) k_vs_sign<T><<<dim3(nb), blk, 0, st>>>(B.vY, (const T *)F->sgn, F->n);
-    for (int32_t L = S.nlevels - 1; L >= 0; L--) {
-        const RsLevel &P = B.lev[L];
-        parts(2, P);
-        parts(3, P);
+        parts(0, P, st, vpart);
+        parts(1, P, st, vpart);
+    };
+    auto bwd = [&](const RsLevel &P, hipStream_t st, double *vpart) {
+        parts(2, P, st, vpart);
+        parts(3, P, st, vpart);
         if (P.sm_cnt)
             k_vs_tiny_bwd<T><<<dim3((unsigned)((P.sm_cnt + 3) / 4)), blk, 0, st>>>(pool, B.small + P.sm_off,
                                                                                   (int)P.sm_cnt, a, B.vY, B.vX);
+    };
+    // the halves below the root on two streams, joined at the root and at the end
+    const bool two = B.split && !B.one_stream;
+    hipStream_t s2 = two ? B.up_st : st;
+    const int32_t lj = B.split ? B.ljoin : S.nlevels;
+    if (two) {
+        SEHIP_CHECK(hipEventRecord(B.sev[0], st));
+        SEHIP_CHECK(hipStreamWaitEvent(s2, B.sev[0], 0));
+    }
+    for (int32_t L = 0; L < lj; L++) {
+        fwd(B.lev[L], st, B.vpart);
+        fwd(B.levR[L], s2, two ? B.vpart2 : B.vpart);
+    }
+    // the root's level pulls both halves' update rows
+    if (two) {
+        SEHIP_CHECK(hipEventRecord(B.sev[1], s2));
+        SEHIP_CHECK(hipStreamWaitEvent(st, B.sev[1], 0));
+    }
+    for (int32_t L = lj; L < S.nlevels; L++) fwd(B.lev[L], st, B.vpart);
+    if (F->signed_fac) k_vs_sign<T><<<dim3(nb), blk, 0, st>>>(B.vY, (const T *)F->sgn, F->n);
+    for (int32_t L = S.nlevels - 1; L >= lj; L--) bwd(B.lev[L], st, B.vpart);
+    if (two) {
+        SEHIP_CHECK(hipEventRecord(B.sev[2], st));
+        SEHIP_CHECK(hipStreamWaitEvent(s2, B.sev[2], 0));
+    }
+    for (int32_t L = std::min(lj, S.nlevels) - 1; L >= 0; L--) {
+        bwd(B.lev[L], st, B.vpart);
+        bwd(B.levR[L], s2, two ? B.vpart2 : B.vpart);
+    }
+    if (two) {
+        SEHIP_CHECK(hipEventRecord(B.sev[3], s2));
+        SEHIP_CHECK(hipStreamWaitEvent(st, B.sev[3], 0));
     }
     k_vs_out<<<dim3(nb), blk, 0, st>>>(B.vX, F->perm, F->n, x);
+    return 0;
 }
 
 // join the plan's helper thread and its upload stream
@@ -1983,9 +2109,7 @@ int factor_vec_solve(Factor *F, const double *b, double *x)
     int rc = factor_vec_prepare(F);
     if (rc) return rc;
     RsState &B = *reinterpret_cast<RsState *>(F->rs);
-    if (F->fp32) vs_run<float>(F, B, b, x);
-    else vs_run<double>(F, B, b, x);
-    return 0;
+    return F->fp32 ? vs_run<float>(F, B, b, x) : vs_run<double>(F, B, b, x);
 }
 
 int factor_block_solve_rows(Factor *F, const double *Bin, int64_t brs, int64_t bcs, double *Xout, int64_t xrs,

@@ -267,6 +267,39 @@ def test_potrf_waves_bitwise(hiplib, monkeypatch, prec):
     assert diff <= (1e-11 if prec == 0 else 1e-3)
 
 
+@pytest.mark.parametrize("prec,solve_mode", [(2, None), (0, "3")])
+def test_solve_halves_two_streams_bitwise(hiplib, monkeypatch, prec, solve_mode):
+    """Below the root the elimination tree splits into two halves that share no
+    front, and the solves run the halves' level chains on two streams (joined
+    before the root's forward step and after the backward sweep).  Each front
+    does the same work either way: the FP32 multi-right-hand-side solve and the
+    FP64 single-vector solve (inverted diagonal blocks) give the same bits as
+    with the split planned away (SPEIGS_HIP_RS_SPLIT=0) and as with both halves
+    on one stream (=1), over two applies of one operator."""
+    A = gen.weighted_lap3d(28, seed=12, upper=True)
+    n = len(A[0]) - 1
+    if solve_mode:
+        monkeypatch.setenv("SPEIGS_HIP_SOLVE_MODE", solve_mode)
+    B = np.asfortranarray(np.random.default_rng(8).standard_normal((n, 3 if prec == 2 else 1)))
+    Bv = B if prec == 2 else np.ascontiguousarray(B[:, 0])
+    out = {}
+    for split in ("2", "1", "0"):
+        monkeypatch.setenv("SPEIGS_HIP_RS_SPLIT", split)
+        rc_, op, hi = _op(hiplib, A, 1, prec, 0.0)
+        assert rc_ == 0
+        xs = []
+        for _ in range(2):
+            X = np.zeros_like(Bv, order="F")
+            assert _apply(hiplib, op, Bv, X) == 0
+            xs.append(X)
+        hiplib.speigs_hip_op_destroy(op)
+        assert np.array_equal(xs[0], xs[1])
+        out[split] = xs[0]
+    R = gen.spmv_upper(*A, out["2"]) - Bv
+    assert np.linalg.norm(R) / np.linalg.norm(Bv) <= (1e-12 if prec == 0 else 1e-3)
+    assert np.array_equal(out["2"], out["0"]) and np.array_equal(out["1"], out["0"])
+
+
 @pytest.mark.parametrize("mode,mfma,prec", [("0", "0", 0), ("1", "0", 0), ("1", "1", 0), ("1", "0", 1), ("1", "1", 1),
                                             ("2", "1", 0), ("2", "1", 1), ("3", "1", 0), ("3", "1", 1)])
 def test_kernel_variants_agree(hiplib, monkeypatch, mode, mfma, prec):
