@@ -170,11 +170,36 @@ __device__ __forceinline__ int lower_bound_i32(const int32_t *a, int n, int v)
     return lo;
 }
 
+// Block boundaries of the children's update rows in their parents: for child
+// ch (rc = its update rows, rl = rel + relptr[ch] increasing parent rows) and k
+// = 0 .. ceil(F_parent / 64), bnd[bndptr[ch] + k] = the first j with rl[j] >=
+// 64 k (rc if none) -- the lower bounds the extend-add needs for its 64-column
+// and 256-row blocks, written once per factor by one thread per rel entry.
+__global__ void __launch_bounds__(256) k_ea_bounds(const int64_t *__restrict__ relptr, const int32_t *__restrict__ rel,
+                                                   const int64_t *__restrict__ bndptr, int64_t ns, int32_t *__restrict__ bnd)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= relptr[ns]) return;
+    int64_t lo = 0, hi = ns;                  // the child whose list holds e: relptr[ch] <= e < relptr[ch + 1]
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (relptr[mid] <= e) lo = mid; else hi = mid;
+    }
+    const int64_t ch = lo, j = e - relptr[ch], rc = relptr[ch + 1] - relptr[ch];
+    const int64_t nb = bndptr[ch + 1] - bndptr[ch];
+    int32_t *bd = bnd + bndptr[ch];
+    const int kcur = rel[e] >> 6, kprev = j == 0 ? -1 : rel[e - 1] >> 6;
+    for (int k = kprev + 1; k <= kcur && k < nb; k++) bd[k] = (int32_t)j;
+    if (j == rc - 1)
+        for (int64_t k = kcur + 1; k < nb; k++) bd[k] = (int32_t)rc;
+}
+
 // extend-add into the parent block (columns [64 cb, +64), rows [256 rb, +256)):
 // children in order with a barrier between them, so every element's sum is
 // deterministic.  The rel maps are increasing, so a child contributes a
-// contiguous range of its columns and of its rows; thread t takes child row
-// ilo + t, 16 columns at a time with all 16 loads issued before the stores.
+// contiguous range of its columns and of its rows (from the block boundaries
+// of k_ea_bounds); thread t takes child row ilo + t, 16 columns at a time with
+// all 16 loads issued before the stores.
 template <class T>
 __global__ void __launch_bounds__(256) k_extend_add_blk(T *pool, const EaBlk *__restrict__ tasks,
                                                         const int64_t *__restrict__ foff,
@@ -182,7 +207,9 @@ __global__ void __launch_bounds__(256) k_extend_add_blk(T *pool, const EaBlk *__
                                                         const int64_t *__restrict__ relptr,
                                                         const int32_t *__restrict__ rel,
                                                         const int64_t *__restrict__ chptr,
-                                                        const int32_t *__restrict__ chl)
+                                                        const int32_t *__restrict__ chl,
+                                                        const int64_t *__restrict__ bndptr,
+                                                        const int32_t *__restrict__ bnd)
 {
     constexpr int NB = 16;
     const EaBlk t = tasks[blockIdx.x];
@@ -194,23 +221,31 @@ __global__ void __launch_bounds__(256) k_extend_add_blk(T *pool, const EaBlk *__
         const int ch = chl[q];
         const int32_t Fc = Fs[ch], cc = ncol[ch], rc = Fc - cc;
         const int32_t *rl = rel + relptr[ch];
-        const int jlo = lower_bound_i32(rl, rc, c0), jhi = lower_bound_i32(rl, rc, c1);
-        const int ilo = lower_bound_i32(rl, rc, R0), ihi = lower_bound_i32(rl, rc, R1);
+        const int32_t *bd = bnd + bndptr[ch];
+        const int nbd = (int)(bndptr[ch + 1] - bndptr[ch]) - 1;   // = ceil(Fp / 64)
+        const bool any = rc > 0;                                  // (no boundaries are written for rc = 0)
+        const int jlo = any ? bd[t.cb] : 0, jhi = any ? bd[t.cb + 1] : 0;
+        const int ilo = any ? bd[4 * t.rb] : 0, ihi = any ? bd[min(4 * t.rb + 4, nbd)] : 0;
         const T *U = pool + foff[ch] + (int64_t)cc * Fc + cc;     // r x r, ld Fc
         const int i = ilo + threadIdx.x;
         if (jlo < jhi && i < ihi) {
             const int64_t prow = rl[i];
             const int jend = min(jhi, i + 1);      // U is lower triangular
             for (int j = jlo; j < jend; j += NB) {
+                // every load unconditional at an index clamped into [j, jend)
+                // (a select after each load, or a use between loads, makes
+                // each wait for the one before); only the stores are predicated
                 T u[NB], pv[NB];
                 int64_t pos[NB];
+                int32_t rj[NB];
+#pragma unroll
+                for (int b = 0; b < NB; b++) rj[b] = rl[min(j + b, jend - 1)];
 #pragma unroll
                 for (int b = 0; b < NB; b++) {
-                    const int jj = j + b;
-                    const bool ok = jj < jend;
-                    pos[b] = ok ? (int64_t)rl[jj] * Fp + prow : 0;
-                    u[b] = ok ? U[(int64_t)jj * Fc + i] : T(0);
-                    pv[b] = ok ? P[pos[b]] : T(0);
+                    const int jj = min(j + b, jend - 1);
+                    pos[b] = (int64_t)rj[b] * Fp + prow;
+                    u[b] = U[(int64_t)jj * Fc + i];
+                    pv[b] = P[pos[b]];
                 }
 #pragma unroll
                 for (int b = 0; b < NB; b++)
@@ -1515,7 +1550,8 @@ int run_numeric(Factor *F, double delta)
         const LevelPlan &P = F->plan[L];
         if (P.pcb_cnt > 0)
             k_extend_add_blk<T><<<dim3((unsigned)P.pcb_cnt), dim3(256), 0, st>>>(
-                pool, F->eab + P.pcb_off, F->foff, F->Fs, F->ncol, F->relptr, F->rel, F->chptr, F->chl);
+                pool, F->eab + P.pcb_off, F->foff, F->Fs, F->ncol, F->relptr, F->rel, F->chptr, F->chl, F->bndptr,
+                F->bnd);
         for (size_t k = 0; k < P.k_potrf_cnt.size(); k++) {
             if (F->use_reg_panel) {
                 // panel in registers (+ its inverse), then TRSM as a product with the inverse
@@ -1900,6 +1936,12 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     if ((rc = stage(&F->rel, S.rel.data(), S.rel.size()))) return rc;
     if ((rc = stage(&F->chl, chl32.data(), chl32.size()))) return rc;
     if ((rc = stage(&F->perm, S.perm.data(), S.perm.size()))) return rc;
+    // the extend-add's block boundaries: ceil(F_parent / 64) + 1 per child
+    std::vector<int64_t> bndptr(ns + 1, 0);
+    for (int64_t s2 = 0; s2 < ns; s2++)
+        bndptr[s2 + 1] = bndptr[s2] + (S.parent[s2] >= 0 ? (S.F[S.parent[s2]] + 63) / 64 + 1 : 0);
+    if ((rc = stage(&F->bndptr, bndptr.data(), bndptr.size()))) return rc;
+    if ((rc = F->mem.alloc(&F->bnd, (size_t)std::max<int64_t>(bndptr[ns], 1)))) return rc;
     up_th.join();
     if (up_rc) {
         note_error(up_msg.c_str());
@@ -1911,6 +1953,9 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     if ((rc = F->mem.alloc(&F->snode, (size_t)n))) return rc;
     if ((rc = F->mem.alloc(&F->diagA, (size_t)n))) return rc;
     k_snode_fill<<<dim3((unsigned)((ns + 3) / 4)), dim3(256), 0, st>>>(F->col0, F->ncol, ns, F->snode);
+    if (S.relptr[ns] > 0)
+        k_ea_bounds<<<dim3((unsigned)((S.relptr[ns] + 255) / 256)), dim3(256), 0, st>>>(F->relptr, F->rel, F->bndptr,
+                                                                                       ns, F->bnd);
     SEHIP_CHECK(hipMemsetAsync(F->diagA, 0, (size_t)n * sizeof(double), st));
     k_diag_values<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st>>>(F->Apd, F->Aid, F->Ax, F->iperm, n,
                                                                           F->diagA);

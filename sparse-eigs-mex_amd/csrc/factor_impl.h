@@ -76,6 +76,8 @@ struct Factor {
     double *Ax = nullptr, *diagA = nullptr;
     int64_t *Apd = nullptr, *Aid = nullptr, *iperm = nullptr;   // A's CSC arrays, iperm[old] = new
     int32_t *snode = nullptr;                                     // supernode of each permuted column
+    int64_t *bndptr = nullptr;                                    // extend-add block boundaries (k_ea_bounds)
+    int32_t *bnd = nullptr;
     int64_t *foff = nullptr, *wsoff = nullptr, *col0 = nullptr, *rptr = nullptr, *relptr = nullptr, *chptr = nullptr;
     int32_t *Fs = nullptr, *ncol = nullptr, *rows = nullptr, *rel = nullptr, *chl = nullptr, *fronts = nullptr;
     int64_t *perm = nullptr;
