@@ -194,12 +194,24 @@ __global__ void __launch_bounds__(256) k_ea_bounds(const int64_t *__restrict__ r
         for (int64_t k = kcur + 1; k < nb; k++) bd[k] = (int32_t)rc;
 }
 
+__device__ __forceinline__ int32_t lane_i32(int32_t v, int q) { return __builtin_amdgcn_readlane(v, q); }
+__device__ __forceinline__ int64_t lane_i64(int64_t v, int q)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, q);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(v >> 32), q);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // extend-add into the parent block (columns [64 cb, +64), rows [256 rb, +256)):
-// children in order with a barrier between them, so every element's sum is
-// deterministic.  The rel maps are increasing, so a child contributes a
-// contiguous range of its columns and of its rows (from the block boundaries
-// of k_ea_bounds); thread t takes child row ilo + t, 16 columns at a time with
-// all 16 loads issued before the stores.
+// each wave takes one 64-row block of it and applies the children in order,
+// so every element's sum is deterministic.  The lanes first read the
+// children's sizes, offsets and block boundaries together (lane q for the
+// q-th child of a run of 64; the rel maps are increasing, so a child
+// contributes a contiguous range of its columns and of its rows, from the
+// boundaries of k_ea_bounds), then child by child lane t takes child row ilo
+// + t, 16 columns at a time with all 16 loads issued before the stores.  Two
+// children may add into the same parent element from different lanes: the
+// wave's stores complete before the next child's loads.
 template <class T>
 __global__ void __launch_bounds__(256) k_extend_add_blk(T *pool, const EaBlk *__restrict__ tasks,
                                                         const int64_t *__restrict__ foff,
@@ -213,46 +225,63 @@ __global__ void __launch_bounds__(256) k_extend_add_blk(T *pool, const EaBlk *__
 {
     constexpr int NB = 16;
     const EaBlk t = tasks[blockIdx.x];
+    const int lane = threadIdx.x & 63;
+    const int r64 = 4 * t.rb + (int)(threadIdx.x >> 6);      // the wave's 64-row block of the parent
     const int32_t Fp = Fs[t.parent];
+    // rows above the column block get nothing (a child's U is lower triangular)
+    if (r64 < t.cb || 64 * r64 >= Fp) return;
     T *P = pool + foff[t.parent];
-    const int c0 = t.cb * 64, c1 = min(Fp, c0 + 64);
-    const int R0 = t.rb * 256, R1 = min(Fp, R0 + 256);
-    for (int64_t q = chptr[t.parent]; q < chptr[t.parent + 1]; q++) {
-        const int ch = chl[q];
-        const int32_t Fc = Fs[ch], cc = ncol[ch], rc = Fc - cc;
-        const int32_t *rl = rel + relptr[ch];
+    const int64_t q0 = chptr[t.parent], q1 = chptr[t.parent + 1];
+    for (int64_t qb = q0; qb < q1; qb += 64) {
+        const int nq = (int)min((int64_t)64, q1 - qb);
+        const int ch = chl[qb + min(lane, nq - 1)];
+        const int32_t Fc = Fs[ch], cc = ncol[ch];
+        const int64_t fo = foff[ch], ro = relptr[ch];
+        // every child has ceil(Fp / 64) + 1 boundaries and cb, r64 < ceil(Fp / 64);
+        // none is written for a child with no update rows
         const int32_t *bd = bnd + bndptr[ch];
-        const int nbd = (int)(bndptr[ch + 1] - bndptr[ch]) - 1;   // = ceil(Fp / 64)
-        const bool any = rc > 0;                                  // (no boundaries are written for rc = 0)
-        const int jlo = any ? bd[t.cb] : 0, jhi = any ? bd[t.cb + 1] : 0;
-        const int ilo = any ? bd[4 * t.rb] : 0, ihi = any ? bd[min(4 * t.rb + 4, nbd)] : 0;
-        const T *U = pool + foff[ch] + (int64_t)cc * Fc + cc;     // r x r, ld Fc
-        const int i = ilo + threadIdx.x;
-        if (jlo < jhi && i < ihi) {
-            const int64_t prow = rl[i];
-            const int jend = min(jhi, i + 1);      // U is lower triangular
-            for (int j = jlo; j < jend; j += NB) {
-                // every load unconditional at an index clamped into [j, jend)
-                // (a select after each load, or a use between loads, makes
-                // each wait for the one before); only the stores are predicated
-                T u[NB], pv[NB];
-                int64_t pos[NB];
-                int32_t rj[NB];
+        const int32_t b0 = bd[t.cb], b1 = bd[t.cb + 1], b2 = bd[r64], b3 = bd[r64 + 1];
+        const bool any = Fc > cc;
+        const int32_t jlo_l = any ? b0 : 0, jhi_l = any ? b1 : 0, ilo_l = any ? b2 : 0, ihi_l = any ? b3 : 0;
+        for (int q = 0; q < nq; q++) {
+            // (read out of lane q here, where every lane is active)
+            const int jlo = lane_i32(jlo_l, q), jhi = lane_i32(jhi_l, q);
+            const int ilo = lane_i32(ilo_l, q), ihi = lane_i32(ihi_l, q);
+            const int32_t Fcq = lane_i32(Fc, q), ccq = lane_i32(cc, q);
+            const int64_t roq = lane_i64(ro, q), foq = lane_i64(fo, q);
+            const int i = ilo + lane;
+            if (jlo < jhi && i < ihi) {
+                const int32_t *rl = rel + roq;
+                const T *U = pool + foq + (int64_t)ccq * Fcq + ccq;     // r x r, ld Fc
+                const int64_t prow = rl[i];
+                const int jend = min(jhi, i + 1);
+                for (int j = jlo; j < jend; j += NB) {
+                    // every load unconditional at an index clamped into [j, jend)
+                    // (a select after each load, or a use between loads, makes
+                    // each wait for the one before); only the stores are predicated
+                    T u[NB], pv[NB];
+                    int64_t pos[NB];
+                    int32_t rj[NB];
 #pragma unroll
-                for (int b = 0; b < NB; b++) rj[b] = rl[min(j + b, jend - 1)];
+                    for (int b = 0; b < NB; b++) rj[b] = rl[min(j + b, jend - 1)];
 #pragma unroll
-                for (int b = 0; b < NB; b++) {
-                    const int jj = min(j + b, jend - 1);
-                    pos[b] = (int64_t)rj[b] * Fp + prow;
-                    u[b] = U[(int64_t)jj * Fc + i];
-                    pv[b] = P[pos[b]];
+                    for (int b = 0; b < NB; b++) {
+                        const int jj = min(j + b, jend - 1);
+                        pos[b] = (int64_t)rj[b] * Fp + prow;
+                        u[b] = U[(int64_t)jj * Fcq + i];
+                        pv[b] = P[pos[b]];
+                    }
+#pragma unroll
+                    for (int b = 0; b < NB; b++)
+                        if (j + b < jend) P[pos[b]] = pv[b] + u[b];
                 }
-#pragma unroll
-                for (int b = 0; b < NB; b++)
-                    if (j + b < jend) P[pos[b]] = pv[b] + u[b];
             }
+            // the stores done before the next child's loads (other lanes may
+            // read these elements); the fences also keep the compiler from
+            // moving memory operations across
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         }
-        __syncthreads();
     }
 }
 
@@ -1960,21 +1989,34 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
     k_diag_values<<<dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st>>>(F->Apd, F->Aid, F->Ax, F->iperm, n,
                                                                           F->diagA);
     trace_phase("assembly data", tp);
-    // plans: extend-add tasks, tile tasks, fronts per level
-    std::vector<EaTask> ea;
-    std::vector<TileTask> tiles;
-    std::vector<int32_t> ufid;
-    std::vector<int64_t> upref;
+    // plans: extend-add tasks, tile tasks, fronts per level.  The levels'
+    // lists are built in parallel, each with offsets into lists of its own,
+    // then placed one after another (the offsets moved by the lengths of the
+    // levels before; front ordinals and tile prefixes are per step already)
     std::vector<int32_t> fronts;
     std::vector<std::vector<int64_t>> bylevel(S.nlevels);
     for (int64_t s = 0; s < ns; s++) bylevel[S.level[s]].push_back(s);
     F->plan.resize(S.nlevels);
-    int64_t xinv_max = 1;
     for (int32_t L = 0; L < S.nlevels; L++) {
-        LevelPlan &P = F->plan[L];
-        P.fr_off = (int64_t)fronts.size();
+        F->plan[L].fr_off = (int64_t)fronts.size();
         for (int64_t s : bylevel[L]) fronts.push_back((int32_t)s);
-        P.fr_cnt = (int64_t)bylevel[L].size();
+        F->plan[L].fr_cnt = (int64_t)bylevel[L].size();
+    }
+    struct LevelLists {
+        std::vector<EaTask> ea;
+        std::vector<TileTask> tiles;
+        std::vector<int32_t> ufid;
+        std::vector<int64_t> upref;
+        int64_t xinv_max = 1;
+    };
+    std::vector<LevelLists> LL(S.nlevels);
+    parallel_for(S.nlevels, nt, 1, [&](int64_t L, int) {
+        LevelPlan &P = F->plan[L];
+        std::vector<EaTask> &ea = LL[L].ea;
+        std::vector<TileTask> &tiles = LL[L].tiles;
+        std::vector<int32_t> &ufid = LL[L].ufid;
+        std::vector<int64_t> &upref = LL[L].upref;
+        int64_t &xinv_max = LL[L].xinv_max;
         int64_t maxch = 0, maxpan = 0;
         for (int64_t s : bylevel[L]) {
             maxch = std::max(maxch, S.chptr[s + 1] - S.chptr[s]);
@@ -2075,7 +2117,39 @@ int factor_create(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
             upref.push_back(accf);
             P.k_out2_cnt.push_back(accf);
         }
+    });
+    int64_t xinv_max = 1;
+    std::vector<int64_t> b_ea(S.nlevels + 1, 0), b_tl(S.nlevels + 1, 0), b_uf(S.nlevels + 1, 0), b_ur(S.nlevels + 1, 0);
+    for (int32_t L = 0; L < S.nlevels; L++) {
+        b_ea[L + 1] = b_ea[L] + (int64_t)LL[L].ea.size();
+        b_tl[L + 1] = b_tl[L] + (int64_t)LL[L].tiles.size();
+        b_uf[L + 1] = b_uf[L] + (int64_t)LL[L].ufid.size();
+        b_ur[L + 1] = b_ur[L] + (int64_t)LL[L].upref.size();
+        xinv_max = std::max(xinv_max, LL[L].xinv_max);
+        LevelPlan &P = F->plan[L];
+        for (int64_t &x : P.ea_off) x += b_ea[L];
+        for (int64_t &x : P.k_potrf_off) x += b_tl[L];
+        for (int64_t &x : P.k_trsm_off) x += b_tl[L];
+        for (auto &cut : P.k_trsm_cut)
+            for (int64_t &x : cut) x += b_tl[L];
+        for (int64_t &x : P.k_upd_off) x += b_uf[L];
+        for (int64_t &x : P.k_out_off) x += b_uf[L];
+        for (int64_t &x : P.k_out2_off) x += b_uf[L];
+        for (int64_t &x : P.k_upref_off) x += b_ur[L];
+        for (int64_t &x : P.k_outref_off) x += b_ur[L];
+        for (int64_t &x : P.k_out2ref_off) x += b_ur[L];
     }
+    std::vector<EaTask, NoInitAlloc<EaTask>> ea(b_ea[S.nlevels]);
+    std::vector<TileTask, NoInitAlloc<TileTask>> tiles(b_tl[S.nlevels]);
+    std::vector<int32_t, NoInitAlloc<int32_t>> ufid(b_uf[S.nlevels]);
+    std::vector<int64_t, NoInitAlloc<int64_t>> upref(b_ur[S.nlevels]);
+    parallel_for(S.nlevels, nt, 1, [&](int64_t L, int) {
+        std::copy(LL[L].ea.begin(), LL[L].ea.end(), ea.begin() + b_ea[L]);
+        std::copy(LL[L].tiles.begin(), LL[L].tiles.end(), tiles.begin() + b_tl[L]);
+        std::copy(LL[L].ufid.begin(), LL[L].ufid.end(), ufid.begin() + b_uf[L]);
+        std::copy(LL[L].upref.begin(), LL[L].upref.end(), upref.begin() + b_ur[L]);
+        LL[L] = LevelLists();
+    });
     trace_phase("factor task lists", tp);
     if (trace_on())
         for (int32_t L = 0; L < S.nlevels; L++) {
