@@ -30,6 +30,7 @@ constexpr int RG_AHEAD = 4;        // steps of 4 rows loaded ahead
 
 typedef double rd4 __attribute__((ext_vector_type(4)));
 
+template <int GA, int GB>
 __global__ void __launch_bounds__(256) k_rgram_partial(const double *__restrict__ A, int64_t lda, int p,
                                                        const double *__restrict__ B, int64_t ldb, int q, int64_t n,
                                                        int64_t rows_per_wave, double *__restrict__ partial)
@@ -37,53 +38,102 @@ __global__ void __launch_bounds__(256) k_rgram_partial(const double *__restrict_
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int lr = lane & 15, lk = lane >> 4;
     const int pt = (p + 15) >> 4, qt = (q + 15) >> 4;
-    // halves of the tile rows and of the tile columns (<= 3 each): the waves'
-    // shares of the products differ by at most one row or column of tiles
-    const int ga = (pt + 1) >> 1, gb = (qt + 1) >> 1;
-    const int a0 = ga * (wv & 1), b0 = gb * (wv >> 1);   // this wave's tile group
+    // halves of the tile rows and of the tile columns (GA = ceil(pt / 2), GB =
+    // ceil(qt / 2) each): the waves' shares differ by at most one row or
+    // column of tiles, and a wave's group is always GA x GB products (a tile
+    // past the matrix multiplies clamped columns into outputs never stored),
+    // so no product waits on a branch
+    const int a0 = GA * (wv & 1), b0 = GB * (wv >> 1);   // this wave's tile group
     if (a0 >= pt || b0 >= qt) return;                     // no workgroup barrier below
-    rd4 acc[3][3];
+    rd4 acc[GA][GB];
 #pragma unroll
-    for (int a = 0; a < 3; a++)
+    for (int a = 0; a < GA; a++)
 #pragma unroll
-        for (int b = 0; b < 3; b++) acc[a][b] = rd4{0.0, 0.0, 0.0, 0.0};
-    const int64_t r0 = (int64_t)blockIdx.x * rows_per_wave, r1 = min(n, r0 + rows_per_wave);
-    // ring of RG_AHEAD steps: slot s holds the rows r + 4 s of the current round
-    double xa[RG_AHEAD][3], xb[RG_AHEAD][3];
+        for (int b = 0; b < GB; b++) acc[a][b] = rd4{0.0, 0.0, 0.0, 0.0};
+    // (the last workgroups may start past n: no rows)
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_wave, r1 = max(r0, min(n, r0 + rows_per_wave));
+    const int64_t rfull = r0 + ((r1 - r0) & ~(int64_t)3);  // whole steps of 4 rows
+    // columns clamped into the matrices: a column past p (q) feeds only rows
+    // (columns) of the product past p (q), which are not stored
+    int64_t ofa[GA], ofb[GB];
+#pragma unroll
+    for (int a = 0; a < GA; a++) ofa[a] = min(16 * (a0 + a) + lr, p - 1);
+#pragma unroll
+    for (int b = 0; b < GB; b++) ofb[b] = min(16 * (b0 + b) + lr, q - 1);
+    // ring of RG_AHEAD steps: slot s holds the rows r + 4 s of the current
+    // round.  While the next round is whole its loads are unconditional (a
+    // load under a branch makes the step wait for every load in flight); the
+    // last round in the ring is then multiplied out, and the steps left (fewer
+    // than a round) go one at a time
+    double xa[RG_AHEAD][GA], xb[RG_AHEAD][GB];
     auto load = [&](int s, int64_t r) {
         const int64_t row = r + lk;
-        const bool in = row < r1;
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
-            const int ca = 16 * (a0 + a) + lr, cb = 16 * (b0 + a) + lr;
-            xa[s][a] = (a < ga && a0 + a < pt && in && ca < p) ? A[row * lda + ca] : 0.0;
-            xb[s][a] = (a < gb && b0 + a < qt && in && cb < q) ? B[row * ldb + cb] : 0.0;
-        }
+        for (int a = 0; a < GA; a++) xa[s][a] = A[row * lda + ofa[a]];
+#pragma unroll
+        for (int b = 0; b < GB; b++) xb[s][b] = B[row * ldb + ofb[b]];
     };
+    auto mul = [&](int s) {
 #pragma unroll
-    for (int s = 0; s < RG_AHEAD; s++) load(s, r0 + 4 * s);
-    for (int64_t r = r0; r < r1; r += 4 * RG_AHEAD) {
+        for (int a = 0; a < GA; a++)
 #pragma unroll
-        for (int s = 0; s < RG_AHEAD; s++) {
-            if (r + 4 * s >= r1) break;
-            double ca[3], cb[3];
+            for (int b = 0; b < GB; b++)
+                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[s][a], xb[s][b], acc[a][b], 0, 0, 0);
+    };
+    constexpr int64_t RND = 4 * RG_AHEAD;
+    int64_t r = r0;
+    if (r + RND <= rfull) {
 #pragma unroll
-            for (int a = 0; a < 3; a++) { ca[a] = xa[s][a]; cb[a] = xb[s][a]; }
-            if (r + 4 * s + 4 * RG_AHEAD < r1) load(s, r + 4 * s + 4 * RG_AHEAD);
+        for (int s = 0; s < RG_AHEAD; s++) load(s, r + 4 * s);
+        for (; r + 2 * RND <= rfull; r += RND) {
 #pragma unroll
-            for (int a = 0; a < 3; a++)
+            for (int s = 0; s < RG_AHEAD; s++) {
+                double ca[GA], cb[GB];
 #pragma unroll
-                for (int b = 0; b < 3; b++)
-                    if (a < ga && b < gb && a0 + a < pt && b0 + b < qt)
+                for (int a = 0; a < GA; a++) ca[a] = xa[s][a];
+#pragma unroll
+                for (int b = 0; b < GB; b++) cb[b] = xb[s][b];
+                load(s, r + RND + 4 * s);
+#pragma unroll
+                for (int a = 0; a < GA; a++)
+#pragma unroll
+                    for (int b = 0; b < GB; b++)
                         acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[a], cb[b], acc[a][b], 0, 0, 0);
+            }
         }
+#pragma unroll
+        for (int s = 0; s < RG_AHEAD; s++) mul(s);
+        r += RND;
+    }
+    for (; r < rfull; r += 4) {
+        load(0, r);
+        mul(0);
+    }
+    if (rfull < r1) {            // the last 1-3 rows: the step's other rows are zero
+        const int64_t row = rfull + lk;
+        const bool in = row < r1;
+        const int64_t rc = in ? row : r1 - 1;
+        double ca[GA], cb[GB];
+#pragma unroll
+        for (int a = 0; a < GA; a++) ca[a] = A[rc * lda + ofa[a]];
+#pragma unroll
+        for (int b = 0; b < GB; b++) cb[b] = B[rc * ldb + ofb[b]];
+#pragma unroll
+        for (int a = 0; a < GA; a++) ca[a] = in ? ca[a] : 0.0;
+#pragma unroll
+        for (int b = 0; b < GB; b++) cb[b] = in ? cb[b] : 0.0;
+#pragma unroll
+        for (int a = 0; a < GA; a++)
+#pragma unroll
+            for (int b = 0; b < GB; b++)
+                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ca[a], cb[b], acc[a][b], 0, 0, 0);
     }
     double *out = partial + (int64_t)blockIdx.x * p * q;
 #pragma unroll
-    for (int a = 0; a < 3; a++)
+    for (int a = 0; a < GA; a++)
 #pragma unroll
-        for (int b = 0; b < 3; b++) {
-            if (a >= ga || b >= gb || a0 + a >= pt || b0 + b >= qt) continue;
+        for (int b = 0; b < GB; b++) {
+            if (a0 + a >= pt || b0 + b >= qt) continue;
             const int j = 16 * (b0 + b) + lr;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
@@ -130,7 +180,19 @@ void rgram(const double *A, int64_t lda, int p, const double *B, int64_t ldb, in
            double *G, hipStream_t st)
 {
     const int64_t rpw = rgram_rpw(n), nw = rgram_blocks(n);
-    k_rgram_partial<<<dim3((unsigned)nw), dim3(256), 0, st>>>(A, lda, p, B, ldb, q, n, rpw, partial);
+    const int ga = ((p + 15) / 16 + 1) / 2, gb = ((q + 15) / 16 + 1) / 2;    // <= 3 each (p, q <= 96)
+    const dim3 g((unsigned)nw), t(256);
+    switch (3 * (ga - 1) + (gb - 1)) {
+    case 0: k_rgram_partial<1, 1><<<g, t, 0, st>>>(A, lda, p, B, ldb, q, n, rpw, partial); break;
+    case 1: k_rgram_partial<1, 2><<<g, t, 0, st>>>(A, lda, p, B, ldb, q, n, rpw, partial); break;
+    case 2: k_rgram_partial<1, 3><<<g, t, 0, st>>>(A, lda, p, B, ldb, q, n, rpw, partial); break;
+    case 3: k_rgram_partial<2, 1><<<g, t, 0, st>>>(A, lda, p, B, ldb, q, n, rpw, partial); break;
+    case 4: k_rgram_partial<2, 2><<<g, t, 0, st>>>(A, lda, p, B, ldb, q, n, rpw, partial); break;
+    case 5: k_rgram_partial<2, 3><<<g, t, 0, st>>>(A, lda, p, B, ldb, q, n, rpw, partial); break;
+    case 6: k_rgram_partial<3, 1><<<g, t, 0, st>>>(A, lda, p, B, ldb, q, n, rpw, partial); break;
+    case 7: k_rgram_partial<3, 2><<<g, t, 0, st>>>(A, lda, p, B, ldb, q, n, rpw, partial); break;
+    default: k_rgram_partial<3, 3><<<g, t, 0, st>>>(A, lda, p, B, ldb, q, n, rpw, partial); break;
+    }
     k_rgram_reduce<<<dim3((unsigned)((p * q + 63) / 64)), dim3(256), 0, st>>>(partial, nw, p * q, G);
 }
 

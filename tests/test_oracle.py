@@ -240,14 +240,20 @@ def test_jacobi_graded_relative_accuracy(oracle):
     # and a normwise solver cannot promise that: its bound is u*||S|| ~ 1e-8
 
 
-def test_sparse_cholesky_vs_dense(oracle):
-    """L L^T = P(A+dI)P^T and the solve, against numpy on a small weighted grid."""
-    o = oracle
+def _chol(o):
+    """The oracle's sparse Cholesky entry points with their argument types (each
+    test binds them itself: the tests may run in any order or process)."""
     o.oracle_chol_factor.restype = ctypes.c_void_p
     o.oracle_chol_factor.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
     o.oracle_chol_solve.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     o.oracle_chol_free.argtypes = [ctypes.c_void_p]
+    return o
+
+
+def test_sparse_cholesky_vs_dense(oracle):
+    """L L^T = P(A+dI)P^T and the solve, against numpy on a small weighted grid."""
+    o = _chol(oracle)
     for upper in (True, False):
         Ap, Ai, Ax = gen.weighted_lap3d(7, seed=5, upper=upper)
         n = len(Ap) - 1
@@ -266,7 +272,7 @@ def test_sparse_cholesky_vs_dense(oracle):
 
 def test_sparse_cholesky_known_answer(oracle):
     """(A + d I)^-1 v = v / (lambda + d) for a separable Dirichlet eigenvector."""
-    o = oracle
+    o = _chol(oracle)
     N = 14
     Ap, Ai, Ax = gen.lap3d(N, True)
     n = N ** 3
@@ -282,7 +288,7 @@ def test_sparse_cholesky_known_answer(oracle):
 
 
 def test_neumann_not_pd_detected(oracle):
-    o = oracle
+    o = _chol(oracle)
     Ap, Ai, Ax = gen.lap3d(8, False)
     st = ctypes.c_int(0)
     F = o.oracle_chol_factor(512, Ap.ctypes.data, Ai.ctypes.data, Ax.ctypes.data, 0.0, ctypes.byref(st))
