@@ -208,7 +208,12 @@ constexpr int RM_CP = 100;   // LDS pitch of C rows
 constexpr int RM_GROUPS = 4;   // 16-row groups per wave
 
 // X may be V itself (same rows, same stride): a wave reads all of its 16 rows'
-// inputs before it writes any output row, and no other wave touches them
+// inputs before it writes any output row, and no other wave touches them.
+// PT = ceil(p / 16) output tiles and NS >= ceil(m / 4) steps are fixed per
+// instance, so no product or load waits on a branch: rows past n are read at
+// row n - 1 (they feed only output rows that are not stored) and inner indices
+// past m at column m - 1 (C's rows past m are zero in LDS)
+template <int PT, int NS>
 __global__ void __launch_bounds__(256) k_rgemm(const double *V, int64_t ldv, int m,
                                                const double *__restrict__ C, int p, double *X,
                                                int64_t ldx, int64_t n, double beta)
@@ -219,16 +224,16 @@ __global__ void __launch_bounds__(256) k_rgemm(const double *V, int64_t ldv, int
         const int c = e / m, k = e - c * m;
         Cs[k][c] = C[e];
     }
-    // zero the pad of the last step of 4 and of the last 16-column tile
-    const int m4 = (m + 3) & ~3, p16 = (p + 15) & ~15;
-    for (int e = threadIdx.x; e < m4 * p16; e += 256) {
-        const int k = e / p16, c = e - k * p16;
+    // zero the rows past m up to the last step and the columns past p up to
+    // the last 16-column tile
+    constexpr int K4 = 4 * NS, P16 = 16 * PT;
+    for (int e = threadIdx.x; e < K4 * P16; e += 256) {
+        const int k = e / P16, c = e - k * P16;
         if (k >= m || c >= p) Cs[k][c] = 0.0;
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int lr = lane & 15, lk = lane >> 4;
-    const int pt = (p + 15) >> 4;
     const int64_t rbase = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * RM_GROUPS * 16;
     // one group's 16 x m slice of V in registers at a time: loading the next
     // group's during the current products (double the registers, occupancy 2
@@ -236,28 +241,20 @@ __global__ void __launch_bounds__(256) k_rgemm(const double *V, int64_t ldv, int
     for (int g = 0; g < RM_GROUPS; g++) {
         const int64_t r0 = rbase + g * 16;
         if (r0 >= n) break;
-        double va[24];
-        const int64_t arow = r0 + lr;
-        const bool rin = arow < n;
+        const int64_t arow = min(r0 + lr, n - 1);
+        double va[NS];
 #pragma unroll
-        for (int s2 = 0; s2 < 24; s2++) {
-            const int kk = 4 * s2 + lk;
-            va[s2] = (rin && kk < m) ? V[arow * ldv + kk] : 0.0;
-        }
-        rd4 acc[6];
+        for (int s2 = 0; s2 < NS; s2++) va[s2] = V[arow * ldv + min(4 * s2 + lk, m - 1)];
+        rd4 acc[PT];
 #pragma unroll
-        for (int b = 0; b < 6; b++) acc[b] = rd4{0.0, 0.0, 0.0, 0.0};
+        for (int b = 0; b < PT; b++) acc[b] = rd4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int s2 = 0; s2 < 24; s2++) {
-            if (4 * s2 >= m) break;
+        for (int s2 = 0; s2 < NS; s2++)
 #pragma unroll
-            for (int b = 0; b < 6; b++)
-                if (b < pt)
-                    acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[s2], Cs[4 * s2 + lk][16 * b + lr], acc[b], 0, 0, 0);
-        }
+            for (int b = 0; b < PT; b++)
+                acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(va[s2], Cs[4 * s2 + lk][16 * b + lr], acc[b], 0, 0, 0);
 #pragma unroll
-        for (int b = 0; b < 6; b++) {
-            if (b >= pt) continue;
+        for (int b = 0; b < PT; b++) {
             const int c = 16 * b + lr;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
@@ -271,11 +268,30 @@ __global__ void __launch_bounds__(256) k_rgemm(const double *V, int64_t ldv, int
     }
 }
 
+template <int PT>
+static void rgemm_steps(const double *V, int64_t ldv, int m, const double *C, int p, double *X, int64_t ldx,
+                        int64_t n, double beta, hipStream_t st)
+{
+    const int64_t rows = 64 * RM_GROUPS;
+    const dim3 g((unsigned)((n + rows - 1) / rows)), t(256);
+    if (m <= 28) k_rgemm<PT, 7><<<g, t, 0, st>>>(V, ldv, m, C, p, X, ldx, n, beta);
+    else if (m <= 56) k_rgemm<PT, 14><<<g, t, 0, st>>>(V, ldv, m, C, p, X, ldx, n, beta);
+    else if (m <= 84) k_rgemm<PT, 21><<<g, t, 0, st>>>(V, ldv, m, C, p, X, ldx, n, beta);
+    else k_rgemm<PT, 24><<<g, t, 0, st>>>(V, ldv, m, C, p, X, ldx, n, beta);
+}
+
 void rgemm(const double *V, int64_t ldv, int m, const double *C, int p, double *X, int64_t ldx, int64_t n,
            double beta, hipStream_t st)
 {
-    const int64_t rows = 64 * RM_GROUPS;
-    k_rgemm<<<dim3((unsigned)((n + rows - 1) / rows)), dim3(256), 0, st>>>(V, ldv, m, C, p, X, ldx, n, beta);
+    if (n <= 0 || m <= 0 || p <= 0) return;     // (every caller has 1 <= m, p <= 96)
+    switch ((p + 15) / 16) {
+    case 1: rgemm_steps<1>(V, ldv, m, C, p, X, ldx, n, beta, st); break;
+    case 2: rgemm_steps<2>(V, ldv, m, C, p, X, ldx, n, beta, st); break;
+    case 3: rgemm_steps<3>(V, ldv, m, C, p, X, ldx, n, beta, st); break;
+    case 4: rgemm_steps<4>(V, ldv, m, C, p, X, ldx, n, beta, st); break;
+    case 5: rgemm_steps<5>(V, ldv, m, C, p, X, ldx, n, beta, st); break;
+    default: rgemm_steps<6>(V, ldv, m, C, p, X, ldx, n, beta, st); break;
+    }
 }
 
 // ------------------------------------------------------------------ Ritz step
