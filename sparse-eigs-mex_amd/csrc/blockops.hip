@@ -539,7 +539,12 @@ __global__ void __launch_bounds__(256) k_sell_rspmm(int64_t nslices, const int64
 // and gathers them as two 16-byte loads, one 512-byte load instruction per
 // nonzero for four rows instead of four 128-byte ones (the sums per column run
 // in the same nonzero order as above; 554 -> 494 us for 56 columns at 10^6
-// rows).  X and Y rows must be 16-byte aligned.
+// rows).  X and Y rows must be 16-byte aligned.  FULL (nv a multiple of 4: a
+// lane has four columns or none) issues every load with no branch: the slots
+// past a row's width in its last chunk read the row's last slot again with a
+// zero coefficient (adding exact zeros), a lane with no columns reads columns
+// 0..3 and stores nothing.
+template <bool FULL>
 __global__ void __launch_bounds__(256) k_sell_rspmm_q(int64_t nslices, const int64_t *__restrict__ sptr,
                                                       const int32_t *__restrict__ swid,
                                                       const int32_t *__restrict__ col,
@@ -555,6 +560,41 @@ __global__ void __launch_bounds__(256) k_sell_rspmm_q(int64_t nslices, const int
     int64_t pp = sptr[sl] + (row & 63);
     const int w = swid[sl];
     double s[4] = {0.0, 0.0, 0.0, 0.0};
+    if constexpr (FULL) {
+        const int cx = nc > 0 ? c0 : 0;
+        for (int k0 = 0; k0 < w; k0 += 8, pp += 8 * 64) {
+            const int kc = min(8, w - k0);
+            double a[8];
+            int64_t cc[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int64_t q = pp + 64 * min(u, kc - 1);
+                a[u] = val[q];
+                cc[u] = col[q];
+            }
+            double2 xv[8][2];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const double2 *xr = reinterpret_cast<const double2 *>(X + cc[u] * ldx + cx);
+                xv[u][0] = xr[0];
+                xv[u][1] = xr[1];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const double au = u < kc ? a[u] : 0.0;
+                s[0] = fma(au, xv[u][0].x, s[0]);
+                s[1] = fma(au, xv[u][0].y, s[1]);
+                s[2] = fma(au, xv[u][1].x, s[2]);
+                s[3] = fma(au, xv[u][1].y, s[3]);
+            }
+        }
+        if (nc > 0) {
+            double2 *yr = reinterpret_cast<double2 *>(Y + row * ldy + c0);
+            yr[0] = double2{s[0], s[1]};
+            yr[1] = double2{s[2], s[3]};
+        }
+        return;
+    }
     for (int k0 = 0; k0 < w; k0 += 8, pp += 8 * 64) {
         const int kc = min(8, w - k0);
         double a[8];
@@ -600,7 +640,10 @@ void rspmm(const SellA &S, const double *X, int64_t ldx, double *Y, int64_t ldy,
     const dim3 g((unsigned)((S.n + 15) / 16)), b(256);
     const bool al16 = ((uintptr_t)X | (uintptr_t)Y) % 16 == 0 && ldx % 2 == 0 && ldy % 2 == 0;
     if (nv > 32 && nv <= 64 && al16) {
-        k_sell_rspmm_q<<<g, b, 0, st>>>(S.nslices, S.sptr, S.swid, S.col, S.val, X, ldx, Y, ldy, S.n, nv);
+        if (nv % 4 == 0)
+            k_sell_rspmm_q<true><<<g, b, 0, st>>>(S.nslices, S.sptr, S.swid, S.col, S.val, X, ldx, Y, ldy, S.n, nv);
+        else
+            k_sell_rspmm_q<false><<<g, b, 0, st>>>(S.nslices, S.sptr, S.swid, S.col, S.val, X, ldx, Y, ldy, S.n, nv);
         return;
     }
     switch ((nv + 15) / 16) {
